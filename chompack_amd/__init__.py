@@ -1,0 +1,24 @@
+"""chompack_amd -- MI355X-native supernodal multifrontal chordal matrix library.
+
+Drop-in for the numeric hot path of CHOMPACK (src/python/__init__.py:24-44):
+``symbolic``/``cspmatrix`` objects with the reference's block storage, and
+``cholesky``, ``llt``, ``projected_inverse``, ``completion``, ``hessian``,
+``trsm`` and ``dot`` running as hand-written HIP kernels on gfx950 through the
+C-ABI in include/chompack_amd.h.  Symbolic analysis (AMD, elimination tree,
+supernodes, relative indices) runs on the host and is integer bit-exact.
+"""
+from .symbolic import symbolic, cspmatrix, merge_size_fill, peo
+from .misc import tril, triu, symmetrize, perm, eye, lmerge, \
+    frontal_add_update, frontal_get_update
+from . import amd
+
+try:
+    from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, dot
+except ImportError:  # numeric module not built into this checkout yet
+    pass
+
+__version__ = "0.1.0"
+
+__all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
+           "completion", "projected_inverse", "hessian", "trsm", "dot", "tril", "triu",
+           "symmetrize", "perm", "eye", "lmerge", "amd"]

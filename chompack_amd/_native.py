@@ -1,0 +1,86 @@
+"""ctypes binding of libchompack_amd.so (the C-ABI declared in include/chompack_amd.h).
+
+The library holds both the host-side symbolic routines (etree, counts,
+supernodes, embedding, AMD) and the HIP/gfx950 numeric kernels.  There is no
+fallback: if the library is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libchompack_amd.so")
+
+_lib = None
+
+i64 = ctypes.c_int64
+i32 = ctypes.c_int
+dbl = ctypes.c_double
+vp = ctypes.c_void_p
+P64 = ctypes.POINTER(ctypes.c_int64)
+
+
+def lib():
+    """Return the loaded library, loading it on first use (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(
+                "libchompack_amd.so not found at %s; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`" % LIBPATH)
+        L = ctypes.CDLL(LIBPATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    # host symbolic routines
+    L.chp_etree.argtypes = [i64, vp, vp, vp]
+    L.chp_etree.restype = None
+    L.chp_post_order.argtypes = [i64, vp, vp]
+    L.chp_post_order.restype = None
+    L.chp_counts.argtypes = [i64, vp, vp, vp, vp, vp]
+    L.chp_counts.restype = None
+    L.chp_supernodes.argtypes = [i64, vp, vp, vp, vp, vp, vp]
+    L.chp_supernodes.restype = i64
+    L.chp_embed.argtypes = [i64, vp, vp, vp, vp, vp, vp]
+    L.chp_embed.restype = i64
+    L.chp_relative_idx.argtypes = [i64, vp, vp, vp, vp, vp, vp]
+    L.chp_relative_idx.restype = i64
+    L.chp_amd_order.argtypes = [i64, vp, vp, vp]
+    L.chp_amd_order.restype = i32
+    # device plan and numeric sweeps (declared when present)
+    _opt(L, "chp_version", [], i32)
+    _opt(L, "chp_plan_create", [i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32], vp)
+    _opt(L, "chp_plan_destroy", [vp], None)
+    _opt(L, "chp_plan_info", [vp, vp], i32)
+    _opt(L, "chp_cholesky", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_llt", [vp, vp, vp], i32)
+    _opt(L, "chp_projected_inverse", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_trsm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
+    _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)
+    _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
+    _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
+    _opt(L, "chp_frontal_add_update", [vp, i64, vp, i64, vp, vp, dbl, vp], i32)
+    _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)
+    _opt(L, "chp_set_mode", [vp, i32], i32)
+    _opt(L, "chp_last_error", [], ctypes.c_char_p)
+
+
+def _opt(L, name, argtypes, restype):
+    if hasattr(L, name):
+        f = getattr(L, name)
+        f.argtypes = argtypes
+        f.restype = restype
+
+
+def ptr(a):
+    """Host pointer of a C-contiguous numpy array."""
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def as_i64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
