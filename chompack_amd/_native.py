@@ -53,7 +53,7 @@ def _declare(L):
     L.chp_amd_order.restype = i32
     # device plan and numeric sweeps (declared when present)
     _opt(L, "chp_version", [], i32)
-    _opt(L, "chp_plan_create", [i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32], vp)
+    _opt(L, "chp_plan_create", [i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, i32], vp)
     _opt(L, "chp_plan_destroy", [vp], None)
     _opt(L, "chp_plan_info", [vp, vp], i32)
     _opt(L, "chp_cholesky", [vp, vp, vp, vp], i32)
@@ -63,7 +63,7 @@ def _declare(L):
     _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)
     _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
     _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
-    _opt(L, "chp_frontal_add_update", [vp, i64, vp, i64, vp, vp, dbl, vp], i32)
+    _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)
     _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)
     _opt(L, "chp_set_mode", [vp, i32], i32)
     _opt(L, "chp_last_error", [], ctypes.c_char_p)

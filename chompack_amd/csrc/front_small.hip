@@ -1,0 +1,268 @@
+// Small-front kernels: one workgroup owns one frontal matrix, held entirely
+// in LDS, for fronts whose working set fits (the many small fronts near the
+// leaves of the clique tree).  All arithmetic is FP64.
+//
+// Frontal layout in LDS: column-major, leading dimension nj, matching the
+// block layout of blkval so block loads/stores are straight copies.
+#include "common.h"
+#include "kernels.h"
+
+#define TPB 256
+
+__global__ void k_status_init(unsigned long long* status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *status = CHP_STATUS_OK;
+}
+
+// Extend-add of all children of front k (reverse child order, one child at a
+// time) into the LDS front F (ld = nj).  F[r[i] + r[j]*nj] += U[i + j*N],
+// i >= j.  Follows cholesky.c:41-52 / cbase.c:11-34.
+__device__ inline void lds_extend_add(double* F, int nj, const FrontInfo& f,
+                                      const FrontInfo* __restrict__ fi,
+                                      const int* __restrict__ chidx,
+                                      const int* __restrict__ relidx,
+                                      const double* __restrict__ upd) {
+    for (int l = f.chend - 1; l >= f.chbeg; --l) {
+        const FrontInfo g = fi[chidx[l]];
+        const int N = g.na;
+        const int* r = relidx + g.relptr;
+        const double* U = upd + g.updptr;
+        for (int j = 0; j < N; ++j) {
+            const int cj = r[j] * nj;
+            for (int i = j + (int)threadIdx.x; i < N; i += TPB)
+                F[r[i] + cj] += U[i + (i64)j * N];
+        }
+        __syncthreads();
+    }
+}
+
+// Supernodal Cholesky of one small front (cholesky.c:30-81):
+// assemble, factor F11, L21 = F21 L11^-T, F22 -= L21 L21^T,
+// B21 = L21 L11^-1 (stored convention), write block and update matrix.
+__global__ __launch_bounds__(TPB) void k_chol_small(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx, const int* __restrict__ chidx,
+    unsigned long long* status) {
+    extern __shared__ double F[];
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    double* blk = blkval + f.blkptr;
+
+    for (int e = tid; e < nj * nj; e += TPB) {
+        int i = e % nj, j = e / nj;
+        F[e] = (j < nn && i >= j) ? blk[e] : 0.0;
+    }
+    __syncthreads();
+    lds_extend_add(F, nj, f, fi, chidx, relidx, upd);
+
+    // right-looking unblocked factorization of the first nn columns,
+    // updating the trailing part (including F22) as it goes
+    for (int c = 0; c < nn; ++c) {
+        const double d = F[c + c * nj];
+        if (!(d > 0.0)) {
+            if (tid == 0) chp_record_failure(status, k, c + 1);
+            return;
+        }
+        const double s = sqrt(d);
+        const double rs = 1.0 / s;
+        __syncthreads();
+        if (tid == 0) F[c + c * nj] = s;
+        for (int i = c + 1 + tid; i < nj; i += TPB) F[i + c * nj] *= rs;
+        __syncthreads();
+        const int m = nj - c - 1;
+        for (int e = tid; e < m * m; e += TPB) {
+            int i = c + 1 + e % m, j = c + 1 + e / m;
+            if (i >= j) F[i + j * nj] -= F[i + c * nj] * F[j + c * nj];
+        }
+        __syncthreads();
+    }
+    // B21 := L21 L11^{-1}, one row per thread (backward substitution)
+    for (int i = nn + tid; i < nj; i += TPB) {
+        for (int j = nn - 1; j >= 0; --j) {
+            double x = F[i + j * nj];
+            for (int m = j + 1; m < nn; ++m) x -= F[i + m * nj] * F[m + j * nj];
+            F[i + j * nj] = x / F[j + j * nj];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nj * nn; e += TPB) {
+        int i = e % nj, j = e / nj;
+        if (i >= j) blk[e] = F[e];
+    }
+    if (na > 0) {
+        double* U = upd + f.updptr;
+        for (int e = tid; e < na * na; e += TPB) {
+            int i = e % na, j = e / na;
+            if (i >= j) U[e] = F[(nn + i) + (nn + j) * nj];
+        }
+    }
+}
+
+// Gather V_k = F_p[r, r] (lower) from the parent's finished front into k's
+// update slot (projected_inverse.c:60-70 / cbase.c:39-59, done by the child).
+// The parent's front is its blkval block (columns < nn_p) and its own V_p
+// (columns >= nn_p).  One workgroup per (front, 16-column tile).
+__global__ __launch_bounds__(TPB) void k_extract(
+    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx) {
+    const int2 t = tiles[blockIdx.x];
+    const int k = t.x, j0 = t.y;
+    const FrontInfo f = fi[k];
+    const int na = f.na;
+    const FrontInfo p = fi[f.parent];
+    const int nnp = p.nn, njp = p.nn + p.na, nap = p.na;
+    const int* r = relidx + f.relptr;
+    const double* pblk = blkval + p.blkptr;
+    const double* pV = upd + p.updptr;
+    double* V = upd + f.updptr;
+    const int j1 = min(j0 + 16, na);
+    for (int j = j0; j < j1; ++j) {
+        const int rj = r[j];
+        for (int i = j + (int)threadIdx.x; i < na; i += TPB) {
+            const int ri = r[i];
+            double v = (rj < nnp) ? pblk[ri + (i64)rj * njp]
+                                  : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
+            V[i + (i64)j * na] = v;
+        }
+    }
+}
+
+// Projected inverse of one small front (projected_inverse.c:29-72), with
+// V_k already gathered into k's update slot.
+// LDS: Li (nn x nn), C (nn x nn), B (na x nn), T (na x nn), tmp (nn).
+__global__ __launch_bounds__(TPB) void k_projinv_small(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, const double* __restrict__ upd,
+    unsigned long long* status) {
+    extern __shared__ double S[];
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    double* blk = blkval + f.blkptr;
+    double* Li = S;
+    double* C = Li + nn * nn;
+    double* B = C + nn * nn;
+    double* T = B + na * nn;
+    double* tmp = T + na * nn;
+
+    for (int e = tid; e < nn * nn; e += TPB) {
+        int i = e % nn, j = e / nn;
+        Li[e] = (i >= j) ? blk[i + j * nj] : 0.0;
+    }
+    for (int e = tid; e < na * nn; e += TPB) {
+        int i = e % na, j = e / na;
+        B[e] = blk[nn + i + j * nj];
+    }
+    __syncthreads();
+    // dtrtri: first zero on the diagonal is an error
+    for (int j = 0; j < nn; ++j) {
+        if (Li[j + j * nn] == 0.0) {
+            if (tid == 0) chp_record_failure(status, k, j + 1);
+            return;
+        }
+    }
+    for (int j = nn - 1; j >= 0; --j) {
+        if (tid == 0) Li[j + j * nn] = 1.0 / Li[j + j * nn];
+        __syncthreads();
+        const double ajj = -Li[j + j * nn];
+        for (int i = j + 1 + tid; i < nn; i += TPB) {
+            double y = 0.0;
+            for (int m = j + 1; m <= i; ++m) y += Li[i + m * nn] * Li[m + j * nn];
+            tmp[i] = y;
+        }
+        __syncthreads();
+        for (int i = j + 1 + tid; i < nn; i += TPB) Li[i + j * nn] = tmp[i] * ajj;
+        __syncthreads();
+    }
+    // C = Li^T Li (lower)
+    for (int e = tid; e < nn * nn; e += TPB) {
+        int i = e % nn, j = e / nn;
+        if (i < j) continue;
+        double s = 0.0;
+        for (int m = i; m < nn; ++m) s += Li[m + i * nn] * Li[m + j * nn];
+        C[e] = s;
+    }
+    if (na > 0) {
+        const double* V = upd + f.updptr;
+        // T = V B21 with V symmetric (lower stored, ld na)
+        for (int e = tid; e < na * nn; e += TPB) {
+            int i = e % na, j = e / na;
+            double s = 0.0;
+            for (int m = 0; m < na; ++m) {
+                double v = (i >= m) ? V[i + (i64)m * na] : V[m + (i64)i * na];
+                s += v * B[m + j * na];
+            }
+            T[e] = s;
+        }
+        __syncthreads();
+        // C += T^T B21
+        for (int e = tid; e < nn * nn; e += TPB) {
+            int i = e % nn, j = e / nn;
+            if (i < j) continue;
+            double s = 0.0;
+            for (int m = 0; m < na; ++m) s += T[m + i * na] * B[m + j * na];
+            C[e] += s;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nn * nn; e += TPB) {
+        int i = e % nn, j = e / nn;
+        blk[i + j * nj] = (i >= j) ? C[e] : 0.0;
+    }
+    for (int e = tid; e < na * nn; e += TPB) {
+        int i = e % na, j = e / na;
+        blk[nn + i + j * nj] = -T[e];
+    }
+}
+
+// Chordal product X = L L^T for one small front (llt.c:30-57).
+// LDS: A (nj x nn, the block as stored) and F (nj x nj).
+__global__ __launch_bounds__(TPB) void k_llt_small(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx, const int* __restrict__ chidx) {
+    extern __shared__ double S[];
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    double* blk = blkval + f.blkptr;
+    double* A = S;
+    double* F = A + nj * nn;
+    for (int e = tid; e < nj * nn; e += TPB) A[e] = blk[e];
+    __syncthreads();
+    // B21 := B21 * L11 (L11 lower triangular), one row per thread
+    for (int i = nn + tid; i < nj; i += TPB) {
+        for (int j = 0; j < nn; ++j) {
+            double s = 0.0;
+            for (int m = j; m < nn; ++m) s += A[i + m * nj] * A[m + j * nj];
+            A[i + j * nj] = s;
+        }
+    }
+    __syncthreads();
+    // F = A A^T (lower)
+    for (int e = tid; e < nj * nj; e += TPB) {
+        int i = e % nj, j = e / nj;
+        double s = 0.0;
+        if (i >= j)
+            for (int m = 0; m < nn; ++m) s += A[i + m * nj] * A[j + m * nj];
+        F[e] = s;
+    }
+    __syncthreads();
+    lds_extend_add(F, nj, f, fi, chidx, relidx, upd);
+    for (int e = tid; e < nj * nn; e += TPB) {
+        int i = e % nj, j = e / nj;
+        if (i >= j) blk[e] = F[e];
+    }
+    if (na > 0) {
+        double* U = upd + f.updptr;
+        for (int e = tid; e < na * na; e += TPB) {
+            int i = e % na, j = e / na;
+            if (i >= j) U[e] = F[(nn + i) + (nn + j) * nj];
+        }
+    }
+}

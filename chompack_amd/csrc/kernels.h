@@ -1,0 +1,53 @@
+// Kernel declarations shared between the kernel sources and the host
+// scheduler (plan.hip).
+#pragma once
+#include "common.h"
+
+#define NB 64          // panel width of the large-front blocked algorithms
+#define TILE 64        // GEMM output tile (TILE x TILE)
+
+__global__ void k_status_init(unsigned long long* status);
+
+__global__ void k_chol_small(const int* fronts, const FrontInfo* fi, double* blkval,
+                             double* upd, const int* relidx, const int* chidx,
+                             unsigned long long* status);
+__global__ void k_extract(const int2* tiles, const FrontInfo* fi, const double* blkval,
+                          double* upd, const int* relidx);
+__global__ void k_projinv_small(const int* fronts, const FrontInfo* fi, double* blkval,
+                                const double* upd, unsigned long long* status);
+__global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkval,
+                            double* upd, const int* relidx, const int* chidx);
+
+// large fronts
+__global__ void k_extadd_tiles(const int2* tiles, const FrontInfo* fi, double* blkval,
+                               double* upd, const int* relidx, const int* chidx);
+__global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, double* blkval,
+                             double* dinv, const i64* dinvptr, unsigned long long* status);
+__global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,
+                             double* dinv, const i64* dinvptr, double* X, const i64* xptr,
+                             unsigned long long* status);
+__global__ void k_panel_mul(const int4* tiles, const FrontInfo* fi, double* blkval,
+                            const double* dinv, const i64* dinvptr);
+__global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, double* b0,
+                             double* b1, double* b2, double* b3);
+__global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkval,
+                           const double* scratch, const i64* sptr);
+
+// triangular solve with the chordal factor
+__global__ void k_solve_fwd(const int* fronts, const FrontInfo* fi, const double* blkval,
+                            const i64* snptr, const int* relidx, const int* chidx,
+                            const i64* perm, double* B, int ldb, int nrhs, double alpha,
+                            double* work, const i64* wptr);
+__global__ void k_solve_bwd(const int* fronts, const FrontInfo* fi, const double* blkval,
+                            const i64* snptr, const int* relidx, const i64* perm,
+                            double* B, int ldb, int nrhs, double alpha, double* work,
+                            const i64* wptr);
+
+// utilities
+__global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,
+                              const int* colfront, const double* x, const double* y,
+                              double* partial);
+__global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,
+                              i64 fr, i64 fc, i64 ur, i64 uc, double alpha);
+__global__ void k_frontal_get(const double* F, i64 N, const i64* r, double* U,
+                              i64 fr, i64 fc, i64 ur, i64 uc);

@@ -1,0 +1,627 @@
+// Device plan: level schedule + work lists for every sweep, built once per
+// symbolic factorization, and the C-ABI entry points (include/chompack_amd.h).
+//
+// A plan copies the symbolic arrays into HBM and precomputes, per level of
+// the supernodal elimination tree (level = height above the leaves), the
+// launch sequence of each sweep:
+//   * small fronts (LDS-resident)  -> one k_*_small launch per level;
+//   * large fronts                 -> column-tiled extend-add, 64-wide panel
+//     steps (diagonal factor + panel solve + MFMA trailing update), and the
+//     stored-convention solve, each step one batched launch for all large
+//     fronts of the level.
+// Executing a sweep replays the precomputed launches on the caller's stream;
+// there is no host work per front, only the final status read.
+#include "common.h"
+#include "kernels.h"
+#include "../../include/chompack_amd.h"
+
+#include <algorithm>
+#include <cstring>
+
+static thread_local std::string g_err;
+void chp_set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+enum Kid {
+    K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
+    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD
+};
+
+struct Launch {
+    Kid kid;
+    int grid;
+    int block;
+    size_t lds;
+    i64 list;   // byte offset of the work list in the list buffer
+    int iarg;
+};
+
+struct ListBuf {
+    std::vector<char> bytes;
+    template <class T>
+    i64 add(const std::vector<T>& v) {
+        i64 off = (i64)((bytes.size() + 255) / 256 * 256);
+        bytes.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+}  // namespace
+
+struct chp_plan {
+    int device = 0;
+    i64 n = 0, nsn = 0;
+    int nlevels = 0;
+    int small_max = 64;
+    std::vector<FrontInfo> fi;
+    std::vector<std::vector<int>> bylevel;
+    std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit;
+    // device arrays
+    FrontInfo* d_fi = nullptr;
+    int* d_relidx = nullptr;
+    int* d_chidx = nullptr;
+    i64* d_perm = nullptr;
+    i64* d_snptr = nullptr;
+    i64* d_dinvptr = nullptr;
+    i64* d_xptr = nullptr;
+    i64* d_tptr = nullptr;
+    i64* d_wunit = nullptr;
+    int* d_colfront = nullptr;
+    int* d_colidx = nullptr;
+    double* d_upd = nullptr;
+    double* d_dinv = nullptr;
+    double* d_scratch = nullptr;
+    double* d_partial = nullptr;
+    double* d_work = nullptr;
+    i64 work_len = 0;
+    unsigned long long* d_status = nullptr;
+    char* d_lists = nullptr;
+    GemmTask* d_tasks = nullptr;
+    i64 upd_len = 0, dinv_len = 0, scratch_len = 0, solve_len = 0;
+    std::vector<Launch> chol, projinv, llt, solve_fwd, solve_bwd;
+    bool llt_ok = true;
+    std::vector<GemmTask> tasks;
+    ListBuf lists;
+    int n_large_chol = 0, n_large_projinv = 0;
+    i64 n_gemm_tiles = 0;
+};
+
+namespace {
+
+const size_t LDS_SMALL_CAP = 64 * 1024;
+
+size_t chol_small_lds(const FrontInfo& f) {
+    size_t nj = f.nn + f.na;
+    return nj * nj * sizeof(double);
+}
+size_t projinv_small_lds(const FrontInfo& f) {
+    size_t nn = f.nn, na = f.na;
+    return (2 * nn * nn + 2 * na * nn + nn) * sizeof(double);
+}
+size_t llt_small_lds(const FrontInfo& f) {
+    size_t nn = f.nn, nj = f.nn + f.na;
+    return (nj * nn + nj * nj) * sizeof(double);
+}
+bool chol_is_small(const chp_plan* P, const FrontInfo& f) {
+    return f.nn + f.na <= P->small_max && chol_small_lds(f) <= LDS_SMALL_CAP;
+}
+bool projinv_is_small(const chp_plan* P, const FrontInfo& f) {
+    return f.nn + f.na <= P->small_max && projinv_small_lds(f) <= LDS_SMALL_CAP;
+}
+
+GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, int opB,
+            int bufC, i64 offC, int ldc, int M, int N, int K, double alpha, double beta,
+            int tri, int triA = 0, int triB = 0) {
+    GemmTask t;
+    std::memset(&t, 0, sizeof(t));
+    t.offA = offA; t.offB = offB; t.offC = offC;
+    t.lda = lda; t.ldb = ldb; t.ldc = ldc;
+    t.M = M; t.N = N; t.K = K;
+    t.bufA = (unsigned char)bufA; t.bufB = (unsigned char)bufB; t.bufC = (unsigned char)bufC;
+    t.opA = (unsigned char)opA; t.opB = (unsigned char)opB;
+    t.tri = (unsigned char)tri; t.triA = (unsigned char)triA; t.pad = (unsigned char)triB;
+    t.alpha = alpha; t.beta = beta;
+    return t;
+}
+
+void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, const GemmTask& t,
+                    bool skip_upper) {
+    if (t.M <= 0 || t.N <= 0) return;
+    int id = (int)P->tasks.size();
+    P->tasks.push_back(t);
+    int mt = (t.M + TILE - 1) / TILE, nt = (t.N + TILE - 1) / TILE;
+    for (int tn = 0; tn < nt; ++tn)
+        for (int tm = 0; tm < mt; ++tm) {
+            if (skip_upper && tm < tn) continue;
+            tiles.push_back({id, tm, tn, 0});
+        }
+}
+
+void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles) {
+    if (tiles.empty()) return;
+    P->n_gemm_tiles += (i64)tiles.size();
+    seq.push_back({K_GEMM, (int)tiles.size(), 256, 0, P->lists.add(tiles), 0});
+}
+
+// Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
+void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
+    std::vector<int> small, large;
+    size_t lds = 0;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        if (chol_is_small(P, f)) { small.push_back(k); lds = std::max(lds, chol_small_lds(f)); }
+        else large.push_back(k);
+    }
+    if (!small.empty())
+        P->chol.push_back({K_CHOL_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+    if (large.empty()) return;
+    P->n_large_chol += (int)large.size();
+    std::vector<int2> et;
+    for (int k : large) {
+        int nj = P->fi[k].nn + P->fi[k].na;
+        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
+    }
+    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 0});
+    int maxnn = 0;
+    for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
+    for (int c0 = 0; c0 < maxnn; c0 += NB) {
+        std::vector<int> act;
+        for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
+        P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 64, 0, P->lists.add(act), c0});
+        std::vector<int4> pt;
+        std::vector<GemmTile> gt;
+        for (int k : act) {
+            const FrontInfo& f = P->fi[k];
+            int nn = f.nn, na = f.na, nj = nn + na;
+            int c1 = std::min(c0 + NB, nn);
+            for (int r0 = c1; r0 < nj; r0 += 64) pt.push_back(make_int4(k, r0, c0, 0));
+            // trailing update F[c1:, c1:] -= L[c1:, c0:c1] L[c1:, c0:c1]^T
+            if (c1 < nn)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
+                                         0, f.blkptr + c1 + (i64)c0 * nj, nj, 1,
+                                         0, f.blkptr + c1 + (i64)c1 * nj, nj,
+                                         nj - c1, nn - c1, c1 - c0, -1.0, 1.0, 1), true);
+            if (na > 0)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)c0 * nj, nj, 0,
+                                         0, f.blkptr + nn + (i64)c0 * nj, nj, 1,
+                                         1, f.updptr, na, na, na, c1 - c0, -1.0, 1.0, 1), true);
+        }
+        if (!pt.empty())
+            P->chol.push_back({K_PANEL_MUL, (int)pt.size(), 256, 0, P->lists.add(pt), 0});
+        push_gemm(P, P->chol, gt);
+    }
+    // stored convention B21 = L21 L11^{-1}: column blocks from the right
+    int maxblk = 0;
+    for (int k : large)
+        if (P->fi[k].na > 0) maxblk = std::max(maxblk, (P->fi[k].nn + NB - 1) / NB);
+    for (int s = 0; s < maxblk; ++s) {
+        std::vector<int4> pt;
+        std::vector<GemmTile> gt;
+        for (int k : large) {
+            const FrontInfo& f = P->fi[k];
+            int nn = f.nn, na = f.na, nj = nn + na;
+            int nbk = (nn + NB - 1) / NB;
+            if (na == 0 || s >= nbk) continue;
+            int b = nbk - 1 - s, b0 = b * NB, b1 = std::min(b0 + NB, nn);
+            for (int r0 = nn; r0 < nj; r0 += 64) pt.push_back(make_int4(k, r0, b0, 1));
+            if (b0 > 0)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
+                                         0, f.blkptr + b0, nj, 0,
+                                         0, f.blkptr + nn, nj, na, b0, b1 - b0, -1.0, 1.0, 0),
+                               false);
+        }
+        if (!pt.empty())
+            P->chol.push_back({K_PANEL_MUL, (int)pt.size(), 256, 0, P->lists.add(pt), 0});
+        push_gemm(P, P->chol, gt);
+    }
+}
+
+// Projected-inverse launches for one level (projected_inverse.c:25-73).
+void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need) {
+    std::vector<int2> xt;
+    for (int k : fr)
+        for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
+    if (!xt.empty())
+        P->projinv.push_back({K_EXTRACT, (int)xt.size(), 256, 0, P->lists.add(xt), 0});
+    std::vector<int> small, large;
+    size_t lds = 0;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        if (projinv_is_small(P, f)) { small.push_back(k); lds = std::max(lds, projinv_small_lds(f)); }
+        else large.push_back(k);
+    }
+    if (!small.empty())
+        P->projinv.push_back({K_PROJINV_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+    if (large.empty()) return;
+    P->n_large_projinv += (int)large.size();
+    // per-front scratch: X = L11^{-1} (nn x nn), T = V B21 (na x nn), W (NB x nn)
+    i64 off = 0;
+    std::vector<i64> woff(large.size());
+    for (size_t q = 0; q < large.size(); ++q) {
+        const FrontInfo& f = P->fi[large[q]];
+        P->h_xptr[large[q]] = off; off += (i64)f.nn * f.nn;
+        P->h_tptr[large[q]] = off; off += (i64)f.na * f.nn;
+        woff[q] = off; off += (i64)NB * f.nn;
+        off = (off + 31) / 32 * 32;
+    }
+    scratch_need = std::max(scratch_need, off);
+    std::vector<int2> bt;
+    for (int k : large) {
+        int nbk = (P->fi[k].nn + NB - 1) / NB;
+        for (int b = 0; b < nbk; ++b) bt.push_back(make_int2(k, b));
+    }
+    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 64, 0, P->lists.add(bt), 0});
+    // block rows of X = L11^{-1}:  W = L[bi, :b0] X[:b0, :b0];  X[bi, :b0] = -Dinv_i W
+    int maxblk = 0;
+    for (int k : large) maxblk = std::max(maxblk, (P->fi[k].nn + NB - 1) / NB);
+    for (int i = 1; i < maxblk; ++i) {
+        std::vector<GemmTile> g1, g2;
+        for (size_t q = 0; q < large.size(); ++q) {
+            int k = large[q];
+            const FrontInfo& f = P->fi[k];
+            int nn = f.nn, nj = nn + f.na;
+            int nbk = (nn + NB - 1) / NB;
+            if (i >= nbk) continue;
+            int b0 = i * NB, b1 = std::min(b0 + NB, nn), nb = b1 - b0;
+            i64 xo = P->h_xptr[k];
+            add_gemm_tiles(P, g1, mk(0, f.blkptr + b0, nj, 0, 2, xo, nn, 0,
+                                     2, woff[q], NB, nb, b0, b0, 1.0, 0.0, 0, 0, 1), false);
+            add_gemm_tiles(P, g2, mk(3, P->h_dinvptr[k] + (i64)i * NB * NB, NB, 0,
+                                     2, woff[q], NB, 0, 2, xo + b0, nn, nb, b0, nb,
+                                     -1.0, 0.0, 0, 1, 0), false);
+        }
+        push_gemm(P, P->projinv, g1);
+        push_gemm(P, P->projinv, g2);
+    }
+    // F11 = X^T X (lower, strict upper zeroed) and T = V B21
+    std::vector<GemmTile> g3, g4;
+    for (int k : large) {
+        const FrontInfo& f = P->fi[k];
+        int nn = f.nn, na = f.na, nj = nn + na;
+        i64 xo = P->h_xptr[k], to = P->h_tptr[k];
+        add_gemm_tiles(P, g3, mk(2, xo, nn, 1, 2, xo, nn, 0, 0, f.blkptr, nj,
+                                 nn, nn, nn, 1.0, 0.0, 2, 2, 1), false);
+        if (na > 0) {
+            add_gemm_tiles(P, g3, mk(1, f.updptr, na, 2, 0, f.blkptr + nn, nj, 0,
+                                     2, to, na, na, nn, na, 1.0, 0.0, 0), false);
+            add_gemm_tiles(P, g4, mk(2, to, na, 1, 0, f.blkptr + nn, nj, 0,
+                                     0, f.blkptr, nj, nn, nn, na, 1.0, 1.0, 1), true);
+        }
+    }
+    push_gemm(P, P->projinv, g3);
+    push_gemm(P, P->projinv, g4);
+    std::vector<int2> nt;
+    for (int k : large)
+        if (P->fi[k].na > 0)
+            for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
+    if (!nt.empty())
+        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->lists.add(nt), 0});
+}
+
+void build_llt_level(chp_plan* P, const std::vector<int>& fr) {
+    std::vector<int> small;
+    size_t lds = 0;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        if (llt_small_lds(f) <= LDS_SMALL_CAP) { small.push_back(k); lds = std::max(lds, llt_small_lds(f)); }
+        else P->llt_ok = false;
+    }
+    if (!small.empty())
+        P->llt.push_back({K_LLT_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+}
+
+template <class T>
+int upload(T** dptr, const std::vector<T>& v) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIP_CHECK(hipMalloc((void**)dptr, bytes));
+    if (!v.empty()) HIP_CHECK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int alloc_doubles(double** d, i64 len) {
+    HIP_CHECK(hipMalloc((void**)d, (size_t)std::max<i64>(len, 1) * sizeof(double)));
+    return 0;
+}
+
+int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStream_t s) {
+    for (const Launch& L : seq) {
+        const void* list = P->d_lists + L.list;
+        dim3 g(L.grid), b(L.block);
+        switch (L.kid) {
+        case K_CHOL_SMALL:
+            hipLaunchKernelGGL(k_chol_small, g, b, L.lds, s, (const int*)list, P->d_fi, blkval,
+                               P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
+            break;
+        case K_EXTADD:
+            hipLaunchKernelGGL(k_extadd_tiles, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+                               P->d_upd, P->d_relidx, P->d_chidx);
+            break;
+        case K_POTRF_DIAG:
+            hipLaunchKernelGGL(k_potrf_diag, g, b, 0, s, (const int*)list, L.iarg, P->d_fi,
+                               blkval, P->d_dinv, P->d_dinvptr, P->d_status);
+            break;
+        case K_PANEL_MUL:
+            hipLaunchKernelGGL(k_panel_mul, g, b, 0, s, (const int4*)list, P->d_fi, blkval,
+                               P->d_dinv, P->d_dinvptr);
+            break;
+        case K_GEMM:
+            hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
+                               blkval, P->d_upd, P->d_scratch, P->d_dinv);
+            break;
+        case K_EXTRACT:
+            hipLaunchKernelGGL(k_extract, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+                               P->d_upd, P->d_relidx);
+            break;
+        case K_PROJINV_SMALL:
+            hipLaunchKernelGGL(k_projinv_small, g, b, L.lds, s, (const int*)list, P->d_fi,
+                               blkval, P->d_upd, P->d_status);
+            break;
+        case K_TRTRI_DIAG:
+            hipLaunchKernelGGL(k_trtri_diag, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+                               P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
+            break;
+        case K_NEG_COPY:
+            hipLaunchKernelGGL(k_neg_copy, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+                               P->d_scratch, P->d_tptr);
+            break;
+        case K_LLT_SMALL:
+            hipLaunchKernelGGL(k_llt_small, g, b, L.lds, s, (const int*)list, P->d_fi, blkval,
+                               P->d_upd, P->d_relidx, P->d_chidx);
+            break;
+        default:
+            chp_set_error("internal: unexpected launch kind");
+            return -3;
+        }
+    }
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int read_status(chp_plan* P, hipStream_t s, i64* info) {
+    unsigned long long h = CHP_STATUS_OK;
+    HIP_CHECK(hipMemcpyAsync(&h, P->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h == CHP_STATUS_OK) {
+        if (info) { info[0] = -1; info[1] = 0; }
+        return 0;
+    }
+    if (info) { info[0] = (i64)(h >> 32); info[1] = (i64)(h & 0xffffffffULL); }
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* chp_last_error(void) { return g_err.c_str(); }
+
+int chp_version(void) { return 1; }
+
+// See include/chompack_amd.h.
+chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
+                          const i64* relptr, const i64* relidx, const i64* chptr,
+                          const i64* chidx, const i64* blkptr, const i64* perm,
+                          int small_max) {
+    chp_plan* P = new chp_plan();
+    auto fail = [&](const std::string& m) -> chp_plan* { chp_set_error(m); delete P; return nullptr; };
+    if (hipGetDevice(&P->device) != hipSuccess) return fail("no HIP device");
+    P->n = n; P->nsn = nsn;
+    P->small_max = small_max > 0 ? small_max : 64;
+    std::vector<int> parent(nsn), level(nsn, 0);
+    for (i64 k = 0; k < nsn; ++k) parent[k] = (int)k;
+    for (i64 k = 0; k < nsn; ++k)
+        for (i64 l = chptr[k]; l < chptr[k + 1]; ++l) {
+            if (chidx[l] < 0 || chidx[l] >= nsn) return fail("invalid chidx");
+            parent[chidx[l]] = (int)k;
+        }
+    for (i64 t = 0; t < nsn; ++t) {
+        i64 k = snpost[t];
+        if (k < 0 || k >= nsn) return fail("invalid snpost");
+        for (i64 l = chptr[k]; l < chptr[k + 1]; ++l)
+            level[k] = std::max(level[k], level[chidx[l]] + 1);
+    }
+    int H = 0;
+    for (i64 k = 0; k < nsn; ++k) H = std::max(H, level[k] + 1);
+    P->nlevels = H;
+    P->bylevel.assign(H, {});
+    for (i64 t = 0; t < nsn; ++t) P->bylevel[level[snpost[t]]].push_back((int)snpost[t]);
+    P->fi.resize(nsn);
+    i64 upd = 0, dinv = 0, wun = 0;
+    P->h_dinvptr.assign(nsn, 0);
+    P->h_xptr.assign(nsn, 0);
+    P->h_tptr.assign(nsn, 0);
+    P->h_wunit.assign(nsn, 0);
+    for (i64 k = 0; k < nsn; ++k) {
+        FrontInfo& f = P->fi[k];
+        f.nn = (int)(snptr[k + 1] - snptr[k]);
+        f.na = (int)(relptr[k + 1] - relptr[k]);
+        if ((blkptr[k + 1] - blkptr[k]) != (i64)f.nn * (f.nn + f.na))
+            return fail("blkptr inconsistent with snptr/relptr");
+        f.chbeg = (int)chptr[k];
+        f.chend = (int)chptr[k + 1];
+        f.parent = parent[k];
+        f.level = level[k];
+        f.blkptr = blkptr[k];
+        f.updptr = upd;
+        f.relptr = relptr[k];
+        upd += (i64)f.na * f.na;
+        if (!chol_is_small(P, f) || !projinv_is_small(P, f)) {
+            P->h_dinvptr[k] = dinv;
+            dinv += (i64)((f.nn + NB - 1) / NB) * NB * NB;
+        }
+        P->h_wunit[k] = wun;
+        wun += f.nn + f.na;
+    }
+    P->upd_len = upd;
+    P->dinv_len = dinv;
+    P->solve_len = wun;
+    for (int L = 0; L < H; ++L) build_chol_level(P, P->bylevel[L]);
+    i64 scratch = 0;
+    for (int L = H - 1; L >= 0; --L) build_projinv_level(P, P->bylevel[L], scratch);
+    P->scratch_len = scratch;
+    for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L]);
+    for (int L = 0; L < H; ++L) {
+        const auto& fr = P->bylevel[L];
+        P->solve_fwd.push_back({K_SOLVE_FWD, (int)fr.size(), 256, 0, P->lists.add(fr), 0});
+    }
+    for (int L = H - 1; L >= 0; --L) {
+        const auto& fr = P->bylevel[L];
+        P->solve_bwd.push_back({K_SOLVE_BWD, (int)fr.size(), 256, 0, P->lists.add(fr), 0});
+    }
+    // device copies
+    std::vector<int> rel32(relptr[nsn]), ch32(chptr[nsn]);
+    for (i64 i = 0; i < relptr[nsn]; ++i) rel32[i] = (int)relidx[i];
+    for (i64 i = 0; i < chptr[nsn]; ++i) ch32[i] = (int)chidx[i];
+    std::vector<i64> pv(perm, perm + n), sp(snptr, snptr + nsn + 1);
+    std::vector<int> colfront(n), colidx(n);
+    for (i64 k = 0; k < nsn; ++k)
+        for (i64 c = snptr[k]; c < snptr[k + 1]; ++c) {
+            colfront[c] = (int)k;
+            colidx[c] = (int)(c - snptr[k]);
+        }
+    int rc = 0;
+    rc |= upload(&P->d_fi, P->fi);
+    rc |= upload(&P->d_relidx, rel32);
+    rc |= upload(&P->d_chidx, ch32);
+    rc |= upload(&P->d_perm, pv);
+    rc |= upload(&P->d_snptr, sp);
+    rc |= upload(&P->d_dinvptr, P->h_dinvptr);
+    rc |= upload(&P->d_xptr, P->h_xptr);
+    rc |= upload(&P->d_tptr, P->h_tptr);
+    rc |= upload(&P->d_wunit, P->h_wunit);
+    rc |= upload(&P->d_colfront, colfront);
+    rc |= upload(&P->d_colidx, colidx);
+    rc |= upload(&P->d_lists, P->lists.bytes);
+    rc |= upload(&P->d_tasks, P->tasks);
+    rc |= alloc_doubles(&P->d_upd, P->upd_len);
+    rc |= alloc_doubles(&P->d_dinv, P->dinv_len);
+    rc |= alloc_doubles(&P->d_scratch, P->scratch_len);
+    rc |= alloc_doubles(&P->d_partial, n);
+    if (hipMalloc((void**)&P->d_status, sizeof(unsigned long long)) != hipSuccess) rc = -1;
+    if (rc) {
+        std::string e = g_err;
+        chp_plan_destroy(P);
+        chp_set_error("plan allocation failed: " + e);
+        return nullptr;
+    }
+    return P;
+}
+
+void chp_plan_destroy(chp_plan* P) {
+    if (!P) return;
+    void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
+                    P->d_xptr, P->d_tptr, P->d_wunit, P->d_colfront, P->d_colidx, P->d_upd,
+                    P->d_dinv, P->d_scratch, P->d_partial, P->d_work, P->d_status,
+                    P->d_lists, P->d_tasks};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete P;
+}
+
+// out[0..7] = nlevels, n_large_chol, n_large_projinv, n_gemm_tiles,
+//             upd_len, dinv_len, scratch_len, n launches (cholesky)
+int chp_plan_info(chp_plan* P, i64* out) {
+    if (!P || !out) return -1;
+    out[0] = P->nlevels; out[1] = P->n_large_chol; out[2] = P->n_large_projinv;
+    out[3] = P->n_gemm_tiles; out[4] = P->upd_len; out[5] = P->dinv_len;
+    out[6] = P->scratch_len; out[7] = (i64)P->chol.size();
+    return 0;
+}
+
+int chp_cholesky(chp_plan* P, double* blkval, void* stream, i64* info) {
+    if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    int rc = run_seq(P, P->chol, blkval, s);
+    if (rc) return rc;
+    return read_status(P, s, info);
+}
+
+int chp_projected_inverse(chp_plan* P, double* blkval, void* stream, i64* info) {
+    if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    int rc = run_seq(P, P->projinv, blkval, s);
+    if (rc) return rc;
+    return read_status(P, s, info);
+}
+
+int chp_llt(chp_plan* P, double* blkval, void* stream) {
+    if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    if (!P->llt_ok) { chp_set_error("llt: fronts larger than the LDS path are not supported yet"); return -4; }
+    int rc = run_seq(P, P->llt, blkval, (hipStream_t)stream);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
+             int trans, void* stream) {
+    if (!P || !blkval || !B) { chp_set_error("null argument"); return -1; }
+    if (trans != 'N' && trans != 'T') { chp_set_error("trans must be 'N' or 'T'"); return -1; }
+    if (nrhs <= 0) return 0;
+    if (ldb < P->n) { chp_set_error("ldB is smaller than n"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    i64 need = P->solve_len * nrhs;
+    if (need > P->work_len) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (P->d_work) HIP_CHECK(hipFree(P->d_work));
+        P->d_work = nullptr;
+        HIP_CHECK(hipMalloc((void**)&P->d_work, need * sizeof(double)));
+        P->work_len = need;
+    }
+    const std::vector<Launch>& seq = trans == 'N' ? P->solve_fwd : P->solve_bwd;
+    for (const Launch& L : seq) {
+        const int* list = (const int*)(P->d_lists + L.list);
+        if (trans == 'N')
+            hipLaunchKernelGGL(k_solve_fwd, dim3(L.grid), dim3(256), 0, s, list, P->d_fi, blkval,
+                               P->d_snptr, P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs,
+                               alpha, P->d_work, P->d_wunit);
+        else
+            hipLaunchKernelGGL(k_solve_bwd, dim3(L.grid), dim3(256), 0, s, list, P->d_fi, blkval,
+                               P->d_snptr, P->d_relidx, P->d_perm, B, ldb, nrhs, alpha,
+                               P->d_work, P->d_wunit);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int chp_dot(chp_plan* P, const double* x, const double* y, double* out, void* stream) {
+    if (!P || !x || !y || !out) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    if (P->n == 0) { *out = 0.0; return 0; }
+    hipLaunchKernelGGL(k_dot_partial, dim3((unsigned)P->n), dim3(256), 0, s, P->d_colidx,
+                       (int)P->n, P->d_fi, P->d_colfront, x, y, P->d_partial);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> h(P->n);
+    HIP_CHECK(hipMemcpyAsync(h.data(), P->d_partial, P->n * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double v = 0.0;
+    for (i64 j = 0; j < P->n; ++j) v += h[j];
+    *out = v;
+    return 0;
+}
+
+int chp_frontal_add_update(double* F, i64 N, const i64* relidx, const double* U,
+                           const i64* strides, double alpha, void* stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_frontal_add, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream, F,
+                       N, relidx, U, strides[0], strides[1], strides[2], strides[3], alpha);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int chp_frontal_get_update(const double* F, i64 N, const i64* relidx, double* U,
+                           const i64* strides, void* stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_frontal_get, dim3((unsigned)N), dim3(256), 0, (hipStream_t)stream, F,
+                       N, relidx, U, strides[0], strides[1], strides[2], strides[3]);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// Multi-RHS triangular solve with the chordal factor (trsm.c:32-136) and
+// small utility kernels (trace inner product, frontal add/extract).
+//
+// Each front k owns a column-major nj x nrhs slab of the work buffer
+// (offset wptr[k] * nrhs, ld nj).  In the forward sweep a parent reads its
+// children's update vectors straight out of the children's slabs (rows
+// nn_c..nj_c); in the backward sweep a child reads its separator values out
+// of its parent's slab.  One workgroup per front, one launch per level.
+#include "common.h"
+#include "kernels.h"
+
+#define TPB 256
+
+// B := alpha L^{-1} B, leaves to root (trsm.c:36-81).
+__global__ __launch_bounds__(TPB) void k_solve_fwd(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, const i64* __restrict__ snptr,
+    const int* __restrict__ relidx, const int* __restrict__ chidx,
+    const i64* __restrict__ perm, double* __restrict__ B, int ldb, int nrhs,
+    double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    const double* L = blkval + f.blkptr;
+    double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];  // original rows of this supernode's columns
+    for (int e = tid; e < nj * nrhs; e += TPB) {
+        int i = e % nj, j = e / nj;
+        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
+    }
+    __syncthreads();
+    for (int l = f.chend - 1; l >= f.chbeg; --l) {
+        const int c = chidx[l];
+        const FrontInfo g = fi[c];
+        const int N = g.na, njc = g.nn + g.na;
+        const int* r = relidx + g.relptr;
+        const double* Wc = work + wptr[c] * nrhs + g.nn;
+        for (int e = tid; e < N * nrhs; e += TPB) {
+            int i = e % N, j = e / N;
+            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
+        }
+        __syncthreads();
+    }
+    // x_A -= B21 x_N  (uses x_N before the solve: B21 = L21 L11^-1)
+    for (int e = tid; e < na * nrhs; e += TPB) {
+        int i = e % na, j = e / na;
+        double s = W[nn + i + (i64)j * nj];
+        for (int m = 0; m < nn; ++m) s -= L[nn + i + (i64)m * nj] * W[m + (i64)j * nj];
+        W[nn + i + (i64)j * nj] = s;
+    }
+    // x_N := L11^{-1} x_N, column oriented
+    for (int c = 0; c < nn; ++c) {
+        __syncthreads();
+        const double d = L[c + (i64)c * nj];
+        for (int j = tid; j < nrhs; j += TPB) W[c + (i64)j * nj] /= d;
+        __syncthreads();
+        const int m = nn - c - 1;
+        for (int e = tid; e < m * nrhs; e += TPB) {
+            int i = c + 1 + e % m, j = e / m;
+            W[i + (i64)j * nj] -= L[i + (i64)c * nj] * W[c + (i64)j * nj];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nn * nrhs; e += TPB) {
+        int i = e % nn, j = e / nn;
+        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+    }
+}
+
+// B := alpha L^{-T} B, root to leaves (trsm.c:84-135).
+__global__ __launch_bounds__(TPB) void k_solve_bwd(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, const i64* __restrict__ snptr,
+    const int* __restrict__ relidx, const i64* __restrict__ perm,
+    double* __restrict__ B, int ldb, int nrhs, double alpha,
+    double* __restrict__ work, const i64* __restrict__ wptr) {
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    const double* L = blkval + f.blkptr;
+    double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];
+    for (int e = tid; e < nn * nrhs; e += TPB) {
+        int i = e % nn, j = e / nn;
+        W[i + (i64)j * nj] = alpha * B[pk[i] + (i64)j * ldb];
+    }
+    // x_N := L11^{-T} x_N (backward substitution, axpy form)
+    for (int c = nn - 1; c >= 0; --c) {
+        __syncthreads();
+        const double d = L[c + (i64)c * nj];
+        for (int j = tid; j < nrhs; j += TPB) W[c + (i64)j * nj] /= d;
+        __syncthreads();
+        for (int e = tid; e < c * nrhs; e += TPB) {
+            int i = e % c, j = e / c;
+            W[i + (i64)j * nj] -= L[c + (i64)i * nj] * W[c + (i64)j * nj];
+        }
+    }
+    if (na > 0) {
+        const FrontInfo p = fi[f.parent];
+        const int njp = p.nn + p.na;
+        const double* Wp = work + wptr[f.parent] * nrhs;
+        const int* r = relidx + f.relptr;
+        for (int e = tid; e < na * nrhs; e += TPB) {
+            int i = e % na, j = e / na;
+            W[nn + i + (i64)j * nj] = Wp[r[i] + (i64)j * njp];
+        }
+        __syncthreads();
+        // x_N -= B21^T x_A
+        for (int e = tid; e < nn * nrhs; e += TPB) {
+            int m = e % nn, j = e / nn;
+            double s = W[m + (i64)j * nj];
+            for (int i = 0; i < na; ++i) s -= L[nn + i + (i64)m * nj] * W[nn + i + (i64)j * nj];
+            W[m + (i64)j * nj] = s;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nn * nrhs; e += TPB) {
+        int i = e % nn, j = e / nn;
+        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+    }
+}
+
+// Partial trace inner products, one per supernode column j of the listed
+// range: 2 * sum_{i >= j} X[i,j] Y[i,j] - X[j,j] Y[j,j] (dot.c:9-20).
+// colfront[j] = supernode of column j; partial[j] receives the value.
+__global__ __launch_bounds__(TPB) void k_dot_partial(
+    const int* __restrict__ cols, int ncols, const FrontInfo* __restrict__ fi,
+    const int* __restrict__ colfront, const double* __restrict__ x,
+    const double* __restrict__ y, double* __restrict__ partial) {
+    __shared__ double red[TPB];
+    const int j = blockIdx.x;
+    if (j >= ncols) return;
+    const int k = colfront[j];
+    const FrontInfo f = fi[k];
+    const int nj = f.nn + f.na;
+    const int jl = cols[j];  // column index inside the supernode
+    const i64 off = f.blkptr + jl + (i64)jl * nj;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nj - jl; i += TPB) s += x[off + i] * y[off + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[j] = 2.0 * red[0] - x[off] * y[off];
+}
+
+// F[r[i], r[j]] += alpha U[i, j] (i >= j), general strides (cbase.c:11-34).
+__global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,
+                              i64 fr, i64 fc, i64 ur, i64 uc, double alpha) {
+    const i64 j = blockIdx.x;
+    for (i64 i = j + threadIdx.x; i < N; i += blockDim.x)
+        F[r[i] * fr + r[j] * fc] += alpha * U[i * ur + j * uc];
+}
+
+// U[i, j] = F[r[i], r[j]] (i >= j) (cbase.c:39-59).
+__global__ void k_frontal_get(const double* F, i64 N, const i64* r, double* U,
+                              i64 fr, i64 fc, i64 ur, i64 uc) {
+    const i64 j = blockIdx.x;
+    for (i64 i = j + threadIdx.x; i < N; i += blockDim.x)
+        U[i * ur + j * uc] = F[r[i] * fr + r[j] * fc];
+}

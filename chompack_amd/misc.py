@@ -125,8 +125,8 @@ def frontal_add_update(F, U, relidx, relptr, i, alpha=1.0):
     d_r = torch.as_tensor(r, device=F.device)
     stream = torch.cuda.current_stream(F.device).cuda_stream
     st = (ctypes_i64 * 4)(fr, fc, ur, uc)
-    rc = lib.chp_frontal_add_update(F.data_ptr(), N, d_r.data_ptr(), 0,
-                                    U.data_ptr(), st, float(alpha), stream)
+    rc = lib.chp_frontal_add_update(F.data_ptr(), N, d_r.data_ptr(), U.data_ptr(), st,
+                                    float(alpha), stream)
     _check(rc)
     return None
 

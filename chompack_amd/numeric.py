@@ -1,0 +1,198 @@
+"""Numeric sweeps on the GPU: the drop-in for chompack.cbase's
+``cholesky, llt, projected_inverse, completion, hessian, trsm`` and
+``base.dot`` (src/python/__init__.py:27-33, src/C/cbase.c:106-1077).
+
+Every function has the reference's signature, in-place semantics,
+``is_factor`` state machine and exception types/messages; the work runs as
+hand-written HIP kernels through the C-ABI (include/chompack_amd.h).  There is
+no CPU fallback: without the HIP library or a GPU these functions raise.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .symbolic import cspmatrix
+
+__all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian",
+           "trsm", "dot"]
+
+SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("chompack_amd numeric routines need a ROCm GPU (none visible)")
+    return torch
+
+
+def _check_blkval(X):
+    torch = _torch()
+    b = X.blkval
+    if not (isinstance(b, torch.Tensor) and b.is_cuda and b.dtype == torch.float64
+            and b.is_contiguous()):
+        raise TypeError("blkval must be a contiguous float64 tensor in GPU memory")
+    return b
+
+
+def _stream(t):
+    torch = _torch()
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _raise_rc(rc, what):
+    if rc < 0:
+        msg = _native.lib().chp_last_error()
+        raise RuntimeError("%s: %s" % (what, msg.decode() if msg else "error %d" % rc))
+
+
+class Plan(object):
+    """Device plan of one symbolic factorization (level schedule, index arrays
+    and work arrays resident in HBM).  Built on first use, cached on the
+    symbolic object, freed with it."""
+
+    def __init__(self, symb, device, small_max=None):
+        torch = _torch()
+        lib = _native.lib()
+        self.symb_arrays = [_native.as_i64(a) for a in (
+            symb.snpost, symb.snptr, symb.relptr, symb.relidx, symb.chptr, symb.chidx,
+            symb.blkptr, symb.p)]
+        a = self.symb_arrays
+        self.device = torch.device(device)
+        with torch.cuda.device(self.device):
+            h = lib.chp_plan_create(symb.n, symb.Nsn, *[_native.ptr(x) for x in a],
+                                    int(small_max or SMALL_MAX))
+        if not h:
+            raise RuntimeError("plan creation failed: %s" % lib.chp_last_error().decode())
+        self.handle = h
+        self._lib = lib
+
+    def info(self):
+        out = np.zeros(8, dtype=np.int64)
+        self._lib.chp_plan_info(self.handle, _native.ptr(out))
+        keys = ["levels", "large_chol", "large_projinv", "gemm_tiles", "upd_doubles",
+                "dinv_doubles", "scratch_doubles", "chol_launches"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                self._lib.chp_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def _plan(X):
+    b = _check_blkval(X)
+    return X.symb._plan(b.device), b
+
+
+def cholesky(X):
+    """Supernodal multifrontal Cholesky factorization X = L L^T, in place
+    (cbase.c:187-284).  On exit X holds the factor (stored convention: the
+    (2,1) block of supernode k is L_{Ak,Nk} L_{Nk,Nk}^{-1})."""
+    if not (isinstance(X, cspmatrix) and X.is_factor is False):
+        raise ValueError("X must be a cspmatrix")
+    plan, b = _plan(X)
+    info = np.zeros(2, dtype=np.int64)
+    rc = _native.lib().chp_cholesky(plan.handle, b.data_ptr(), _stream(b), _native.ptr(info))
+    X.is_factor = True
+    _raise_rc(rc, "cholesky")
+    if rc == 1:
+        raise ArithmeticError("factorization failed")
+
+
+def llt(L):
+    """Chordal product X = L L^T, in place (cbase.c:298-390)."""
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("X must be a cspmatrix")
+    plan, b = _plan(L)
+    rc = _native.lib().chp_llt(plan.handle, b.data_ptr(), _stream(b))
+    _raise_rc(rc, "llt")
+    L.is_factor = False
+
+
+def projected_inverse(L):
+    """Projected inverse Y = P(L^{-T} L^{-1}), in place (cbase.c:404-500)."""
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("X must be a cspmatrix")
+    plan, b = _plan(L)
+    info = np.zeros(2, dtype=np.int64)
+    rc = _native.lib().chp_projected_inverse(plan.handle, b.data_ptr(), _stream(b),
+                                             _native.ptr(info))
+    L.is_factor = False
+    _raise_rc(rc, "projected_inverse")
+    if rc == 1:
+        raise ArithmeticError("partial inverse failed")
+
+
+def completion(X, factored_updates=True):
+    """Maximum-determinant positive definite completion (cbase.c:521-621)."""
+    raise NotImplementedError("completion is not available in this build yet")
+
+
+def hessian(L, Y, U, adj=False, inv=False, factored_updates=True):
+    """Hessian of the log-det barrier and its factors (cbase.c:675-857)."""
+    raise NotImplementedError("hessian is not available in this build yet")
+
+
+def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):
+    """B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B ('T') with the chordal
+    factor L, B in the original ordering (cbase.c:965-1077).
+
+    B is a float64 CUDA tensor: a vector of length n, or an (n, k) matrix.
+    As in the reference, ``nrhs``, ``offsetB`` and ``ldB`` address B's
+    column-major storage; a row-major (n, k) tensor is solved through a
+    column-major copy that is written back."""
+    torch = _torch()
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("L must be a cspmatrix factor")
+    if trans not in ('N', 'T'):
+        raise ValueError("trans must be 'N' or 'T'")
+    plan, b = _plan(L)
+    n = L.symb.n
+    if not (isinstance(B, torch.Tensor) and B.is_cuda and B.dtype == torch.float64):
+        raise TypeError("B must be a float64 tensor in GPU memory")
+    if B.dim() == 1:
+        work, cols, ld = B, 1, B.shape[0]
+    elif B.dim() == 2:
+        cols = B.shape[1]
+        if B.stride(0) == 1:          # column-major storage
+            work, ld = B, (B.stride(1) if cols > 1 else B.shape[0])
+        else:                         # row-major: solve a column-major copy
+            work, ld = B.t().contiguous(), B.shape[0]
+    else:
+        raise TypeError("B must be a vector or a matrix")
+    if ldB != -1:
+        ld = int(ldB)
+    k = cols if nrhs == -1 else int(nrhs)
+    if ld < n:
+        raise ValueError("ldB must be at least n")
+    if k > 0 and int(offsetB) + (k - 1) * ld + n > work.numel():
+        raise ValueError("B is too small for n, nrhs, offsetB and ldB")
+    rc = _native.lib().chp_trsm(plan.handle, b.data_ptr(), work.data_ptr() + 8 * int(offsetB),
+                                int(ld), int(k), float(alpha), ord(trans), _stream(b))
+    _raise_rc(rc, "trsm")
+    if work is not B:
+        B.copy_(work.t())
+
+
+def dot(X, Y):
+    """tr(XY) of two chordal matrices on the same symbolic object; returns
+    None when the symbolic objects differ (cbase.c:106-174)."""
+    if not (isinstance(X, cspmatrix) and X.is_factor is False):
+        raise ValueError("X must be a cspmatrix")
+    if not (isinstance(Y, cspmatrix) and Y.is_factor is False):
+        raise ValueError("Y must be a cspmatrix")
+    if X.symb is not Y.symb:
+        return None
+    plan, bx = _plan(X)
+    by = _check_blkval(Y)
+    out = ctypes.c_double(0.0)
+    rc = _native.lib().chp_dot(plan.handle, bx.data_ptr(), by.data_ptr(), ctypes.byref(out),
+                               _stream(bx))
+    _raise_rc(rc, "dot")
+    return out.value

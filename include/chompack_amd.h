@@ -1,0 +1,116 @@
+/*
+ * chompack_amd C-ABI: MI355X (gfx950) supernodal multifrontal numeric sweeps.
+ *
+ * This is the boundary a CHOMPACK-style host binds (ctypes from Python, see
+ * chompack_amd/_native.py).  All index arrays are int64 host arrays with the
+ * meaning of the reference's symbolic object (src/python/symbolic.py:499-890);
+ * all value arrays (blkval, B, U, F) are float64 DEVICE pointers in HBM;
+ * `stream` is a hipStream_t (NULL = default stream).  No torch types appear.
+ *
+ * Return codes: 0 success; 1 numerical failure (see info); negative = usage or
+ * runtime error, with a message from chp_last_error().
+ */
+#ifndef CHOMPACK_AMD_H
+#define CHOMPACK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chp_plan chp_plan;
+
+/* Last error message of the calling thread. */
+const char* chp_last_error(void);
+
+/* ABI version of this library. */
+int chp_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Host-side symbolic routines (integer, bit-exact with the reference).
+ * ------------------------------------------------------------------------- */
+
+/* Elimination tree (symbolic.py:49-72).  cp/ri: CSC of the symmetric pattern. */
+void chp_etree(int64_t n, const int64_t* cp, const int64_t* ri, int64_t* parent);
+/* Postorder of a forest, roots have parent[j] == j (symbolic.py:26-47). */
+void chp_post_order(int64_t n, const int64_t* parent, int64_t* post);
+/* Column counts of the Cholesky factor (symbolic.py:95-131). */
+void chp_counts(int64_t n, const int64_t* cp, const int64_t* ri, const int64_t* parent,
+                const int64_t* post, int64_t* colcount);
+/* Pothen-Sun maximal supernodes (symbolic.py:133-244); returns Nsn. */
+int64_t chp_supernodes(int64_t n, const int64_t* parent, const int64_t* post,
+                       const int64_t* colcount, int64_t* snode, int64_t* snptr,
+                       int64_t* snpar);
+/* Clique row sets (symbolic.py:328-384); returns 0 or -1-k on a size mismatch. */
+int64_t chp_embed(int64_t nsn, const int64_t* cpl, const int64_t* ril, const int64_t* snptr,
+                  const int64_t* snpar, const int64_t* sncolptr, int64_t* snrowidx);
+/* Relative indices (symbolic.py:386-411); returns 0 or -1-k. */
+int64_t chp_relative_idx(int64_t nsn, const int64_t* snptr, const int64_t* snpar,
+                         const int64_t* sncolptr, const int64_t* snrowidx,
+                         const int64_t* relptr, int64_t* relidx);
+/* Approximate minimum degree ordering; replaces cvxopt.amd.order as called
+ * through symbolic(A, p=amd.order) (symbolic.py:540-546). */
+int chp_amd_order(int64_t n, const int64_t* cp, const int64_t* ri, int64_t* perm);
+
+/* ---------------------------------------------------------------------------
+ * Device plan: the symbolic arrays of one factorization in HBM plus the
+ * precomputed level schedule.  Replaces the per-call argument extraction and
+ * workspace malloc of cbase.c (e.g. cchol, cbase.c:187-258).
+ * small_max: largest clique order handled by the LDS-resident front kernels
+ * (<= 0 selects the default, 64).
+ * ------------------------------------------------------------------------- */
+chp_plan* chp_plan_create(int64_t n, int64_t nsn, const int64_t* snpost,
+                          const int64_t* snptr, const int64_t* relptr,
+                          const int64_t* relidx, const int64_t* chptr,
+                          const int64_t* chidx, const int64_t* blkptr,
+                          const int64_t* p, int small_max);
+void chp_plan_destroy(chp_plan* plan);
+/* out[8]: levels, large fronts (cholesky), large fronts (projected inverse),
+ * GEMM tiles, update-arena doubles, diagonal-inverse doubles, scratch doubles,
+ * cholesky launches. */
+int chp_plan_info(chp_plan* plan, int64_t* out);
+
+/* ---------------------------------------------------------------------------
+ * Numeric sweeps, in place on a device blkval array of length blkptr[nsn].
+ * ------------------------------------------------------------------------- */
+
+/* Supernodal multifrontal Cholesky; replaces cholesky() (chompack.h:63-76,
+ * cholesky.c:3-84) as called by cbase.c:262-266.  On failure returns 1 with
+ * info[0] = supernode, info[1] = LAPACK-style column info within it. */
+int chp_cholesky(chp_plan* plan, double* blkval, void* stream, int64_t* info);
+
+/* Chordal product X = L L^T; replaces llt() (llt.c:3-60, cbase.c:364-368). */
+int chp_llt(chp_plan* plan, double* blkval, void* stream);
+
+/* Projected inverse Y = P(X^{-1}) from the factor; replaces
+ * projected_inverse() (projected_inverse.c:3-75, cbase.c:470-474). */
+int chp_projected_inverse(chp_plan* plan, double* blkval, void* stream, int64_t* info);
+
+/* B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B (trans 'T'), B a device
+ * column-major n x nrhs array with leading dimension ldb in the original
+ * ordering; replaces trsm() (trsm.c:3-138, cbase.c:1042-1049). */
+int chp_trsm(chp_plan* plan, const double* blkval, double* B, int ldb, int nrhs,
+             double alpha, int trans, void* stream);
+
+/* *out = tr(X Y) for two chordal matrices on the plan's pattern; replaces
+ * dot() (dot.c:3-22, cbase.c:106-174). */
+int chp_dot(chp_plan* plan, const double* x, const double* y, double* out, void* stream);
+
+/* F[r[i], r[j]] += alpha U[i, j] for i >= j, with element (a, b) of F at
+ * F + a*strides[0] + b*strides[1] and of U at U + a*strides[2] + b*strides[3];
+ * r is a device int64 array of length N.  Replaces frontal_add_update
+ * (cbase.c:11-34). */
+int chp_frontal_add_update(double* F, int64_t N, const int64_t* relidx, const double* U,
+                           const int64_t* strides, double alpha, void* stream);
+
+/* U[i, j] = F[r[i], r[j]] for i >= j (same conventions); replaces
+ * frontal_get_update (cbase.c:39-59). */
+int chp_frontal_get_update(const double* F, int64_t N, const int64_t* relidx, double* U,
+                           const int64_t* strides, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHOMPACK_AMD_H */

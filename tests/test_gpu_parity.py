@@ -1,0 +1,112 @@
+"""GPU parity: HIP sweeps vs the NumPy oracle on the same symbolic object.
+
+Tolerance (FP64 path): max |gpu - oracle| / max |oracle| <= 1e-12 over the
+whole blkval array (reordered sums in the dense kernels move the last bits).
+Each problem runs twice: with the default LDS/large-front split and with
+``small_max = 1``, which routes every front through the large-front kernels
+(column-tiled extend-add, panel factorization, MFMA GEMM tiles).
+"""
+import numpy as np
+import pytest
+
+import chompack_amd as cp
+from chompack_amd import numeric, problems
+from oracle import numeric as on
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+PROBLEMS = {
+    "ref31": lambda: problems.reference_test_matrix(),
+    "grid2d_64": lambda: problems.grid2d(64, seed=0),
+    "grid3d_12": lambda: problems.grid3d(12, seed=1),
+}
+
+
+def _setup(name, small_max):
+    import torch
+    A = PROBLEMS[name]()
+    symb = cp.symbolic(A, p=cp.amd.order)
+    symb._plans["cuda:0"] = numeric.Plan(symb, "cuda:0", small_max=small_max)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    torch.cuda.synchronize()
+    return A, symb, X
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("small_max", [64, 1])
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_cholesky_projinv_solve(name, small_max):
+    import torch
+    print("case", name, small_max, flush=True)
+    A, symb, X = _setup(name, small_max)
+    ref = X.blkval.cpu().numpy().copy()
+    A0 = ref.copy()
+    assert on.cholesky(symb, ref) == 0
+    cp.cholesky(X)
+    assert X.is_factor is True
+    e = _rel(X.blkval.cpu().numpy(), ref)
+    print("cholesky rel err %.3e" % e, flush=True)
+    assert e <= TOL
+
+    # solve with the factor: trsm N then T equals A^{-1} b
+    rng = np.random.default_rng(7)
+    Bh = rng.standard_normal((symb.n, 3))
+    B = torch.as_tensor(Bh.copy(), device="cuda:0")
+    Bref = np.asfortranarray(Bh.copy())
+    on.trsm(symb, ref, Bref, trans="N")
+    cp.trsm(X, B)
+    e = _rel(B.cpu().numpy(), Bref)
+    print("trsm N rel err %.3e" % e, flush=True)
+    assert e <= TOL
+    on.trsm(symb, ref, Bref, trans="T")
+    cp.trsm(X, B, trans="T")
+    e = _rel(B.cpu().numpy(), Bref)
+    print("trsm T rel err %.3e" % e, flush=True)
+    assert e <= TOL
+    As = A + A.T - np.diag(A.diagonal())
+    res = np.abs(As @ B.cpu().numpy() - Bh).max() / np.abs(Bh).max()
+    assert res <= 1e-12
+
+    on.projected_inverse(symb, ref)
+    cp.projected_inverse(X)
+    assert X.is_factor is False
+    e = _rel(X.blkval.cpu().numpy(), ref)
+    print("projected_inverse rel err %.3e" % e, flush=True)
+    assert e <= TOL
+    # <P(A^-1), A> = n  (tests/test_numeric.py:40-46 of the reference)
+    Y = X
+    Xa = cp.cspmatrix(symb, device="cuda:0") + A
+    v = cp.dot(Y, Xa)
+    assert abs(v - symb.n) <= 1e-9 * symb.n
+    assert abs(v - on.dot(symb, Y.blkval.cpu().numpy(), A0)) <= 1e-10 * symb.n
+
+
+@pytest.mark.parametrize("name", ["ref31", "grid2d_64"])
+def test_llt_roundtrip(name):
+    A, symb, X = _setup(name, 64)
+    A0 = X.blkval.cpu().numpy().copy()
+    ref = A0.copy()
+    on.cholesky(symb, ref)
+    on.llt(symb, ref)
+    cp.cholesky(X)
+    cp.llt(X)
+    assert X.is_factor is False
+    e = _rel(X.blkval.cpu().numpy(), ref)
+    assert e <= TOL
+    assert _rel(X.blkval.cpu().numpy(), A0) <= 1e-12
+
+
+def test_cholesky_failure_raises():
+    import scipy.sparse as sp
+    A = problems.grid2d(8, seed=0)
+    A = sp.csc_matrix(A - 30.0 * sp.identity(64))
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    with pytest.raises(ArithmeticError, match="factorization failed"):
+        cp.cholesky(X)
+    assert X.is_factor is True
