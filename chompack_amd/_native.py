@@ -66,6 +66,9 @@ def _declare(L):
     _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)
     _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)
     _opt(L, "chp_set_mode", [vp, i32], i32)
+    _opt(L, "chp_profile", [vp, i32], i32)
+    _opt(L, "chp_profile_read", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_plan_flops", [vp, vp], i32)
     _opt(L, "chp_last_error", [], ctypes.c_char_p)
 
 

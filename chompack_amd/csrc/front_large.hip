@@ -64,130 +64,166 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal-block factorization for panel [c0, c0+nb) of each listed front:
-// unblocked right-looking Cholesky of the nb x nb block in LDS (one thread
-// per row), then its triangular inverse (one row per thread, no barriers),
-// stored for the panel solve and the stored-convention solve.
-__device__ inline void lds_lower_inverse(double (*D)[NB + 1], double (*E)[NB + 1], int nb, int t) {
-    if (t < nb) {
-        E[t][t] = 1.0 / D[t][t];
-        for (int j = t - 1; j >= 0; --j) {
-            double s = 0.0;
-            for (int m = j + 1; m <= t; ++m) s += E[t][m] * D[m][j];
-            E[t][j] = -s / D[j][j];
+// Diagonal blocks.  One 256-thread workgroup per 64 x 64 diagonal block,
+// padded with the identity beyond nb.  Thread (w = tid/64, l = tid%64) owns
+// the 16 elements (w + 4q, l), q = 0..15, in registers.  Both the Cholesky
+// factorization and the inverse of the factor are right-looking with two
+// barriers per column; the column (or row) being eliminated is broadcast
+// through LDS.  Inverses are stored row-major: Dinv(r, j) at dinv[j + r*NB].
+
+#define DQ 16  // elements per thread
+
+// L (lower, padded identity) in registers -> factor in place.  Returns false
+// (uniformly) if a pivot is not positive; *bad receives its column.
+__device__ inline bool block_potrf(double (&a)[DQ], double* colc, int w, int l, int* bad) {
+    __shared__ double sd;
+    __shared__ int sfail;
+    if (w == 0 && l == 0) sfail = -1;
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+        const int qc = c >> 2, wc = c & 3;
+        // owner of (c, c) publishes the pivot
+        if (w == wc && l == c) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < DQ; ++q) if (q == qc) d = a[q];
+            if (!(d > 0.0)) sfail = c;
+            sd = sqrt(d);
         }
+        __syncthreads();
+        if (sfail >= 0) { *bad = sfail; return false; }
+        const double s = sd, rs = 1.0 / s;
+        // column c: L(i, c) = a(i, c) / s for i > c, L(c, c) = s
+        if (l == c) {
+#pragma unroll
+            for (int q = 0; q < DQ; ++q) {
+                const int i = w + 4 * q;
+                if (i > c) { a[q] *= rs; colc[i] = a[q]; }
+                else if (i == c) { a[q] = s; colc[i] = s; }
+            }
+        }
+        __syncthreads();
+        // trailing update a(i, j) -= L(i, c) L(j, c) for i >= j > c
+        if (l > c) {
+            const double lj = colc[l];
+#pragma unroll
+            for (int q = 0; q < DQ; ++q) {
+                const int i = w + 4 * q;
+                if (i >= l) a[q] -= colc[i] * lj;
+            }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// x = L^{-1} for the lower triangular L held (full, padded) in LDS Ls.
+__device__ inline void block_trtri(const double (*Ls)[NB + 1], double (&x)[DQ], double* rowc,
+                                   int w, int l) {
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) x[q] = (w + 4 * q == l) ? 1.0 : 0.0;
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+        const int qc = c >> 2, wc = c & 3;
+        // row c of X is final once divided by L(c, c)
+        if (w == wc) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < DQ; ++q) if (q == qc) { x[q] = x[q] / Ls[c][c]; v = x[q]; }
+            rowc[l] = v;
+        }
+        __syncthreads();
+        // X(i, j) -= L(i, c) X(c, j) for i > c, j <= c
+        if (l <= c) {
+            const double xc = rowc[l];
+#pragma unroll
+            for (int q = 0; q < DQ; ++q) {
+                const int i = w + 4 * q;
+                if (i > c) x[q] -= Ls[i][c] * xc;
+            }
+        }
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(64) void k_potrf_diag(
+__global__ __launch_bounds__(TPB) void k_potrf_diag(
     const int* __restrict__ fronts, int c0, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ dinv,
     const i64* __restrict__ dinvptr, unsigned long long* status) {
-    __shared__ double D[NB][NB + 1];
-    __shared__ double E[NB][NB + 1];
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ double vec[NB];
     const int k = fronts[blockIdx.x];
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
     const int nb = min(NB, nn - c0);
-    const int t = threadIdx.x;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     double* blk = blkval + f.blkptr + c0 + (i64)c0 * nj;
-    for (int j = 0; j < nb; ++j)
-        if (t < nb && t >= j) D[t][j] = blk[t + (i64)j * nj];
-    __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const double d = D[c][c];
-        if (!(d > 0.0)) {
-            if (t == 0) chp_record_failure(status, k, c0 + c + 1);
-            return;
-        }
-        const double s = sqrt(d);
-        const double rs = 1.0 / s;
-        __syncthreads();
-        if (t == c) D[c][c] = s;
-        if (t > c && t < nb) D[t][c] *= rs;
-        __syncthreads();
-        if (t > c && t < nb) {
-            const double lt = D[t][c];
-            for (int j = c + 1; j <= t; ++j) D[t][j] -= lt * D[j][c];
-        }
-        __syncthreads();
+    double a[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int i = w + 4 * q;
+        a[q] = (i < nb && l < nb && i >= l) ? blk[i + (i64)l * nj] : (i == l ? 1.0 : 0.0);
     }
-    lds_lower_inverse(D, E, nb, t);
+    int bad = 0;
+    if (!block_potrf(a, vec, w, l, &bad)) {
+        if (threadIdx.x == 0) chp_record_failure(status, k, c0 + bad + 1);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int i = w + 4 * q;
+        Ls[i][l] = (i >= l) ? a[q] : 0.0;
+        if (i < nb && l < nb && i >= l) blk[i + (i64)l * nj] = a[q];
+    }
     __syncthreads();
+    double x[DQ];
+    block_trtri(Ls, x, vec, w, l);
     double* inv = dinv + dinvptr[k] + (i64)(c0 / NB) * NB * NB;
-    for (int j = 0; j < NB; ++j) {
-        if (j < nb && t < nb && t >= j) blk[t + (i64)j * nj] = D[t][j];
-        inv[t + j * NB] = (j < nb && t < nb && t >= j) ? E[t][j] : 0.0;
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int i = w + 4 * q;
+        inv[l + i * NB] = (i < nb && l < nb) ? x[q] : 0.0;
     }
 }
 
-// Inverse of every diagonal block of L11 (projected inverse, dtrtri's
-// diagonal part).  blocks[i] = (front, block index).
-__global__ __launch_bounds__(64) void k_trtri_diag(
+// Inverse of every diagonal block of L11 (projected inverse, the diagonal
+// part of dtrtri), also written into the diagonal blocks of X = L11^{-1}
+// (column-major, ld nn).  blocks[i] = (front, block index).
+__global__ __launch_bounds__(TPB) void k_trtri_diag(
     const int2* __restrict__ blocks, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ dinv,
     const i64* __restrict__ dinvptr, double* __restrict__ X,
     const i64* __restrict__ xptr, unsigned long long* status) {
-    __shared__ double D[NB][NB + 1];
-    __shared__ double E[NB][NB + 1];
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ double vec[NB];
     const int2 b = blocks[blockIdx.x];
     const int k = b.x, c0 = b.y * NB;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
     const int nb = min(NB, nn - c0);
-    const int t = threadIdx.x;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const double* blk = blkval + f.blkptr + c0 + (i64)c0 * nj;
-    for (int j = 0; j < nb; ++j)
-        if (t < nb && t >= j) D[t][j] = blk[t + (i64)j * nj];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int i = w + 4 * q;
+        Ls[i][l] = (i < nb && l < nb && i >= l) ? blk[i + (i64)l * nj] : (i == l ? 1.0 : 0.0);
+    }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
-        if (D[j][j] == 0.0) {
-            if (t == 0) chp_record_failure(status, k, c0 + j + 1);
+        if (Ls[j][j] == 0.0) {
+            if (threadIdx.x == 0) chp_record_failure(status, k, c0 + j + 1);
             return;
         }
     }
-    lds_lower_inverse(D, E, nb, t);
-    __syncthreads();
+    double x[DQ];
+    block_trtri(Ls, x, vec, w, l);
     double* inv = dinv + dinvptr[k] + (i64)b.y * NB * NB;
-    for (int j = 0; j < NB; ++j)
-        inv[t + j * NB] = (t < nb && j < nb && t >= j) ? E[t][j] : 0.0;
-    // the diagonal block of X = L11^{-1} (ld nn)
-    double* x = X + xptr[k] + c0 + (i64)c0 * nn;
-    for (int j = 0; j < nb; ++j)
-        if (t < nb) x[t + (i64)j * nn] = (t >= j) ? E[t][j] : 0.0;
-}
-
-// In-place right multiplication of a 64-row tile of a panel by a diagonal
-// block inverse: tiles[i] = (front, row0, c0, mode).
-//   mode 0: X = F[rows, c0:c0+nb] * Dinv^T   (L21 = F21 L_dd^-T, panel solve)
-//   mode 1: X = F[rows, c0:c0+nb] * Dinv     (B21 = L21 L11^-1, block step)
-__global__ __launch_bounds__(TPB) void k_panel_mul(
-    const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
-    double* __restrict__ blkval, const double* __restrict__ dinv,
-    const i64* __restrict__ dinvptr) {
-    __shared__ double Fs[64][NB + 1];
-    __shared__ double Ms[NB][NB + 1];
-    const int4 t = tiles[blockIdx.x];
-    const int k = t.x, row0 = t.y, c0 = t.z, mode = t.w;
-    const FrontInfo f = fi[k];
-    const int nn = f.nn, nj = nn + f.na;
-    const int nb = min(NB, nn - c0);
-    const int rows = min(64, nj - row0);
-    const int tid = threadIdx.x;
-    double* blk = blkval + f.blkptr + row0 + (i64)c0 * nj;
-    const double* inv = dinv + dinvptr[k] + (i64)(c0 / NB) * NB * NB;
-    for (int e = tid; e < 64 * NB; e += TPB) {
-        int i = e % 64, m = e / 64;
-        Fs[i][m] = (i < rows && m < nb) ? blk[i + (i64)m * nj] : 0.0;
-        int a = e % NB, b = e / NB;  // Ms[a][b]
-        Ms[a][b] = mode == 0 ? inv[b + a * NB] : inv[a + b * NB];
-    }
-    __syncthreads();
-    const int i = tid % 64;
-    for (int j = tid / 64; j < nb; j += 4) {
-        double s = 0.0;
-        for (int m = 0; m < nb; ++m) s += Fs[i][m] * Ms[m][j];
-        if (i < rows) blk[i + (i64)j * nj] = s;
+    double* xd = X + xptr[k] + c0 + (i64)c0 * nn;
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int i = w + 4 * q;
+        inv[l + i * NB] = (i < nb && l < nb) ? x[q] : 0.0;
+        if (i < nb && l < nb) xd[i + (i64)l * nn] = x[q];
     }
 }
 
@@ -195,85 +231,100 @@ __global__ __launch_bounds__(TPB) void k_panel_mul(
 // Batched FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64).  One workgroup computes
 // one 64 x 64 tile of one task; 4 waves in a 2 x 2 arrangement, each wave a
 // 32 x 32 sub-tile made of 2 x 2 MFMA 16 x 16 accumulators.  K is consumed in
-// chunks of 16 staged through LDS.  Accumulator layout (16 x 16 f64): lane l,
+// chunks of 32 staged through LDS, the next chunk prefetched into registers
+// while the current one is multiplied.  Accumulator layout (16 x 16 f64): lane l,
 // element v holds row (l / 16) + 4 v, column l % 16.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+#define BK 32
 
 __device__ inline const double* bufptr(int b, double* b0, double* b1, double* b2, double* b3) {
     return b == 0 ? b0 : (b == 1 ? b1 : (b == 2 ? b2 : b3));
 }
 
+// Element (gi, gk) of opA and (gk, gj) of opB with the task's masks; 0 outside.
+__device__ inline double ldA(const GemmTask& T, const double* A, int gi, int gk) {
+    bool ok = gi < T.M && gk < T.K;
+    if (T.triA == 1) ok = ok && gk <= gi;
+    else if (T.triA == 2) ok = ok && gk >= gi;
+    if (!ok) return 0.0;
+    if (T.opA == 0) return A[gi + (i64)gk * T.lda];
+    if (T.opA == 1) return A[gk + (i64)gi * T.lda];
+    return (gi >= gk) ? A[gi + (i64)gk * T.lda] : A[gk + (i64)gi * T.lda];
+}
+__device__ inline double ldB(const GemmTask& T, const double* B, int gk, int gj) {
+    bool ok = gj < T.N && gk < T.K;
+    if (T.pad == 1) ok = ok && gk >= gj;
+    if (!ok) return 0.0;
+    return (T.opB == 0) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
+}
+
 __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
     double* b0, double* b1, double* b2, double* b3) {
-    __shared__ double As[16][TILE + 2];
-    __shared__ double Bs[16][TILE + 2];
+    __shared__ double As[BK][TILE + 2];
+    __shared__ double Bs[BK][TILE + 2];
     const GemmTile tl = tiles[blockIdx.x];
     const GemmTask T = tasks[tl.task];
     const double* A = bufptr(T.bufA, b0, b1, b2, b3) + T.offA;
     const double* B = bufptr(T.bufB, b0, b1, b2, b3) + T.offB;
     double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
     const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
-    const int M = T.M, N = T.N, K = T.K;
+    const int K = T.K;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
-    const int triB = T.pad;
 
     if (T.tri == 2 && row0 + TILE - 1 < col0) {
         for (int e = tid; e < TILE * TILE; e += TPB) {
             int i = row0 + e % TILE, j = col0 + e / TILE;
-            if (i < M && j < N) C[i + (i64)j * T.ldc] = 0.0;
+            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
         }
         return;
     }
+    // 8 elements of A and 8 of B per thread and chunk, mapped so that
+    // consecutive lanes read consecutive addresses of the stored operand
+    const bool a_kfast = (T.opA != 0);
+    const bool b_kfast = (T.opB == 0);
+    double ra[8], rb[8];
+    auto fetch = [&](int kb) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * TPB;
+            int i, kk;
+            if (a_kfast) { kk = e & 31; i = e >> 5; } else { i = e & 63; kk = e >> 6; }
+            ra[q] = ldA(T, A, row0 + i, kb + kk);
+            int j, k2;
+            if (b_kfast) { k2 = e & 31; j = e >> 5; } else { j = e & 63; k2 = e >> 6; }
+            rb[q] = ldB(T, B, kb + k2, col0 + j);
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = tid + q * TPB;
+            if (a_kfast) As[e & 31][e >> 5] = ra[q]; else As[e >> 6][e & 63] = ra[q];
+            if (b_kfast) Bs[e & 31][e >> 5] = rb[q]; else Bs[e >> 6][e & 63] = rb[q];
+        }
+    };
     dbl4 acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
-
-    for (int kb = 0; kb < K; kb += 16) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = tid + q * TPB;
-            int i, kk;
-            if (T.opA == 1) { kk = e & 15; i = e >> 4; }
-            else { i = e & 63; kk = e >> 6; }
-            const int gi = row0 + i, gk = kb + kk;
-            double v = 0.0;
-            bool ok = gi < M && gk < K;
-            if (T.triA == 1) ok = ok && gk <= gi;
-            else if (T.triA == 2) ok = ok && gk >= gi;
-            if (ok) {
-                if (T.opA == 0) v = A[gi + (i64)gk * T.lda];
-                else if (T.opA == 1) v = A[gk + (i64)gi * T.lda];
-                else v = (gi >= gk) ? A[gi + (i64)gk * T.lda] : A[gk + (i64)gi * T.lda];
-            }
-            As[kk][i] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = tid + q * TPB;
-            int j, kk;
-            if (T.opB == 0) { kk = e & 15; j = e >> 4; }
-            else { j = e & 63; kk = e >> 6; }
-            const int gj = col0 + j, gk = kb + kk;
-            double v = 0.0;
-            bool ok = gj < N && gk < K;
-            if (triB == 1) ok = ok && gk >= gj;
-            if (ok) v = (T.opB == 0) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
-            Bs[kk][j] = v;
-        }
+    fetch(0);
+    for (int kb = 0; kb < K; kb += BK) {
+        stash();
         __syncthreads();
+        if (kb + BK < K) fetch(kb + BK);
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
+        for (int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
-            double a0 = As[kr][wr * 32 + (lane & 15)];
-            double a1 = As[kr][wr * 32 + 16 + (lane & 15)];
-            double bb0 = Bs[kr][wc * 32 + (lane & 15)];
-            double bb1 = Bs[kr][wc * 32 + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb1, acc[1][1], 0, 0, 0);
+            const double a0 = As[kr][wr * 32 + (lane & 15)];
+            const double a1 = As[kr][wr * 32 + 16 + (lane & 15)];
+            const double q0 = Bs[kr][wc * 32 + (lane & 15)];
+            const double q1 = Bs[kr][wc * 32 + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, q0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, q1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -282,7 +333,7 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
             for (int v = 0; v < 4; ++v) {
                 const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
                 const int j = col0 + wc * 32 + b * 16 + (lane & 15);
-                if (i >= M || j >= N) continue;
+                if (i >= T.M || j >= T.N) continue;
                 double* c = C + i + (i64)j * T.ldc;
                 if (T.tri == 0 || i >= j) {
                     double r = T.alpha * acc[a][b][v];

@@ -26,8 +26,6 @@ __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, dou
 __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,
                              double* dinv, const i64* dinvptr, double* X, const i64* xptr,
                              unsigned long long* status);
-__global__ void k_panel_mul(const int4* tiles, const FrontInfo* fi, double* blkval,
-                            const double* dinv, const i64* dinvptr);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, double* b0,
                              double* b1, double* b2, double* b3);
 __global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkval,

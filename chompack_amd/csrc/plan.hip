@@ -16,7 +16,9 @@
 #include "../../include/chompack_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
+#include <functional>
 
 static thread_local std::string g_err;
 void chp_set_error(const std::string& msg) { g_err = msg; }
@@ -35,7 +37,10 @@ struct Launch {
     size_t lds;
     i64 list;   // byte offset of the work list in the list buffer
     int iarg;
+    double flops = 0.0;  // algorithmic flops of a GEMM launch
 };
+
+const int NKIDS = 12;
 
 struct ListBuf {
     std::vector<char> bytes;
@@ -86,6 +91,13 @@ struct chp_plan {
     ListBuf lists;
     int n_large_chol = 0, n_large_projinv = 0;
     i64 n_gemm_tiles = 0;
+    double pending_flops = 0.0;
+    // profiling: per-kernel-kind time from HIP events around every launch
+    bool prof = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms[NKIDS] = {0};
+    double prof_flops[NKIDS] = {0};
+    i64 prof_cnt[NKIDS] = {0};
 };
 
 namespace {
@@ -126,9 +138,33 @@ GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, i
     return t;
 }
 
+// Useful flops of a task: 2 * sum over computed C(i,j) of the number of k
+// with nonzero opA(i,k) opB(k,j) (triangular operands and outputs excluded).
+double task_flops(const GemmTask& t) {
+    double total = 0.0;
+    for (int i = 0; i < t.M; ++i) {
+        int jmax = t.tri ? std::min(i + 1, t.N) : t.N;
+        if (jmax <= 0) continue;
+        int klo = 0, khi = t.K;
+        if (t.triA == 1) khi = std::min(khi, i + 1);
+        if (t.triA == 2) klo = std::max(klo, i);
+        if (khi <= klo) continue;
+        if (t.pad == 1) {  // B lower triangular: k >= j
+            double sum = (double)std::min(jmax, klo) * (khi - klo);
+            int a = klo, b = std::min(jmax, khi);
+            if (b > a) sum += (double)(b - a) * khi - ((double)(b - 1) * b / 2.0 - (double)(a - 1) * a / 2.0);
+            total += sum;
+        } else {
+            total += (double)jmax * (khi - klo);
+        }
+    }
+    return 2.0 * total;
+}
+
 void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, const GemmTask& t,
                     bool skip_upper) {
     if (t.M <= 0 || t.N <= 0) return;
+    P->pending_flops += task_flops(t);
     int id = (int)P->tasks.size();
     P->tasks.push_back(t);
     int mt = (t.M + TILE - 1) / TILE, nt = (t.N + TILE - 1) / TILE;
@@ -142,7 +178,10 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, const GemmTask& t
 void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles) {
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
-    seq.push_back({K_GEMM, (int)tiles.size(), 256, 0, P->lists.add(tiles), 0});
+    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->lists.add(tiles), 0};
+    L.flops = P->pending_flops;
+    P->pending_flops = 0.0;
+    seq.push_back(L);
 }
 
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
@@ -169,14 +208,18 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     for (int c0 = 0; c0 < maxnn; c0 += NB) {
         std::vector<int> act;
         for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
-        P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 64, 0, P->lists.add(act), c0});
-        std::vector<int4> pt;
-        std::vector<GemmTile> gt;
+        P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->lists.add(act), c0});
+        std::vector<GemmTile> pt, gt;
         for (int k : act) {
             const FrontInfo& f = P->fi[k];
             int nn = f.nn, na = f.na, nj = nn + na;
             int c1 = std::min(c0 + NB, nn);
-            for (int r0 = c1; r0 < nj; r0 += 64) pt.push_back(make_int4(k, r0, c0, 0));
+            // panel solve, in place: L[c1:, c0:c1] = F[c1:, c0:c1] Dinv^T
+            // (Dinv stored row-major, so Dinv^T(m, j) = dinv[m + j*NB])
+            add_gemm_tiles(P, pt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
+                                     3, P->h_dinvptr[k] + (i64)(c0 / NB) * NB * NB, NB, 0,
+                                     0, f.blkptr + c1 + (i64)c0 * nj, nj,
+                                     nj - c1, c1 - c0, c1 - c0, 1.0, 0.0, 0), false);
             // trailing update F[c1:, c1:] -= L[c1:, c0:c1] L[c1:, c0:c1]^T
             if (c1 < nn)
                 add_gemm_tiles(P, gt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
@@ -188,8 +231,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                          0, f.blkptr + nn + (i64)c0 * nj, nj, 1,
                                          1, f.updptr, na, na, na, c1 - c0, -1.0, 1.0, 1), true);
         }
-        if (!pt.empty())
-            P->chol.push_back({K_PANEL_MUL, (int)pt.size(), 256, 0, P->lists.add(pt), 0});
+        push_gemm(P, P->chol, pt);
         push_gemm(P, P->chol, gt);
     }
     // stored convention B21 = L21 L11^{-1}: column blocks from the right
@@ -197,23 +239,25 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     for (int k : large)
         if (P->fi[k].na > 0) maxblk = std::max(maxblk, (P->fi[k].nn + NB - 1) / NB);
     for (int s = 0; s < maxblk; ++s) {
-        std::vector<int4> pt;
-        std::vector<GemmTile> gt;
+        std::vector<GemmTile> pt, gt;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
             int nn = f.nn, na = f.na, nj = nn + na;
             int nbk = (nn + NB - 1) / NB;
             if (na == 0 || s >= nbk) continue;
             int b = nbk - 1 - s, b0 = b * NB, b1 = std::min(b0 + NB, nn);
-            for (int r0 = nn; r0 < nj; r0 += 64) pt.push_back(make_int4(k, r0, b0, 1));
+            // X_b = R_b Dinv_b in place (Dinv(m, j) = dinv[j + m*NB])
+            add_gemm_tiles(P, pt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
+                                     3, P->h_dinvptr[k] + (i64)b * NB * NB, NB, 1,
+                                     0, f.blkptr + nn + (i64)b0 * nj, nj,
+                                     na, b1 - b0, b1 - b0, 1.0, 0.0, 0), false);
             if (b0 > 0)
                 add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
                                          0, f.blkptr + b0, nj, 0,
                                          0, f.blkptr + nn, nj, na, b0, b1 - b0, -1.0, 1.0, 0),
                                false);
         }
-        if (!pt.empty())
-            P->chol.push_back({K_PANEL_MUL, (int)pt.size(), 256, 0, P->lists.add(pt), 0});
+        push_gemm(P, P->chol, pt);
         push_gemm(P, P->chol, gt);
     }
 }
@@ -236,14 +280,14 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         P->projinv.push_back({K_PROJINV_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
     if (large.empty()) return;
     P->n_large_projinv += (int)large.size();
-    // per-front scratch: X = L11^{-1} (nn x nn), T = V B21 (na x nn), W (NB x nn)
+    // per-front scratch: X = L11^{-1} (nn x nn), W (nn x nn), T = V B21 (na x nn)
     i64 off = 0;
     std::vector<i64> woff(large.size());
     for (size_t q = 0; q < large.size(); ++q) {
         const FrontInfo& f = P->fi[large[q]];
         P->h_xptr[large[q]] = off; off += (i64)f.nn * f.nn;
+        woff[q] = off; off += (i64)f.nn * f.nn;
         P->h_tptr[large[q]] = off; off += (i64)f.na * f.nn;
-        woff[q] = off; off += (i64)NB * f.nn;
         off = (off + 31) / 32 * 32;
     }
     scratch_need = std::max(scratch_need, off);
@@ -252,25 +296,41 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         int nbk = (P->fi[k].nn + NB - 1) / NB;
         for (int b = 0; b < nbk; ++b) bt.push_back(make_int2(k, b));
     }
-    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 64, 0, P->lists.add(bt), 0});
-    // block rows of X = L11^{-1}:  W = L[bi, :b0] X[:b0, :b0];  X[bi, :b0] = -Dinv_i W
-    int maxblk = 0;
-    for (int k : large) maxblk = std::max(maxblk, (P->fi[k].nn + NB - 1) / NB);
-    for (int i = 1; i < maxblk; ++i) {
+    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 256, 0, P->lists.add(bt), 0});
+    // X = L11^{-1} by recursive halving of the block range [a, b) at m:
+    //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
+    // all nodes of one height (over all large fronts) form one launch pair.
+    std::vector<std::vector<std::array<int, 4>>> byh;  // (q, a, m, b)
+    std::function<int(int, int, int)> tree = [&](int q, int a, int b) -> int {
+        if (b - a <= 1) return 0;
+        int m = (a + b) / 2;
+        int h = 1 + std::max(tree(q, a, m), tree(q, m, b));
+        if ((int)byh.size() < h) byh.resize(h);
+        byh[h - 1].push_back({q, a, m, b});
+        return h;
+    };
+    for (size_t q = 0; q < large.size(); ++q)
+        tree((int)q, 0, (P->fi[large[q]].nn + NB - 1) / NB);
+    for (auto& level_nodes : byh) {
         std::vector<GemmTile> g1, g2;
-        for (size_t q = 0; q < large.size(); ++q) {
+        for (auto& nd : level_nodes) {
+            int q = nd[0];
             int k = large[q];
             const FrontInfo& f = P->fi[k];
             int nn = f.nn, nj = nn + f.na;
-            int nbk = (nn + NB - 1) / NB;
-            if (i >= nbk) continue;
-            int b0 = i * NB, b1 = std::min(b0 + NB, nn), nb = b1 - b0;
-            i64 xo = P->h_xptr[k];
-            add_gemm_tiles(P, g1, mk(0, f.blkptr + b0, nj, 0, 2, xo, nn, 0,
-                                     2, woff[q], NB, nb, b0, b0, 1.0, 0.0, 0, 0, 1), false);
-            add_gemm_tiles(P, g2, mk(3, P->h_dinvptr[k] + (i64)i * NB * NB, NB, 0,
-                                     2, woff[q], NB, 0, 2, xo + b0, nn, nb, b0, nb,
-                                     -1.0, 0.0, 0, 1, 0), false);
+            int r1 = nd[1] * NB, r2 = nd[2] * NB, r3 = std::min(nd[3] * NB, nn);
+            int m1 = r2 - r1, m2 = r3 - r2;
+            i64 xo = P->h_xptr[k], wo = woff[q];
+            // W[r2:r3, r1:r2] = L[r2:r3, r1:r2] X[r1:r2, r1:r2]   (X lower)
+            add_gemm_tiles(P, g1, mk(0, f.blkptr + r2 + (i64)r1 * nj, nj, 0,
+                                     2, xo + r1 + (i64)r1 * nn, nn, 0,
+                                     2, wo + r2 + (i64)r1 * nn, nn,
+                                     m2, m1, m1, 1.0, 0.0, 0, 0, 1), false);
+            // X[r2:r3, r1:r2] = -X[r2:r3, r2:r3] W[r2:r3, r1:r2]   (X lower)
+            add_gemm_tiles(P, g2, mk(2, xo + r2 + (i64)r2 * nn, nn, 0,
+                                     2, wo + r2 + (i64)r1 * nn, nn, 0,
+                                     2, xo + r2 + (i64)r1 * nn, nn,
+                                     m2, m1, m2, -1.0, 0.0, 0, 1, 0), false);
         }
         push_gemm(P, P->projinv, g1);
         push_gemm(P, P->projinv, g2);
@@ -329,6 +389,7 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
     for (const Launch& L : seq) {
         const void* list = P->d_lists + L.list;
         dim3 g(L.grid), b(L.block);
+        if (P->prof) HIP_CHECK(hipEventRecord(P->ev0, s));
         switch (L.kid) {
         case K_CHOL_SMALL:
             hipLaunchKernelGGL(k_chol_small, g, b, L.lds, s, (const int*)list, P->d_fi, blkval,
@@ -341,10 +402,6 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
         case K_POTRF_DIAG:
             hipLaunchKernelGGL(k_potrf_diag, g, b, 0, s, (const int*)list, L.iarg, P->d_fi,
                                blkval, P->d_dinv, P->d_dinvptr, P->d_status);
-            break;
-        case K_PANEL_MUL:
-            hipLaunchKernelGGL(k_panel_mul, g, b, 0, s, (const int4*)list, P->d_fi, blkval,
-                               P->d_dinv, P->d_dinvptr);
             break;
         case K_GEMM:
             hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
@@ -373,6 +430,15 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
         default:
             chp_set_error("internal: unexpected launch kind");
             return -3;
+        }
+        if (P->prof) {
+            HIP_CHECK(hipEventRecord(P->ev1, s));
+            HIP_CHECK(hipEventSynchronize(P->ev1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+            P->prof_ms[L.kid] += ms;
+            P->prof_cnt[L.kid] += 1;
+            P->prof_flops[L.kid] += L.flops;
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -511,6 +577,8 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
 
 void chp_plan_destroy(chp_plan* P) {
     if (!P) return;
+    if (P->ev0) (void)hipEventDestroy(P->ev0);
+    if (P->ev1) (void)hipEventDestroy(P->ev1);
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_work, P->d_status,
@@ -527,6 +595,37 @@ int chp_plan_info(chp_plan* P, i64* out) {
     out[0] = P->nlevels; out[1] = P->n_large_chol; out[2] = P->n_large_projinv;
     out[3] = P->n_gemm_tiles; out[4] = P->upd_len; out[5] = P->dinv_len;
     out[6] = P->scratch_len; out[7] = (i64)P->chol.size();
+    return 0;
+}
+
+// Per-launch HIP-event timing of the sweeps (adds a synchronization after
+// every launch; for measurement only).  on != 0 enables and clears counters.
+int chp_profile(chp_plan* P, int on) {
+    if (!P) return -1;
+    if (on && !P->ev0) {
+        HIP_CHECK(hipEventCreate(&P->ev0));
+        HIP_CHECK(hipEventCreate(&P->ev1));
+    }
+    P->prof = on != 0;
+    for (int i = 0; i < NKIDS; ++i) { P->prof_ms[i] = 0; P->prof_cnt[i] = 0; P->prof_flops[i] = 0; }
+    return 0;
+}
+
+// ms[NKIDS], cnt[NKIDS], flops[NKIDS] accumulated per kernel kind.  Kinds:
+// 0 chol_small 1 extadd 2 potrf_diag 3 panel_mul 4 gemm 5 extract
+// 6 projinv_small 7 trtri_diag 8 neg_copy 9 llt_small 10 solve_fwd 11 solve_bwd
+int chp_profile_read(chp_plan* P, double* ms, i64* cnt, double* flops) {
+    if (!P) return -1;
+    for (int i = 0; i < NKIDS; ++i) { ms[i] = P->prof_ms[i]; cnt[i] = P->prof_cnt[i]; flops[i] = P->prof_flops[i]; }
+    return NKIDS;
+}
+
+// Algorithmic GEMM flops of one sweep: out[0] cholesky, out[1] projected inverse.
+int chp_plan_flops(chp_plan* P, double* out) {
+    if (!P) return -1;
+    out[0] = out[1] = 0.0;
+    for (const Launch& L : P->chol) out[0] += L.flops;
+    for (const Launch& L : P->projinv) out[1] += L.flops;
     return 0;
 }
 

@@ -1,0 +1,142 @@
+"""Pins the NumPy oracle (oracle/numeric.py) before it is trusted as the
+checker of the GPU path:
+  * against golden vectors written by the reference's own C kernels
+    (tests/golden/*.npz, made by tests/golden/make_golden.py);
+  * against oracle/_ref (the reference kernels) directly, when built;
+  * against closed forms: dense Cholesky of P A P^T, the dense inverse on the
+    filled pattern, dense triangular solves, dense trace;
+  * against the reference suite's identities (tests/test_numeric.py:20-124).
+Each check fails under a plausible oracle mistake (dropped Schur term,
+missing stored-convention solve, transposed operand, wrong child order)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import chompack_amd as cp
+from chompack_amd import problems
+from oracle import numeric as on
+from oracle import ref
+
+GOLD = Path(__file__).parent / "golden"
+FIXTURES = sorted(p.stem for p in GOLD.glob("*.npz"))
+
+
+def _problem(name):
+    g = np.load(GOLD / (name + ".npz"))
+    n = int(g["n"])
+    A = sp.csc_matrix((g["A_vals"], (g["A_rows"], g["A_cols"])), shape=(n, n))
+    p = None if name.endswith("natural") else cp.amd.order
+    symb = cp.symbolic(A, p=p)
+    assert np.array_equal(np.asarray(symb.p), g["symb_p"])
+    assert np.array_equal(np.asarray(symb.relidx), g["symb_relidx"])
+    return g, A, symb
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_vs_reference_golden(name):
+    g, A, symb = _problem(name)
+    b = cp.cspmatrix(symb, device="cpu").__add__(A).blkval.numpy().copy()
+    assert np.array_equal(b, g["blkval_A"])
+    L = b.copy()
+    assert on.cholesky(symb, L) == 0
+    assert _rel(L, g["blkval_L"]) < 1e-13
+    Y = L.copy()
+    assert on.projected_inverse(symb, Y) == 0
+    assert _rel(Y, g["blkval_Y"]) < 1e-13
+    P = L.copy()
+    on.llt(symb, P)
+    assert _rel(P, g["blkval_LLT"]) < 1e-13
+    for tr in "NT":
+        B = g["B"].copy()
+        on.trsm(symb, g["blkval_L"].copy(), B, trans=tr)
+        assert _rel(B, g["B_trsm" + tr]) < 1e-13
+    assert abs(on.dot(symb, b, g["blkval_Y"]) - float(g["dot_AY"])) < 1e-10
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("side", [8, 20])
+def test_oracle_vs_reference_kernels(side):
+    A = problems.grid2d(side, seed=3)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    b = (cp.cspmatrix(symb, device="cpu") + A).blkval.numpy().copy()
+    L1, L2 = b.copy(), b.copy()
+    on.cholesky(symb, L1)
+    ref.cholesky(symb, L2)
+    assert _rel(L1, L2) < 1e-13
+    on.projected_inverse(symb, L1)
+    ref.projected_inverse(symb, L2)
+    assert _rel(L1, L2) < 1e-13
+
+
+def _dense(A):
+    return cp.symmetrize(A).toarray()
+
+
+@pytest.mark.parametrize("maker", [lambda: problems.reference_test_matrix(),
+                                   lambda: problems.grid2d(12, seed=0),
+                                   lambda: problems.grid3d(5, seed=2)])
+def test_oracle_closed_forms(maker):
+    A = maker()
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cpu") + A
+    b = X.blkval.numpy().copy()
+    Ad = _dense(A)
+    PAP = Ad[np.ix_(symb.p, symb.p)]
+    Ld = np.linalg.cholesky(PAP)
+    L = cp.cspmatrix(symb, blkval=b.copy(), device="cpu")
+    assert on.cholesky(symb, L.blkval.numpy()) == 0
+    L.is_factor = True
+    Lsp = L.spmatrix(reordered=True).toarray()
+    assert np.abs(Lsp - Ld).max() < 1e-12 * np.abs(Ld).max()
+    # projected inverse: dense inverse on the filled pattern
+    Y = L.blkval.numpy().copy()
+    on.projected_inverse(symb, Y)
+    Yc = cp.cspmatrix(symb, blkval=Y, device="cpu")
+    Ys = Yc.spmatrix(reordered=True, symmetric=True).toarray()
+    Pat = symb.sparsity_pattern(reordered=True, symmetric=True).toarray() != 0
+    Ainv = np.linalg.inv(PAP)
+    assert np.abs(Ys[Pat] - Ainv[Pat]).max() < 1e-12 * np.abs(Ainv).max()
+    # solves: L^{-1} b and L^{-T} b in the original ordering
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((symb.n, 2))
+    BN = B.copy()
+    on.trsm(symb, L.blkval.numpy(), BN, trans="N")
+    ip = symb.ip
+    ref_N = np.linalg.solve(Ld, B[symb.p])[ip]
+    assert np.abs(BN - ref_N).max() < 1e-12 * np.abs(ref_N).max()
+    BT = B.copy()
+    on.trsm(symb, L.blkval.numpy(), BT, trans="T")
+    ref_T = np.linalg.solve(Ld.T, B[symb.p])[ip]
+    assert np.abs(BT - ref_T).max() < 1e-12 * np.abs(ref_T).max()
+    # llt(cholesky(A)) == A and <P(A^-1), A> = n
+    P = L.blkval.numpy().copy()
+    on.llt(symb, P)
+    assert np.abs(P - b).max() < 1e-12 * np.abs(b).max()
+    assert abs(on.dot(symb, Y, b) - symb.n) < 1e-9 * symb.n
+    Xd = cp.cspmatrix(symb, blkval=b, device="cpu").spmatrix(reordered=True, symmetric=True).toarray()
+    assert abs(on.dot(symb, b, Y) - np.sum(Xd * Ys)) < 1e-9 * symb.n
+
+
+def test_planted_mistake_is_caught():
+    """A copy of the oracle with the Schur-complement update dropped must
+    fail the closed-form check (the checks above are not vacuous)."""
+    import oracle.numeric as m
+    A = problems.grid2d(6, seed=0)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    b = (cp.cspmatrix(symb, device="cpu") + A).blkval.numpy().copy()
+    orig = m._extend_add
+    try:
+        m._extend_add = lambda F, U, r: None
+        bad = b.copy()
+        m.cholesky(symb, bad)
+    finally:
+        m._extend_add = orig
+    good = b.copy()
+    m.cholesky(symb, good)
+    assert _rel(bad, good) > 1e-3
