@@ -95,7 +95,19 @@ def main():
     kcnt = np.zeros(12, dtype=np.int64)
     kfl = np.zeros(12)
     L.chp_profile_read(plan.handle, _native.ptr(kms), _native.ptr(kcnt), _native.ptr(kfl))
+    nrows = int(L.chp_profile_launches(plan.handle, None, 0))
+    rows = np.zeros(4 * max(nrows, 1))
+    L.chp_profile_launches(plan.handle, _native.ptr(rows), nrows)
+    rows = rows.reshape(-1, 4)[:nrows]
     L.chp_profile(plan.handle, 0)
+    g = rows[rows[:, 0] == 4]
+    gemm_classes = {}
+    for lo, hi in [(1, 16), (17, 64), (65, 256), (257, 1024), (1025, 1 << 30)]:
+        m = (g[:, 1] >= lo) & (g[:, 1] <= hi)
+        if m.any():
+            gemm_classes["tiles %d-%d" % (lo, min(hi, int(g[:, 1].max())))] = {
+                "launches": int(m.sum()), "ms": round(float(g[m, 3].sum()), 3),
+                "tflops": round(float(g[m, 2].sum() / (g[m, 3].sum() * 1e-3) / 1e12), 2)}
     comps = {KINDS[i]: {"ms": round(float(kms[i]), 4), "launches": int(kcnt[i])}
              for i in range(12) if kcnt[i]}
     dom = int(np.argmax(kms))
@@ -130,6 +142,7 @@ def main():
         "effective_tflops": round((fl_chol + fl_pinv) / (ms * 1e-3) / 1e12, 3),
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,
+        "gemm_by_size": gemm_classes,
     }
 
     if not args.no_cpu:

@@ -69,6 +69,7 @@ def _declare(L):
     _opt(L, "chp_profile", [vp, i32], i32)
     _opt(L, "chp_profile_read", [vp, vp, vp, vp], i32)
     _opt(L, "chp_plan_flops", [vp, vp], i32)
+    _opt(L, "chp_profile_launches", [vp, vp, i64], i64)
     _opt(L, "chp_last_error", [], ctypes.c_char_p)
 
 

@@ -13,13 +13,14 @@
 // columns [c0, c0+16) of front k: it zeroes the F22 part of those columns,
 // then adds every child's update in reverse child order (the summation order
 // of cholesky.c:41-52), so each target entry is updated by exactly one
-// workgroup, in the reference's order, without atomics.
+// workgroup, in the reference's order, without atomics.  (llt.c:36-46 adds
+// into an F22 already holding the product, so zeroing is optional.)
 // Target (row, col) with col < nn lives in the blkval block (ld nj); with
 // col >= nn in k's update slot (ld na).
 __global__ __launch_bounds__(TPB) void k_extadd_tiles(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx, const int* __restrict__ chidx) {
+    const int* __restrict__ relidx, const int* __restrict__ chidx, int zero22) {
     const int2 t = tiles[blockIdx.x];
     const int k = t.x, c0 = t.y;
     const FrontInfo f = fi[k];
@@ -28,9 +29,11 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
     const int tid = threadIdx.x;
     double* blk = blkval + f.blkptr;
     double* U = upd + f.updptr;
-    for (int tc = max(c0, nn); tc < c1; ++tc)
-        for (int tr = tc + tid; tr < nj; tr += TPB)
-            U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
+    if (zero22) {
+        for (int tc = max(c0, nn); tc < c1; ++tc)
+            for (int tr = tc + tid; tr < nj; tr += TPB)
+                U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
+    }
     __syncthreads();
     __shared__ int s_ja, s_jb;
     for (int l = f.chend - 1; l >= f.chbeg; --l) {
@@ -65,82 +68,124 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
 
 // ---------------------------------------------------------------------------
 // Diagonal blocks.  One 256-thread workgroup per 64 x 64 diagonal block,
-// padded with the identity beyond nb.  Thread (w = tid/64, l = tid%64) owns
-// the 16 elements (w + 4q, l), q = 0..15, in registers.  Both the Cholesky
-// factorization and the inverse of the factor are right-looking with two
-// barriers per column; the column (or row) being eliminated is broadcast
-// through LDS.  Inverses are stored row-major: Dinv(r, j) at dinv[j + r*NB].
+// padded with the identity beyond nb.  Serial steps, not flops, bound these
+// kernels (a barrier step costs about a microsecond once the dependent FP64
+// pivot chain is included), so:
+//   * the factorization eliminates 4 columns per step (16 steps): every
+//     thread factors the 4 x 4 pivot block redundantly from LDS, computes its
+//     own row of the 4-column panel, and applies the rank-4 update to the 16
+//     elements it owns, (l, w + 4q) for thread (w = tid/64, l = tid%64);
+//   * the inverse uses recursive doubling: closed-form inverses of the 16
+//     diagonal 4 x 4 blocks, then 4 levels of X21 = -X22 (L21 X11), 8 barriers.
+// Inverses are stored row-major: Dinv(r, j) at dinv[j + r*NB].
 
 #define DQ 16  // elements per thread
 
-// L (lower, padded identity) in registers -> factor in place.  Returns false
-// (uniformly) if a pivot is not positive; *bad receives its column.
-__device__ inline bool block_potrf(double (&a)[DQ], double* colc, int w, int l, int* bad) {
-    __shared__ double sd;
-    __shared__ int sfail;
-    if (w == 0 && l == 0) sfail = -1;
+// Factor the padded block held in a[] in place.  Returns false (uniformly)
+// on a non-positive pivot, whose column goes to *bad.  rdiag[c] = 1/L(c,c).
+__device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)[5],
+                                    double* rdiag, int w, int l, int* bad) {
 #pragma unroll 1
-    for (int c = 0; c < NB; ++c) {
-        const int qc = c >> 2, wc = c & 3;
-        // owner of (c, c) publishes the pivot
-        if (w == wc && l == c) {
-            double d = 0.0;
+    for (int s = 0; s < NB / 4; ++s) {
+        const int r0 = 4 * s;
+        double v = 0.0;
 #pragma unroll
-            for (int q = 0; q < DQ; ++q) if (q == qc) d = a[q];
-            if (!(d > 0.0)) sfail = c;
-            sd = sqrt(d);
-        }
+        for (int q = 0; q < DQ; ++q) if (q == s) v = a[q];
+        P[l][w] = v;  // column r0 + w
         __syncthreads();
-        if (sfail >= 0) { *bad = sfail; return false; }
-        const double s = sd, rs = 1.0 / s;
-        // column c: L(i, c) = a(i, c) / s for i > c, L(c, c) = s
-        if (l == c) {
+        // 4 x 4 pivot block, redundantly in every thread
+        const double d00 = P[r0][0], d10 = P[r0 + 1][0], d20 = P[r0 + 2][0], d30 = P[r0 + 3][0];
+        const double d11 = P[r0 + 1][1], d21 = P[r0 + 2][1], d31 = P[r0 + 3][1];
+        const double d22 = P[r0 + 2][2], d32 = P[r0 + 3][2], d33 = P[r0 + 3][3];
+        const double i0 = rsqrt(d00);
+        const double L10 = d10 * i0, L20 = d20 * i0, L30 = d30 * i0;
+        const double e11 = d11 - L10 * L10;
+        const double i1 = rsqrt(e11);
+        const double L21 = (d21 - L20 * L10) * i1, L31 = (d31 - L30 * L10) * i1;
+        const double e22 = d22 - L20 * L20 - L21 * L21;
+        const double i2 = rsqrt(e22);
+        const double L32 = (d32 - L30 * L20 - L31 * L21) * i2;
+        const double e33 = d33 - L30 * L30 - L31 * L31 - L32 * L32;
+        const double i3 = rsqrt(e33);
+        const int fail = !(d00 > 0.0) ? 0 : (!(e11 > 0.0) ? 1 : (!(e22 > 0.0) ? 2 : (!(e33 > 0.0) ? 3 : -1)));
+        if (fail >= 0) { *bad = r0 + fail; return false; }
+        // this thread's row of the panel: x = P[l, 0:4] L_pp^{-T}
+        double x0, x1, x2, x3;
+        if (l >= r0 + 4) {
+            const double p0 = P[l][0], p1 = P[l][1], p2 = P[l][2], p3 = P[l][3];
+            x0 = p0 * i0;
+            x1 = (p1 - x0 * L10) * i1;
+            x2 = (p2 - x0 * L20 - x1 * L21) * i2;
+            x3 = (p3 - x0 * L30 - x1 * L31 - x2 * L32) * i3;
+        } else {
+            const int r = l - r0;  // rows of the pivot block, zero above it
+            x0 = r == 0 ? d00 * i0 : (r == 1 ? L10 : (r == 2 ? L20 : (r == 3 ? L30 : 0.0)));
+            x1 = r == 1 ? e11 * i1 : (r == 2 ? L21 : (r == 3 ? L31 : 0.0));
+            x2 = r == 2 ? e22 * i2 : (r == 3 ? L32 : 0.0);
+            x3 = r == 3 ? e33 * i3 : 0.0;
+        }
+        const double xw = w == 0 ? x0 : (w == 1 ? x1 : (w == 2 ? x2 : x3));
 #pragma unroll
-            for (int q = 0; q < DQ; ++q) {
-                const int i = w + 4 * q;
-                if (i > c) { a[q] *= rs; colc[i] = a[q]; }
-                else if (i == c) { a[q] = s; colc[i] = s; }
-            }
-        }
+        for (int q = 0; q < DQ; ++q) if (q == s) a[q] = xw;
+        if (w == 0) { Q[l][0] = x0; Q[l][1] = x1; Q[l][2] = x2; Q[l][3] = x3; }
+        if (threadIdx.x == 0) { rdiag[r0] = i0; rdiag[r0 + 1] = i1; rdiag[r0 + 2] = i2; rdiag[r0 + 3] = i3; }
         __syncthreads();
-        // trailing update a(i, j) -= L(i, c) L(j, c) for i >= j > c
-        if (l > c) {
-            const double lj = colc[l];
+        // rank-4 update of owned a(l, j), j = w + 4q > r0 + 3, l >= j
 #pragma unroll
-            for (int q = 0; q < DQ; ++q) {
-                const int i = w + 4 * q;
-                if (i >= l) a[q] -= colc[i] * lj;
-            }
+        for (int q = 0; q < DQ; ++q) {
+            const int j = w + 4 * q;
+            const double q0 = Q[j][0], q1 = Q[j][1], q2 = Q[j][2], q3 = Q[j][3];
+            const double m = (j >= r0 + 4 && l >= j) ? 1.0 : 0.0;
+            a[q] -= m * (x0 * q0 + x1 * q1 + x2 * q2 + x3 * q3);
         }
-        __syncthreads();
     }
+    __syncthreads();
     return true;
 }
 
-// x = L^{-1} for the lower triangular L held (full, padded) in LDS Ls.
-__device__ inline void block_trtri(const double (*Ls)[NB + 1], double (&x)[DQ], double* rowc,
-                                   int w, int l) {
-#pragma unroll
-    for (int q = 0; q < DQ; ++q) x[q] = (w + 4 * q == l) ? 1.0 : 0.0;
+// X = L^{-1} (64 x 64, in LDS) from L (lower, in LDS) and rdiag = 1/diag(L),
+// by recursive doubling.  W is a 64 x 65 LDS temporary.
+__device__ inline void block_trtri_rd(const double (*Ls)[NB + 1], const double* rdiag,
+                                      double (*X)[NB + 1], double (*W)[NB + 1]) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += TPB) X[e / NB][e % NB] = 0.0;
+    __syncthreads();
+    if (tid < NB / 4) {
+        const int b = 4 * tid;
+        // closed-form inverse of the 4 x 4 lower block at (b, b)
+        const double y0 = rdiag[b], y1 = rdiag[b + 1], y2 = rdiag[b + 2], y3 = rdiag[b + 3];
+        const double l10 = Ls[b + 1][b], l20 = Ls[b + 2][b], l30 = Ls[b + 3][b];
+        const double l21 = Ls[b + 2][b + 1], l31 = Ls[b + 3][b + 1], l32 = Ls[b + 3][b + 2];
+        const double x10 = -y1 * l10 * y0;
+        const double x21 = -y2 * l21 * y1;
+        const double x20 = -y2 * (l20 * y0 + l21 * x10);
+        const double x32 = -y3 * l32 * y2;
+        const double x31 = -y3 * (l31 * y1 + l32 * x21);
+        const double x30 = -y3 * (l30 * y0 + l31 * x10 + l32 * x20);
+        X[b][b] = y0; X[b + 1][b + 1] = y1; X[b + 2][b + 2] = y2; X[b + 3][b + 3] = y3;
+        X[b + 1][b] = x10; X[b + 2][b] = x20; X[b + 3][b] = x30;
+        X[b + 2][b + 1] = x21; X[b + 3][b + 1] = x31; X[b + 3][b + 2] = x32;
+    }
+    __syncthreads();
 #pragma unroll 1
-    for (int c = 0; c < NB; ++c) {
-        const int qc = c >> 2, wc = c & 3;
-        // row c of X is final once divided by L(c, c)
-        if (w == wc) {
-            double v = 0.0;
-#pragma unroll
-            for (int q = 0; q < DQ; ++q) if (q == qc) { x[q] = x[q] / Ls[c][c]; v = x[q]; }
-            rowc[l] = v;
+    for (int bs = 4; bs < NB; bs *= 2) {
+        const int nout = (NB / (2 * bs)) * bs * bs;
+        // W = L21 X11 (X11 lower: k >= j)
+        for (int e = tid; e < nout; e += TPB) {
+            const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
+            const int a0 = pr * 2 * bs;
+            double acc = 0.0;
+            for (int k = j; k < bs; ++k) acc += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
+            W[a0 + bs + i][a0 + j] = acc;
         }
         __syncthreads();
-        // X(i, j) -= L(i, c) X(c, j) for i > c, j <= c
-        if (l <= c) {
-            const double xc = rowc[l];
-#pragma unroll
-            for (int q = 0; q < DQ; ++q) {
-                const int i = w + 4 * q;
-                if (i > c) x[q] -= Ls[i][c] * xc;
-            }
+        // X21 = -X22 W (X22 lower: k <= i)
+        for (int e = tid; e < nout; e += TPB) {
+            const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
+            const int a0 = pr * 2 * bs;
+            double acc = 0.0;
+            for (int k = 0; k <= i; ++k) acc += X[a0 + bs + i][a0 + bs + k] * W[a0 + bs + k][a0 + j];
+            X[a0 + bs + i][a0 + j] = -acc;
         }
         __syncthreads();
     }
@@ -151,7 +196,10 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
     double* __restrict__ blkval, double* __restrict__ dinv,
     const i64* __restrict__ dinvptr, unsigned long long* status) {
     __shared__ double Ls[NB][NB + 1];
-    __shared__ double vec[NB];
+    __shared__ double X[NB][NB + 1];
+    __shared__ double W[NB][NB + 1];
+    __shared__ double P[NB][5], Q[NB][5];
+    __shared__ double rdiag[NB];
     const int k = fronts[blockIdx.x];
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
@@ -161,28 +209,27 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
     double a[DQ];
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
-        const int i = w + 4 * q;
-        a[q] = (i < nb && l < nb && i >= l) ? blk[i + (i64)l * nj] : (i == l ? 1.0 : 0.0);
+        const int j = w + 4 * q;
+        a[q] = (l < nb && j < nb && l >= j) ? blk[l + (i64)j * nj] : (l == j ? 1.0 : 0.0);
     }
     int bad = 0;
-    if (!block_potrf(a, vec, w, l, &bad)) {
+    if (!block_potrf4(a, P, Q, rdiag, w, l, &bad)) {
         if (threadIdx.x == 0) chp_record_failure(status, k, c0 + bad + 1);
         return;
     }
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
-        const int i = w + 4 * q;
-        Ls[i][l] = (i >= l) ? a[q] : 0.0;
-        if (i < nb && l < nb && i >= l) blk[i + (i64)l * nj] = a[q];
+        const int j = w + 4 * q;
+        Ls[l][j] = (l >= j) ? a[q] : 0.0;
+        if (l < nb && j < nb && l >= j) blk[l + (i64)j * nj] = a[q];
     }
     __syncthreads();
-    double x[DQ];
-    block_trtri(Ls, x, vec, w, l);
+    block_trtri_rd(Ls, rdiag, X, W);
     double* inv = dinv + dinvptr[k] + (i64)(c0 / NB) * NB * NB;
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
-        const int i = w + 4 * q;
-        inv[l + i * NB] = (i < nb && l < nb) ? x[q] : 0.0;
+        const int r = w + 4 * q;  // row r, column l of Dinv
+        inv[l + r * NB] = (r < nb && l < nb) ? X[r][l] : 0.0;
     }
 }
 
@@ -192,10 +239,13 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
 __global__ __launch_bounds__(TPB) void k_trtri_diag(
     const int2* __restrict__ blocks, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ dinv,
-    const i64* __restrict__ dinvptr, double* __restrict__ X,
+    const i64* __restrict__ dinvptr, double* __restrict__ Xg,
     const i64* __restrict__ xptr, unsigned long long* status) {
     __shared__ double Ls[NB][NB + 1];
-    __shared__ double vec[NB];
+    __shared__ double X[NB][NB + 1];
+    __shared__ double W[NB][NB + 1];
+    __shared__ double rdiag[NB];
+    __shared__ int zero_at;
     const int2 b = blocks[blockIdx.x];
     const int k = b.x, c0 = b.y * NB;
     const FrontInfo f = fi[k];
@@ -203,27 +253,31 @@ __global__ __launch_bounds__(TPB) void k_trtri_diag(
     const int nb = min(NB, nn - c0);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const double* blk = blkval + f.blkptr + c0 + (i64)c0 * nj;
+    if (threadIdx.x == 0) zero_at = NB;
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
-        const int i = w + 4 * q;
-        Ls[i][l] = (i < nb && l < nb && i >= l) ? blk[i + (i64)l * nj] : (i == l ? 1.0 : 0.0);
+        const int j = w + 4 * q;
+        Ls[l][j] = (l < nb && j < nb && l >= j) ? blk[l + (i64)j * nj] : (l == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (Ls[j][j] == 0.0) {
-            if (threadIdx.x == 0) chp_record_failure(status, k, c0 + j + 1);
-            return;
-        }
+    if (w == 0) {
+        if (l < nb && Ls[l][l] == 0.0) atomicMin(&zero_at, l);
+        rdiag[l] = 1.0 / Ls[l][l];
     }
-    double x[DQ];
-    block_trtri(Ls, x, vec, w, l);
+    __syncthreads();
+    if (zero_at < NB) {
+        if (threadIdx.x == 0) chp_record_failure(status, k, c0 + zero_at + 1);
+        return;
+    }
+    block_trtri_rd(Ls, rdiag, X, W);
     double* inv = dinv + dinvptr[k] + (i64)b.y * NB * NB;
-    double* xd = X + xptr[k] + c0 + (i64)c0 * nn;
+    double* xd = Xg + xptr[k] + c0 + (i64)c0 * nn;
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
-        const int i = w + 4 * q;
-        inv[l + i * NB] = (i < nb && l < nb) ? x[q] : 0.0;
-        if (i < nb && l < nb) xd[i + (i64)l * nn] = x[q];
+        const int j = w + 4 * q;  // element (l, j) of the block of X
+        if (l < nb && j < nb) xd[l + (i64)j * nn] = X[l][j];
+        const int r = w + 4 * q;  // row r, column l of Dinv
+        inv[l + r * NB] = (r < nb && l < nb) ? X[r][l] : 0.0;
     }
 }
 
@@ -242,72 +296,70 @@ __device__ inline const double* bufptr(int b, double* b0, double* b1, double* b2
     return b == 0 ? b0 : (b == 1 ? b1 : (b == 2 ? b2 : b3));
 }
 
-// Element (gi, gk) of opA and (gk, gj) of opB with the task's masks; 0 outside.
-__device__ inline double ldA(const GemmTask& T, const double* A, int gi, int gk) {
+// Element (gi, gk) of opA and (gk, gj) of opB with the task's bounds and
+// triangular masks; 0 outside.
+template <int OPA>
+__device__ inline double ldA_chk(const GemmTask& T, const double* A, int gi, int gk) {
     bool ok = gi < T.M && gk < T.K;
     if (T.triA == 1) ok = ok && gk <= gi;
     else if (T.triA == 2) ok = ok && gk >= gi;
     if (!ok) return 0.0;
-    if (T.opA == 0) return A[gi + (i64)gk * T.lda];
-    if (T.opA == 1) return A[gk + (i64)gi * T.lda];
+    if (OPA == 0) return A[gi + (i64)gk * T.lda];
+    if (OPA == 1) return A[gk + (i64)gi * T.lda];
     return (gi >= gk) ? A[gi + (i64)gk * T.lda] : A[gk + (i64)gi * T.lda];
 }
-__device__ inline double ldB(const GemmTask& T, const double* B, int gk, int gj) {
+template <int OPB>
+__device__ inline double ldB_chk(const GemmTask& T, const double* B, int gk, int gj) {
     bool ok = gj < T.N && gk < T.K;
     if (T.pad == 1) ok = ok && gk >= gj;
     if (!ok) return 0.0;
-    return (T.opB == 0) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
+    return (OPB == 0) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
 }
 
-__global__ __launch_bounds__(TPB) void k_gemm_tiles(
-    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
-    double* b0, double* b1, double* b2, double* b3) {
-    __shared__ double As[BK][TILE + 2];
-    __shared__ double Bs[BK][TILE + 2];
-    const GemmTile tl = tiles[blockIdx.x];
-    const GemmTask T = tasks[tl.task];
-    const double* A = bufptr(T.bufA, b0, b1, b2, b3) + T.offA;
-    const double* B = bufptr(T.bufB, b0, b1, b2, b3) + T.offB;
-    double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
-    const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
-    const int K = T.K;
+// One 64 x 64 tile.  OPA: 0 A(i,k) = A[i + k lda], 1 A[k + i lda],
+// 2 symmetric lower.  OPB: 0 B(k,j) = B[k + j ldb], 1 B[j + k ldb].
+// CHK: per-element bounds and masks (edge tiles, triangular operands);
+// otherwise a check-free interior path.  Threads are mapped so consecutive
+// lanes read consecutive addresses of the stored operand.
+template <int OPA, int OPB, bool CHK>
+__device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A,
+                                 const double* __restrict__ B, double* __restrict__ C,
+                                 int row0, int col0, double (*As)[TILE + 2],
+                                 double (*Bs)[TILE + 2]) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
-
-    if (T.tri == 2 && row0 + TILE - 1 < col0) {
-        for (int e = tid; e < TILE * TILE; e += TPB) {
-            int i = row0 + e % TILE, j = col0 + e / TILE;
-            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
-        }
-        return;
-    }
-    // 8 elements of A and 8 of B per thread and chunk, mapped so that
-    // consecutive lanes read consecutive addresses of the stored operand
-    const bool a_kfast = (T.opA != 0);
-    const bool b_kfast = (T.opB == 0);
+    const int K = T.K;
+    constexpr bool AK = (OPA == 1);   // A read along k
+    constexpr bool BK_ = (OPB == 0);  // B read along k
+    // per-thread coordinates of the 8 A and 8 B elements of a chunk
+    const int ai = AK ? (tid >> 5) : (tid & 63);
+    const int ak = AK ? (tid & 31) : (tid >> 6);
+    const int bj = BK_ ? (tid >> 5) : (tid & 63);
+    const int bk = BK_ ? (tid & 31) : (tid >> 6);
+    // q-th element: AK: (ai + 8q, ak); else (ai, ak + 4q).  Same for B.
     double ra[8], rb[8];
     auto fetch = [&](int kb) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int e = tid + q * TPB;
-            int i, kk;
-            if (a_kfast) { kk = e & 31; i = e >> 5; } else { i = e & 63; kk = e >> 6; }
-            ra[q] = ldA(T, A, row0 + i, kb + kk);
-            int j, k2;
-            if (b_kfast) { k2 = e & 31; j = e >> 5; } else { j = e & 63; k2 = e >> 6; }
-            rb[q] = ldB(T, B, kb + k2, col0 + j);
+            const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+            if (CHK) ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+            else ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+            const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+            if (CHK) rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+            else rb[q] = (OPB == 0) ? B[gk2 + (i64)gj * T.ldb] : B[gj + (i64)gk2 * T.ldb];
         }
     };
     auto stash = [&]() {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int e = tid + q * TPB;
-            if (a_kfast) As[e & 31][e >> 5] = ra[q]; else As[e >> 6][e & 63] = ra[q];
-            if (b_kfast) Bs[e & 31][e >> 5] = rb[q]; else Bs[e >> 6][e & 63] = rb[q];
+            As[ak + (AK ? 0 : 4 * q)][ai + (AK ? 8 * q : 0)] = ra[q];
+            Bs[bk + (BK_ ? 0 : 4 * q)][bj + (BK_ ? 8 * q : 0)] = rb[q];
         }
     };
     dbl4 acc[2][2];
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
     fetch(0);
     for (int kb = 0; kb < K; kb += BK) {
@@ -328,21 +380,77 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
         }
         __syncthreads();
     }
+    const double alpha = T.alpha, beta = T.beta;
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
                 const int j = col0 + wc * 32 + b * 16 + (lane & 15);
-                if (i >= T.M || j >= T.N) continue;
+                if (CHK && (i >= T.M || j >= T.N)) continue;
                 double* c = C + i + (i64)j * T.ldc;
                 if (T.tri == 0 || i >= j) {
-                    double r = T.alpha * acc[a][b][v];
-                    if (T.beta != 0.0) r += T.beta * (*c);
+                    double r = alpha * acc[a][b][v];
+                    if (beta != 0.0) r += beta * (*c);
                     *c = r;
                 } else if (T.tri == 2) {
                     *c = 0.0;
                 }
             }
+}
+
+__global__ __launch_bounds__(TPB) void k_gemm_tiles(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
+    double* b0, double* b1, double* b2, double* b3) {
+    __shared__ double As[BK][TILE + 2];
+    __shared__ double Bs[BK][TILE + 2];
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    const double* A = bufptr(T.bufA, b0, b1, b2, b3) + T.offA;
+    const double* B = bufptr(T.bufB, b0, b1, b2, b3) + T.offB;
+    double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
+    const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
+
+    if (T.tri == 2 && row0 + TILE - 1 < col0) {
+        for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
+            int i = row0 + e % TILE, j = col0 + e / TILE;
+            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
+        }
+        return;
+    }
+    // interior: full tile, K a multiple of the chunk, no operand masks; the
+    // symmetric operand qualifies when the whole tile sits on one side of the
+    // diagonal of A (then it is a plain N or T read)
+    int opa = T.opA;
+    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && (T.K % BK) == 0 &&
+                      T.triA == 0 && T.pad == 0;
+    bool chk = !full;
+    if (opa == 2 && full) {
+        if (row0 >= T.K) opa = 0;            // all gi >= gk: lower part, N read
+        else if (row0 + TILE <= 0) opa = 1;  // never
+        else chk = true;
+    }
+    const int key = opa * 2 + T.opB;
+    if (!chk) {
+        switch (key) {
+        case 0: gemm_tile<0, 0, false>(T, A, B, C, row0, col0, As, Bs); break;
+        case 1: gemm_tile<0, 1, false>(T, A, B, C, row0, col0, As, Bs); break;
+        case 2: gemm_tile<1, 0, false>(T, A, B, C, row0, col0, As, Bs); break;
+        case 3: gemm_tile<1, 1, false>(T, A, B, C, row0, col0, As, Bs); break;
+        default: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
+        }
+    } else {
+        switch (key) {
+        case 0: gemm_tile<0, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 1: gemm_tile<0, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 2: gemm_tile<1, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 3: gemm_tile<1, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 4: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
+        default: gemm_tile<2, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
+        }
+    }
 }
 
 // blk[nn + i, j] = -S[i + j*na] for the listed (front, 16-column) tiles:
@@ -361,4 +469,22 @@ __global__ __launch_bounds__(TPB) void k_neg_copy(
     const int c1 = min(c0 + 16, nn);
     for (int j = c0; j < c1; ++j)
         for (int i = threadIdx.x; i < na; i += TPB) blk[i + (i64)j * nj] = -S[i + (i64)j * na];
+}
+
+// scratch[sptr[k] + i + j*nj] = blk[i + j*nj] for the rows i < nn of the
+// listed (front, 16-column) tiles: the diagonal block as stored, kept for
+// the product [L11; B21 L11][L11; B21 L11]^T of llt.c:32-33.
+__global__ __launch_bounds__(TPB) void k_copy_top(
+    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, double* __restrict__ scratch,
+    const i64* __restrict__ sptr) {
+    const int2 t = tiles[blockIdx.x];
+    const int k = t.x, c0 = t.y;
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, nj = nn + f.na;
+    const double* blk = blkval + f.blkptr;
+    double* S = scratch + sptr[k];
+    const int c1 = min(c0 + 16, nn);
+    for (int j = c0; j < c1; ++j)
+        for (int i = threadIdx.x; i < nn; i += TPB) S[i + (i64)j * nj] = blk[i + (i64)j * nj];
 }

@@ -100,14 +100,17 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
     }
 }
 
-// Gather V_k = F_p[r, r] (lower) from the parent's finished front into k's
-// update slot (projected_inverse.c:60-70 / cbase.c:39-59, done by the child).
-// The parent's front is its blkval block (columns < nn_p) and its own V_p
-// (columns >= nn_p).  One workgroup per (front, 16-column tile).
+// Gather V_k = F_p[r, r] from the parent's finished front into k's update
+// slot, both triangles (projected_inverse.c:60-70 / cbase.c:39-59, done by
+// the child).  The parent's front is its blkval block (columns < nn_p) and
+// its own V_p (columns >= nn_p, lower triangle read).  One workgroup per
+// (front, 16-column tile); rows go through LDS in chunks of 256 so that the
+// mirrored upper-triangle writes are coalesced too.
 __global__ __launch_bounds__(TPB) void k_extract(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ upd,
     const int* __restrict__ relidx) {
+    __shared__ double S[256][17];
     const int2 t = tiles[blockIdx.x];
     const int k = t.x, j0 = t.y;
     const FrontInfo f = fi[k];
@@ -119,13 +122,27 @@ __global__ __launch_bounds__(TPB) void k_extract(
     const double* pV = upd + p.updptr;
     double* V = upd + f.updptr;
     const int j1 = min(j0 + 16, na);
-    for (int j = j0; j < j1; ++j) {
-        const int rj = r[j];
-        for (int i = j + (int)threadIdx.x; i < na; i += TPB) {
-            const int ri = r[i];
-            double v = (rj < nnp) ? pblk[ri + (i64)rj * njp]
-                                  : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
-            V[i + (i64)j * na] = v;
+    const int tid = threadIdx.x;
+    for (int i0 = j0; i0 < na; i0 += 256) {
+        const int i1 = min(i0 + 256, na);
+        __syncthreads();
+        for (int j = j0; j < j1; ++j) {
+            const int rj = r[j];
+            const int i = i0 + tid;
+            if (i < i1 && i >= j) {
+                const int ri = r[i];
+                const double v = (rj < nnp) ? pblk[ri + (i64)rj * njp]
+                                            : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
+                V[i + (i64)j * na] = v;
+                S[tid][j - j0] = v;
+            }
+        }
+        __syncthreads();
+        // mirrored part V[j, i] = V[i, j] for i > j: 16 consecutive j per row i
+        for (int e = tid; e < (i1 - i0) * 16; e += TPB) {
+            const int jj = e & 15, ii = e >> 4;
+            const int i = i0 + ii, j = j0 + jj;
+            if (j < j1 && i > j) V[j + (i64)i * na] = S[ii][jj];
         }
     }
 }

@@ -20,7 +20,9 @@ __global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkv
 
 // large fronts
 __global__ void k_extadd_tiles(const int2* tiles, const FrontInfo* fi, double* blkval,
-                               double* upd, const int* relidx, const int* chidx);
+                               double* upd, const int* relidx, const int* chidx, int zero22);
+__global__ void k_copy_top(const int2* tiles, const FrontInfo* fi, const double* blkval,
+                           double* scratch, const i64* sptr);
 __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, double* blkval,
                              double* dinv, const i64* dinvptr, unsigned long long* status);
 __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,

@@ -27,7 +27,8 @@ namespace {
 
 enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
-    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD
+    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
+    K_COPY_TOP
 };
 
 struct Launch {
@@ -40,7 +41,7 @@ struct Launch {
     double flops = 0.0;  // algorithmic flops of a GEMM launch
 };
 
-const int NKIDS = 12;
+const int NKIDS = 13;
 
 struct ListBuf {
     std::vector<char> bytes;
@@ -62,7 +63,8 @@ struct chp_plan {
     int small_max = 64;
     std::vector<FrontInfo> fi;
     std::vector<std::vector<int>> bylevel;
-    std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit;
+    std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit, h_lltptr;
+    i64* d_lltptr = nullptr;
     // device arrays
     FrontInfo* d_fi = nullptr;
     int* d_relidx = nullptr;
@@ -95,6 +97,7 @@ struct chp_plan {
     // profiling: per-kernel-kind time from HIP events around every launch
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> prof_rows;  // per launch: kind, grid, flops, ms
     double prof_ms[NKIDS] = {0};
     double prof_flops[NKIDS] = {0};
     i64 prof_cnt[NKIDS] = {0};
@@ -202,7 +205,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         int nj = P->fi[k].nn + P->fi[k].na;
         for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
     }
-    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 0});
+    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 1});
     int maxnn = 0;
     for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
     for (int c0 = 0; c0 < maxnn; c0 += NB) {
@@ -344,7 +347,8 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         add_gemm_tiles(P, g3, mk(2, xo, nn, 1, 2, xo, nn, 0, 0, f.blkptr, nj,
                                  nn, nn, nn, 1.0, 0.0, 2, 2, 1), false);
         if (na > 0) {
-            add_gemm_tiles(P, g3, mk(1, f.updptr, na, 2, 0, f.blkptr + nn, nj, 0,
+            // V_k is stored with both triangles by the extract kernel
+            add_gemm_tiles(P, g3, mk(1, f.updptr, na, 0, 0, f.blkptr + nn, nj, 0,
                                      2, to, na, na, nn, na, 1.0, 0.0, 0), false);
             add_gemm_tiles(P, g4, mk(2, to, na, 1, 0, f.blkptr + nn, nj, 0,
                                      0, f.blkptr, nj, nn, nn, na, 1.0, 1.0, 1), true);
@@ -360,16 +364,62 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->lists.add(nt), 0});
 }
 
-void build_llt_level(chp_plan* P, const std::vector<int>& fr) {
-    std::vector<int> small;
+// llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
+// block as stored (A_top), A_bot = B21 L11, F = A A^T on MFMA (lower), then
+// extend-add the children into F without clearing F22.
+void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need) {
+    std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
         const FrontInfo& f = P->fi[k];
-        if (llt_small_lds(f) <= LDS_SMALL_CAP) { small.push_back(k); lds = std::max(lds, llt_small_lds(f)); }
-        else P->llt_ok = false;
+        if (f.nn + f.na <= P->small_max && llt_small_lds(f) <= LDS_SMALL_CAP) {
+            small.push_back(k);
+            lds = std::max(lds, llt_small_lds(f));
+        } else {
+            large.push_back(k);
+        }
     }
     if (!small.empty())
         P->llt.push_back({K_LLT_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+    if (large.empty()) return;
+    // scratch A (nj x nn, ld nj) per large front; offsets in h_xptr are per
+    // sweep and the levels run one after the other, so they are reused
+    i64 off = 0;
+    std::vector<i64> aoff(large.size());
+    for (size_t q = 0; q < large.size(); ++q) {
+        const FrontInfo& f = P->fi[large[q]];
+        aoff[q] = off;
+        off += (i64)(f.nn + f.na) * f.nn;
+        off = (off + 31) / 32 * 32;
+    }
+    scratch_need = std::max(scratch_need, off);
+    std::vector<int2> ct;
+    std::vector<GemmTile> g1, g2;
+    for (size_t q = 0; q < large.size(); ++q) {
+        int k = large[q];
+        const FrontInfo& f = P->fi[k];
+        int nn = f.nn, na = f.na, nj = nn + na;
+        P->h_lltptr[k] = aoff[q];
+        for (int c0 = 0; c0 < nn; c0 += 16) ct.push_back(make_int2(k, c0));
+        if (na > 0)  // A_bot = B21 tril(L11)
+            add_gemm_tiles(P, g1, mk(0, f.blkptr + nn, nj, 0, 0, f.blkptr, nj, 0,
+                                     2, aoff[q] + nn, nj, na, nn, nn, 1.0, 0.0, 0, 0, 1), false);
+        // F[:, :nn] (lower) and F22 = A A^T
+        add_gemm_tiles(P, g2, mk(2, aoff[q], nj, 0, 2, aoff[q], nj, 1,
+                                 0, f.blkptr, nj, nj, nn, nn, 1.0, 0.0, 1), true);
+        if (na > 0)
+            add_gemm_tiles(P, g2, mk(2, aoff[q] + nn, nj, 0, 2, aoff[q] + nn, nj, 1,
+                                     1, f.updptr, na, na, na, nn, 1.0, 0.0, 1), true);
+    }
+    P->llt.push_back({K_COPY_TOP, (int)ct.size(), 256, 0, P->lists.add(ct), 0});
+    push_gemm(P, P->llt, g1);
+    push_gemm(P, P->llt, g2);
+    std::vector<int2> et;
+    for (int k : large) {
+        int nj = P->fi[k].nn + P->fi[k].na;
+        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
+    }
+    P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 0});
 }
 
 template <class T>
@@ -397,7 +447,11 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
             break;
         case K_EXTADD:
             hipLaunchKernelGGL(k_extadd_tiles, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
-                               P->d_upd, P->d_relidx, P->d_chidx);
+                               P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
+            break;
+        case K_COPY_TOP:
+            hipLaunchKernelGGL(k_copy_top, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+                               P->d_scratch, P->d_lltptr);
             break;
         case K_POTRF_DIAG:
             hipLaunchKernelGGL(k_potrf_diag, g, b, 0, s, (const int*)list, L.iarg, P->d_fi,
@@ -439,6 +493,7 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
             P->prof_ms[L.kid] += ms;
             P->prof_cnt[L.kid] += 1;
             P->prof_flops[L.kid] += L.flops;
+            P->prof_rows.insert(P->prof_rows.end(), {(double)L.kid, (double)L.grid, L.flops, (double)ms});
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -499,6 +554,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     P->h_xptr.assign(nsn, 0);
     P->h_tptr.assign(nsn, 0);
     P->h_wunit.assign(nsn, 0);
+    P->h_lltptr.assign(nsn, 0);
     for (i64 k = 0; k < nsn; ++k) {
         FrontInfo& f = P->fi[k];
         f.nn = (int)(snptr[k + 1] - snptr[k]);
@@ -526,8 +582,8 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     for (int L = 0; L < H; ++L) build_chol_level(P, P->bylevel[L]);
     i64 scratch = 0;
     for (int L = H - 1; L >= 0; --L) build_projinv_level(P, P->bylevel[L], scratch);
+    for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
-    for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L]);
     for (int L = 0; L < H; ++L) {
         const auto& fr = P->bylevel[L];
         P->solve_fwd.push_back({K_SOLVE_FWD, (int)fr.size(), 256, 0, P->lists.add(fr), 0});
@@ -557,6 +613,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= upload(&P->d_xptr, P->h_xptr);
     rc |= upload(&P->d_tptr, P->h_tptr);
     rc |= upload(&P->d_wunit, P->h_wunit);
+    rc |= upload(&P->d_lltptr, P->h_lltptr);
     rc |= upload(&P->d_colfront, colfront);
     rc |= upload(&P->d_colidx, colidx);
     rc |= upload(&P->d_lists, P->lists.bytes);
@@ -580,7 +637,7 @@ void chp_plan_destroy(chp_plan* P) {
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
-                    P->d_xptr, P->d_tptr, P->d_wunit, P->d_colfront, P->d_colidx, P->d_upd,
+                    P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_work, P->d_status,
                     P->d_lists, P->d_tasks};
     for (void* p : ptrs)
@@ -608,7 +665,17 @@ int chp_profile(chp_plan* P, int on) {
     }
     P->prof = on != 0;
     for (int i = 0; i < NKIDS; ++i) { P->prof_ms[i] = 0; P->prof_cnt[i] = 0; P->prof_flops[i] = 0; }
+    P->prof_rows.clear();
     return 0;
+}
+
+// Per-launch rows (kind, workgroups, algorithmic flops, ms) of the profiled
+// sweeps; copies at most max rows into out[4*max] and returns the row count.
+i64 chp_profile_launches(chp_plan* P, double* out, i64 max) {
+    if (!P) return -1;
+    i64 rows = (i64)P->prof_rows.size() / 4;
+    for (i64 i = 0; i < std::min(rows, max) * 4; ++i) out[i] = P->prof_rows[i];
+    return rows;
 }
 
 // ms[NKIDS], cnt[NKIDS], flops[NKIDS] accumulated per kernel kind.  Kinds:
@@ -649,7 +716,6 @@ int chp_projected_inverse(chp_plan* P, double* blkval, void* stream, i64* info) 
 
 int chp_llt(chp_plan* P, double* blkval, void* stream) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
-    if (!P->llt_ok) { chp_set_error("llt: fronts larger than the LDS path are not supported yet"); return -4; }
     int rc = run_seq(P, P->llt, blkval, (hipStream_t)stream);
     if (rc) return rc;
     HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));

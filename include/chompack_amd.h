@@ -81,6 +81,9 @@ int chp_plan_info(chp_plan* plan, int64_t* out);
  * (out[0]) and one projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
 int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);
+/* Per-launch rows (kind, workgroups, algorithmic flops, ms) of the profiled
+ * sweeps: copies min(rows, max) rows into out[4*max], returns the row count. */
+int64_t chp_profile_launches(chp_plan* plan, double* out, int64_t max);
 int chp_plan_flops(chp_plan* plan, double* out);
 
 /* ---------------------------------------------------------------------------
