@@ -23,8 +23,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
-KINDS = ["chol_small", "extadd", "potrf_diag", "panel_mul", "gemm", "extract",
-         "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd"]
+KINDS = ["chol_small", "extadd", "potrf_diag", "unused", "gemm", "extract",
+         "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
+         "copy_top", "splitk_reduce"]
+NK = len(KINDS)
 
 
 def algorithmic_flops(symb):
@@ -91,9 +93,9 @@ def main():
     L.chp_profile(plan.handle, 1)
     step()
     torch.cuda.synchronize()
-    kms = np.zeros(12)
-    kcnt = np.zeros(12, dtype=np.int64)
-    kfl = np.zeros(12)
+    kms = np.zeros(NK)
+    kcnt = np.zeros(NK, dtype=np.int64)
+    kfl = np.zeros(NK)
     L.chp_profile_read(plan.handle, _native.ptr(kms), _native.ptr(kcnt), _native.ptr(kfl))
     nrows = int(L.chp_profile_launches(plan.handle, None, 0))
     rows = np.zeros(4 * max(nrows, 1))
@@ -109,7 +111,7 @@ def main():
                 "launches": int(m.sum()), "ms": round(float(g[m, 3].sum()), 3),
                 "tflops": round(float(g[m, 2].sum() / (g[m, 3].sum() * 1e-3) / 1e12), 2)}
     comps = {KINDS[i]: {"ms": round(float(kms[i]), 4), "launches": int(kcnt[i])}
-             for i in range(12) if kcnt[i]}
+             for i in range(NK) if kcnt[i]}
     dom = int(np.argmax(kms))
     gemm_tf = kfl[4] / (kms[4] * 1e-3) / 1e12 if kms[4] > 0 else 0.0
     roof = {"kernel": "k_gemm_tiles (FP64 MFMA Schur/projected-inverse products)",
@@ -143,6 +145,10 @@ def main():
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,
         "gemm_by_size": gemm_classes,
+        "gemm_top": [{"tiles": int(r[1]), "gflop": round(r[2] / 1e9, 3), "ms": round(r[3], 4),
+                      "tflops": round(r[2] / (r[3] * 1e-3) / 1e12, 2),
+                      "k_equiv": int(r[2] / max(r[1], 1) / (2 * 64 * 64))}
+                     for r in sorted(g.tolist(), key=lambda r: -r[3])[:15]],
     }
 
     if not args.no_cpu:

@@ -54,15 +54,20 @@ void chp_set_error(const std::string& msg);
 //   triA: 0 none; 1 A is lower triangular (A(i,k) = 0 for k > i);
 //         2 A is upper triangular in op form (A(i,k) = 0 for k < i);
 //   triB (stored in `pad`): 1 B is lower triangular (B(k,j) = 0 for k < j).
+//   ksplit > 1: deterministic split-K.  Tile slice ks covers
+//        k in [ks*kchunk, min(K, (ks+1)*kchunk)) and writes alpha * partial to
+//        buffer 4 at offP + ks*M*N (column-major, ld M); k_splitk_reduce then
+//        forms C = beta*C + sum_ks partial in slice order.
 struct GemmTask {
-    i64 offA, offB, offC;
+    i64 offA, offB, offC, offP;
     int lda, ldb, ldc;
     int M, N, K;
+    int ksplit, kchunk;
     unsigned char bufA, bufB, bufC, opA, opB, tri, triA, pad;
     double alpha, beta;
 };
 
-// One 64 x 64 output tile of a GemmTask.
+// One 64 x 64 output tile (k-slice ks) of a GemmTask.
 struct GemmTile {
-    int task, tm, tn, pad;
+    int task, tm, tn, ks;
 };

@@ -81,6 +81,14 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
 
 #define DQ 16  // elements per thread
 
+// 1/sqrt(d) from the hardware estimate refined by one Newton step
+// (relative error ~ ulp); the pivot chain is latency bound.
+__device__ inline double prsqrt(double d) {
+    const double r = __builtin_amdgcn_rsq(d);
+    const double h = d * r;
+    return fma(fma(-h, r, 1.0), 0.5 * r, r);
+}
+
 // Factor the padded block held in a[] in place.  Returns false (uniformly)
 // on a non-positive pivot, whose column goes to *bad.  rdiag[c] = 1/L(c,c).
 __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)[5],
@@ -97,16 +105,16 @@ __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)
         const double d00 = P[r0][0], d10 = P[r0 + 1][0], d20 = P[r0 + 2][0], d30 = P[r0 + 3][0];
         const double d11 = P[r0 + 1][1], d21 = P[r0 + 2][1], d31 = P[r0 + 3][1];
         const double d22 = P[r0 + 2][2], d32 = P[r0 + 3][2], d33 = P[r0 + 3][3];
-        const double i0 = rsqrt(d00);
+        const double i0 = prsqrt(d00);
         const double L10 = d10 * i0, L20 = d20 * i0, L30 = d30 * i0;
         const double e11 = d11 - L10 * L10;
-        const double i1 = rsqrt(e11);
+        const double i1 = prsqrt(e11);
         const double L21 = (d21 - L20 * L10) * i1, L31 = (d31 - L30 * L10) * i1;
         const double e22 = d22 - L20 * L20 - L21 * L21;
-        const double i2 = rsqrt(e22);
+        const double i2 = prsqrt(e22);
         const double L32 = (d32 - L30 * L20 - L31 * L21) * i2;
         const double e33 = d33 - L30 * L30 - L31 * L31 - L32 * L32;
-        const double i3 = rsqrt(e33);
+        const double i3 = prsqrt(e33);
         const int fail = !(d00 > 0.0) ? 0 : (!(e11 > 0.0) ? 1 : (!(e22 > 0.0) ? 2 : (!(e33 > 0.0) ? 3 : -1)));
         if (fail >= 0) { *bad = r0 + fail; return false; }
         // this thread's row of the panel: x = P[l, 0:4] L_pp^{-T}
@@ -174,18 +182,32 @@ __device__ inline void block_trtri_rd(const double (*Ls)[NB + 1], const double* 
         for (int e = tid; e < nout; e += TPB) {
             const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
             const int a0 = pr * 2 * bs;
-            double acc = 0.0;
-            for (int k = j; k < bs; ++k) acc += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
-            W[a0 + bs + i][a0 + j] = acc;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int k = j;
+            for (; k + 3 < bs; k += 4) {
+                s0 += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
+                s1 += Ls[a0 + bs + i][a0 + k + 1] * X[a0 + k + 1][a0 + j];
+                s2 += Ls[a0 + bs + i][a0 + k + 2] * X[a0 + k + 2][a0 + j];
+                s3 += Ls[a0 + bs + i][a0 + k + 3] * X[a0 + k + 3][a0 + j];
+            }
+            for (; k < bs; ++k) s0 += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
+            W[a0 + bs + i][a0 + j] = (s0 + s1) + (s2 + s3);
         }
         __syncthreads();
         // X21 = -X22 W (X22 lower: k <= i)
         for (int e = tid; e < nout; e += TPB) {
             const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
             const int a0 = pr * 2 * bs;
-            double acc = 0.0;
-            for (int k = 0; k <= i; ++k) acc += X[a0 + bs + i][a0 + bs + k] * W[a0 + bs + k][a0 + j];
-            X[a0 + bs + i][a0 + j] = -acc;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int k = 0;
+            for (; k + 3 <= i; k += 4) {
+                s0 += X[a0 + bs + i][a0 + bs + k] * W[a0 + bs + k][a0 + j];
+                s1 += X[a0 + bs + i][a0 + bs + k + 1] * W[a0 + bs + k + 1][a0 + j];
+                s2 += X[a0 + bs + i][a0 + bs + k + 2] * W[a0 + bs + k + 2][a0 + j];
+                s3 += X[a0 + bs + i][a0 + bs + k + 3] * W[a0 + bs + k + 3][a0 + j];
+            }
+            for (; k <= i; ++k) s0 += X[a0 + bs + i][a0 + bs + k] * W[a0 + bs + k][a0 + j];
+            X[a0 + bs + i][a0 + j] = -((s0 + s1) + (s2 + s3));
         }
         __syncthreads();
     }
@@ -324,11 +346,10 @@ __device__ inline double ldB_chk(const GemmTask& T, const double* B, int gk, int
 template <int OPA, int OPB, bool CHK>
 __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A,
                                  const double* __restrict__ B, double* __restrict__ C,
-                                 int row0, int col0, double (*As)[TILE + 2],
-                                 double (*Bs)[TILE + 2]) {
+                                 int row0, int col0, int kbase, int kend, double* Pp,
+                                 double (*As)[TILE + 2], double (*Bs)[TILE + 2]) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
-    const int K = T.K;
     constexpr bool AK = (OPA == 1);   // A read along k
     constexpr bool BK_ = (OPB == 0);  // B read along k
     // per-thread coordinates of the 8 A and 8 B elements of a chunk
@@ -338,15 +359,35 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
     const int bk = BK_ ? (tid & 31) : (tid >> 6);
     // q-th element: AK: (ai + 8q, ak); else (ai, ak + 4q).  Same for B.
     double ra[8], rb[8];
+    // chunks that need per-element checks: the whole task if CHK, else none;
+    // with CHK, a chunk away from the triangles' diagonals and inside all
+    // bounds still takes the plain path
+    const bool tile_in = row0 + TILE <= T.M && col0 + TILE <= T.N;
+    auto chunk_plain = [&](int kb) -> bool {
+        if (!CHK) return true;
+        if (!tile_in || kb + BK > T.K) return false;
+        if (T.triA == 1 && !(kb + BK - 1 <= row0)) return false;
+        if (T.triA == 2 && !(kb >= row0 + TILE - 1)) return false;
+        if (T.pad == 1 && !(kb >= col0 + TILE - 1)) return false;
+        return OPA != 2;
+    };
     auto fetch = [&](int kb) {
+        if (chunk_plain(kb)) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
-            if (CHK) ra[q] = ldA_chk<OPA>(T, A, gi, gk);
-            else ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
-            const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
-            if (CHK) rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
-            else rb[q] = (OPB == 0) ? B[gk2 + (i64)gj * T.ldb] : B[gj + (i64)gk2 * T.ldb];
+            for (int q = 0; q < 8; ++q) {
+                const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+                ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+                const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+                rb[q] = (OPB == 0) ? B[gk2 + (i64)gj * T.ldb] : B[gj + (i64)gk2 * T.ldb];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+                ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+                const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+                rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+            }
         }
     };
     auto stash = [&]() {
@@ -361,11 +402,26 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
-    fetch(0);
-    for (int kb = 0; kb < K; kb += BK) {
+    // C is read early (beta != 0) so its latency hides behind the K loop;
+    // split-K slices write unscaled-by-beta partials instead
+    const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
+    dbl4 cin[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
+                const int j = col0 + wc * 32 + b * 16 + (lane & 15);
+                const bool in = !CHK || (i < T.M && j < T.N);
+                cin[a][b][v] = (beta != 0.0 && in && (T.tri == 0 || i >= j)) ? C[i + (i64)j * T.ldc] : 0.0;
+            }
+    if (kbase < kend) fetch(kbase);
+    for (int kb = kbase; kb < kend; kb += BK) {
         stash();
         __syncthreads();
-        if (kb + BK < K) fetch(kb + BK);
+        if (kb + BK < kend) fetch(kb + BK);
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
@@ -380,7 +436,6 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
         }
         __syncthreads();
     }
-    const double alpha = T.alpha, beta = T.beta;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -390,20 +445,16 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
                 const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
                 const int j = col0 + wc * 32 + b * 16 + (lane & 15);
                 if (CHK && (i >= T.M || j >= T.N)) continue;
+                if (Pp) { Pp[i + (i64)j * T.M] = alpha * acc[a][b][v]; continue; }
                 double* c = C + i + (i64)j * T.ldc;
-                if (T.tri == 0 || i >= j) {
-                    double r = alpha * acc[a][b][v];
-                    if (beta != 0.0) r += beta * (*c);
-                    *c = r;
-                } else if (T.tri == 2) {
-                    *c = 0.0;
-                }
+                if (T.tri == 0 || i >= j) *c = alpha * acc[a][b][v] + beta * cin[a][b][v];
+                else if (T.tri == 2) *c = 0.0;
             }
 }
 
 __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
-    double* b0, double* b1, double* b2, double* b3) {
+    double* b0, double* b1, double* b2, double* b3, double* b4) {
     __shared__ double As[BK][TILE + 2];
     __shared__ double Bs[BK][TILE + 2];
     const GemmTile tl = tiles[blockIdx.x];
@@ -412,6 +463,21 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     const double* B = bufptr(T.bufB, b0, b1, b2, b3) + T.offB;
     double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
     const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
+    int kbase = 0, kend = T.K;
+    double* Pp = nullptr;
+    if (T.ksplit > 1) {
+        kbase = tl.ks * T.kchunk;
+        kend = min(T.K, kbase + T.kchunk);
+        Pp = b4 + T.offP + (i64)tl.ks * T.M * T.N;
+        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // reduce handles it
+    }
+
+    // triangular operands: chunks where opA or opB is identically zero are
+    // skipped (A lower: k <= i < row0 + TILE; A upper: k >= i >= row0;
+    // B lower: k >= j >= col0)
+    if (T.triA == 1) kend = min(kend, row0 + TILE);
+    if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
+    if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
 
     if (T.tri == 2 && row0 + TILE - 1 < col0) {
         for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
@@ -424,7 +490,7 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     // symmetric operand qualifies when the whole tile sits on one side of the
     // diagonal of A (then it is a plain N or T read)
     int opa = T.opA;
-    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && (T.K % BK) == 0 &&
+    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && ((kend - kbase) % BK) == 0 &&
                       T.triA == 0 && T.pad == 0;
     bool chk = !full;
     if (opa == 2 && full) {
@@ -435,20 +501,20 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     const int key = opa * 2 + T.opB;
     if (!chk) {
         switch (key) {
-        case 0: gemm_tile<0, 0, false>(T, A, B, C, row0, col0, As, Bs); break;
-        case 1: gemm_tile<0, 1, false>(T, A, B, C, row0, col0, As, Bs); break;
-        case 2: gemm_tile<1, 0, false>(T, A, B, C, row0, col0, As, Bs); break;
-        case 3: gemm_tile<1, 1, false>(T, A, B, C, row0, col0, As, Bs); break;
-        default: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 0: gemm_tile<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 2: gemm_tile<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     } else {
         switch (key) {
-        case 0: gemm_tile<0, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
-        case 1: gemm_tile<0, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
-        case 2: gemm_tile<1, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
-        case 3: gemm_tile<1, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
-        case 4: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, As, Bs); break;
-        default: gemm_tile<2, 1, true>(T, A, B, C, row0, col0, As, Bs); break;
+        case 0: gemm_tile<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 2: gemm_tile<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 4: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     }
 }
@@ -487,4 +553,28 @@ __global__ __launch_bounds__(TPB) void k_copy_top(
     const int c1 = min(c0 + 16, nn);
     for (int j = c0; j < c1; ++j)
         for (int i = threadIdx.x; i < nn; i += TPB) S[i + (i64)j * nj] = blk[i + (i64)j * nj];
+}
+
+// C = beta*C + sum_ks partial_ks over one 64 x 64 tile of a split-K task, in
+// slice order (deterministic), honouring the task's triangular output mode.
+__global__ __launch_bounds__(TPB) void k_splitk_reduce(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
+    double* b0, double* b1, double* b2, double* b3, const double* __restrict__ b4) {
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
+    const double* Pp = b4 + T.offP;
+    const i64 mn = (i64)T.M * T.N;
+    for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
+        const int i = tl.tm * TILE + (e & 63), j = tl.tn * TILE + (e >> 6);
+        if (i >= T.M || j >= T.N) continue;
+        double* c = C + i + (i64)j * T.ldc;
+        if (T.tri != 0 && i < j) {
+            if (T.tri == 2) *c = 0.0;
+            continue;
+        }
+        double s = 0.0;
+        for (int ks = 0; ks < T.ksplit; ++ks) s += Pp[ks * mn + i + (i64)j * T.M];
+        *c = (T.beta != 0.0 ? T.beta * (*c) : 0.0) + s;
+    }
 }

@@ -3,7 +3,8 @@
 #pragma once
 #include "common.h"
 
-#define NB 64          // panel width of the large-front blocked algorithms
+#define NB 64          // inner panel width / diagonal block order of the large-front algorithms
+#define NB2 256        // outer panel width (K of the big trailing updates)
 #define TILE 64        // GEMM output tile (TILE x TILE)
 
 __global__ void k_status_init(unsigned long long* status);
@@ -29,7 +30,9 @@ __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const doub
                              double* dinv, const i64* dinvptr, double* X, const i64* xptr,
                              unsigned long long* status);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, double* b0,
-                             double* b1, double* b2, double* b3);
+                             double* b1, double* b2, double* b3, double* b4);
+__global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, double* b0,
+                                double* b1, double* b2, double* b3, const double* b4);
 __global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkval,
                            const double* scratch, const i64* sptr);
 

@@ -28,7 +28,7 @@ namespace {
 enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
-    K_COPY_TOP
+    K_COPY_TOP, K_SPLITK_REDUCE
 };
 
 struct Launch {
@@ -41,7 +41,7 @@ struct Launch {
     double flops = 0.0;  // algorithmic flops of a GEMM launch
 };
 
-const int NKIDS = 13;
+const int NKIDS = 14;
 
 struct ListBuf {
     std::vector<char> bytes;
@@ -93,7 +93,9 @@ struct chp_plan {
     ListBuf lists;
     int n_large_chol = 0, n_large_projinv = 0;
     i64 n_gemm_tiles = 0;
-    double pending_flops = 0.0;
+    i64 splitp_cur = 0, splitp_len = 0;  // split-K partials: per-launch bump, max
+    double* d_splitp = nullptr;
+    std::vector<GemmTile> split_out;     // output tiles of split tasks, current launch
     // profiling: per-kernel-kind time from HIP events around every launch
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -138,6 +140,7 @@ GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, i
     t.opA = (unsigned char)opA; t.opB = (unsigned char)opB;
     t.tri = (unsigned char)tri; t.triA = (unsigned char)triA; t.pad = (unsigned char)triB;
     t.alpha = alpha; t.beta = beta;
+    t.ksplit = 1; t.kchunk = K; t.offP = 0;
     return t;
 }
 
@@ -164,17 +167,37 @@ double task_flops(const GemmTask& t) {
     return 2.0 * total;
 }
 
-void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, const GemmTask& t,
-                    bool skip_upper) {
+// Adds the tiles of one task.  Tasks with few output tiles and a long K are
+// split along K (deterministic split-K) so the launch fills the GPU.
+void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool skip_upper) {
     if (t.M <= 0 || t.N <= 0) return;
-    P->pending_flops += task_flops(t);
     int id = (int)P->tasks.size();
-    P->tasks.push_back(t);
     int mt = (t.M + TILE - 1) / TILE, nt = (t.N + TILE - 1) / TILE;
+    int ntiles = 0;
+    for (int tn = 0; tn < nt; ++tn)
+        for (int tm = 0; tm < mt; ++tm)
+            if (!(skip_upper && tm < tn)) ++ntiles;
+    int ks = 1;
+    if (ntiles < 128 && t.K >= 512) {
+        ks = std::max(1, std::min(t.K / 256, (512 + ntiles - 1) / ntiles));
+        ks = std::min(ks, 16);
+    }
+    if (ks > 1) {
+        int chunk = (t.K + ks - 1) / ks;
+        chunk = (chunk + 31) / 32 * 32;
+        ks = (t.K + chunk - 1) / chunk;
+        t.ksplit = ks;
+        t.kchunk = chunk;
+        t.offP = P->splitp_cur;
+        P->splitp_cur += (i64)ks * t.M * t.N;
+        P->splitp_len = std::max(P->splitp_len, P->splitp_cur);
+    }
+    P->tasks.push_back(t);
     for (int tn = 0; tn < nt; ++tn)
         for (int tm = 0; tm < mt; ++tm) {
             if (skip_upper && tm < tn) continue;
-            tiles.push_back({id, tm, tn, 0});
+            for (int k = 0; k < ks; ++k) tiles.push_back({id, tm, tn, k});
+            if (ks > 1) P->split_out.push_back({id, tm, tn, 0});
         }
 }
 
@@ -182,9 +205,24 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
     Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->lists.add(tiles), 0};
-    L.flops = P->pending_flops;
-    P->pending_flops = 0.0;
+    double fl = 0.0;
+    int last = -1;
+    for (const GemmTile& g : tiles)
+        if (g.task != last) { fl += task_flops(P->tasks[g.task]); last = g.task; }
+    L.flops = fl;
     seq.push_back(L);
+    // the split tasks whose tiles are in this list get their reduction now
+    std::vector<GemmTile> red;
+    std::vector<GemmTile> keep;
+    for (const GemmTile& g : P->split_out) {
+        bool mine = false;
+        for (const GemmTile& h : tiles) if (h.task == g.task) { mine = true; break; }
+        (mine ? red : keep).push_back(g);
+    }
+    P->split_out.swap(keep);
+    if (!red.empty())
+        seq.push_back({K_SPLITK_REDUCE, (int)red.size(), 256, 0, P->lists.add(red), 0});
+    if (P->split_out.empty()) P->splitp_cur = 0;
 }
 
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
@@ -206,61 +244,106 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
     }
     P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 1});
+    // Two-level blocking: 64-wide inner panels (diagonal factor, panel solve,
+    // update confined to the current outer panel), then one trailing update
+    // with K = the outer panel width for everything right of it.
     int maxnn = 0;
     for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
-    for (int c0 = 0; c0 < maxnn; c0 += NB) {
-        std::vector<int> act;
-        for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
-        P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->lists.add(act), c0});
-        std::vector<GemmTile> pt, gt;
-        for (int k : act) {
-            const FrontInfo& f = P->fi[k];
-            int nn = f.nn, na = f.na, nj = nn + na;
-            int c1 = std::min(c0 + NB, nn);
-            // panel solve, in place: L[c1:, c0:c1] = F[c1:, c0:c1] Dinv^T
-            // (Dinv stored row-major, so Dinv^T(m, j) = dinv[m + j*NB])
-            add_gemm_tiles(P, pt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
-                                     3, P->h_dinvptr[k] + (i64)(c0 / NB) * NB * NB, NB, 0,
-                                     0, f.blkptr + c1 + (i64)c0 * nj, nj,
-                                     nj - c1, c1 - c0, c1 - c0, 1.0, 0.0, 0), false);
-            // trailing update F[c1:, c1:] -= L[c1:, c0:c1] L[c1:, c0:c1]^T
-            if (c1 < nn)
-                add_gemm_tiles(P, gt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
-                                         0, f.blkptr + c1 + (i64)c0 * nj, nj, 1,
-                                         0, f.blkptr + c1 + (i64)c1 * nj, nj,
-                                         nj - c1, nn - c1, c1 - c0, -1.0, 1.0, 1), true);
-            if (na > 0)
-                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)c0 * nj, nj, 0,
-                                         0, f.blkptr + nn + (i64)c0 * nj, nj, 1,
-                                         1, f.updptr, na, na, na, c1 - c0, -1.0, 1.0, 1), true);
+    for (int p0 = 0; p0 < maxnn; p0 += NB2) {
+        for (int c0 = p0; c0 < std::min(p0 + NB2, maxnn); c0 += NB) {
+            std::vector<int> act;
+            for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
+            P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->lists.add(act), c0});
+            std::vector<GemmTile> pt, it;
+            for (int k : act) {
+                const FrontInfo& f = P->fi[k];
+                int nn = f.nn, nj = nn + f.na;
+                int c1 = std::min(c0 + NB, nn), p1 = std::min(p0 + NB2, nn);
+                // panel solve, in place: L[c1:, c0:c1] = F[c1:, c0:c1] Dinv^T
+                // (Dinv row-major, so Dinv^T(m, j) = dinv[m + j*NB])
+                add_gemm_tiles(P, pt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
+                                         3, P->h_dinvptr[k] + (i64)(c0 / NB) * NB * NB, NB, 0,
+                                         0, f.blkptr + c1 + (i64)c0 * nj, nj,
+                                         nj - c1, c1 - c0, c1 - c0, 1.0, 0.0, 0), false);
+                // update inside the outer panel: F[c1:, c1:p1] -= L[c1:, c0:c1] L[c1:p1, c0:c1]^T
+                if (c1 < p1)
+                    add_gemm_tiles(P, it, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
+                                             0, f.blkptr + c1 + (i64)c0 * nj, nj, 1,
+                                             0, f.blkptr + c1 + (i64)c1 * nj, nj,
+                                             nj - c1, p1 - c1, c1 - c0, -1.0, 1.0, 1), true);
+            }
+            push_gemm(P, P->chol, pt);
+            push_gemm(P, P->chol, it);
         }
-        push_gemm(P, P->chol, pt);
-        push_gemm(P, P->chol, gt);
-    }
-    // stored convention B21 = L21 L11^{-1}: column blocks from the right
-    int maxblk = 0;
-    for (int k : large)
-        if (P->fi[k].na > 0) maxblk = std::max(maxblk, (P->fi[k].nn + NB - 1) / NB);
-    for (int s = 0; s < maxblk; ++s) {
-        std::vector<GemmTile> pt, gt;
+        // trailing update right of the outer panel, K = p1 - p0
+        std::vector<GemmTile> gt;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
             int nn = f.nn, na = f.na, nj = nn + na;
-            int nbk = (nn + NB - 1) / NB;
-            if (na == 0 || s >= nbk) continue;
-            int b = nbk - 1 - s, b0 = b * NB, b1 = std::min(b0 + NB, nn);
-            // X_b = R_b Dinv_b in place (Dinv(m, j) = dinv[j + m*NB])
-            add_gemm_tiles(P, pt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
-                                     3, P->h_dinvptr[k] + (i64)b * NB * NB, NB, 1,
-                                     0, f.blkptr + nn + (i64)b0 * nj, nj,
-                                     na, b1 - b0, b1 - b0, 1.0, 0.0, 0), false);
-            if (b0 > 0)
-                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
-                                         0, f.blkptr + b0, nj, 0,
-                                         0, f.blkptr + nn, nj, na, b0, b1 - b0, -1.0, 1.0, 0),
-                               false);
+            if (nn <= p0) continue;
+            int p1 = std::min(p0 + NB2, nn);
+            if (p1 < nn)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + p1 + (i64)p0 * nj, nj, 0,
+                                         0, f.blkptr + p1 + (i64)p0 * nj, nj, 1,
+                                         0, f.blkptr + p1 + (i64)p1 * nj, nj,
+                                         nj - p1, nn - p1, p1 - p0, -1.0, 1.0, 1), true);
+            if (na > 0)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)p0 * nj, nj, 0,
+                                         0, f.blkptr + nn + (i64)p0 * nj, nj, 1,
+                                         1, f.updptr, na, na, na, p1 - p0, -1.0, 1.0, 1), true);
         }
-        push_gemm(P, P->chol, pt);
+        push_gemm(P, P->chol, gt);
+    }
+    // stored convention B21 = L21 L11^{-1}: groups of NB2/NB column blocks
+    // from the right; inside a group block by block (X_b = R_b Dinv_b, update
+    // of the group's remaining columns), then one update of everything left
+    // of the group with K = the group width.
+    const int GB = NB2 / NB;
+    int maxgrp = 0;
+    for (int k : large)
+        if (P->fi[k].na > 0) maxgrp = std::max(maxgrp, ((P->fi[k].nn + NB - 1) / NB + GB - 1) / GB);
+    for (int sg = 0; sg < maxgrp; ++sg) {
+        for (int sb = 0; sb < GB; ++sb) {
+            std::vector<GemmTile> pt, it;
+            for (int k : large) {
+                const FrontInfo& f = P->fi[k];
+                int nn = f.nn, na = f.na, nj = nn + na;
+                int nbk = (nn + NB - 1) / NB, ng = (nbk + GB - 1) / GB;
+                if (na == 0 || sg >= ng) continue;
+                int g = ng - 1 - sg;
+                int gb0 = g * GB, gb1 = std::min(gb0 + GB, nbk);
+                int b = gb1 - 1 - sb;
+                if (b < gb0) continue;
+                int b0 = b * NB, b1 = std::min(b0 + NB, nn), q0 = gb0 * NB;
+                // X_b = R_b Dinv_b in place (Dinv(m, j) = dinv[j + m*NB])
+                add_gemm_tiles(P, pt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
+                                         3, P->h_dinvptr[k] + (i64)b * NB * NB, NB, 1,
+                                         0, f.blkptr + nn + (i64)b0 * nj, nj,
+                                         na, b1 - b0, b1 - b0, 1.0, 0.0, 0), false);
+                // R[:, q0:b0] -= X_b L[b0:b1, q0:b0]
+                if (b0 > q0)
+                    add_gemm_tiles(P, it, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
+                                             0, f.blkptr + b0 + (i64)q0 * nj, nj, 0,
+                                             0, f.blkptr + nn + (i64)q0 * nj, nj,
+                                             na, b0 - q0, b1 - b0, -1.0, 1.0, 0), false);
+            }
+            push_gemm(P, P->chol, pt);
+            push_gemm(P, P->chol, it);
+        }
+        std::vector<GemmTile> gt;
+        for (int k : large) {
+            const FrontInfo& f = P->fi[k];
+            int nn = f.nn, na = f.na, nj = nn + na;
+            int nbk = (nn + NB - 1) / NB, ng = (nbk + GB - 1) / GB;
+            if (na == 0 || sg >= ng) continue;
+            int g = ng - 1 - sg;
+            int q0 = g * GB * NB, q1 = std::min((g * GB + GB) * NB, nn);
+            // R[:, 0:q0] -= X[:, q0:q1] L[q0:q1, 0:q0]
+            if (q0 > 0)
+                add_gemm_tiles(P, gt, mk(0, f.blkptr + nn + (i64)q0 * nj, nj, 0,
+                                         0, f.blkptr + q0, nj, 0,
+                                         0, f.blkptr + nn, nj, na, q0, q1 - q0, -1.0, 1.0, 0), false);
+        }
         push_gemm(P, P->chol, gt);
     }
 }
@@ -459,7 +542,11 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
             break;
         case K_GEMM:
             hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
-                               blkval, P->d_upd, P->d_scratch, P->d_dinv);
+                               blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
+            break;
+        case K_SPLITK_REDUCE:
+            hipLaunchKernelGGL(k_splitk_reduce, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
+                               blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
             break;
         case K_EXTRACT:
             hipLaunchKernelGGL(k_extract, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
@@ -622,6 +709,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= alloc_doubles(&P->d_dinv, P->dinv_len);
     rc |= alloc_doubles(&P->d_scratch, P->scratch_len);
     rc |= alloc_doubles(&P->d_partial, n);
+    rc |= alloc_doubles(&P->d_splitp, P->splitp_len);
     if (hipMalloc((void**)&P->d_status, sizeof(unsigned long long)) != hipSuccess) rc = -1;
     if (rc) {
         std::string e = g_err;
@@ -638,7 +726,7 @@ void chp_plan_destroy(chp_plan* P) {
     if (P->ev1) (void)hipEventDestroy(P->ev1);
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
-                    P->d_dinv, P->d_scratch, P->d_partial, P->d_work, P->d_status,
+                    P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->d_lists, P->d_tasks};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -13,7 +13,7 @@ import numpy as np
 import scipy.sparse as sp
 
 __all__ = ["grid_edges", "spd_from_edges", "grid2d", "grid3d", "rgg", "band_arrow",
-           "batch_grids"]
+           "batch_grids", "two_cliques", "reference_test_matrix"]
 
 
 def grid_edges(shape):
@@ -115,6 +115,27 @@ def band_arrow(n=20000, bandwidth=8, arrows=50, seed=4):
         I.append(np.full(r, r, dtype=np.int64))
         J.append(j)
     return spd_from_edges(n, np.concatenate(I), np.concatenate(J), rng)
+
+
+def two_cliques(n1=300, n2=100, n3=200, seed=6):
+    """Nodes 0..n1-1 form a clique fully joined to the clique on
+    n1..n1+n2+n3-1 through its first n2 nodes.  In natural order this gives a
+    supernode of n1 columns with an n2-row separator and a root supernode of
+    n2 + n3 columns: dense fronts larger than one 256-wide panel."""
+    rng = np.random.default_rng(seed)
+    def clique(lo, hi):
+        idx = np.arange(lo, hi)
+        I, J = np.meshgrid(idx, idx, indexing="ij")
+        m = I > J
+        return I[m], J[m]
+    I1, J1 = clique(0, n1)
+    I3, J3 = clique(n1, n1 + n2 + n3)
+    a = np.arange(n1)
+    b = np.arange(n1, n1 + n2)
+    Ib, Jb = np.meshgrid(b, a, indexing="ij")
+    I = np.concatenate([I1, Ib.ravel(), I3])
+    J = np.concatenate([J1, Jb.ravel(), J3])
+    return spd_from_edges(n1 + n2 + n3, I, J, rng)
 
 
 def reference_test_matrix(n=31, density=0.15, seed=1):

@@ -74,10 +74,10 @@ int chp_plan_info(chp_plan* plan, int64_t* out);
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
  * (synchronizing after each launch) and clears the counters;
- * chp_profile_read fills ms/cnt/flops[12] per kernel kind (0 chol_small,
- * 1 extadd, 2 potrf_diag, 3 panel_mul, 4 gemm, 5 extract, 6 projinv_small,
- * 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd, 11 solve_bwd) and
- * returns 12.  chp_plan_flops: algorithmic GEMM flops of one cholesky
+ * chp_profile_read fills ms/cnt/flops[14] per kernel kind (0 chol_small,
+ * 1 extadd, 2 potrf_diag, 3 unused, 4 gemm, 5 extract, 6 projinv_small,
+ * 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd, 11 solve_bwd,
+ * 12 copy_top, 13 splitk_reduce) and returns 14.  chp_plan_flops: algorithmic GEMM flops of one cholesky
  * (out[0]) and one projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
 int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);

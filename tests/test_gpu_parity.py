@@ -21,13 +21,15 @@ PROBLEMS = {
     "ref31": lambda: problems.reference_test_matrix(),
     "grid2d_64": lambda: problems.grid2d(64, seed=0),
     "grid3d_12": lambda: problems.grid3d(12, seed=1),
+    "two_cliques": lambda: problems.two_cliques(),
 }
+ORDER = {"two_cliques": None}
 
 
 def _setup(name, small_max):
     import torch
     A = PROBLEMS[name]()
-    symb = cp.symbolic(A, p=cp.amd.order)
+    symb = cp.symbolic(A, p=ORDER.get(name, cp.amd.order))
     symb._plans["cuda:0"] = numeric.Plan(symb, "cuda:0", small_max=small_max)
     X = cp.cspmatrix(symb, device="cuda:0") + A
     torch.cuda.synchronize()
