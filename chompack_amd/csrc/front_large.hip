@@ -54,12 +54,23 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
         for (int j = ja; j < jb; ++j) {
             const int tc = r[j];
             const double* col = Uc + (i64)j * N;
-            if (tc < nn) {
-                double* dst = blk + (i64)tc * nj;
-                for (int i = j + tid; i < N; i += TPB) dst[r[i]] += col[i];
-            } else {
-                double* dst = U + (i64)(tc - nn) * na - nn;
-                for (int i = j + tid; i < N; i += TPB) dst[r[i]] += col[i];
+            double* dst = (tc < nn) ? blk + (i64)tc * nj : U + (i64)(tc - nn) * na - nn;
+            // 4 rows per thread in flight: the loads of r, U and F are independent
+            for (int i0 = j + tid; i0 < N; i0 += 4 * TPB) {
+                int ri[4];
+                double u[4], fv[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int i = i0 + t * TPB;
+                    ri[t] = (i < N) ? r[i] : 0;
+                    u[t] = (i < N) ? col[i] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    fv[t] = (i0 + t * TPB < N) ? dst[ri[t]] : 0.0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (i0 + t * TPB < N) dst[ri[t]] = fv[t] + u[t];
             }
         }
         __syncthreads();

@@ -126,15 +126,26 @@ __global__ __launch_bounds__(TPB) void k_extract(
     for (int i0 = j0; i0 < na; i0 += 256) {
         const int i1 = min(i0 + 256, na);
         __syncthreads();
-        for (int j = j0; j < j1; ++j) {
-            const int rj = r[j];
-            const int i = i0 + tid;
-            if (i < i1 && i >= j) {
-                const int ri = r[i];
-                const double v = (rj < nnp) ? pblk[ri + (i64)rj * njp]
-                                            : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
-                V[i + (i64)j * na] = v;
-                S[tid][j - j0] = v;
+        const int i = i0 + tid;
+        const int ri = (i < i1) ? r[i] : 0;
+        // all 16 columns' loads issued before any store
+        double v[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = j0 + jj;
+            v[jj] = 0.0;
+            if (j < j1 && i < i1 && i >= j) {
+                const int rj = r[j];
+                v[jj] = (rj < nnp) ? pblk[ri + (i64)rj * njp]
+                                   : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = j0 + jj;
+            if (j < j1 && i < i1 && i >= j) {
+                V[i + (i64)j * na] = v[jj];
+                S[tid][jj] = v[jj];
             }
         }
         __syncthreads();

@@ -96,6 +96,13 @@ struct chp_plan {
     i64 splitp_cur = 0, splitp_len = 0;  // split-K partials: per-launch bump, max
     double* d_splitp = nullptr;
     std::vector<GemmTile> split_out;     // output tiles of split tasks, current launch
+    // HIP graphs of the sweeps, replayed on a plan-owned stream that is
+    // joined to the caller's stream with events; cached per blkval pointer
+    bool use_graphs = true;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev_in = nullptr, gev_out = nullptr;
+    struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
+    Cached g_chol, g_projinv, g_llt;
     // profiling: per-kernel-kind time from HIP events around every launch
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -587,6 +594,43 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
     return 0;
 }
 
+// Runs `seq` (preceded by a status reset) on stream s, through a cached HIP
+// graph when enabled.
+int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, double* blkval,
+              hipStream_t s, bool reset_status) {
+    if (!P->use_graphs || P->prof) {
+        if (reset_status)
+            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+        return run_seq(P, seq, blkval, s);
+    }
+    if (!P->gstream) {
+        HIP_CHECK(hipStreamCreateWithFlags(&P->gstream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&P->gev_in, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
+    }
+    if (G.key != (const void*)blkval || !G.exec) {
+        if (G.exec) { HIP_CHECK(hipGraphExecDestroy(G.exec)); G.exec = nullptr; }
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
+        if (reset_status)
+            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
+        int rc = run_seq(P, seq, blkval, P->gstream);
+        hipError_t e = hipStreamEndCapture(P->gstream, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
+        e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) { chp_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return -2; }
+        G.key = blkval;
+    }
+    HIP_CHECK(hipEventRecord(P->gev_in, s));
+    HIP_CHECK(hipStreamWaitEvent(P->gstream, P->gev_in, 0));
+    HIP_CHECK(hipGraphLaunch(G.exec, P->gstream));
+    HIP_CHECK(hipEventRecord(P->gev_out, P->gstream));
+    HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
+    return 0;
+}
+
 int read_status(chp_plan* P, hipStream_t s, i64* info) {
     unsigned long long h = CHP_STATUS_OK;
     HIP_CHECK(hipMemcpyAsync(&h, P->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -724,6 +768,11 @@ void chp_plan_destroy(chp_plan* P) {
     if (!P) return;
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
+    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt})
+        if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (P->gev_in) (void)hipEventDestroy(P->gev_in);
+    if (P->gev_out) (void)hipEventDestroy(P->gev_out);
+    if (P->gstream) (void)hipStreamDestroy(P->gstream);
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
@@ -775,6 +824,13 @@ int chp_profile_read(chp_plan* P, double* ms, i64* cnt, double* flops) {
     return NKIDS;
 }
 
+// Enable (on != 0, the default) or disable HIP-graph replay of the sweeps.
+int chp_set_graphs(chp_plan* P, int on) {
+    if (!P) return -1;
+    P->use_graphs = on != 0;
+    return 0;
+}
+
 // Algorithmic GEMM flops of one sweep: out[0] cholesky, out[1] projected inverse.
 int chp_plan_flops(chp_plan* P, double* out) {
     if (!P) return -1;
@@ -787,8 +843,7 @@ int chp_plan_flops(chp_plan* P, double* out) {
 int chp_cholesky(chp_plan* P, double* blkval, void* stream, i64* info) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-    int rc = run_seq(P, P->chol, blkval, s);
+    int rc = run_sweep(P, P->chol, P->g_chol, blkval, s, true);
     if (rc) return rc;
     return read_status(P, s, info);
 }
@@ -796,15 +851,14 @@ int chp_cholesky(chp_plan* P, double* blkval, void* stream, i64* info) {
 int chp_projected_inverse(chp_plan* P, double* blkval, void* stream, i64* info) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-    int rc = run_seq(P, P->projinv, blkval, s);
+    int rc = run_sweep(P, P->projinv, P->g_projinv, blkval, s, true);
     if (rc) return rc;
     return read_status(P, s, info);
 }
 
 int chp_llt(chp_plan* P, double* blkval, void* stream) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
-    int rc = run_seq(P, P->llt, blkval, (hipStream_t)stream);
+    int rc = run_sweep(P, P->llt, P->g_llt, blkval, (hipStream_t)stream, false);
     if (rc) return rc;
     HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
     return 0;

@@ -112,3 +112,31 @@ def test_cholesky_failure_raises():
     with pytest.raises(ArithmeticError, match="factorization failed"):
         cp.cholesky(X)
     assert X.is_factor is True
+
+
+def test_graph_replays_match_eager():
+    """The sweeps replay a captured HIP graph; every replay (with eager work
+    and synchronizations in between) must equal the eager launch sequence."""
+    import torch
+    from chompack_amd import _native
+    A = problems.grid3d(10, seed=4)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    symb._plans["cuda:0"] = numeric.Plan(symb, "cuda:0", small_max=32)
+    plan = symb._plans["cuda:0"]
+    X0 = cp.cspmatrix(symb, device="cuda:0") + A
+    lib = _native.lib()
+    lib.chp_set_graphs(plan.handle, 0)
+    E = X0.copy()
+    cp.cholesky(E)
+    cp.projected_inverse(E)
+    eager = E.blkval.cpu().numpy()
+    lib.chp_set_graphs(plan.handle, 1)
+    X = cp.cspmatrix(symb, blkval=torch.empty_like(X0.blkval))
+    for rep in range(4):
+        X.blkval.copy_(X0.blkval)
+        X.is_factor = False
+        cp.cholesky(X)
+        cp.projected_inverse(X)
+        torch.cuda.synchronize()
+        assert np.array_equal(X.blkval.cpu().numpy(), eager), rep
+        _ = torch.ones(1000, device="cuda:0").sum().item()  # eager work between replays
