@@ -13,6 +13,7 @@
 // there is no host work per front, only the final status read.
 #include "common.h"
 #include "kernels.h"
+#include <hip/hip_ext.h>
 #include "../../include/chompack_amd.h"
 
 #include <algorithm>
@@ -525,54 +526,63 @@ int alloc_doubles(double** d, i64 len) {
     return 0;
 }
 
+// Profiled launches take their start/stop timestamps from the kernel's own
+// dispatch (hipExtLaunchKernelGGL events), not from separate event packets.
+#define CHP_LAUNCH(K, LDS, ...)                                                        \
+    do {                                                                               \
+        if (P->prof)                                                                   \
+            hipExtLaunchKernelGGL(K, g, b, LDS, s, P->ev0, P->ev1, 0, __VA_ARGS__);    \
+        else                                                                           \
+            hipLaunchKernelGGL(K, g, b, LDS, s, __VA_ARGS__);                          \
+    } while (0)
+
 int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStream_t s) {
     for (const Launch& L : seq) {
         const void* list = P->d_lists + L.list;
         dim3 g(L.grid), b(L.block);
-        if (P->prof) HIP_CHECK(hipEventRecord(P->ev0, s));
         switch (L.kid) {
         case K_CHOL_SMALL:
-            hipLaunchKernelGGL(k_chol_small, g, b, L.lds, s, (const int*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, P->d_fi, blkval,
                                P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
             break;
         case K_EXTADD:
-            hipLaunchKernelGGL(k_extadd_tiles, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extadd_tiles, 0, (const int2*)list, P->d_fi, blkval,
                                P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
             break;
         case K_COPY_TOP:
-            hipLaunchKernelGGL(k_copy_top, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_copy_top, 0, (const int2*)list, P->d_fi, blkval,
                                P->d_scratch, P->d_lltptr);
             break;
         case K_POTRF_DIAG:
-            hipLaunchKernelGGL(k_potrf_diag, g, b, 0, s, (const int*)list, L.iarg, P->d_fi,
+            CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, P->d_fi,
                                blkval, P->d_dinv, P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
+            CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, P->d_tasks,
                                blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
             break;
         case K_SPLITK_REDUCE:
-            hipLaunchKernelGGL(k_splitk_reduce, g, b, 0, s, (const GemmTile*)list, P->d_tasks,
+            CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, P->d_tasks,
                                blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
             break;
         case K_EXTRACT:
-            hipLaunchKernelGGL(k_extract, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extract, 0, (const int2*)list, P->d_fi, blkval,
                                P->d_upd, P->d_relidx);
             break;
         case K_PROJINV_SMALL:
-            hipLaunchKernelGGL(k_projinv_small, g, b, L.lds, s, (const int*)list, P->d_fi,
+            CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
                                blkval, P->d_upd, P->d_status);
             break;
         case K_TRTRI_DIAG:
-            hipLaunchKernelGGL(k_trtri_diag, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, P->d_fi, blkval,
                                P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
             break;
         case K_NEG_COPY:
-            hipLaunchKernelGGL(k_neg_copy, g, b, 0, s, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_neg_copy, 0, (const int2*)list, P->d_fi, blkval,
                                P->d_scratch, P->d_tptr);
             break;
         case K_LLT_SMALL:
-            hipLaunchKernelGGL(k_llt_small, g, b, L.lds, s, (const int*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, P->d_fi, blkval,
                                P->d_upd, P->d_relidx, P->d_chidx);
             break;
         default:
@@ -580,7 +590,6 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
             return -3;
         }
         if (P->prof) {
-            HIP_CHECK(hipEventRecord(P->ev1, s));
             HIP_CHECK(hipEventSynchronize(P->ev1));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
@@ -593,6 +602,8 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
     HIP_CHECK(hipGetLastError());
     return 0;
 }
+
+#undef CHP_LAUNCH
 
 // Runs `seq` (preceded by a status reset) on stream s, through a cached HIP
 // graph when enabled.
