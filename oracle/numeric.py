@@ -20,7 +20,8 @@ blkptr[k+1]], a column-major nj x nn array; ``_blk`` returns a writable
 import numpy as np
 import scipy.linalg as sla
 
-__all__ = ["cholesky", "llt", "projected_inverse", "trsm", "dot", "logdet"]
+__all__ = ["cholesky", "llt", "projected_inverse", "completion", "update_factor", "hessian",
+           "trsm", "dot", "logdet"]
 
 
 class _S(object):
@@ -218,3 +219,235 @@ def logdet(symb, lblkval):
     for k in range(symb.Nsn):
         s += float(np.sum(np.log(np.diag(_blk(lblkval, S, k)[:S.dims(k)[0]]))))
     return 2.0 * s
+
+
+def _larfg(alpha, x):
+    """Householder reflector H = I - tau [1; v][1; v]^T with
+    H [alpha; x] = [beta; 0], beta = -sign(alpha) ||[alpha; x]|| (LAPACK dlarfg).
+    Returns (tau, beta, v)."""
+    xnorm = float(np.linalg.norm(x))
+    if xnorm == 0.0:
+        return 0.0, alpha, x.copy()
+    beta = -np.copysign(np.hypot(alpha, xnorm), alpha)
+    tau = (beta - alpha) / beta
+    return tau, beta, x * (1.0 / (alpha - beta))
+
+
+def _rev_chol_inverse_factor(S):
+    """Lower-triangular Lr with S = Lr^T Lr: Cholesky of the reversed matrix
+    J S J = U^T U, then Lr = J U J (completion.c:93-104, update_factor.c:67-103).
+    Raises LinAlgError if S is not positive definite."""
+    U = np.linalg.cholesky(S[::-1, ::-1]).T
+    return np.ascontiguousarray(U[::-1, ::-1])
+
+
+def update_factor(r, nn, na, F):
+    """Factor of the child update V = F[r, r] = Li^T Li, Li lower triangular,
+    when the (2,2) block of the frontal matrix F holds the parent's factor Lk
+    (V_k = Lk^T Lk) instead of V_k (update_factor.c:3-108, misc.py:84-176).
+    The separator part C = Lk[:, r2 - nn] is reduced to lower-triangular form
+    by Householder reflections from the last column, Li21 solves
+    Li22^T Li21 = B, and Li11 is the reversed Cholesky factor of
+    A - Li21^T Li21.  Returns Li (N x N, zero above the diagonal)."""
+    r = np.asarray(r)
+    N = len(r)
+    N1 = int(np.sum(r < nn))
+    N2 = N - N1
+    Li = np.zeros((N, N))
+    Li[:, :N1] = F[np.ix_(r, r[:N1])]
+    if N2 > 0:
+        C = np.zeros((na, N2))
+        for j in range(N2):
+            jj = int(r[N1 + j])
+            C[jj - nn:, j] = F[jj:, jj]
+        for j in range(N2):
+            col = N2 - 1 - j
+            jj = int(r[N - 1 - j]) - nn   # first structural row of this column
+            t = (na - jj) - (j + 1)       # entries to annihilate above the pivot
+            if t == 0:
+                continue
+            piv = na - 1 - j
+            tau, beta, v = _larfg(C[piv, col], C[jj:piv, col])
+            C[jj:piv, col] = v
+            C[piv, col] = beta
+            if col > 0 and tau != 0.0:
+                w = np.concatenate([v, [1.0]])
+                blk = C[jj:piv + 1, :col]
+                blk -= tau * np.outer(w, w @ blk)
+        L22 = np.tril(C[na - N2:, :])
+        Li[N1:, N1:] = L22
+        L21 = sla.solve_triangular(L22, Li[N1:, :N1], lower=True, trans="T")
+        Li[N1:, :N1] = L21
+        A = np.tril(Li[:N1, :N1]) - np.tril(L21.T @ L21)
+    else:
+        A = np.tril(Li[:N1, :N1])
+    Li[:N1, :N1] = _rev_chol_inverse_factor(_sym_lower(A))
+    return Li
+
+
+def completion(symb, blkval, factored_updates=True):
+    """Maximum-determinant positive definite completion, root to leaves
+    (completion.c:3-120): on exit blkval holds the factor L (stored
+    convention) with P((L L^T)^{-1}) = X.  Returns 0 or a LAPACK-style info
+    (the caller raises and still marks the matrix a factor, cbase.c:612-617)."""
+    S = _S(symb)
+    stack = []
+    for k in reversed(S.snpost):
+        nn, na, nj = S.dims(k)
+        B = _blk(blkval, S, k)
+        F = np.zeros((nj, nj))
+        F[:, :nn] = np.tril(B, 0) if nn == nj else np.vstack([np.tril(B[:nn]), B[nn:]])
+        if na > 0:
+            F[nn:, nn:] = stack.pop()
+        try:
+            for c in S.children(k):
+                if factored_updates:
+                    stack.append(update_factor(S.rel(c), nn, na, F))
+                else:
+                    stack.append(_extract(F, S.rel(c)))
+            if na > 0:
+                X21 = B[nn:, :].copy()
+                if factored_updates:      # V_k = Lk^T Lk
+                    Lk = np.tril(F[nn:, nn:])
+                    Z = sla.solve_triangular(Lk, X21, lower=True, trans="T")
+                    W = lambda Z: sla.solve_triangular(Lk, Z, lower=True)
+                else:                     # V_k = Lk Lk^T
+                    Lk = np.linalg.cholesky(_sym_lower(F[nn:, nn:]))
+                    Z = sla.solve_triangular(Lk, X21, lower=True)
+                    W = lambda Z: sla.solve_triangular(Lk, Z, lower=True, trans="T")
+                B[:nn, :] = np.tril(B[:nn, :]) - np.tril(Z.T @ Z) + np.triu(B[:nn, :], 1)
+                B[nn:, :] = -W(Z)
+            Lr = _rev_chol_inverse_factor(_sym_lower(B[:nn, :]))
+        except np.linalg.LinAlgError:
+            return 1
+        Linv = sla.solve_triangular(Lr, np.eye(nn), lower=True)
+        iu = np.triu_indices(nn, 1)
+        keep = B[:nn, :][iu]
+        B[:nn, :] = np.tril(Linv)
+        B[:nn, :][iu] = keep
+    return 0
+
+
+def _y2k(S, lblkval, ub, inv):
+    """Leaves-to-root half of the Hessian factor G (hessian.c:3-101)."""
+    alpha = 1.0 if inv else -1.0
+    stack = []
+    for k in S.snpost:
+        nn, na, nj = S.dims(k)
+        L = _blk(lblkval, S, k)
+        U = _blk(ub, S, k)
+        F = np.zeros((nj, nj))
+        F[:, :nn] = np.tril(U, 0) if nn == nj else np.vstack([np.tril(U[:nn]), U[nn:]])
+        def ext_add():
+            for c in reversed(S.children(k)):
+                _extend_add(F, stack.pop(), S.rel(c))
+        if not inv:
+            ext_add()
+        if na > 0:
+            L21 = L[nn:, :]
+            F[nn:, nn:] += alpha * (L21 @ F[nn:, :nn].T)
+            F[nn:, :nn] += alpha * (L21 @ _sym_lower(F[:nn, :nn]))
+            F[nn:, nn:] += alpha * (F[nn:, :nn] @ L21.T)
+        if inv:
+            ext_add()
+        if na > 0:
+            stack.append(np.tril(F[nn:, nn:]))
+        _write_lower(U, F[:, :nn], nn)
+
+
+def _m2t(S, lblkval, ub, inv):
+    """Root-to-leaves half of the Hessian factor (hessian.c:103-198)."""
+    alpha = 1.0 if inv else -1.0
+    stack = []
+    for k in reversed(S.snpost):
+        nn, na, nj = S.dims(k)
+        L = _blk(lblkval, S, k)
+        U = _blk(ub, S, k)
+        F = np.zeros((nj, nj))
+        F[:, :nn] = np.tril(U, 0) if nn == nj else np.vstack([np.tril(U[:nn]), U[nn:]])
+        if na > 0:
+            F[nn:, nn:] = _sym_lower(stack.pop())
+        if inv:
+            for c in S.children(k):
+                stack.append(_extract(F, S.rel(c)))
+        if na > 0:
+            L21 = L[nn:, :]
+            F[:nn, :nn] += alpha * (F[nn:, :nn].T @ L21)
+            F[nn:, :nn] += alpha * (_sym_lower(F[nn:, nn:]) @ L21)
+            F[:nn, :nn] += alpha * (L21.T @ F[nn:, :nn])
+        _write_lower(U, F[:, :nn], nn)
+        if not inv:
+            for c in S.children(k):
+                stack.append(_extract(F, S.rel(c)))
+
+
+def _scale(S, lblkval, yblkval, ubs, inv, adj, factored_updates):
+    """Diagonal scaling part of the Hessian factor, root to leaves
+    (hessian.c:201-352).  Returns 0 or the info of a failed update
+    factorization (which the reference's hessian() discards)."""
+    stack = []
+    tr = lambda A, t: A.T if t == "T" else A
+    for k in reversed(S.snpost):
+        nn, na, nj = S.dims(k)
+        L = _blk(lblkval, S, k)
+        Y = _blk(yblkval, S, k)
+        F = np.zeros((nj, nj))
+        F[:, :nn] = np.tril(Y, 0) if nn == nj else np.vstack([np.tril(Y[:nn]), Y[nn:]])
+        if na > 0:
+            F[nn:, nn:] = stack.pop()
+        try:
+            for c in S.children(k):
+                if factored_updates:
+                    stack.append(update_factor(S.rel(c), nn, na, F))
+                else:
+                    stack.append(_extract(F, S.rel(c)))
+            Lk = None
+            if na > 0:
+                if factored_updates:
+                    Lk = np.tril(F[nn:, nn:])
+                    tr3 = "T" if adj else "N"
+                else:
+                    Lk = np.linalg.cholesky(_sym_lower(F[nn:, nn:]))
+                    tr3 = "N" if adj else "T"
+        except np.linalg.LinAlgError:
+            return 1
+        tr1, tr2 = ("N", "T") if adj else ("T", "N")
+        L11 = np.tril(L[:nn, :])
+        for ub in ubs:
+            U = _blk(ub, S, k)
+            U[:nn, :] = _sym_lower(U[:nn, :])
+            if not inv:
+                # dtrsm R: U := U op1(L11)^{-1};  dtrsm L: U11 := op2(L11)^{-1} U11
+                U[:, :] = sla.solve_triangular(tr(L11, tr1).T, U.T,
+                                               lower=(tr1 == "T")).T
+                U[:nn, :] = sla.solve_triangular(L11, U[:nn, :], lower=True,
+                                                 trans=tr2)
+                U[:nn, :] = np.tril(U[:nn, :])
+                if na > 0:
+                    U[nn:, :] = tr(Lk, tr3) @ U[nn:, :]
+            else:
+                U[:, :] = U @ tr(L11, tr1)
+                U[:nn, :] = tr(L11, tr2) @ U[:nn, :]
+                U[:nn, :] = np.tril(U[:nn, :])
+                if na > 0:
+                    U[nn:, :] = sla.solve_triangular(Lk, U[nn:, :], lower=True, trans=tr3)
+    return 0
+
+
+def hessian(symb, lblkval, yblkval, ublkvals, adj=False, inv=False, factored_updates=True):
+    """Hessian factor G_X / its adjoint / their inverses on each array of
+    ``ublkvals`` in place (hessian.c:354-374); ``adj=None`` applies the full
+    map G^adj G (or its inverse), as the reference's wrapper does
+    (cbase.c:701-713, 833-842).  L is the factor, Y the projected inverse."""
+    S = _S(symb)
+    inv = bool(inv)
+    a0 = inv if adj is None else bool(adj)
+    passes = [a0] if adj is not None else [a0, not a0]
+    for a in passes:
+        if a != inv:
+            _scale(S, lblkval, yblkval, ublkvals, inv, a, factored_updates)
+        for ub in ublkvals:
+            (_m2t if a else _y2k)(S, lblkval, ub, inv)
+        if a == inv:
+            _scale(S, lblkval, yblkval, ublkvals, inv, a, factored_updates)
+    return 0

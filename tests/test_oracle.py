@@ -140,3 +140,116 @@ def test_planted_mistake_is_caught():
     good = b.copy()
     m.cholesky(symb, good)
     assert _rel(bad, good) > 1e-3
+
+
+def _factor_and_inverse(symb, A):
+    b = (cp.cspmatrix(symb, device="cpu") + A).blkval.numpy().copy()
+    L = b.copy()
+    assert on.cholesky(symb, L) == 0
+    Y = L.copy()
+    assert on.projected_inverse(symb, Y) == 0
+    return b, L, Y
+
+
+def _dense_sym(symb, blkval):
+    return cp.cspmatrix(symb, blkval=blkval, device="cpu").spmatrix(
+        reordered=True, symmetric=True).toarray()
+
+
+def _lower_mask(symb):
+    """True on the meaningful (lower-trapezoid) entries of blkval."""
+    m = np.zeros(int(symb.blkptr[-1]), dtype=bool)
+    for k in range(symb.Nsn):
+        nn = int(symb.snptr[k + 1] - symb.snptr[k])
+        nj = nn + int(symb.relptr[k + 1] - symb.relptr[k])
+        blk = m[symb.blkptr[k]:symb.blkptr[k + 1]].reshape((nn, nj)).T
+        blk[np.tril_indices(nj, 0, nn)] = True
+    return m
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("maker", [lambda: problems.reference_test_matrix(),
+                                   lambda: problems.grid2d(14, seed=0),
+                                   lambda: problems.grid3d(6, seed=1)])
+def test_completion_hessian_vs_reference_kernels(maker):
+    A = maker()
+    symb = cp.symbolic(A, p=cp.amd.order)
+    b, L, Y = _factor_and_inverse(symb, A)
+    for fu in (True, False):
+        x1, x2 = Y.copy(), Y.copy()
+        assert ref.completion(symb, x1, fu) == 0
+        assert on.completion(symb, x2, fu) == 0
+        assert _rel(x2, x1) < 1e-12
+    rng = np.random.default_rng(0)
+    for fu in (True, False):
+        for adj in (False, True, None):
+            for inv in (False, True):
+                us = [rng.standard_normal(len(b)) for _ in range(2)]
+                u1 = [u.copy() for u in us]
+                u2 = [u.copy() for u in us]
+                ref.hessian(symb, L, Y, u1, adj=adj, inv=inv, factored_updates=fu)
+                on.hessian(symb, L, Y, u2, adj=adj, inv=inv, factored_updates=fu)
+                for a, c in zip(u2, u1):
+                    assert _rel(a, c) < 1e-12, (fu, adj, inv)
+
+
+def test_update_factor_closed_form():
+    """Li^T Li = F[r, r] when the (2,2) block of F holds Lk with
+    V_k = Lk^T Lk; Li is lower triangular."""
+    rng = np.random.default_rng(3)
+    nn, na = 5, 9
+    nj = nn + na
+    M = rng.standard_normal((nj, nj))
+    Vfull = M @ M.T + nj * np.eye(nj)
+    Lk = np.linalg.cholesky(Vfull[nn:, nn:][::-1, ::-1]).T[::-1, ::-1]  # V22 = Lk^T Lk
+    F = np.tril(Vfull)
+    F[nn:, nn:] = np.tril(Lk)
+    for r in ([0, 2, 3, 6, 7, 10, 13], [1, 4], [5, 8, 12], [0, 1, 2, 3, 4]):
+        r = np.array(r)
+        Li = on.update_factor(r, nn, na, F)
+        assert np.allclose(np.triu(Li, 1), 0.0)
+        assert np.abs(Li.T @ Li - Vfull[np.ix_(r, r)]).max() < 1e-12 * np.abs(Vfull).max()
+
+
+@pytest.mark.parametrize("maker", [lambda: problems.reference_test_matrix(),
+                                   lambda: problems.grid2d(10, seed=0),
+                                   lambda: problems.grid3d(5, seed=2)])
+def test_completion_hessian_closed_forms(maker):
+    """completion(P(X^{-1})) = L (tests/test_numeric.py:48-64 of the
+    reference); the full Hessian map is P(X^{-1} U X^{-1}) (dense); G and its
+    inverse compose to the identity and G^adj is the adjoint of G under the
+    trace inner product (tests/test_numeric.py:66-90)."""
+    A = maker()
+    symb = cp.symbolic(A, p=cp.amd.order)
+    b, L, Y = _factor_and_inverse(symb, A)
+    mask = _lower_mask(symb)
+    for fu in (True, False):
+        Lc = Y.copy()
+        assert on.completion(symb, Lc, fu) == 0
+        assert np.abs(Lc - L)[mask].max() < 1e-12 * np.abs(L).max()
+    PAP = _dense(A)[np.ix_(symb.p, symb.p)]
+    Xi = np.linalg.inv(PAP)
+    Pat = symb.sparsity_pattern(reordered=True, symmetric=True).toarray() != 0
+    rng = np.random.default_rng(5)
+    U0 = rng.standard_normal(len(b))
+    Ud = _dense_sym(symb, U0)
+    for fu in (True, False):
+        H = U0.copy()
+        on.hessian(symb, L, Y, [H], adj=None, factored_updates=fu)
+        Hd = Xi @ Ud @ Xi
+        assert np.abs(_dense_sym(symb, H)[Pat] - Hd[Pat]).max() < 1e-11 * np.abs(Hd).max()
+        Hi = H.copy()
+        on.hessian(symb, L, Y, [Hi], adj=None, inv=True, factored_updates=fu)
+        assert np.abs(Hi - U0)[mask].max() < 1e-10 * np.abs(U0).max()
+        for adj in (False, True):
+            G = U0.copy()
+            on.hessian(symb, L, Y, [G], adj=adj, factored_updates=fu)
+            on.hessian(symb, L, Y, [G], adj=adj, inv=True, factored_updates=fu)
+            assert np.abs(G - U0)[mask].max() < 1e-10 * np.abs(U0).max()
+        V0 = rng.standard_normal(len(b))
+        GU, GaV = U0.copy(), V0.copy()
+        on.hessian(symb, L, Y, [GU], adj=False, factored_updates=fu)
+        on.hessian(symb, L, Y, [GaV], adj=True, factored_updates=fu)
+        lhs = np.sum(_dense_sym(symb, GU) * _dense_sym(symb, V0))
+        rhs = np.sum(_dense_sym(symb, U0) * _dense_sym(symb, GaV))
+        assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1.0)
