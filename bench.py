@@ -25,7 +25,7 @@ sys.path.insert(0, HERE)
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 KINDS = ["chol_small", "extadd", "potrf_diag", "unused", "gemm", "extract",
          "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
-         "copy_top", "splitk_reduce"]
+         "copy_top", "splitk_reduce", "diag", "matop", "update_factor"]
 NK = len(KINDS)
 
 

@@ -45,15 +45,19 @@ void chp_set_error(const std::string& msg);
 // A GEMM-like task: C[i,j] = alpha * sum_k opA(i,k) opB(k,j) + beta * C[i,j]
 // over an M x N block, used for the large-front Schur updates and the
 // projected-inverse products.  Operands are given as (buffer, offset):
-// buffer 0 = blkval, 1 = update arena, 2 = scratch, 3 = diagonal-block inverses.
+// buffer 0 = blkval (or the direction / matrix the sweep works on),
+// 1 = update arena, 2 = scratch, 3 = diagonal-block inverses, 4 = split-K
+// partials, 5 = factor L, 6 = projected inverse Y (see BufTab).
 //   opA: 0 A(i,k) = A[i + k*lda]; 1 A(i,k) = A[k + i*lda];
 //        2 symmetric, lower stored: A(i,k) = A[max + min*lda].
-//   opB: 0 B(k,j) = B[k + j*ldb]; 1 B(k,j) = B[j + k*ldb].
+//   opB: 0 B(k,j) = B[k + j*ldb]; 1 B(k,j) = B[j + k*ldb];
+//        2 symmetric, lower stored: B(k,j) = B[max + min*ldb].
 //   tri: 0 full; 1 only C(i,j) with i >= j written; 2 as 1, and C(i,j) = 0
 //        for i < j (tiles above the diagonal only write zeros);
 //   triA: 0 none; 1 A is lower triangular (A(i,k) = 0 for k > i);
 //         2 A is upper triangular in op form (A(i,k) = 0 for k < i);
-//   triB (stored in `pad`): 1 B is lower triangular (B(k,j) = 0 for k < j).
+//   triB (stored in `pad`): 1 B is lower triangular (B(k,j) = 0 for k < j);
+//         2 B is upper triangular (B(k,j) = 0 for k > j).
 //   ksplit > 1: deterministic split-K.  Tile slice ks covers
 //        k in [ks*kchunk, min(K, (ks+1)*kchunk)) and writes alpha * partial to
 //        buffer 4 at offP + ks*M*N (column-major, ld M); k_splitk_reduce then
@@ -70,4 +74,48 @@ struct GemmTask {
 // One 64 x 64 output tile (k-slice ks) of a GemmTask.
 struct GemmTile {
     int task, tm, tn, ks;
+};
+
+// Base pointers of the buffers a launch addresses (kernel argument by value).
+struct BufTab {
+    double* p[8];
+};
+
+// One diagonal block (order nb <= 64) of a dense lower-triangular operation,
+// addressed by descriptor: element (i, j) at p[buf][off + i + j*ld].
+//   mode 0: Cholesky in place; the inverse of the factor goes row-major to
+//           p[obuf][outoff + j + i*64] (64 x 64, zero padded).
+//   mode 1: inverse of the lower-triangular block, written column-major to
+//           p[obuf][outoff + i + j*ldo] (full nb x nb, zero above the diagonal).
+// A failure records (key, col + LAPACK-style column) in the status word.
+struct DiagTask {
+    i64 off, outoff;
+    int ld, ldo, nb, key, col;
+    unsigned char buf, obuf, mode, pad;
+};
+
+// Element-wise operation on one 64 x 64 tile (origin i0, j0) of an m x n
+// matrix: source element (i, j) at p[sbuf][src + i + j*lds], destination at
+// p[dbuf][dst + i + j*ldd].  Kinds: MO_COPY dst = alpha src; MO_COPY_LOWER
+// the same for i >= j only; MO_COPY_LOWER_Z also writes 0 for i < j;
+// MO_RT dst(i, j) = src(n-1-j, n-1-i) for i >= j and 0 above (reversed
+// transpose of a lower triangle, square n); MO_SYMM dst(i, j) = dst(j, i)
+// for i < j; MO_ZERO_UPPER dst(i, j) = 0 for i < j; MO_ZERO dst = 0.
+enum MatOpKind { MO_COPY, MO_COPY_LOWER, MO_COPY_LOWER_Z, MO_RT, MO_SYMM, MO_ZERO_UPPER, MO_ZERO };
+struct MatOp {
+    i64 src, dst;
+    int lds, ldd, m, n, i0, j0;
+    unsigned char sbuf, dbuf, kind, pad;
+    int pad2;
+    double alpha;
+};
+
+// update_factor for one child c of front k: F11/F21 = the nj x nn block of k
+// at p[fbuf][foff] (ld nj), F22 = k's factor Lk (na x na, lower, ld na) at
+// p[1][lkoff]; writes Li (N x N, lower, zero above) to p[1][outoff] and uses
+// p[2][woff .. + na*N2] as workspace.  r = relidx + relpos (N entries).
+struct UFTask {
+    i64 foff, lkoff, outoff, woff, relpos;
+    int nn, na, N, key;
+    unsigned char fbuf, pad[7];
 };

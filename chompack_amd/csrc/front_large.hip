@@ -314,6 +314,99 @@ __global__ __launch_bounds__(TPB) void k_trtri_diag(
     }
 }
 
+// Descriptor-addressed diagonal blocks (dense operations of the completion
+// and Hessian sweeps): mode 0 factors the block in place and writes the
+// inverse of the factor row-major to the output; mode 1 writes the inverse
+// of the lower-triangular block column-major (zero above the diagonal).
+__global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__ tasks,
+                                                    BufTab bt, unsigned long long* status) {
+    __shared__ double Ls[NB][NB + 1];
+    __shared__ double X[NB][NB + 1];
+    __shared__ double W[NB][NB + 1];
+    __shared__ double P[NB][5], Q[NB][5];
+    __shared__ double rdiag[NB];
+    __shared__ int zero_at;
+    const DiagTask t = tasks[blockIdx.x];
+    const int nb = t.nb, ld = t.ld;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double* blk = bt.p[t.buf] + t.off;
+    double* out = bt.p[t.obuf] + t.outoff;
+    if (t.mode == 0) {
+        double a[DQ];
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int j = w + 4 * q;
+            a[q] = (l < nb && j < nb && l >= j) ? blk[l + (i64)j * ld] : (l == j ? 1.0 : 0.0);
+        }
+        int bad = 0;
+        if (!block_potrf4(a, P, Q, rdiag, w, l, &bad)) {
+            if (threadIdx.x == 0) chp_record_failure(status, t.key, t.col + bad + 1);
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int j = w + 4 * q;
+            Ls[l][j] = (l >= j) ? a[q] : 0.0;
+            if (l < nb && j < nb && l >= j) blk[l + (i64)j * ld] = a[q];
+        }
+        __syncthreads();
+        block_trtri_rd(Ls, rdiag, X, W);
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int r = w + 4 * q;
+            out[l + r * NB] = (r < nb && l < nb) ? X[r][l] : 0.0;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) zero_at = NB;
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int j = w + 4 * q;
+        Ls[l][j] = (l < nb && j < nb && l >= j) ? blk[l + (i64)j * ld] : (l == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (w == 0) {
+        if (l < nb && Ls[l][l] == 0.0) atomicMin(&zero_at, l);
+        rdiag[l] = 1.0 / Ls[l][l];
+    }
+    __syncthreads();
+    if (zero_at < NB) {
+        if (threadIdx.x == 0) chp_record_failure(status, t.key, t.col + zero_at + 1);
+        return;
+    }
+    block_trtri_rd(Ls, rdiag, X, W);
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int j = w + 4 * q;
+        if (l < nb && j < nb) out[l + (i64)j * t.ldo] = X[l][j];
+    }
+}
+
+// Element-wise tile operations (see MatOpKind).  One workgroup per 64 x 64
+// tile; consecutive threads walk rows.
+__global__ __launch_bounds__(TPB) void k_matop(const MatOp* __restrict__ ops, BufTab bt) {
+    const MatOp o = ops[blockIdx.x];
+    const double* S = bt.p[o.sbuf] + o.src;
+    double* D = bt.p[o.dbuf] + o.dst;
+    const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+    const int i = o.i0 + il;
+    if (i >= o.m) return;
+    for (int jl = jl0; jl < 64; jl += 4) {
+        const int j = o.j0 + jl;
+        if (j >= o.n) break;
+        double* d = D + i + (i64)j * o.ldd;
+        switch (o.kind) {
+        case MO_COPY: *d = o.alpha * S[i + (i64)j * o.lds]; break;
+        case MO_COPY_LOWER: if (i >= j) *d = o.alpha * S[i + (i64)j * o.lds]; break;
+        case MO_COPY_LOWER_Z: *d = (i >= j) ? o.alpha * S[i + (i64)j * o.lds] : 0.0; break;
+        case MO_RT: *d = (i >= j) ? S[(o.n - 1 - j) + (i64)(o.n - 1 - i) * o.lds] : 0.0; break;
+        case MO_SYMM: if (i < j) *d = D[j + (i64)i * o.ldd]; break;
+        case MO_ZERO_UPPER: if (i < j) *d = 0.0; break;
+        default: *d = 0.0; break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64).  One workgroup computes
 // one 64 x 64 tile of one task; 4 waves in a 2 x 2 arrangement, each wave a
@@ -325,9 +418,6 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 #define BK 32
 
-__device__ inline const double* bufptr(int b, double* b0, double* b1, double* b2, double* b3) {
-    return b == 0 ? b0 : (b == 1 ? b1 : (b == 2 ? b2 : b3));
-}
 
 // Element (gi, gk) of opA and (gk, gj) of opB with the task's bounds and
 // triangular masks; 0 outside.
@@ -345,12 +435,16 @@ template <int OPB>
 __device__ inline double ldB_chk(const GemmTask& T, const double* B, int gk, int gj) {
     bool ok = gj < T.N && gk < T.K;
     if (T.pad == 1) ok = ok && gk >= gj;
+    else if (T.pad == 2) ok = ok && gk <= gj;
     if (!ok) return 0.0;
-    return (OPB == 0) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
+    if (OPB == 0) return B[gk + (i64)gj * T.ldb];
+    if (OPB == 1) return B[gj + (i64)gk * T.ldb];
+    return (gk >= gj) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
 }
 
 // One 64 x 64 tile.  OPA: 0 A(i,k) = A[i + k lda], 1 A[k + i lda],
-// 2 symmetric lower.  OPB: 0 B(k,j) = B[k + j ldb], 1 B[j + k ldb].
+// 2 symmetric lower.  OPB: 0 B(k,j) = B[k + j ldb], 1 B[j + k ldb],
+// 2 symmetric lower.
 // CHK: per-element bounds and masks (edge tiles, triangular operands);
 // otherwise a check-free interior path.  Threads are mapped so consecutive
 // lanes read consecutive addresses of the stored operand.
@@ -362,7 +456,7 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     constexpr bool AK = (OPA == 1);   // A read along k
-    constexpr bool BK_ = (OPB == 0);  // B read along k
+    constexpr bool BK_ = (OPB != 1);  // B read along k
     // per-thread coordinates of the 8 A and 8 B elements of a chunk
     const int ai = AK ? (tid >> 5) : (tid & 63);
     const int ak = AK ? (tid & 31) : (tid >> 6);
@@ -380,7 +474,9 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
         if (T.triA == 1 && !(kb + BK - 1 <= row0)) return false;
         if (T.triA == 2 && !(kb >= row0 + TILE - 1)) return false;
         if (T.pad == 1 && !(kb >= col0 + TILE - 1)) return false;
-        return OPA != 2;
+        if (T.pad == 2 && !(kb + BK - 1 <= col0)) return false;
+        // a symmetric B chunk below the diagonal of B is a plain N read
+        return OPA != 2 && (OPB != 2 || kb >= col0 + TILE - 1);
     };
     auto fetch = [&](int kb) {
         if (chunk_plain(kb)) {
@@ -389,7 +485,7 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
                 const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
                 ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
                 const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
-                rb[q] = (OPB == 0) ? B[gk2 + (i64)gj * T.ldb] : B[gj + (i64)gk2 * T.ldb];
+                rb[q] = (OPB == 1) ? B[gj + (i64)gk2 * T.ldb] : B[gk2 + (i64)gj * T.ldb];
             }
         } else {
 #pragma unroll
@@ -464,31 +560,31 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
 }
 
 __global__ __launch_bounds__(TPB) void k_gemm_tiles(
-    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
-    double* b0, double* b1, double* b2, double* b3, double* b4) {
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     __shared__ double As[BK][TILE + 2];
     __shared__ double Bs[BK][TILE + 2];
     const GemmTile tl = tiles[blockIdx.x];
     const GemmTask T = tasks[tl.task];
-    const double* A = bufptr(T.bufA, b0, b1, b2, b3) + T.offA;
-    const double* B = bufptr(T.bufB, b0, b1, b2, b3) + T.offB;
-    double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
+    const double* A = bt.p[T.bufA] + T.offA;
+    const double* B = bt.p[T.bufB] + T.offB;
+    double* C = bt.p[T.bufC] + T.offC;
     const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
     int kbase = 0, kend = T.K;
     double* Pp = nullptr;
     if (T.ksplit > 1) {
         kbase = tl.ks * T.kchunk;
         kend = min(T.K, kbase + T.kchunk);
-        Pp = b4 + T.offP + (i64)tl.ks * T.M * T.N;
+        Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
         if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // reduce handles it
     }
 
     // triangular operands: chunks where opA or opB is identically zero are
     // skipped (A lower: k <= i < row0 + TILE; A upper: k >= i >= row0;
-    // B lower: k >= j >= col0)
+    // B lower: k >= j >= col0; B upper: k <= j < col0 + TILE)
     if (T.triA == 1) kend = min(kend, row0 + TILE);
     if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
     if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
+    if (T.pad == 2) kend = min(kend, col0 + TILE);
 
     if (T.tri == 2 && row0 + TILE - 1 < col0) {
         for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
@@ -502,30 +598,31 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     // diagonal of A (then it is a plain N or T read)
     int opa = T.opA;
     const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && ((kend - kbase) % BK) == 0 &&
-                      T.triA == 0 && T.pad == 0;
+                      T.triA == 0 && T.pad == 0 && T.opB != 2;
     bool chk = !full;
     if (opa == 2 && full) {
         if (row0 >= T.K) opa = 0;            // all gi >= gk: lower part, N read
-        else if (row0 + TILE <= 0) opa = 1;  // never
         else chk = true;
     }
-    const int key = opa * 2 + T.opB;
+    const int key = opa * 3 + T.opB;
     if (!chk) {
         switch (key) {
         case 0: gemm_tile<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         case 1: gemm_tile<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        case 2: gemm_tile<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        case 3: gemm_tile<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        default: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     } else {
         switch (key) {
         case 0: gemm_tile<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         case 1: gemm_tile<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        case 2: gemm_tile<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        case 3: gemm_tile<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        case 4: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
-        default: gemm_tile<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 2: gemm_tile<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 4: gemm_tile<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 5: gemm_tile<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 6: gemm_tile<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 7: gemm_tile<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     }
 }
@@ -569,12 +666,11 @@ __global__ __launch_bounds__(TPB) void k_copy_top(
 // C = beta*C + sum_ks partial_ks over one 64 x 64 tile of a split-K task, in
 // slice order (deterministic), honouring the task's triangular output mode.
 __global__ __launch_bounds__(TPB) void k_splitk_reduce(
-    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks,
-    double* b0, double* b1, double* b2, double* b3, const double* __restrict__ b4) {
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     const GemmTile tl = tiles[blockIdx.x];
     const GemmTask T = tasks[tl.task];
-    double* C = const_cast<double*>(bufptr(T.bufC, b0, b1, b2, b3)) + T.offC;
-    const double* Pp = b4 + T.offP;
+    double* C = bt.p[T.bufC] + T.offC;
+    const double* Pp = bt.p[4] + T.offP;
     const i64 mn = (i64)T.M * T.N;
     for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
         const int i = tl.tm * TILE + (e & 63), j = tl.tn * TILE + (e >> 6);

@@ -29,10 +29,13 @@ __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, dou
 __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,
                              double* dinv, const i64* dinvptr, double* X, const i64* xptr,
                              unsigned long long* status);
-__global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, double* b0,
-                             double* b1, double* b2, double* b3, double* b4);
-__global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, double* b0,
-                                double* b1, double* b2, double* b3, const double* b4);
+__global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+__global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+// descriptor-addressed dense operations (completion, Hessian)
+__global__ void k_diag_tasks(const DiagTask* tasks, BufTab bt, unsigned long long* status);
+__global__ void k_matop(const MatOp* ops, BufTab bt);
+__global__ void k_update_factor(const UFTask* tasks, BufTab bt, const int* relidx,
+                                unsigned long long* status);
 __global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkval,
                            const double* scratch, const i64* sptr);
 

@@ -1,0 +1,138 @@
+// Device plan shared by the scheduler sources (plan.hip: cholesky, llt,
+// projected inverse, solve; plan_dense.hip: completion and the Hessian maps).
+#pragma once
+#include "common.h"
+#include "kernels.h"
+
+#include <cstring>
+#include <vector>
+
+namespace chpi {
+
+enum Kid {
+    K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
+    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
+    K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR
+};
+const int NKIDS = 17;
+
+struct Launch {
+    Kid kid;
+    int grid;
+    int block;
+    size_t lds;
+    i64 list;            // byte offset of the work list in the program's list buffer
+    int iarg;
+    double flops = 0.0;  // algorithmic flops of a GEMM launch
+    int bsel = 0;        // buffer (BufTab index) used as blkval by front kernels
+};
+
+struct ListBuf {
+    std::vector<char> bytes;
+    template <class T>
+    i64 add(const std::vector<T>& v) {
+        i64 off = (i64)((bytes.size() + 255) / 256 * 256);
+        bytes.resize(off + v.size() * sizeof(T));
+        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+// Work lists + GEMM tasks of a family of sweeps, with their device copies and
+// split-K partial bookkeeping.
+struct Program {
+    ListBuf lists;
+    std::vector<GemmTask> tasks;
+    char* d_lists = nullptr;
+    GemmTask* d_tasks = nullptr;
+    size_t up_lists = 0, up_tasks = 0;  // sizes of the device copies
+    i64 splitp_cur = 0, splitp_len = 0;
+    std::vector<GemmTile> split_out;    // output tiles of split tasks, current launch
+};
+
+}  // namespace chpi
+
+struct chp_plan {
+    int device = 0;
+    i64 n = 0, nsn = 0;
+    int nlevels = 0;
+    int small_max = 64;
+    std::vector<FrontInfo> fi;
+    std::vector<std::vector<int>> bylevel;
+    std::vector<int> h_relidx, h_chidx;
+    std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit, h_lltptr;
+    i64* d_lltptr = nullptr;
+    // device arrays
+    FrontInfo* d_fi = nullptr;
+    int* d_relidx = nullptr;
+    int* d_chidx = nullptr;
+    i64* d_perm = nullptr;
+    i64* d_snptr = nullptr;
+    i64* d_dinvptr = nullptr;
+    i64* d_xptr = nullptr;
+    i64* d_tptr = nullptr;
+    i64* d_wunit = nullptr;
+    int* d_colfront = nullptr;
+    int* d_colidx = nullptr;
+    double* d_upd = nullptr;
+    double* d_dinv = nullptr;
+    double* d_scratch = nullptr;
+    double* d_partial = nullptr;
+    double* d_work = nullptr;
+    double* d_splitp = nullptr;
+    i64 work_len = 0;
+    unsigned long long* d_status = nullptr;
+    i64 upd_len = 0, dinv_len = 0, scratch_len = 0, solve_len = 0;
+    std::vector<chpi::Launch> chol, projinv, llt, solve_fwd, solve_bwd;
+    chpi::Program main, dense;
+    chpi::Program* pg = &main;  // program the builders append to
+    int n_large_chol = 0, n_large_projinv = 0;
+    i64 n_gemm_tiles = 0;
+    // completion / Hessian sweeps, built on first use.  Index of scale:
+    // 4 factored + 2 inv + adj; of y2k / m2t: inv.
+    struct Seq { bool built = false; std::vector<chpi::Launch> seq; };
+    struct LevelSeq {
+        bool built = false;
+        std::vector<std::vector<chpi::Launch>> shared, dir;  // per level, top-down
+    };
+    Seq comp[2], y2k[2], m2t[2];
+    LevelSeq scale[8];
+    double* d_dscratch = nullptr;
+    double* d_ddinv = nullptr;
+    double* d_dsplitp = nullptr;
+    i64 dscratch_len = 0, ddinv_len = 0;                       // required
+    i64 dscratch_cap = 0, ddinv_cap = 0, dsplitp_cap = 0;      // allocated
+    // HIP graphs of the sweeps, replayed on a plan-owned stream that is
+    // joined to the caller's stream with events; cached per blkval pointer
+    bool use_graphs = true;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev_in = nullptr, gev_out = nullptr;
+    struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
+    Cached g_chol, g_projinv, g_llt;
+    // profiling: per-kernel-kind time from HIP events around every launch
+    bool prof = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<double> prof_rows;  // per launch: kind, grid, flops, ms
+    double prof_ms[chpi::NKIDS] = {0};
+    double prof_flops[chpi::NKIDS] = {0};
+    i64 prof_cnt[chpi::NKIDS] = {0};
+};
+
+namespace chpi {
+
+GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, int opB,
+            int bufC, i64 offC, int ldc, int M, int N, int K, double alpha, double beta,
+            int tri, int triA = 0, int triB = 0);
+double task_flops(const GemmTask& t);
+void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool skip_upper);
+void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles);
+// Uploads a program's lists and tasks if they grew since the last upload.
+int sync_program(Program& pg);
+// Replays `seq` of program `pg` on stream s with buffer table bt.
+int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
+            hipStream_t s);
+int read_status(chp_plan* P, hipStream_t s, i64* info);
+BufTab main_tab(chp_plan* P, double* blkval);
+int alloc_doubles(double** d, i64 len);
+
+}  // namespace chpi

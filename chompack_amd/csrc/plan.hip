@@ -11,8 +11,7 @@
 //     fronts of the level.
 // Executing a sweep replays the precomputed launches on the caller's stream;
 // there is no host work per front, only the final status read.
-#include "common.h"
-#include "kernels.h"
+#include "plan.h"
 #include <hip/hip_ext.h>
 #include "../../include/chompack_amd.h"
 
@@ -24,96 +23,7 @@
 static thread_local std::string g_err;
 void chp_set_error(const std::string& msg) { g_err = msg; }
 
-namespace {
-
-enum Kid {
-    K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
-    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
-    K_COPY_TOP, K_SPLITK_REDUCE
-};
-
-struct Launch {
-    Kid kid;
-    int grid;
-    int block;
-    size_t lds;
-    i64 list;   // byte offset of the work list in the list buffer
-    int iarg;
-    double flops = 0.0;  // algorithmic flops of a GEMM launch
-};
-
-const int NKIDS = 14;
-
-struct ListBuf {
-    std::vector<char> bytes;
-    template <class T>
-    i64 add(const std::vector<T>& v) {
-        i64 off = (i64)((bytes.size() + 255) / 256 * 256);
-        bytes.resize(off + v.size() * sizeof(T));
-        if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
-        return off;
-    }
-};
-
-}  // namespace
-
-struct chp_plan {
-    int device = 0;
-    i64 n = 0, nsn = 0;
-    int nlevels = 0;
-    int small_max = 64;
-    std::vector<FrontInfo> fi;
-    std::vector<std::vector<int>> bylevel;
-    std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit, h_lltptr;
-    i64* d_lltptr = nullptr;
-    // device arrays
-    FrontInfo* d_fi = nullptr;
-    int* d_relidx = nullptr;
-    int* d_chidx = nullptr;
-    i64* d_perm = nullptr;
-    i64* d_snptr = nullptr;
-    i64* d_dinvptr = nullptr;
-    i64* d_xptr = nullptr;
-    i64* d_tptr = nullptr;
-    i64* d_wunit = nullptr;
-    int* d_colfront = nullptr;
-    int* d_colidx = nullptr;
-    double* d_upd = nullptr;
-    double* d_dinv = nullptr;
-    double* d_scratch = nullptr;
-    double* d_partial = nullptr;
-    double* d_work = nullptr;
-    i64 work_len = 0;
-    unsigned long long* d_status = nullptr;
-    char* d_lists = nullptr;
-    GemmTask* d_tasks = nullptr;
-    i64 upd_len = 0, dinv_len = 0, scratch_len = 0, solve_len = 0;
-    std::vector<Launch> chol, projinv, llt, solve_fwd, solve_bwd;
-    bool llt_ok = true;
-    std::vector<GemmTask> tasks;
-    ListBuf lists;
-    int n_large_chol = 0, n_large_projinv = 0;
-    i64 n_gemm_tiles = 0;
-    i64 splitp_cur = 0, splitp_len = 0;  // split-K partials: per-launch bump, max
-    double* d_splitp = nullptr;
-    std::vector<GemmTile> split_out;     // output tiles of split tasks, current launch
-    // HIP graphs of the sweeps, replayed on a plan-owned stream that is
-    // joined to the caller's stream with events; cached per blkval pointer
-    bool use_graphs = true;
-    hipStream_t gstream = nullptr;
-    hipEvent_t gev_in = nullptr, gev_out = nullptr;
-    struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
-    Cached g_chol, g_projinv, g_llt;
-    // profiling: per-kernel-kind time from HIP events around every launch
-    bool prof = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<double> prof_rows;  // per launch: kind, grid, flops, ms
-    double prof_ms[NKIDS] = {0};
-    double prof_flops[NKIDS] = {0};
-    i64 prof_cnt[NKIDS] = {0};
-};
-
-namespace {
+namespace chpi {
 
 const size_t LDS_SMALL_CAP = 64 * 1024;
 
@@ -138,7 +48,7 @@ bool projinv_is_small(const chp_plan* P, const FrontInfo& f) {
 
 GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, int opB,
             int bufC, i64 offC, int ldc, int M, int N, int K, double alpha, double beta,
-            int tri, int triA = 0, int triB = 0) {
+            int tri, int triA, int triB) {
     GemmTask t;
     std::memset(&t, 0, sizeof(t));
     t.offA = offA; t.offB = offB; t.offC = offC;
@@ -179,7 +89,7 @@ double task_flops(const GemmTask& t) {
 // split along K (deterministic split-K) so the launch fills the GPU.
 void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool skip_upper) {
     if (t.M <= 0 || t.N <= 0) return;
-    int id = (int)P->tasks.size();
+    int id = (int)P->pg->tasks.size();
     int mt = (t.M + TILE - 1) / TILE, nt = (t.N + TILE - 1) / TILE;
     int ntiles = 0;
     for (int tn = 0; tn < nt; ++tn)
@@ -196,41 +106,41 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
         ks = (t.K + chunk - 1) / chunk;
         t.ksplit = ks;
         t.kchunk = chunk;
-        t.offP = P->splitp_cur;
-        P->splitp_cur += (i64)ks * t.M * t.N;
-        P->splitp_len = std::max(P->splitp_len, P->splitp_cur);
+        t.offP = P->pg->splitp_cur;
+        P->pg->splitp_cur += (i64)ks * t.M * t.N;
+        P->pg->splitp_len = std::max(P->pg->splitp_len, P->pg->splitp_cur);
     }
-    P->tasks.push_back(t);
+    P->pg->tasks.push_back(t);
     for (int tn = 0; tn < nt; ++tn)
         for (int tm = 0; tm < mt; ++tm) {
             if (skip_upper && tm < tn) continue;
             for (int k = 0; k < ks; ++k) tiles.push_back({id, tm, tn, k});
-            if (ks > 1) P->split_out.push_back({id, tm, tn, 0});
+            if (ks > 1) P->pg->split_out.push_back({id, tm, tn, 0});
         }
 }
 
 void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles) {
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
-    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->lists.add(tiles), 0};
+    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->pg->lists.add(tiles), 0};
     double fl = 0.0;
     int last = -1;
     for (const GemmTile& g : tiles)
-        if (g.task != last) { fl += task_flops(P->tasks[g.task]); last = g.task; }
+        if (g.task != last) { fl += task_flops(P->pg->tasks[g.task]); last = g.task; }
     L.flops = fl;
     seq.push_back(L);
     // the split tasks whose tiles are in this list get their reduction now
     std::vector<GemmTile> red;
     std::vector<GemmTile> keep;
-    for (const GemmTile& g : P->split_out) {
+    for (const GemmTile& g : P->pg->split_out) {
         bool mine = false;
         for (const GemmTile& h : tiles) if (h.task == g.task) { mine = true; break; }
         (mine ? red : keep).push_back(g);
     }
-    P->split_out.swap(keep);
+    P->pg->split_out.swap(keep);
     if (!red.empty())
-        seq.push_back({K_SPLITK_REDUCE, (int)red.size(), 256, 0, P->lists.add(red), 0});
-    if (P->split_out.empty()) P->splitp_cur = 0;
+        seq.push_back({K_SPLITK_REDUCE, (int)red.size(), 256, 0, P->pg->lists.add(red), 0});
+    if (P->pg->split_out.empty()) P->pg->splitp_cur = 0;
 }
 
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
@@ -243,7 +153,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         else large.push_back(k);
     }
     if (!small.empty())
-        P->chol.push_back({K_CHOL_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+        P->chol.push_back({K_CHOL_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
     if (large.empty()) return;
     P->n_large_chol += (int)large.size();
     std::vector<int2> et;
@@ -251,7 +161,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         int nj = P->fi[k].nn + P->fi[k].na;
         for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
     }
-    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 1});
+    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1});
     // Two-level blocking: 64-wide inner panels (diagonal factor, panel solve,
     // update confined to the current outer panel), then one trailing update
     // with K = the outer panel width for everything right of it.
@@ -261,7 +171,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         for (int c0 = p0; c0 < std::min(p0 + NB2, maxnn); c0 += NB) {
             std::vector<int> act;
             for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
-            P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->lists.add(act), c0});
+            P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->pg->lists.add(act), c0});
             std::vector<GemmTile> pt, it;
             for (int k : act) {
                 const FrontInfo& f = P->fi[k];
@@ -362,7 +272,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
     for (int k : fr)
         for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
     if (!xt.empty())
-        P->projinv.push_back({K_EXTRACT, (int)xt.size(), 256, 0, P->lists.add(xt), 0});
+        P->projinv.push_back({K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0});
     std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
@@ -371,7 +281,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         else large.push_back(k);
     }
     if (!small.empty())
-        P->projinv.push_back({K_PROJINV_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+        P->projinv.push_back({K_PROJINV_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
     if (large.empty()) return;
     P->n_large_projinv += (int)large.size();
     // per-front scratch: X = L11^{-1} (nn x nn), W (nn x nn), T = V B21 (na x nn)
@@ -390,7 +300,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         int nbk = (P->fi[k].nn + NB - 1) / NB;
         for (int b = 0; b < nbk; ++b) bt.push_back(make_int2(k, b));
     }
-    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 256, 0, P->lists.add(bt), 0});
+    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 256, 0, P->pg->lists.add(bt), 0});
     // X = L11^{-1} by recursive halving of the block range [a, b) at m:
     //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
     // all nodes of one height (over all large fronts) form one launch pair.
@@ -452,7 +362,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         if (P->fi[k].na > 0)
             for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
     if (!nt.empty())
-        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->lists.add(nt), 0});
+        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0});
 }
 
 // llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
@@ -471,7 +381,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
         }
     }
     if (!small.empty())
-        P->llt.push_back({K_LLT_SMALL, (int)small.size(), 256, lds, P->lists.add(small), 0});
+        P->llt.push_back({K_LLT_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
     if (large.empty()) return;
     // scratch A (nj x nn, ld nj) per large front; offsets in h_xptr are per
     // sweep and the levels run one after the other, so they are reused
@@ -502,7 +412,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
             add_gemm_tiles(P, g2, mk(2, aoff[q] + nn, nj, 0, 2, aoff[q] + nn, nj, 1,
                                      1, f.updptr, na, na, na, nn, 1.0, 0.0, 1), true);
     }
-    P->llt.push_back({K_COPY_TOP, (int)ct.size(), 256, 0, P->lists.add(ct), 0});
+    P->llt.push_back({K_COPY_TOP, (int)ct.size(), 256, 0, P->pg->lists.add(ct), 0});
     push_gemm(P, P->llt, g1);
     push_gemm(P, P->llt, g2);
     std::vector<int2> et;
@@ -510,7 +420,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
         int nj = P->fi[k].nn + P->fi[k].na;
         for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
     }
-    P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->lists.add(et), 0});
+    P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0});
 }
 
 template <class T>
@@ -536,54 +446,86 @@ int alloc_doubles(double** d, i64 len) {
             hipLaunchKernelGGL(K, g, b, LDS, s, __VA_ARGS__);                          \
     } while (0)
 
-int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStream_t s) {
+BufTab main_tab(chp_plan* P, double* blkval) {
+    BufTab t;
+    t.p[0] = blkval; t.p[1] = P->d_upd; t.p[2] = P->d_scratch; t.p[3] = P->d_dinv;
+    t.p[4] = P->d_splitp; t.p[5] = nullptr; t.p[6] = nullptr; t.p[7] = nullptr;
+    return t;
+}
+
+int sync_program(Program& pg) {
+    if (pg.lists.bytes.size() != pg.up_lists || !pg.d_lists) {
+        if (pg.d_lists) HIP_CHECK(hipFree(pg.d_lists));
+        pg.d_lists = nullptr;
+        if (upload(&pg.d_lists, pg.lists.bytes)) return -2;
+        pg.up_lists = pg.lists.bytes.size();
+    }
+    if (pg.tasks.size() != pg.up_tasks || !pg.d_tasks) {
+        if (pg.d_tasks) HIP_CHECK(hipFree(pg.d_tasks));
+        pg.d_tasks = nullptr;
+        if (upload(&pg.d_tasks, pg.tasks)) return -2;
+        pg.up_tasks = pg.tasks.size();
+    }
+    return 0;
+}
+
+int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
+            hipStream_t s) {
     for (const Launch& L : seq) {
-        const void* list = P->d_lists + L.list;
+        const void* list = pg.d_lists + L.list;
+        double* blkval = bt.p[L.bsel];
         dim3 g(L.grid), b(L.block);
         switch (L.kid) {
         case K_CHOL_SMALL:
             CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, P->d_fi, blkval,
-                               P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
+                       P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
             break;
         case K_EXTADD:
             CHP_LAUNCH(k_extadd_tiles, 0, (const int2*)list, P->d_fi, blkval,
-                               P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
+                       P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
             break;
         case K_COPY_TOP:
             CHP_LAUNCH(k_copy_top, 0, (const int2*)list, P->d_fi, blkval,
-                               P->d_scratch, P->d_lltptr);
+                       P->d_scratch, P->d_lltptr);
             break;
         case K_POTRF_DIAG:
             CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, P->d_fi,
-                               blkval, P->d_dinv, P->d_dinvptr, P->d_status);
+                       blkval, P->d_dinv, P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, P->d_tasks,
-                               blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
+            CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_SPLITK_REDUCE:
-            CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, P->d_tasks,
-                               blkval, P->d_upd, P->d_scratch, P->d_dinv, P->d_splitp);
+            CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int2*)list, P->d_fi, blkval,
-                               P->d_upd, P->d_relidx);
+                       P->d_upd, P->d_relidx);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
-                               blkval, P->d_upd, P->d_status);
+                       blkval, P->d_upd, P->d_status);
             break;
         case K_TRTRI_DIAG:
             CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, P->d_fi, blkval,
-                               P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
+                       P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
             break;
         case K_NEG_COPY:
             CHP_LAUNCH(k_neg_copy, 0, (const int2*)list, P->d_fi, blkval,
-                               P->d_scratch, P->d_tptr);
+                       P->d_scratch, P->d_tptr);
             break;
         case K_LLT_SMALL:
             CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, P->d_fi, blkval,
-                               P->d_upd, P->d_relidx, P->d_chidx);
+                       P->d_upd, P->d_relidx, P->d_chidx);
+            break;
+        case K_DIAG:
+            CHP_LAUNCH(k_diag_tasks, 0, (const DiagTask*)list, bt, P->d_status);
+            break;
+        case K_MATOP:
+            CHP_LAUNCH(k_matop, 0, (const MatOp*)list, bt);
+            break;
+        case K_UPDATE_FACTOR:
+            CHP_LAUNCH(k_update_factor, 0, (const UFTask*)list, bt, P->d_relidx, P->d_status);
             break;
         default:
             chp_set_error("internal: unexpected launch kind");
@@ -602,17 +544,17 @@ int run_seq(chp_plan* P, const std::vector<Launch>& seq, double* blkval, hipStre
     HIP_CHECK(hipGetLastError());
     return 0;
 }
-
 #undef CHP_LAUNCH
 
 // Runs `seq` (preceded by a status reset) on stream s, through a cached HIP
 // graph when enabled.
 int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, double* blkval,
               hipStream_t s, bool reset_status) {
+    const BufTab bt = main_tab(P, blkval);
     if (!P->use_graphs || P->prof) {
         if (reset_status)
             hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-        return run_seq(P, seq, blkval, s);
+        return run_seq(P, P->main, seq, bt, s);
     }
     if (!P->gstream) {
         HIP_CHECK(hipStreamCreateWithFlags(&P->gstream, hipStreamNonBlocking));
@@ -625,7 +567,7 @@ int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, 
         HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
         if (reset_status)
             hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
-        int rc = run_seq(P, seq, blkval, P->gstream);
+        int rc = run_seq(P, P->main, seq, bt, P->gstream);
         hipError_t e = hipStreamEndCapture(P->gstream, &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
         if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
@@ -654,7 +596,9 @@ int read_status(chp_plan* P, hipStream_t s, i64* info) {
     return 1;
 }
 
-}  // namespace
+}  // namespace chpi
+
+using namespace chpi;
 
 extern "C" {
 
@@ -728,11 +672,11 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     P->scratch_len = scratch;
     for (int L = 0; L < H; ++L) {
         const auto& fr = P->bylevel[L];
-        P->solve_fwd.push_back({K_SOLVE_FWD, (int)fr.size(), 256, 0, P->lists.add(fr), 0});
+        P->solve_fwd.push_back({K_SOLVE_FWD, (int)fr.size(), 256, 0, P->pg->lists.add(fr), 0});
     }
     for (int L = H - 1; L >= 0; --L) {
         const auto& fr = P->bylevel[L];
-        P->solve_bwd.push_back({K_SOLVE_BWD, (int)fr.size(), 256, 0, P->lists.add(fr), 0});
+        P->solve_bwd.push_back({K_SOLVE_BWD, (int)fr.size(), 256, 0, P->pg->lists.add(fr), 0});
     }
     // device copies
     std::vector<int> rel32(relptr[nsn]), ch32(chptr[nsn]);
@@ -749,6 +693,8 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= upload(&P->d_fi, P->fi);
     rc |= upload(&P->d_relidx, rel32);
     rc |= upload(&P->d_chidx, ch32);
+    P->h_relidx.swap(rel32);
+    P->h_chidx.swap(ch32);
     rc |= upload(&P->d_perm, pv);
     rc |= upload(&P->d_snptr, sp);
     rc |= upload(&P->d_dinvptr, P->h_dinvptr);
@@ -758,13 +704,12 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= upload(&P->d_lltptr, P->h_lltptr);
     rc |= upload(&P->d_colfront, colfront);
     rc |= upload(&P->d_colidx, colidx);
-    rc |= upload(&P->d_lists, P->lists.bytes);
-    rc |= upload(&P->d_tasks, P->tasks);
+    rc |= sync_program(P->main);
     rc |= alloc_doubles(&P->d_upd, P->upd_len);
     rc |= alloc_doubles(&P->d_dinv, P->dinv_len);
     rc |= alloc_doubles(&P->d_scratch, P->scratch_len);
     rc |= alloc_doubles(&P->d_partial, n);
-    rc |= alloc_doubles(&P->d_splitp, P->splitp_len);
+    rc |= alloc_doubles(&P->d_splitp, P->main.splitp_len);
     if (hipMalloc((void**)&P->d_status, sizeof(unsigned long long)) != hipSuccess) rc = -1;
     if (rc) {
         std::string e = g_err;
@@ -787,7 +732,8 @@ void chp_plan_destroy(chp_plan* P) {
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
-                    P->d_lists, P->d_tasks};
+                    P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
+                    P->d_dscratch, P->d_ddinv, P->d_dsplitp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete P;
@@ -892,7 +838,7 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
     }
     const std::vector<Launch>& seq = trans == 'N' ? P->solve_fwd : P->solve_bwd;
     for (const Launch& L : seq) {
-        const int* list = (const int*)(P->d_lists + L.list);
+        const int* list = (const int*)(P->main.d_lists + L.list);
         if (trans == 'N')
             hipLaunchKernelGGL(k_solve_fwd, dim3(L.grid), dim3(256), 0, s, list, P->d_fi, blkval,
                                P->d_snptr, P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs,

@@ -1,0 +1,587 @@
+// Completion and Hessian sweeps (completion.c, hessian.c, update_factor.c of
+// the reference) on the level schedule of a plan.
+//
+// Every front-local dense operation is expressed on descriptors and batched
+// over all fronts of a level:
+//   * Cholesky of a set of matrices: 64 x 64 diagonal-block factorizations
+//     (k_diag_tasks, mode 0), in-place panel products with the block inverse
+//     and trailing updates (k_gemm_tiles);
+//   * inverse of a set of lower-triangular matrices: diagonal-block inverses
+//     (mode 1) and recursive halving X21 = -X22 (L21 X11) on MFMA;
+//   * triangular solves and products: GEMM tasks with a triangular operand
+//     (a solve multiplies by the explicit inverse);
+//   * the reversed Cholesky factor of the completion (S = Lr^T Lr): reversed
+//     transposed copy, Cholesky, reversed transposed copy;
+//   * update matrices: k_extract (unfactored) or k_update_factor
+//     (Householder-factored updates), issued at the parent's level before
+//     the parent's own (2,2) block is factored in place.
+// The sweeps run on their own program (work lists, tasks) and scratch; the
+// update arena is the plan's.
+#include "plan.h"
+#include "../../include/chompack_amd.h"
+
+#include <algorithm>
+#include <array>
+#include <functional>
+
+namespace chpi {
+namespace {
+
+struct Mat {
+    int buf;
+    i64 off;
+    int ld, n;
+};
+
+struct Bump {
+    i64 cur = 0;
+    i64 take(i64 len) {
+        i64 o = cur;
+        cur += (std::max<i64>(len, 1) + 31) / 32 * 32;
+        return o;
+    }
+};
+
+template <class T>
+void push_tasks(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vector<T>& v,
+                int bsel = 0) {
+    if (v.empty()) return;
+    Launch L{kid, (int)v.size(), 256, 0, P->pg->lists.add(v), 0};
+    L.bsel = bsel;
+    seq.push_back(L);
+}
+
+void add_matop(std::vector<MatOp>& ops, MatOpKind kind, int sbuf, i64 src, int lds, int dbuf,
+               i64 dst, int ldd, int m, int n, double alpha = 1.0) {
+    for (int j0 = 0; j0 < n; j0 += 64)
+        for (int i0 = 0; i0 < m; i0 += 64) {
+            const bool above = i0 + 63 < j0;  // every element has i < j
+            const bool below = i0 > j0 + 63;  // every element has i > j
+            if (kind == MO_COPY_LOWER && above) continue;
+            if ((kind == MO_SYMM || kind == MO_ZERO_UPPER) && below) continue;
+            MatOp o;
+            std::memset(&o, 0, sizeof(o));
+            o.src = src; o.dst = dst; o.lds = lds; o.ldd = ldd; o.m = m; o.n = n;
+            o.i0 = i0; o.j0 = j0;
+            o.sbuf = (unsigned char)sbuf; o.dbuf = (unsigned char)dbuf;
+            o.kind = (unsigned char)kind; o.alpha = alpha;
+            ops.push_back(o);
+        }
+}
+
+// In-place lower Cholesky of each matrix (right-looking, 64-wide blocks).
+void emit_potrf(chp_plan* P, std::vector<Launch>& seq, const std::vector<Mat>& ms,
+                const std::vector<int>& keys, Bump& dv) {
+    int maxn = 0;
+    std::vector<i64> dvo(ms.size());
+    for (size_t q = 0; q < ms.size(); ++q) {
+        maxn = std::max(maxn, ms[q].n);
+        dvo[q] = dv.take((i64)((ms[q].n + NB - 1) / NB) * NB * NB);
+    }
+    for (int c0 = 0; c0 < maxn; c0 += NB) {
+        std::vector<DiagTask> dt;
+        std::vector<GemmTile> pt, ut;
+        for (size_t q = 0; q < ms.size(); ++q) {
+            const Mat& m = ms[q];
+            if (m.n <= c0) continue;
+            const int c1 = std::min(c0 + NB, m.n);
+            DiagTask t;
+            std::memset(&t, 0, sizeof(t));
+            t.off = m.off + c0 + (i64)c0 * m.ld;
+            t.outoff = dvo[q] + (i64)(c0 / NB) * NB * NB;
+            t.ld = m.ld; t.ldo = NB; t.nb = c1 - c0; t.key = keys[q]; t.col = c0;
+            t.buf = (unsigned char)m.buf; t.obuf = 3; t.mode = 0;
+            dt.push_back(t);
+            if (c1 < m.n) {
+                const i64 pan = m.off + c1 + (i64)c0 * m.ld;
+                // L[c1:, c0:c1] = A[c1:, c0:c1] Dinv^T (in place; Dinv row-major)
+                add_gemm_tiles(P, pt, mk(m.buf, pan, m.ld, 0, 3, t.outoff, NB, 0, m.buf, pan, m.ld,
+                                         m.n - c1, c1 - c0, c1 - c0, 1.0, 0.0, 0), false);
+                // A[c1:, c1:] -= L[c1:, c0:c1] L[c1:, c0:c1]^T (lower)
+                add_gemm_tiles(P, ut, mk(m.buf, pan, m.ld, 0, m.buf, pan, m.ld, 1, m.buf,
+                                         m.off + c1 + (i64)c1 * m.ld, m.ld, m.n - c1, m.n - c1,
+                                         c1 - c0, -1.0, 1.0, 1), true);
+            }
+        }
+        push_tasks(P, seq, K_DIAG, dt);
+        push_gemm(P, seq, pt);
+        push_gemm(P, seq, ut);
+    }
+}
+
+// X = L^{-1} for lower-triangular L (only its lower triangle is read); X is
+// written in full (zero above the diagonal).  W: scratch n x n (ld n) each.
+void emit_trtri(chp_plan* P, std::vector<Launch>& seq, const std::vector<Mat>& Ls,
+                const std::vector<Mat>& Xs, const std::vector<i64>& Ws,
+                const std::vector<int>& keys) {
+    std::vector<MatOp> z;
+    std::vector<DiagTask> dt;
+    for (size_t q = 0; q < Ls.size(); ++q) {
+        const Mat &L = Ls[q], &X = Xs[q];
+        const int n = L.n;
+        for (int j0 = 64; j0 < n; j0 += 64)
+            for (int i0 = 0; i0 < j0; i0 += 64) {
+                MatOp o;
+                std::memset(&o, 0, sizeof(o));
+                o.dst = X.off; o.ldd = X.ld; o.m = n; o.n = n; o.i0 = i0; o.j0 = j0;
+                o.dbuf = (unsigned char)X.buf; o.sbuf = (unsigned char)X.buf; o.kind = MO_ZERO;
+                z.push_back(o);
+            }
+        for (int b = 0; b * NB < n; ++b) {
+            DiagTask t;
+            std::memset(&t, 0, sizeof(t));
+            t.off = L.off + (i64)b * NB * (1 + (i64)L.ld);
+            t.outoff = X.off + (i64)b * NB * (1 + (i64)X.ld);
+            t.ld = L.ld; t.ldo = X.ld; t.nb = std::min(NB, n - b * NB);
+            t.key = keys[q]; t.col = b * NB;
+            t.buf = (unsigned char)L.buf; t.obuf = (unsigned char)X.buf; t.mode = 1;
+            dt.push_back(t);
+        }
+    }
+    push_tasks(P, seq, K_MATOP, z);
+    push_tasks(P, seq, K_DIAG, dt);
+    std::vector<std::vector<std::array<int, 4>>> byh;  // (q, a, m, b) in blocks
+    std::function<int(int, int, int)> tree = [&](int q, int a, int b) -> int {
+        if (b - a <= 1) return 0;
+        int m = (a + b) / 2;
+        int h = 1 + std::max(tree(q, a, m), tree(q, m, b));
+        if ((int)byh.size() < h) byh.resize(h);
+        byh[h - 1].push_back({q, a, m, b});
+        return h;
+    };
+    for (size_t q = 0; q < Ls.size(); ++q) tree((int)q, 0, (Ls[q].n + NB - 1) / NB);
+    for (auto& nodes : byh) {
+        std::vector<GemmTile> g1, g2;
+        for (auto& nd : nodes) {
+            const Mat &L = Ls[nd[0]], &X = Xs[nd[0]];
+            const int n = L.n;
+            const i64 W = Ws[nd[0]];
+            const int r1 = nd[1] * NB, r2 = nd[2] * NB, r3 = std::min(nd[3] * NB, n);
+            const int m1 = r2 - r1, m2 = r3 - r2;
+            // W[r2:r3, r1:r2] = L[r2:r3, r1:r2] X[r1:r2, r1:r2]   (X11 lower)
+            add_gemm_tiles(P, g1, mk(L.buf, L.off + r2 + (i64)r1 * L.ld, L.ld, 0,
+                                     X.buf, X.off + r1 + (i64)r1 * X.ld, X.ld, 0,
+                                     2, W + r2 + (i64)r1 * n, n, m2, m1, m1, 1.0, 0.0, 0, 0, 1),
+                           false);
+            // X[r2:r3, r1:r2] = -X[r2:r3, r2:r3] W[r2:r3, r1:r2]   (X22 lower)
+            add_gemm_tiles(P, g2, mk(X.buf, X.off + r2 + (i64)r2 * X.ld, X.ld, 0,
+                                     2, W + r2 + (i64)r1 * n, n, 0,
+                                     X.buf, X.off + r2 + (i64)r1 * X.ld, X.ld, m2, m1, m2,
+                                     -1.0, 0.0, 0, 1, 0), false);
+        }
+        push_gemm(P, seq, g1);
+        push_gemm(P, seq, g2);
+    }
+}
+
+std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
+    std::vector<int> ch;
+    for (int k : fr)
+        for (int l = P->fi[k].chbeg; l < P->fi[k].chend; ++l) ch.push_back(P->h_chidx[l]);
+    return ch;
+}
+
+// Update matrices of the children of the fronts `fr`, read from the fronts
+// [blk (buffer fbuf); V or its factor (update arena)].
+void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
+                        bool factored, int fbuf, Bump& sc) {
+    const std::vector<int> ch = children_of(P, fr);
+    if (ch.empty()) return;
+    if (!factored) {
+        std::vector<int2> xt;
+        for (int c : ch)
+            for (int j0 = 0; j0 < P->fi[c].na; j0 += 16) xt.push_back(make_int2(c, j0));
+        push_tasks(P, seq, K_EXTRACT, xt, fbuf);
+        return;
+    }
+    std::vector<UFTask> ut;
+    for (int c : ch) {
+        const FrontInfo& g = P->fi[c];
+        const FrontInfo& f = P->fi[g.parent];
+        int n1 = 0;
+        while (n1 < g.na && P->h_relidx[g.relptr + n1] < f.nn) ++n1;
+        UFTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.foff = f.blkptr; t.lkoff = f.updptr; t.outoff = g.updptr;
+        t.woff = sc.take((i64)f.na * (g.na - n1));
+        t.relpos = g.relptr;
+        t.nn = f.nn; t.na = f.na; t.N = g.na; t.key = g.parent;
+        t.fbuf = (unsigned char)fbuf;
+        ut.push_back(t);
+    }
+    push_tasks(P, seq, K_UPDATE_FACTOR, ut);
+}
+
+// One level of completion (completion.c:24-118), root to leaves.
+void build_completion_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
+                            bool fu, Bump& sc, Bump& dv) {
+    emit_child_updates(P, seq, fr, fu, 0, sc);
+    std::vector<int> fa;
+    for (int k : fr) if (P->fi[k].na > 0) fa.push_back(k);
+    if (!fa.empty()) {
+        std::vector<Mat> V, Xv;
+        std::vector<i64> Wv, Zo;
+        for (int k : fa) {
+            const FrontInfo& f = P->fi[k];
+            V.push_back({1, f.updptr, f.na, f.na});
+            Xv.push_back({2, sc.take((i64)f.na * f.na), f.na, f.na});
+            Wv.push_back(sc.take((i64)f.na * f.na));
+            Zo.push_back(sc.take((i64)f.na * f.nn));
+        }
+        if (!fu) emit_potrf(P, seq, V, fa, dv);  // V_k = Lv Lv^T
+        emit_trtri(P, seq, V, Xv, Wv, fa);
+        std::vector<GemmTile> gz, gs;
+        for (size_t q = 0; q < fa.size(); ++q) {
+            const FrontInfo& f = P->fi[fa[q]];
+            const int nn = f.nn, na = f.na, nj = nn + na;
+            // Z = Lv^{-1} X21 (unfactored) or Lk^{-T} X21 (factored, V_k = Lk^T Lk)
+            add_gemm_tiles(P, gz, mk(2, Xv[q].off, na, fu ? 1 : 0, 0, f.blkptr + nn, nj, 0,
+                                     2, Zo[q], na, na, nn, na, 1.0, 0.0, 0, fu ? 2 : 1), false);
+            // X11 -= Z^T Z (lower) and X21 = -V_k^{-1} X21
+            add_gemm_tiles(P, gs, mk(2, Zo[q], na, 1, 2, Zo[q], na, 0, 0, f.blkptr, nj,
+                                     nn, nn, na, -1.0, 1.0, 1), true);
+            add_gemm_tiles(P, gs, mk(2, Xv[q].off, na, fu ? 0 : 1, 2, Zo[q], na, 0,
+                                     0, f.blkptr + nn, nj, na, nn, na, -1.0, 0.0, 0,
+                                     fu ? 1 : 2), false);
+        }
+        push_gemm(P, seq, gz);
+        push_gemm(P, seq, gs);
+    }
+    // L11 = Lr^{-1} with S11 = Lr^T Lr (completion.c:93-116)
+    std::vector<MatOp> rt1, rt2, cp;
+    std::vector<Mat> T, T2, X11;
+    std::vector<i64> W11;
+    std::vector<int> keys;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        const int nn = f.nn, nj = nn + f.na;
+        const i64 t1 = sc.take((i64)nn * nn), t2 = sc.take((i64)nn * nn);
+        const i64 x = sc.take((i64)nn * nn), w = sc.take((i64)nn * nn);
+        T.push_back({2, t1, nn, nn});
+        T2.push_back({2, t2, nn, nn});
+        X11.push_back({2, x, nn, nn});
+        W11.push_back(w);
+        keys.push_back(k);
+        add_matop(rt1, MO_RT, 0, f.blkptr, nj, 2, t1, nn, nn, nn);
+        add_matop(rt2, MO_RT, 2, t1, nn, 2, t2, nn, nn, nn);
+        add_matop(cp, MO_COPY_LOWER, 2, x, nn, 0, f.blkptr, nj, nn, nn);
+    }
+    push_tasks(P, seq, K_MATOP, rt1);
+    emit_potrf(P, seq, T, keys, dv);
+    push_tasks(P, seq, K_MATOP, rt2);
+    emit_trtri(P, seq, T2, X11, W11, keys);
+    push_tasks(P, seq, K_MATOP, cp);
+}
+
+// One level of the scaling map (hessian.c:201-352): shared part (update
+// matrices, factors, inverses) and the per-direction part.
+void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>& dir,
+                       const std::vector<int>& fr, bool fu, bool inv, bool adj, Bump& sc,
+                       Bump& dv) {
+    emit_child_updates(P, sh, fr, fu, 6, sc);
+    std::vector<int> fa;
+    for (int k : fr) if (P->fi[k].na > 0) fa.push_back(k);
+    std::vector<Mat> V, Xv;
+    std::vector<i64> Wv;
+    for (int k : fa) {
+        const FrontInfo& f = P->fi[k];
+        V.push_back({1, f.updptr, f.na, f.na});
+        if (inv) {
+            Xv.push_back({2, sc.take((i64)f.na * f.na), f.na, f.na});
+            Wv.push_back(sc.take((i64)f.na * f.na));
+        }
+    }
+    if (!fa.empty() && !fu) emit_potrf(P, sh, V, fa, dv);
+    if (!fa.empty() && inv) emit_trtri(P, sh, V, Xv, Wv, fa);
+    // M1 = L11^{-1} (trsm, !inv) or L11 itself (trmm, inv)
+    std::vector<Mat> L11, X11;
+    std::vector<i64> W11;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        L11.push_back({5, f.blkptr, f.nn + f.na, f.nn});
+        if (!inv) {
+            X11.push_back({2, sc.take((i64)f.nn * f.nn), f.nn, f.nn});
+            W11.push_back(sc.take((i64)f.nn * f.nn));
+        }
+    }
+    if (!inv) emit_trtri(P, sh, L11, X11, W11, std::vector<int>(fr.begin(), fr.end()));
+    // per direction: U11 symmetric, Tu = U op1(M1), U11 = op2(M1) Tu11 (upper
+    // zeroed), U21 = op3(M3) Tu21
+    std::vector<MatOp> sy;
+    std::vector<GemmTile> g1, g2;
+    const bool t1 = !adj, t2 = adj;                       // op1 / op2 transposed
+    const bool t3 = fu ? adj : !adj;                      // op3 transposed
+    size_t qa = 0;
+    for (size_t q = 0; q < fr.size(); ++q) {
+        const int k = fr[q];
+        const FrontInfo& f = P->fi[k];
+        const int nn = f.nn, na = f.na, nj = nn + na;
+        const Mat M1 = inv ? L11[q] : X11[q];
+        const i64 tu = sc.take((i64)nj * nn);
+        add_matop(sy, MO_SYMM, 0, f.blkptr, nj, 0, f.blkptr, nj, nn, nn);
+        add_gemm_tiles(P, g1, mk(0, f.blkptr, nj, 0, M1.buf, M1.off, M1.ld, t1 ? 1 : 0,
+                                 2, tu, nj, nj, nn, nn, 1.0, 0.0, 0, 0, t1 ? 2 : 1), false);
+        add_gemm_tiles(P, g2, mk(M1.buf, M1.off, M1.ld, t2 ? 1 : 0, 2, tu, nj, 0,
+                                 0, f.blkptr, nj, nn, nn, nn, 1.0, 0.0, 2, t2 ? 2 : 1), false);
+        if (na > 0) {
+            const Mat M3 = inv ? Xv[qa] : V[qa];
+            add_gemm_tiles(P, g2, mk(M3.buf, M3.off, M3.ld, t3 ? 1 : 0, 2, tu + nn, nj, 0,
+                                     0, f.blkptr + nn, nj, na, nn, na, 1.0, 0.0, 0, t3 ? 2 : 1),
+                           false);
+            ++qa;
+        }
+    }
+    push_tasks(P, dir, K_MATOP, sy);
+    push_gemm(P, dir, g1);
+    push_gemm(P, dir, g2);
+}
+
+// One level of _Y2K (hessian.c:3-101), leaves to root, for one direction.
+void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv) {
+    const double alpha = inv ? 1.0 : -1.0;
+    std::vector<int2> et;
+    for (int k : fr) {
+        const int nj = P->fi[k].nn + P->fi[k].na;
+        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
+    }
+    if (!inv) {
+        Launch L{K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1};
+        seq.push_back(L);
+    } else {
+        std::vector<MatOp> z;
+        for (int k : fr) {
+            const FrontInfo& f = P->fi[k];
+            if (f.na > 0) add_matop(z, MO_ZERO, 1, f.updptr, f.na, 1, f.updptr, f.na, f.na, f.na);
+        }
+        push_tasks(P, seq, K_MATOP, z);
+    }
+    std::vector<GemmTile> g1, g2, g3;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        const int nn = f.nn, na = f.na, nj = nn + na;
+        if (na == 0) continue;
+        const i64 b21 = f.blkptr + nn;
+        // F22 += alpha L21 F21^T
+        add_gemm_tiles(P, g1, mk(5, b21, nj, 0, 0, b21, nj, 1, 1, f.updptr, na,
+                                 na, na, nn, alpha, 1.0, 1), true);
+        // F21 += alpha L21 sym(F11)
+        add_gemm_tiles(P, g2, mk(5, b21, nj, 0, 0, f.blkptr, nj, 2, 0, b21, nj,
+                                 na, nn, nn, alpha, 1.0, 0), false);
+        // F22 += alpha F21 L21^T
+        add_gemm_tiles(P, g3, mk(0, b21, nj, 0, 5, b21, nj, 1, 1, f.updptr, na,
+                                 na, na, nn, alpha, 1.0, 1), true);
+    }
+    push_gemm(P, seq, g1);
+    push_gemm(P, seq, g2);
+    push_gemm(P, seq, g3);
+    if (inv) {
+        Launch L{K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0};
+        seq.push_back(L);
+    }
+}
+
+// One level of _M2T (hessian.c:103-198), root to leaves, for one direction.
+void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv,
+                     Bump& sc) {
+    const double alpha = inv ? 1.0 : -1.0;
+    if (inv) {
+        emit_child_updates(P, seq, fr, false, 0, sc);  // before this level's update
+    } else {
+        std::vector<int2> xt;  // this level's V from the (finished) parents
+        for (int k : fr)
+            for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
+        push_tasks(P, seq, K_EXTRACT, xt, 0);
+    }
+    std::vector<GemmTile> g1, g2, g3;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        const int nn = f.nn, na = f.na, nj = nn + na;
+        if (na == 0) continue;
+        const i64 b21 = f.blkptr + nn;
+        // F11 += alpha F21^T L21 (lower)
+        add_gemm_tiles(P, g1, mk(0, b21, nj, 1, 5, b21, nj, 0, 0, f.blkptr, nj,
+                                 nn, nn, na, alpha, 1.0, 1), true);
+        // F21 += alpha V_k L21 (V_k stored with both triangles)
+        add_gemm_tiles(P, g2, mk(1, f.updptr, na, 0, 5, b21, nj, 0, 0, b21, nj,
+                                 na, nn, na, alpha, 1.0, 0), false);
+        // F11 += alpha L21^T F21 (lower)
+        add_gemm_tiles(P, g3, mk(5, b21, nj, 1, 0, b21, nj, 0, 0, f.blkptr, nj,
+                                 nn, nn, na, alpha, 1.0, 1), true);
+    }
+    push_gemm(P, seq, g1);
+    push_gemm(P, seq, g2);
+    push_gemm(P, seq, g3);
+}
+
+int ensure_dense(chp_plan* P, hipStream_t s) {
+    if (sync_program(P->dense)) return -2;
+    struct { double** d; i64* have; i64 need; } bufs[] = {
+        {&P->d_dscratch, &P->dscratch_cap, P->dscratch_len},
+        {&P->d_ddinv, &P->ddinv_cap, P->ddinv_len},
+        {&P->d_dsplitp, &P->dsplitp_cap, P->dense.splitp_len},
+    };
+    for (auto& b : bufs) {
+        if (*b.d && b.need <= *b.have) continue;
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (*b.d) HIP_CHECK(hipFree(*b.d));
+        *b.d = nullptr;
+        if (alloc_doubles(b.d, b.need)) return -2;
+        *b.have = std::max<i64>(b.need, 1);
+    }
+    return 0;
+}
+
+// Builds the requested sweeps on first use; the needs (scratch, inverses)
+// grow the plan's dense buffers.
+struct DenseBuild {
+    chp_plan* P;
+    i64 sc_need = 0, dv_need = 0;
+    explicit DenseBuild(chp_plan* p) : P(p) { P->pg = &P->dense; }
+    ~DenseBuild() { P->pg = &P->main; }
+    void level_done(const Bump& sc, const Bump& dv) {
+        sc_need = std::max(sc_need, sc.cur);
+        dv_need = std::max(dv_need, dv.cur);
+    }
+};
+
+void note_needs(chp_plan* P, const DenseBuild& b, i64& sc_req, i64& dv_req) {
+    sc_req = std::max(sc_req, b.sc_need);
+    dv_req = std::max(dv_req, b.dv_need);
+}
+
+}  // namespace
+
+int build_completion(chp_plan* P, int fu, i64& sc_req, i64& dv_req) {
+    DenseBuild b(P);
+    auto& seq = P->comp[fu].seq;
+    for (int L = P->nlevels - 1; L >= 0; --L) {
+        Bump sc, dv;
+        build_completion_level(P, seq, P->bylevel[L], fu != 0, sc, dv);
+        b.level_done(sc, dv);
+    }
+    P->comp[fu].built = true;
+    note_needs(P, b, sc_req, dv_req);
+    return 0;
+}
+
+int build_scale(chp_plan* P, int idx, i64& sc_req, i64& dv_req) {
+    DenseBuild b(P);
+    const bool fu = idx & 4, inv = idx & 2, adj = idx & 1;
+    auto& S = P->scale[idx];
+    S.shared.assign(P->nlevels, {});
+    S.dir.assign(P->nlevels, {});
+    for (int L = P->nlevels - 1; L >= 0; --L) {
+        Bump sc, dv;
+        build_scale_level(P, S.shared[L], S.dir[L], P->bylevel[L], fu, inv, adj, sc, dv);
+        b.level_done(sc, dv);
+    }
+    S.built = true;
+    note_needs(P, b, sc_req, dv_req);
+    return 0;
+}
+
+int build_y2k(chp_plan* P, int inv) {
+    DenseBuild b(P);
+    for (int L = 0; L < P->nlevels; ++L) build_y2k_level(P, P->y2k[inv].seq, P->bylevel[L], inv != 0);
+    P->y2k[inv].built = true;
+    return 0;
+}
+
+int build_m2t(chp_plan* P, int inv, i64& sc_req) {
+    DenseBuild b(P);
+    for (int L = P->nlevels - 1; L >= 0; --L) {
+        Bump sc, dv;
+        build_m2t_level(P, P->m2t[inv].seq, P->bylevel[L], inv != 0, sc);
+        b.level_done(sc, dv);
+    }
+    P->m2t[inv].built = true;
+    sc_req = std::max(sc_req, b.sc_need);
+    return 0;
+}
+
+BufTab dense_tab(chp_plan* P, double* x, const double* L, const double* Y) {
+    BufTab t;
+    t.p[0] = x; t.p[1] = P->d_upd; t.p[2] = P->d_dscratch; t.p[3] = P->d_ddinv;
+    t.p[4] = P->d_dsplitp; t.p[5] = const_cast<double*>(L); t.p[6] = const_cast<double*>(Y);
+    t.p[7] = nullptr;
+    return t;
+}
+
+}  // namespace chpi
+
+using namespace chpi;
+
+namespace {
+// Requested dense buffer sizes (scratch, inverses) before allocation.
+int prepare(chp_plan* P, i64 sc_req, i64 dv_req, hipStream_t s) {
+    P->dscratch_len = std::max(P->dscratch_len, sc_req);
+    P->ddinv_len = std::max(P->ddinv_len, dv_req);
+    return ensure_dense(P, s);
+}
+}  // namespace
+
+extern "C" {
+
+int chp_completion(chp_plan* P, double* blkval, int factored, void* stream, i64* info) {
+    if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    const int fu = factored ? 1 : 0;
+    i64 sc = 0, dv = 0;
+    if (!P->comp[fu].built) build_completion(P, fu, sc, dv);
+    if (prepare(P, sc, dv, s)) return -2;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    int rc = run_seq(P, P->dense, P->comp[fu].seq, dense_tab(P, blkval, nullptr, nullptr), s);
+    if (rc) return rc;
+    return read_status(P, s, info);
+}
+
+int chp_hessian(chp_plan* P, const double* lblkval, const double* yblkval, double* const* ublkval,
+                int nu, int adj, int inv, int factored, void* stream, i64* info) {
+    if (!P || !lblkval || !yblkval || (nu > 0 && !ublkval)) {
+        chp_set_error("null argument");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int fu = factored ? 1 : 0, iv = inv ? 1 : 0;
+    const int a0 = adj < 0 ? iv : (adj ? 1 : 0);
+    const int npass = adj < 0 ? 2 : 1;
+    i64 sc = 0, dv = 0;
+    for (int pass = 0; pass < npass; ++pass) {
+        const int a = pass == 0 ? a0 : 1 - a0;
+        const int idx = 4 * fu + 2 * iv + a;
+        if (!P->scale[idx].built) build_scale(P, idx, sc, dv);
+        if (a) { if (!P->m2t[iv].built) build_m2t(P, iv, sc); }
+        else if (!P->y2k[iv].built) build_y2k(P, iv);
+    }
+    if (prepare(P, sc, dv, s)) return -2;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    auto run_scale = [&](int idx) -> int {
+        const auto& S = P->scale[idx];
+        for (int L = P->nlevels - 1; L >= 0; --L) {
+            int rc = run_seq(P, P->dense, S.shared[L], dense_tab(P, nullptr, lblkval, yblkval), s);
+            if (rc) return rc;
+            for (int u = 0; u < nu; ++u) {
+                rc = run_seq(P, P->dense, S.dir[L], dense_tab(P, ublkval[u], lblkval, yblkval), s);
+                if (rc) return rc;
+            }
+        }
+        return 0;
+    };
+    for (int pass = 0; pass < npass; ++pass) {
+        const int a = pass == 0 ? a0 : 1 - a0;
+        const int idx = 4 * fu + 2 * iv + a;
+        int rc = 0;
+        if (a != iv && (rc = run_scale(idx))) return rc;
+        const auto& seq = a ? P->m2t[iv].seq : P->y2k[iv].seq;
+        for (int u = 0; u < nu; ++u)
+            if ((rc = run_seq(P, P->dense, seq, dense_tab(P, ublkval[u], lblkval, yblkval), s)))
+                return rc;
+        if (a == iv && (rc = run_scale(idx))) return rc;
+    }
+    // a failed update factorization is reported in info but, as in the
+    // reference (hessian.c:354-374 returns 0), is not an error
+    read_status(P, s, info);
+    return 0;
+}
+
+}  // extern "C"

@@ -130,13 +130,70 @@ def projected_inverse(L):
 
 
 def completion(X, factored_updates=True):
-    """Maximum-determinant positive definite completion (cbase.c:521-621)."""
-    raise NotImplementedError("completion is not available in this build yet")
+    """Maximum-determinant positive definite completion, in place
+    (cbase.c:521-621, completion.c:3-120): on exit X holds the Cholesky
+    factor L of the inverse of the completion, P((L L^T)^{-1}) = X.
+    ``factored_updates`` selects the Householder-factored update matrices
+    (update_factor.c) or plain extraction plus a Cholesky per front.  As in
+    the reference, X is marked a factor even when the completion fails."""
+    if not (isinstance(X, cspmatrix) and X.is_factor is False):
+        raise ValueError("X must be a cspmatrix")
+    plan, b = _plan(X)
+    info = np.zeros(2, dtype=np.int64)
+    rc = _native.lib().chp_completion(plan.handle, b.data_ptr(), 1 if factored_updates else 0,
+                                      _stream(b), _native.ptr(info))
+    X.is_factor = True
+    _raise_rc(rc, "completion")
+    if rc == 1:
+        raise ArithmeticError("completion failed")
 
 
 def hessian(L, Y, U, adj=False, inv=False, factored_updates=True):
-    """Hessian of the log-det barrier and its factors (cbase.c:675-857)."""
-    raise NotImplementedError("hessian is not available in this build yet")
+    """Factors of the Hessian of the log-det barrier, applied in place to U
+    (a cspmatrix or a list of them) (cbase.c:675-857, hessian.c:3-374):
+
+    ======  =====  ==============================
+    adj     inv    U :=
+    ======  =====  ==============================
+    False   False  G_X(U)
+    True    False  G_X^adj(U)
+    False   True   G_X^{-1}(U)
+    True    True   (G_X^adj)^{-1}(U)
+    None    False  G_X^adj(G_X(U)) = P(X^{-1} U X^{-1})
+    None    True   the inverse of the above
+    ======  =====  ==============================
+
+    L is the factor of X and Y = P(X^{-1}); all arguments must share one
+    symbolic object.  ``adj`` and ``inv`` default to False (the reference's
+    C wrapper leaves them uninitialized when omitted, cbase.c:701-713)."""
+    torch = _torch()
+    syms = [Y.symb] if isinstance(Y, cspmatrix) else [None]
+    if isinstance(U, list):
+        Us = U
+    else:
+        Us = [U]
+    syms += [getattr(u, "symb", None) for u in Us]
+    if any(s is not L.symb for s in syms):
+        raise ValueError("symbolic factorizations must be the same")
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("L must be a cspmatrix factor")
+    if not (isinstance(Y, cspmatrix) and Y.is_factor is False):
+        raise ValueError("Y must be a cspmatrix")
+    plan, lb = _plan(L)
+    yb = _check_blkval(Y)
+    ubs = [_check_blkval(u) for u in Us]
+    for t in [yb] + ubs:
+        if t.device != lb.device:
+            raise ValueError("all blkval arrays must be on the same device")
+    arr = (ctypes.c_void_p * max(len(ubs), 1))(*[u.data_ptr() for u in ubs])
+    a = -1 if adj is None else (1 if adj else 0)
+    info = np.zeros(2, dtype=np.int64)
+    rc = _native.lib().chp_hessian(plan.handle, lb.data_ptr(), yb.data_ptr(),
+                                   ctypes.cast(arr, ctypes.c_void_p), len(ubs), a,
+                                   1 if inv else 0, 1 if factored_updates else 0,
+                                   _stream(lb), _native.ptr(info))
+    _raise_rc(rc, "hessian")
+    del torch
 
 
 def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):

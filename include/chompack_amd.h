@@ -79,10 +79,11 @@ int chp_set_graphs(chp_plan* plan, int on);
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
  * (synchronizing after each launch) and clears the counters;
- * chp_profile_read fills ms/cnt/flops[14] per kernel kind (0 chol_small,
+ * chp_profile_read fills ms/cnt/flops[17] per kernel kind (0 chol_small,
  * 1 extadd, 2 potrf_diag, 3 unused, 4 gemm, 5 extract, 6 projinv_small,
  * 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd, 11 solve_bwd,
- * 12 copy_top, 13 splitk_reduce) and returns 14.  chp_plan_flops: algorithmic GEMM flops of one cholesky
+ * 12 copy_top, 13 splitk_reduce, 14 diag, 15 matop, 16 update_factor) and
+ * returns 17.  chp_plan_flops: algorithmic GEMM flops of one cholesky
  * (out[0]) and one projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
 int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);
@@ -106,6 +107,24 @@ int chp_llt(chp_plan* plan, double* blkval, void* stream);
 /* Projected inverse Y = P(X^{-1}) from the factor; replaces
  * projected_inverse() (projected_inverse.c:3-75, cbase.c:470-474). */
 int chp_projected_inverse(chp_plan* plan, double* blkval, void* stream, int64_t* info);
+
+/* Maximum-determinant positive definite completion: on exit blkval holds the
+ * factor L with P((L L^T)^{-1}) = X; replaces completion() (completion.c:3-120,
+ * update_factor.c:3-108) as called by cbase.c:521-621.  factored != 0 selects
+ * Householder-factored update matrices.  Returns 1 on a failed factorization
+ * (info as for chp_cholesky). */
+int chp_completion(chp_plan* plan, double* blkval, int factored, void* stream, int64_t* info);
+
+/* Hessian factors of the log-det barrier applied in place to nu device blkval
+ * arrays ublkval[0..nu-1]; lblkval is the factor L, yblkval the projected
+ * inverse Y.  adj: 0 G, 1 G^adj, -1 both (G^adj G, the reference's adj=None);
+ * inv != 0 applies the inverses.  Replaces hessian() (hessian.c:3-374) as
+ * called by cbase.c:675-857, including its second pass for adj=None
+ * (cbase.c:833-842).  A failed update factorization is reported in info but,
+ * as in the reference, not as an error. */
+int chp_hessian(chp_plan* plan, const double* lblkval, const double* yblkval,
+                double* const* ublkval, int nu, int adj, int inv, int factored,
+                void* stream, int64_t* info);
 
 /* B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B (trans 'T'), B a device
  * column-major n x nrhs array with leading dimension ldb in the original

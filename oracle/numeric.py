@@ -221,13 +221,21 @@ def logdet(symb, lblkval):
     return 2.0 * s
 
 
+PIVOT_NOISE = 1e-13
+
+
 def _larfg(alpha, x):
     """Householder reflector H = I - tau [1; v][1; v]^T with
     H [alpha; x] = [beta; 0], beta = -sign(alpha) ||[alpha; x]|| (LAPACK dlarfg).
-    Returns (tau, beta, v)."""
+    A pivot with |alpha| <= PIVOT_NOISE * ||x|| is taken as +0 (beta = -||x||),
+    the exact-arithmetic value of pivots that vanish structurally: their
+    computed sign is rounding noise, which would otherwise decide the sign of
+    a row of the factor (see update_factor).  Returns (tau, beta, v)."""
     xnorm = float(np.linalg.norm(x))
     if xnorm == 0.0:
         return 0.0, alpha, x.copy()
+    if abs(alpha) <= PIVOT_NOISE * xnorm:
+        alpha = 0.0
     beta = -np.copysign(np.hypot(alpha, xnorm), alpha)
     tau = (beta - alpha) / beta
     return tau, beta, x * (1.0 / (alpha - beta))
@@ -248,7 +256,13 @@ def update_factor(r, nn, na, F):
     The separator part C = Lk[:, r2 - nn] is reduced to lower-triangular form
     by Householder reflections from the last column, Li21 solves
     Li22^T Li21 = B, and Li11 is the reversed Cholesky factor of
-    A - Li21^T Li21.  Returns Li (N x N, zero above the diagonal)."""
+    A - Li21^T Li21.  Returns Li (N x N, zero above the diagonal).
+
+    Li is unique up to the signs of the rows of Li22.  Li^T Li (all that the
+    completion and the full Hessian map use) does not depend on them, but the
+    Hessian factors G_X, G_X^adj with factored updates do; the reflector
+    pivots can vanish structurally, and then the reference's signs follow
+    the rounding of its BLAS.  Here such pivots count as +0 (_larfg)."""
     r = np.asarray(r)
     N = len(r)
     N1 = int(np.sum(r < nn))

@@ -140,3 +140,67 @@ def test_graph_replays_match_eager():
         torch.cuda.synchronize()
         assert np.array_equal(X.blkval.cpu().numpy(), eager), rep
         _ = torch.ones(1000, device="cuda:0").sum().item()  # eager work between replays
+
+
+def _lower_mask(symb):
+    m = np.zeros(int(symb.blkptr[-1]), dtype=bool)
+    for k in range(symb.Nsn):
+        nn = int(symb.snptr[k + 1] - symb.snptr[k])
+        nj = nn + int(symb.relptr[k + 1] - symb.relptr[k])
+        blk = m[symb.blkptr[k]:symb.blkptr[k + 1]].reshape((nn, nj)).T
+        blk[np.tril_indices(nj, 0, nn)] = True
+    return m
+
+
+def _factor_pair(name):
+    """Oracle factor L and projected inverse Y of a test problem."""
+    A = PROBLEMS[name]()
+    symb = cp.symbolic(A, p=ORDER.get(name, cp.amd.order))
+    b = (cp.cspmatrix(symb, device="cpu") + A).blkval.numpy().copy()
+    L = b.copy()
+    assert on.cholesky(symb, L) == 0
+    Y = L.copy()
+    assert on.projected_inverse(symb, Y) == 0
+    return symb, L, Y
+
+
+@pytest.mark.parametrize("factored", [True, False])
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_completion(name, factored):
+    import torch
+    print("case completion", name, factored, flush=True)
+    symb, L, Y = _factor_pair(name)
+    X = cp.cspmatrix(symb, blkval=torch.as_tensor(Y.copy(), device="cuda:0"))
+    cp.completion(X, factored_updates=factored)
+    assert X.is_factor is True
+    ref = Y.copy()
+    assert on.completion(symb, ref, factored) == 0
+    m = _lower_mask(symb)
+    got = X.blkval.cpu().numpy()
+    e = _rel(got[m], ref[m])
+    print("completion rel err %.3e" % e, flush=True)
+    assert e <= TOL
+    assert _rel(got[m], L[m]) <= 1e-11   # completion(P(X^-1)) recovers the factor
+
+
+@pytest.mark.parametrize("factored", [True, False])
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_hessian_modes(name, factored):
+    import torch
+    symb, L, Y = _factor_pair(name)
+    Lc = cp.cspmatrix(symb, blkval=torch.as_tensor(L, device="cuda:0"), factor=True)
+    Yc = cp.cspmatrix(symb, blkval=torch.as_tensor(Y, device="cuda:0"))
+    rng = np.random.default_rng(11)
+    for adj in (False, True, None):
+        for inv in (False, True):
+            print("case hessian", name, factored, adj, inv, flush=True)
+            us = [rng.standard_normal(len(L)) for _ in range(2)]
+            Us = [cp.cspmatrix(symb, blkval=torch.as_tensor(u.copy(), device="cuda:0")) for u in us]
+            cp.hessian(Lc, Yc, Us, adj=adj, inv=inv, factored_updates=factored)
+            refs = [u.copy() for u in us]
+            on.hessian(symb, L, Y, refs, adj=adj, inv=inv, factored_updates=factored)
+            m = _lower_mask(symb)
+            for U, r in zip(Us, refs):
+                e = _rel(U.blkval.cpu().numpy()[m], r[m])
+                print("hessian rel err %.3e" % e, flush=True)
+                assert e <= TOL, (adj, inv)

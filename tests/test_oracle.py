@@ -253,3 +253,28 @@ def test_completion_hessian_closed_forms(maker):
         lhs = np.sum(_dense_sym(symb, GU) * _dense_sym(symb, V0))
         rhs = np.sum(_dense_sym(symb, U0) * _dense_sym(symb, GaV))
         assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1.0)
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
+def test_factored_hessian_sign_quirk():
+    """With factored updates the reference's G_X depends on the signs of
+    Householder pivots that vanish structurally (|alpha| ~ 1e-23 ||x|| on a
+    40 x 40 grid), i.e. on BLAS rounding; the oracle takes such pivots as +0.
+    G_X is then the reference's up to row signs of the update factors, so
+    every sign-invariant quantity agrees: <G(U), G(V)>, G^adj G and G^{-1} G."""
+    A = problems.grid2d(40, seed=0)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    b, L, Y = _factor_and_inverse(symb, A)
+    rng = np.random.default_rng(2)
+    U0, V0 = rng.standard_normal(len(b)), rng.standard_normal(len(b))
+    gr = [U0.copy(), V0.copy()]
+    go = [U0.copy(), V0.copy()]
+    ref.hessian(symb, L, Y, gr, adj=False, factored_updates=True)
+    on.hessian(symb, L, Y, go, adj=False, factored_updates=True)
+    d = lambda x, y: on.dot(symb, x, y)
+    assert abs(d(*gr) - d(*go)) < 1e-11 * abs(d(*gr))
+    assert abs(d(gr[0], gr[0]) - d(go[0], go[0])) < 1e-11 * d(gr[0], gr[0])
+    hr, ho = [U0.copy()], [U0.copy()]
+    ref.hessian(symb, L, Y, hr, adj=None, factored_updates=True)
+    on.hessian(symb, L, Y, ho, adj=None, factored_updates=True)
+    assert _rel(ho[0], hr[0]) < 1e-12
