@@ -13,12 +13,14 @@ from .misc import tril, triu, symmetrize, perm, eye, lmerge, \
 from . import amd
 
 try:
-    from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, dot
+    from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, dot, logdet
 except ImportError:  # numeric module not built into this checkout yet
     pass
+from . import autograd
 
 __version__ = "0.1.0"
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
-           "completion", "projected_inverse", "hessian", "trsm", "dot", "tril", "triu",
+           "completion", "projected_inverse", "hessian", "trsm", "dot", "logdet", "autograd",
+           "tril", "triu",
            "symmetrize", "perm", "eye", "lmerge", "amd"]

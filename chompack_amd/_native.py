@@ -29,6 +29,12 @@ def lib():
             raise RuntimeError(
                 "libchompack_amd.so not found at %s; build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'`" % LIBPATH)
+        # torch's HIP runtime (libamdhip64 / libhsa-runtime64 shipped with
+        # torch) must be loaded first: the library's NEEDED entries then
+        # resolve to that same runtime by soname, so device pointers, streams
+        # and the device context are shared with torch.  Loaded the other
+        # way round, the process would hold two HIP runtimes.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIBPATH)
         _declare(L)
         _lib = L
@@ -61,11 +67,11 @@ def _declare(L):
     _opt(L, "chp_projected_inverse", [vp, vp, vp, vp], i32)
     _opt(L, "chp_trsm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
     _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)
+    _opt(L, "chp_logdet", [vp, vp, vp, vp], i32)
     _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
     _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
     _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)
     _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)
-    _opt(L, "chp_set_mode", [vp, i32], i32)
     _opt(L, "chp_profile", [vp, i32], i32)
     _opt(L, "chp_set_graphs", [vp, i32], i32)
     _opt(L, "chp_profile_read", [vp, vp, vp, vp], i32)

@@ -53,6 +53,8 @@ __global__ void k_solve_bwd(const int* fronts, const FrontInfo* fi, const double
 __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,
                               const int* colfront, const double* x, const double* y,
                               double* partial);
+__global__ void k_logdiag(const int* cols, int ncols, const FrontInfo* fi, const int* colfront,
+                          const double* x, double* partial);
 __global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,
                               i64 fr, i64 fc, i64 ur, i64 uc, double alpha);
 __global__ void k_frontal_get(const double* F, i64 N, const i64* r, double* U,

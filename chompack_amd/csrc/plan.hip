@@ -16,6 +16,7 @@
 #include "../../include/chompack_amd.h"
 
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstring>
 #include <functional>
@@ -868,6 +869,23 @@ int chp_dot(chp_plan* P, const double* x, const double* y, double* out, void* st
     for (i64 j = 0; j < P->n; ++j) v += h[j];
     *out = v;
     return 0;
+}
+
+int chp_logdet(chp_plan* P, const double* lblkval, double* out, void* stream) {
+    if (!P || !lblkval || !out) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    if (P->n == 0) { *out = 0.0; return 0; }
+    hipLaunchKernelGGL(k_logdiag, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0, s,
+                       P->d_colidx, (int)P->n, P->d_fi, P->d_colfront, lblkval, P->d_partial);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> h(P->n);
+    HIP_CHECK(hipMemcpyAsync(h.data(), P->d_partial, P->n * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double v = 0.0;
+    for (i64 j = 0; j < P->n; ++j) v += h[j];
+    *out = 2.0 * v;
+    return std::isfinite(v) ? 0 : 1;
 }
 
 int chp_frontal_add_update(double* F, i64 N, const i64* relidx, const double* U,

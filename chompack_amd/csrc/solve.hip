@@ -148,6 +148,20 @@ __global__ __launch_bounds__(TPB) void k_dot_partial(
     if (threadIdx.x == 0) partial[j] = 2.0 * red[0] - x[off] * y[off];
 }
 
+// partial[j] = log L(j, j) for every column j of a factor (the diagonal of
+// the stored factor is the diagonal of L, symbolic.py:1113-1128).
+__global__ __launch_bounds__(TPB) void k_logdiag(const int* __restrict__ cols, int ncols,
+                                                 const FrontInfo* __restrict__ fi,
+                                                 const int* __restrict__ colfront,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ partial) {
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= ncols) return;
+    const FrontInfo f = fi[colfront[j]];
+    const int jl = cols[j];
+    partial[j] = log(x[f.blkptr + jl + (i64)jl * (f.nn + f.na)]);
+}
+
 // F[r[i], r[j]] += alpha U[i, j] (i >= j), general strides (cbase.c:11-34).
 __global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,
                               i64 fr, i64 fc, i64 ur, i64 uc, double alpha) {

@@ -15,7 +15,7 @@ from . import _native
 from .symbolic import cspmatrix
 
 __all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian",
-           "trsm", "dot"]
+           "trsm", "dot", "logdet"]
 
 SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
 
@@ -252,4 +252,18 @@ def dot(X, Y):
     rc = _native.lib().chp_dot(plan.handle, bx.data_ptr(), by.data_ptr(), ctypes.byref(out),
                                _stream(bx))
     _raise_rc(rc, "dot")
+    return out.value
+
+
+def logdet(L):
+    """log det X from the Cholesky factor L of X: 2 sum log diag(L), with the
+    diagonal as cspmatrix.diag() reads it from a factor (symbolic.py:1113-1128)."""
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("L must be a cspmatrix factor")
+    plan, b = _plan(L)
+    out = ctypes.c_double(0.0)
+    rc = _native.lib().chp_logdet(plan.handle, b.data_ptr(), ctypes.byref(out), _stream(b))
+    _raise_rc(rc, "logdet")
+    if rc == 1:
+        raise ArithmeticError("factor has a non-positive diagonal entry")
     return out.value

@@ -136,6 +136,11 @@ int chp_trsm(chp_plan* plan, const double* blkval, double* B, int ldb, int nrhs,
  * dot() (dot.c:3-22, cbase.c:106-174). */
 int chp_dot(chp_plan* plan, const double* x, const double* y, double* out, void* stream);
 
+/* *out = log det(L L^T) = 2 sum_j log L(j, j) from a factor; returns 1 when a
+ * diagonal entry is not positive.  The reference computes it from
+ * cspmatrix.diag() of the factor (symbolic.py:1113-1128); no kernel there. */
+int chp_logdet(chp_plan* plan, const double* lblkval, double* out, void* stream);
+
 /* F[r[i], r[j]] += alpha U[i, j] for i >= j, with element (a, b) of F at
  * F + a*strides[0] + b*strides[1] and of U at U + a*strides[2] + b*strides[3];
  * r is a device int64 array of length N.  Replaces frontal_add_update
