@@ -25,7 +25,8 @@ sys.path.insert(0, HERE)
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 KINDS = ["chol_small", "extadd", "potrf_diag", "unused", "gemm", "extract",
          "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
-         "copy_top", "splitk_reduce", "diag", "matop", "update_factor"]
+         "copy_top", "splitk_reduce", "diag", "matop", "update_factor", "sgather_fwd",
+         "sgather_bwd", "sscatter"]
 NK = len(KINDS)
 
 
@@ -38,6 +39,29 @@ def algorithmic_flops(symb):
     chol = float(np.sum(nn ** 3 / 3 + 2 * na * nn ** 2 + na ** 2 * nn))
     pinv = float(np.sum(nn ** 3 / 3 + nn ** 3 + 2 * na ** 2 * nn + 2 * nn ** 2 * na))
     return chol, pinv
+
+
+def memory_bytes(symb, small_max=64):
+    """Algorithmic HBM bytes of the memory-bound front kernels of one step:
+    extend-add into large fronts (12 N (N + 1) B per child of N rows: read the
+    child's update triangle, read and write the target entries) and the
+    extraction of update matrices for large-front children (read the parent
+    triangle, write both triangles: 8 N (N + 1) / 2 + 8 N^2 B)."""
+    import numpy as np
+    nn = np.diff(symb.snptr)
+    na = np.diff(symb.relptr)
+    nj = nn + na
+    large = (nj > small_max) | (nj.astype(float) ** 2 * 8 > 64 * 1024)
+    ext = 0.0
+    chptr, chidx = np.asarray(symb.chptr), np.asarray(symb.chidx)
+    for k in np.nonzero(large)[0]:
+        for c in chidx[chptr[k]:chptr[k + 1]]:
+            N = float(na[c])
+            ext += 12.0 * N * (N + 1)
+    # k_extract runs for every front with a separator (small and large parents)
+    N = na.astype(float)
+    xtr = float(np.sum(4.0 * N * (N + 1) + 8.0 * N * N))
+    return ext, xtr
 
 
 def main():
@@ -123,6 +147,14 @@ def main():
             "dominant_kind_by_time": KINDS[dom]}
 
     fl_chol, fl_pinv = algorithmic_flops(symb)
+    ext_b, xtr_b = memory_bytes(symb)
+    hbm = {}
+    for kind, nbytes in (("extadd", ext_b), ("extract", xtr_b)):
+        i = KINDS.index(kind)
+        if kcnt[i] and kms[i] > 0:
+            gbs = nbytes / (kms[i] * 1e-3) / 1e9
+            hbm[kind] = {"bytes_per_step": nbytes, "ms": round(float(kms[i]), 4),
+                         "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / PEAK["hbm_gbs"], 4)}
     out = {
         "metric": "Cholesky+projinv wall time, 40³ grid; mixed vs FP64 speedup at 1e-10 resid",
         "value": round(ms, 4),
@@ -144,6 +176,7 @@ def main():
         "effective_tflops": round((fl_chol + fl_pinv) / (ms * 1e-3) / 1e12, 3),
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,
+        "hbm_kernels": hbm,
         "gemm_by_size": gemm_classes,
         "gemm_top": [{"tiles": int(r[1]), "gflop": round(r[2] / 1e9, 3), "ms": round(r[3], 4),
                       "tflops": round(r[2] / (r[3] * 1e-3) / 1e12, 2),

@@ -12,9 +12,10 @@ namespace chpi {
 enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
-    K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR
+    K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
+    K_SGATHER_F, K_SGATHER_B, K_SSCATTER
 };
-const int NKIDS = 17;
+const int NKIDS = 20;
 
 struct Launch {
     Kid kid;
@@ -50,6 +51,15 @@ struct Program {
     std::vector<GemmTile> split_out;    // output tiles of split tasks, current launch
 };
 
+// Solve schedule for one right-hand-side count (GEMM shapes and work offsets
+// depend on it).
+struct SolvePlan {
+    int nrhs = 0;
+    Program pg;
+    std::vector<Launch> fwd, bwd;
+    double* d_splitp = nullptr;
+};
+
 }  // namespace chpi
 
 struct chp_plan {
@@ -83,7 +93,7 @@ struct chp_plan {
     i64 work_len = 0;
     unsigned long long* d_status = nullptr;
     i64 upd_len = 0, dinv_len = 0, scratch_len = 0, solve_len = 0;
-    std::vector<chpi::Launch> chol, projinv, llt, solve_fwd, solve_bwd;
+    std::vector<chpi::Launch> chol, projinv, llt;
     chpi::Program main, dense;
     chpi::Program* pg = &main;  // program the builders append to
     int n_large_chol = 0, n_large_projinv = 0;
@@ -97,6 +107,10 @@ struct chp_plan {
     };
     Seq comp[2], y2k[2], m2t[2];
     LevelSeq scale[8];
+    std::vector<chpi::SolvePlan*> solves;  // cached per nrhs
+    double* d_sdinv = nullptr;               // diagonal-block inverses of the solve
+    i64 sdinv_cap = 0;                       // largest request of the solve schedules
+    i64 dinv_len_solve = 0;                  // allocated length of d_sdinv
     double* d_dscratch = nullptr;
     double* d_ddinv = nullptr;
     double* d_dsplitp = nullptr;

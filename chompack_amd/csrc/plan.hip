@@ -671,14 +671,6 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     for (int L = H - 1; L >= 0; --L) build_projinv_level(P, P->bylevel[L], scratch);
     for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
-    for (int L = 0; L < H; ++L) {
-        const auto& fr = P->bylevel[L];
-        P->solve_fwd.push_back({K_SOLVE_FWD, (int)fr.size(), 256, 0, P->pg->lists.add(fr), 0});
-    }
-    for (int L = H - 1; L >= 0; --L) {
-        const auto& fr = P->bylevel[L];
-        P->solve_bwd.push_back({K_SOLVE_BWD, (int)fr.size(), 256, 0, P->pg->lists.add(fr), 0});
-    }
     // device copies
     std::vector<int> rel32(relptr[nsn]), ch32(chptr[nsn]);
     for (i64 i = 0; i < relptr[nsn]; ++i) rel32[i] = (int)relidx[i];
@@ -734,7 +726,13 @@ void chp_plan_destroy(chp_plan* P) {
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
-                    P->d_dscratch, P->d_ddinv, P->d_dsplitp};
+                    P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_sdinv};
+    for (SolvePlan* S : P->solves) {
+        if (S->pg.d_lists) (void)hipFree(S->pg.d_lists);
+        if (S->pg.d_tasks) (void)hipFree(S->pg.d_tasks);
+        if (S->d_splitp) (void)hipFree(S->d_splitp);
+        delete S;
+    }
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete P;
@@ -819,38 +817,6 @@ int chp_llt(chp_plan* P, double* blkval, void* stream) {
     int rc = run_sweep(P, P->llt, P->g_llt, blkval, (hipStream_t)stream, false);
     if (rc) return rc;
     HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
-    return 0;
-}
-
-int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
-             int trans, void* stream) {
-    if (!P || !blkval || !B) { chp_set_error("null argument"); return -1; }
-    if (trans != 'N' && trans != 'T') { chp_set_error("trans must be 'N' or 'T'"); return -1; }
-    if (nrhs <= 0) return 0;
-    if (ldb < P->n) { chp_set_error("ldB is smaller than n"); return -1; }
-    hipStream_t s = (hipStream_t)stream;
-    i64 need = P->solve_len * nrhs;
-    if (need > P->work_len) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        if (P->d_work) HIP_CHECK(hipFree(P->d_work));
-        P->d_work = nullptr;
-        HIP_CHECK(hipMalloc((void**)&P->d_work, need * sizeof(double)));
-        P->work_len = need;
-    }
-    const std::vector<Launch>& seq = trans == 'N' ? P->solve_fwd : P->solve_bwd;
-    for (const Launch& L : seq) {
-        const int* list = (const int*)(P->main.d_lists + L.list);
-        if (trans == 'N')
-            hipLaunchKernelGGL(k_solve_fwd, dim3(L.grid), dim3(256), 0, s, list, P->d_fi, blkval,
-                               P->d_snptr, P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs,
-                               alpha, P->d_work, P->d_wunit);
-        else
-            hipLaunchKernelGGL(k_solve_bwd, dim3(L.grid), dim3(256), 0, s, list, P->d_fi, blkval,
-                               P->d_snptr, P->d_relidx, P->d_perm, B, ldb, nrhs, alpha,
-                               P->d_work, P->d_wunit);
-    }
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(s));
     return 0;
 }
 

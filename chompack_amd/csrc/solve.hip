@@ -122,6 +122,84 @@ __global__ __launch_bounds__(TPB) void k_solve_bwd(
     }
 }
 
+// Large-front solve steps (the dense products run as GEMM tasks between them).
+// Forward gather: W[:nn] = alpha B[perm], W[nn:] = 0, then the children's
+// update rows added in reverse child order (trsm.c:42-60).
+__global__ __launch_bounds__(TPB) void k_sgather_fwd(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const i64* __restrict__ snptr, const int* __restrict__ relidx,
+    const int* __restrict__ chidx, const i64* __restrict__ perm, const double* __restrict__ B,
+    int ldb, int nrhs, double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, nj = nn + f.na;
+    const int tid = threadIdx.x;
+    double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];
+    for (i64 e = tid; e < (i64)nj * nrhs; e += TPB) {
+        const int i = (int)(e % nj), j = (int)(e / nj);
+        W[e] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
+    }
+    __syncthreads();
+    for (int l = f.chend - 1; l >= f.chbeg; --l) {
+        const int c = chidx[l];
+        const FrontInfo g = fi[c];
+        const int N = g.na, njc = g.nn + g.na;
+        const int* r = relidx + g.relptr;
+        const double* Wc = work + wptr[c] * nrhs + g.nn;
+        for (i64 e = tid; e < (i64)N * nrhs; e += TPB) {
+            const int i = (int)(e % N), j = (int)(e / N);
+            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
+        }
+        __syncthreads();
+    }
+}
+
+// Backward gather: W[:nn] = alpha B[perm], W[nn:] = the parent's values of
+// the separator rows (trsm.c:97-115).
+__global__ __launch_bounds__(TPB) void k_sgather_bwd(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const i64* __restrict__ snptr, const int* __restrict__ relidx,
+    const i64* __restrict__ perm, const double* __restrict__ B, int ldb, int nrhs,
+    double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int tid = threadIdx.x;
+    double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];
+    for (i64 e = tid; e < (i64)nn * nrhs; e += TPB) {
+        const int i = (int)(e % nn), j = (int)(e / nn);
+        W[i + (i64)j * nj] = alpha * B[pk[i] + (i64)j * ldb];
+    }
+    if (na > 0) {
+        const FrontInfo p = fi[f.parent];
+        const int njp = p.nn + p.na;
+        const double* Wp = work + wptr[f.parent] * nrhs;
+        const int* r = relidx + f.relptr;
+        for (i64 e = tid; e < (i64)na * nrhs; e += TPB) {
+            const int i = (int)(e % na), j = (int)(e / na);
+            W[nn + i + (i64)j * nj] = Wp[r[i] + (i64)j * njp];
+        }
+    }
+}
+
+// B[perm] = W[:nn] for the listed fronts.
+__global__ __launch_bounds__(TPB) void k_sscatter(
+    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const i64* __restrict__ snptr, const i64* __restrict__ perm, double* __restrict__ B,
+    int ldb, int nrhs, const double* __restrict__ work, const i64* __restrict__ wptr) {
+    const int k = fronts[blockIdx.x];
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, nj = nn + f.na;
+    const double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];
+    for (i64 e = threadIdx.x; e < (i64)nn * nrhs; e += TPB) {
+        const int i = (int)(e % nn), j = (int)(e / nn);
+        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+    }
+}
+
 // Partial trace inner products, one per supernode column j of the listed
 // range: 2 * sum_{i >= j} X[i,j] Y[i,j] - X[j,j] Y[j,j] (dot.c:9-20).
 // colfront[j] = supernode of column j; partial[j] receives the value.

@@ -231,3 +231,51 @@ def test_logdet_and_gradient():
     fd = (f(h) - f(-h)) / (2 * h)
     print("directional derivative %.12e fd %.12e" % (dd, fd), flush=True)
     assert abs(dd - fd) <= 1e-6 * abs(dd)
+
+
+def test_grid40_full_size_vs_reference():
+    """The 40^3 7-point grid (n = 64000, seed 1, AMD) at full size: cholesky,
+    the solve with 16 N(0,1) right-hand sides (L^{-1} then L^{-T}) and the
+    projected inverse against the reference's own C kernels (oracle/_ref),
+    relative error <= 1e-12; plus the size-independent identities
+    ||A x - b|| / ||b|| and <P(A^{-1}), A> = n."""
+    import torch
+    from oracle import ref
+    A = problems.grid3d(40, seed=1)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    b0 = X.blkval.cpu().numpy().copy()
+    cp.cholesky(X)
+    rng = np.random.default_rng(0)
+    Bh = rng.standard_normal((symb.n, 16))
+    B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+    cp.trsm(X, B)
+    cp.trsm(X, B, trans="T")
+    xs = B.cpu().numpy()
+    As = cp.symmetrize(A).tocsr()
+    res = np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
+    print("grid40 solve residual %.3e" % res.max(), flush=True)
+    assert res.max() <= 1e-12
+    Lg = X.blkval.cpu().numpy().copy()
+    cp.projected_inverse(X)
+    Yg = X.blkval.cpu().numpy()
+    Xa = cp.cspmatrix(symb, device="cuda:0") + A
+    assert abs(cp.dot(X, Xa) - symb.n) <= 1e-9 * symb.n
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    m = _lower_mask(symb)
+    L = b0.copy()
+    assert ref.cholesky(symb, L) == 0
+    e = _rel(Lg[m], L[m])
+    print("grid40 cholesky rel err vs reference %.3e" % e, flush=True)
+    assert e <= TOL
+    Br = np.asfortranarray(Bh.copy())
+    ref.trsm(symb, L, Br, trans="N")
+    ref.trsm(symb, L, Br, trans="T")
+    e = _rel(xs, Br)
+    print("grid40 solve rel err vs reference %.3e" % e, flush=True)
+    assert e <= TOL
+    assert ref.projected_inverse(symb, L) == 0
+    e = _rel(Yg[m], L[m])
+    print("grid40 projected_inverse rel err vs reference %.3e" % e, flush=True)
+    assert e <= TOL
