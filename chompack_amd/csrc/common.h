@@ -119,3 +119,10 @@ struct UFTask {
     int nn, na, N, key;
     unsigned char fbuf, pad[7];
 };
+
+// Row tile of a front in the solve's scatter: rows r0 .. r0 + 255 of front k,
+// whose solved block sits at offset tofs of the solve's block buffer.
+struct STile {
+    int k, r0;
+    i64 tofs;
+};

@@ -49,16 +49,16 @@ __global__ void k_solve_bwd(const int* fronts, const FrontInfo* fi, const double
                             double* B, int ldb, int nrhs, double alpha, double* work,
                             const i64* wptr);
 
-__global__ void k_sgather_fwd(const int* fronts, const FrontInfo* fi, const i64* snptr,
+__global__ void k_sgather_fwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
                               const int* relidx, const int* chidx, const i64* perm,
                               const double* B, int ldb, int nrhs, double alpha, double* work,
                               const i64* wptr);
-__global__ void k_sgather_bwd(const int* fronts, const FrontInfo* fi, const i64* snptr,
+__global__ void k_sgather_bwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
                               const int* relidx, const i64* perm, const double* B, int ldb,
                               int nrhs, double alpha, double* work, const i64* wptr);
-__global__ void k_sscatter(const int* fronts, const FrontInfo* fi, const i64* snptr,
-                           const i64* perm, double* B, int ldb, int nrhs, const double* work,
-                           const i64* wptr);
+__global__ void k_sscatter(const STile* tiles, const FrontInfo* fi, const i64* snptr,
+                           const i64* perm, double* B, int ldb, int nrhs, double* work,
+                           const i64* wptr, const double* tmp, int copyback);
 
 // utilities
 __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,

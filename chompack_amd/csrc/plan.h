@@ -58,6 +58,24 @@ struct SolvePlan {
     Program pg;
     std::vector<Launch> fwd, bwd;
     double* d_splitp = nullptr;
+    // inverses of the large diagonal blocks, their workspace, solved blocks
+    double* d_inv = nullptr;
+    double* d_w = nullptr;
+    double* d_tmp = nullptr;
+    i64 inv_len = 0, w_len = 0, tmp_len = 0;
+    // graphs of repeated solves, keyed by every kernel argument that varies
+    struct Key {
+        const void* blk = nullptr;
+        const void* B = nullptr;
+        int ldb = 0, trans = 0;
+        double alpha = 0.0;
+        bool operator==(const Key& o) const {
+            return blk == o.blk && B == o.B && ldb == o.ldb && trans == o.trans && alpha == o.alpha;
+        }
+    };
+    struct Graph { Key key; hipGraphExec_t exec = nullptr; };
+    std::vector<Graph> graphs;  // most recent last
+    Key last;                   // an eager call; the next identical call is captured
 };
 
 }  // namespace chpi
@@ -108,9 +126,6 @@ struct chp_plan {
     Seq comp[2], y2k[2], m2t[2];
     LevelSeq scale[8];
     std::vector<chpi::SolvePlan*> solves;  // cached per nrhs
-    double* d_sdinv = nullptr;               // diagonal-block inverses of the solve
-    i64 sdinv_cap = 0;                       // largest request of the solve schedules
-    i64 dinv_len_solve = 0;                  // allocated length of d_sdinv
     double* d_dscratch = nullptr;
     double* d_ddinv = nullptr;
     double* d_dsplitp = nullptr;
@@ -148,5 +163,34 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
 int read_status(chp_plan* P, hipStream_t s, i64* info);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
+
+// Dense operations on descriptors (plan_dense.hip): element (i, j) of a
+// matrix at BufTab p[buf][off + i + j*ld], order n.
+struct Mat {
+    int buf;
+    i64 off;
+    int ld, n;
+};
+struct Bump {
+    i64 cur = 0;
+    i64 take(i64 len) {
+        i64 o = cur;
+        const i64 l = len < 1 ? 1 : len;
+        cur += (l + 31) / 32 * 32;
+        return o;
+    }
+};
+template <class T>
+void push_tasks(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vector<T>& v,
+                int bsel = 0);
+void add_matop(std::vector<MatOp>& ops, MatOpKind kind, int sbuf, i64 src, int lds, int dbuf,
+               i64 dst, int ldd, int m, int n, double alpha = 1.0);
+// In-place lower Cholesky of each matrix (block inverses to buffer 3 via dv).
+void emit_potrf(chp_plan* P, std::vector<Launch>& seq, const std::vector<Mat>& ms,
+                const std::vector<int>& keys, Bump& dv);
+// X = L^{-1}; W[q]: n x n workspace (ld n) in buffer 2.
+void emit_trtri(chp_plan* P, std::vector<Launch>& seq, const std::vector<Mat>& Ls,
+                const std::vector<Mat>& Xs, const std::vector<i64>& Ws,
+                const std::vector<int>& keys);
 
 }  // namespace chpi

@@ -715,6 +715,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
 
 void chp_plan_destroy(chp_plan* P) {
     if (!P) return;
+    (void)hipDeviceSynchronize();  // no launch of this plan may still be in flight
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
     for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt})
@@ -726,11 +727,13 @@ void chp_plan_destroy(chp_plan* P) {
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
-                    P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_sdinv};
+                    P->d_dscratch, P->d_ddinv, P->d_dsplitp};
     for (SolvePlan* S : P->solves) {
+        for (auto& g : S->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (S->pg.d_lists) (void)hipFree(S->pg.d_lists);
         if (S->pg.d_tasks) (void)hipFree(S->pg.d_tasks);
-        if (S->d_splitp) (void)hipFree(S->d_splitp);
+        for (double* d : {S->d_splitp, S->d_inv, S->d_w, S->d_tmp})
+            if (d) (void)hipFree(d);
         delete S;
     }
     for (void* p : ptrs)

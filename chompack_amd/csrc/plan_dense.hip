@@ -25,26 +25,10 @@
 #include <functional>
 
 namespace chpi {
-namespace {
-
-struct Mat {
-    int buf;
-    i64 off;
-    int ld, n;
-};
-
-struct Bump {
-    i64 cur = 0;
-    i64 take(i64 len) {
-        i64 o = cur;
-        cur += (std::max<i64>(len, 1) + 31) / 32 * 32;
-        return o;
-    }
-};
 
 template <class T>
 void push_tasks(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vector<T>& v,
-                int bsel = 0) {
+                int bsel) {
     if (v.empty()) return;
     Launch L{kid, (int)v.size(), 256, 0, P->pg->lists.add(v), 0};
     L.bsel = bsel;
@@ -52,7 +36,7 @@ void push_tasks(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vecto
 }
 
 void add_matop(std::vector<MatOp>& ops, MatOpKind kind, int sbuf, i64 src, int lds, int dbuf,
-               i64 dst, int ldd, int m, int n, double alpha = 1.0) {
+               i64 dst, int ldd, int m, int n, double alpha) {
     for (int j0 = 0; j0 < n; j0 += 64)
         for (int i0 = 0; i0 < m; i0 += 64) {
             const bool above = i0 + 63 < j0;  // every element has i < j
@@ -173,6 +157,11 @@ void emit_trtri(chp_plan* P, std::vector<Launch>& seq, const std::vector<Mat>& L
         push_gemm(P, seq, g2);
     }
 }
+
+template void push_tasks<DiagTask>(chp_plan*, std::vector<Launch>&, Kid, const std::vector<DiagTask>&, int);
+template void push_tasks<MatOp>(chp_plan*, std::vector<Launch>&, Kid, const std::vector<MatOp>&, int);
+
+namespace {
 
 std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
     std::vector<int> ch;

@@ -3,17 +3,18 @@
 // schedule of a plan.
 //
 // Fronts with a small diagonal block and little separator work are solved by
-// one workgroup each (k_solve_fwd / k_solve_bwd).  Larger fronts run as a
-// short sequence per level, batched over the level's fronts:
-//   forward:  gather (alpha B rows + children's update rows), x_A -= B21 x_N
-//             on MFMA (B21 = L21 L11^{-1}, so x_N is used before its solve),
-//             then x_N := L11^{-1} x_N by 64-row blocks: x_b := Dinv_b x_b,
-//             x_below -= L[below, b] x_b; scatter x_N to B;
-//   backward: gather (alpha B rows + the parent's separator values),
-//             x_N := L11^{-T} x_N by blocks from the last, x_N -= B21^T x_A,
-//             scatter.
-// Dinv_b, the inverses of the 64 x 64 diagonal blocks of L11, are formed at
-// the start of each solve from the factor passed in (k_diag_tasks, mode 1).
+// one workgroup each (k_solve_fwd / k_solve_bwd).  For the larger fronts the
+// inverses X_k = L11^{-1} are formed first, all at once (diagonal-block
+// inverses and recursive halving on MFMA, from the factor passed in), so that
+// every level is a few batched launches:
+//   forward:  gather (alpha B rows + children's update rows); one GEMM launch
+//             with x_A -= B21 x_N (B21 = L21 L11^{-1}, so x_N is used before
+//             its solve) and T = X_k x_N; scatter T to B;
+//   backward: gather (alpha B rows + the parent's separator values);
+//             T = X_k^T x_N; T -= B21^T x_A; scatter T to B and back into the
+//             slab, where the children read their separator values.
+// Solving with the explicit inverse instead of substitution changes the last
+// bits only (forward error of the order of cond(L11) eps either way).
 // The schedule depends on the number of right-hand sides and is cached per
 // count.
 #include "plan.h"
@@ -25,7 +26,16 @@ namespace chpi {
 namespace {
 
 bool solve_large(const FrontInfo& f, int nrhs) {
-    return f.nn > 32 || (i64)f.na * f.nn * nrhs > 256 * 1024;
+    return f.nn > 16 || (i64)f.na * f.nn * nrhs > 32 * 1024;
+}
+
+// (front, first row) tiles of 256 rows over rows [0, nrow(front)).
+template <class F>
+std::vector<int2> row_tiles(const std::vector<int>& fr, F nrow) {
+    std::vector<int2> t;
+    for (int k : fr)
+        for (int r0 = 0; r0 < nrow(k); r0 += 256) t.push_back(make_int2(k, r0));
+    return t;
 }
 
 template <class T>
@@ -39,106 +49,80 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
     S->nrhs = nrhs;
     P->pg = &S->pg;
     const int H = P->nlevels;
-    // diagonal-block inverses of all large fronts (mode 1, column-major, ld 64)
-    std::vector<i64> dvo(P->fi.size(), -1);
-    std::vector<DiagTask> dt;
-    i64 dv = 0;
+    // X_k = L11^{-1} of every large front, one batched recursive-halving
+    // inverse at the start of each solve (buffer 6, ld nn; workspace buffer 2)
+    std::vector<i64> xo(P->fi.size(), -1), to(P->fi.size(), -1);
+    std::vector<Mat> Ls, Xs;
+    std::vector<i64> Ws;
+    std::vector<int> keys;
+    Bump bx, bw, bt;
     for (size_t k = 0; k < P->fi.size(); ++k) {
         const FrontInfo& f = P->fi[k];
         if (!solve_large(f, nrhs)) continue;
-        dvo[k] = dv;
-        for (int b = 0; b * NB < f.nn; ++b) {
-            DiagTask t;
-            std::memset(&t, 0, sizeof(t));
-            t.off = f.blkptr + (i64)b * NB * (1 + (i64)(f.nn + f.na));
-            t.outoff = dv + (i64)b * NB * NB;
-            t.ld = f.nn + f.na; t.ldo = NB; t.nb = std::min(NB, f.nn - b * NB);
-            t.key = (int)k; t.col = b * NB;
-            t.buf = 0; t.obuf = 3; t.mode = 1;
-            dt.push_back(t);
-        }
-        dv += (i64)((f.nn + NB - 1) / NB) * NB * NB;
+        xo[k] = bx.take((i64)f.nn * f.nn);
+        to[k] = bt.take((i64)f.nn * nrhs);
+        Ls.push_back({0, f.blkptr, f.nn + f.na, f.nn});
+        Xs.push_back({6, xo[k], f.nn, f.nn});
+        Ws.push_back(bw.take((i64)f.nn * f.nn));
+        keys.push_back((int)k);
     }
-    P->sdinv_cap = std::max(P->sdinv_cap, dv);  // requested; allocated by the caller
-    push_list(P, S->fwd, K_DIAG, dt);
+    S->inv_len = bx.cur;
+    S->w_len = bw.cur;
+    S->tmp_len = bt.cur;
+    if (!Ls.empty()) emit_trtri(P, S->fwd, Ls, Xs, Ws, keys);
     S->bwd = S->fwd;
     auto wofs = [&](int k) { return P->h_wunit[k] * nrhs; };
+    auto stiles = [&](const std::vector<int>& large) {
+        std::vector<STile> t;
+        for (int k : large)
+            for (int r0 = 0; r0 < P->fi[k].nn; r0 += 256) t.push_back({k, r0, to[k]});
+        return t;
+    };
     for (int L = 0; L < H; ++L) {
         std::vector<int> small, large;
         for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
         push_list(P, S->fwd, K_SOLVE_FWD, small);
         if (large.empty()) continue;
-        push_list(P, S->fwd, K_SGATHER_F, large);
+        auto nj_of = [&](int k) { return P->fi[k].nn + P->fi[k].na; };
+        push_list(P, S->fwd, K_SGATHER_F, row_tiles(large, nj_of));
         std::vector<GemmTile> g;
-        int maxnb = 0;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
             const int nn = f.nn, na = f.na, nj = nn + na;
-            maxnb = std::max(maxnb, (nn + NB - 1) / NB);
-            if (na > 0)  // x_A -= B21 x_N
+            if (na > 0)  // x_A -= B21 x_N  (B21 = L21 L11^{-1}: x_N before its solve)
                 add_gemm_tiles(P, g, mk(0, f.blkptr + nn, nj, 0, 7, wofs(k), nj, 0, 7, wofs(k) + nn,
                                         nj, na, nrhs, nn, -1.0, 1.0, 0), false);
+            // T = L11^{-1} x_N
+            add_gemm_tiles(P, g, mk(6, xo[k], nn, 0, 7, wofs(k), nj, 0, 5, to[k], nn,
+                                    nn, nrhs, nn, 1.0, 0.0, 0, 1), false);
         }
         push_gemm(P, S->fwd, g);
-        for (int b = 0; b < maxnb; ++b) {
-            std::vector<GemmTile> gd, gu;
-            for (int k : large) {
-                const FrontInfo& f = P->fi[k];
-                const int nn = f.nn, nj = nn + f.na;
-                const int b0 = b * NB;
-                if (b0 >= nn) continue;
-                const int b1 = std::min(b0 + NB, nn);
-                // x_b := Dinv_b x_b (in place: one tile row reads all of x_b first)
-                add_gemm_tiles(P, gd, mk(3, dvo[k] + (i64)b * NB * NB, NB, 0, 7, wofs(k) + b0, nj, 0,
-                                         7, wofs(k) + b0, nj, b1 - b0, nrhs, b1 - b0, 1.0, 0.0, 0, 1),
-                               false);
-                if (b1 < nn)  // x[b1:nn] -= L[b1:nn, b0:b1] x_b
-                    add_gemm_tiles(P, gu, mk(0, f.blkptr + b1 + (i64)b0 * nj, nj, 0, 7, wofs(k) + b0,
-                                             nj, 0, 7, wofs(k) + b1, nj, nn - b1, nrhs, b1 - b0,
-                                             -1.0, 1.0, 0), false);
-            }
-            push_gemm(P, S->fwd, gd);
-            push_gemm(P, S->fwd, gu);
-        }
-        push_list(P, S->fwd, K_SSCATTER, large);
+        std::vector<STile> st = stiles(large);
+        push_list(P, S->fwd, K_SSCATTER, st);
     }
     for (int L = H - 1; L >= 0; --L) {
         std::vector<int> small, large;
         for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
         push_list(P, S->bwd, K_SOLVE_BWD, small);
         if (large.empty()) continue;
-        push_list(P, S->bwd, K_SGATHER_B, large);
-        int maxnb = 0;
-        for (int k : large) maxnb = std::max(maxnb, (P->fi[k].nn + NB - 1) / NB);
-        for (int bb = 0; bb < maxnb; ++bb) {
-            std::vector<GemmTile> gd, gu;
-            for (int k : large) {
-                const FrontInfo& f = P->fi[k];
-                const int nn = f.nn, nj = nn + f.na;
-                const int nb = (nn + NB - 1) / NB;
-                if (bb >= nb) continue;
-                const int b = nb - 1 - bb, b0 = b * NB, b1 = std::min(b0 + NB, nn);
-                // x_b := Dinv_b^T x_b
-                add_gemm_tiles(P, gd, mk(3, dvo[k] + (i64)b * NB * NB, NB, 1, 7, wofs(k) + b0, nj, 0,
-                                         7, wofs(k) + b0, nj, b1 - b0, nrhs, b1 - b0, 1.0, 0.0, 0, 2),
-                               false);
-                if (b0 > 0)  // x[0:b0] -= L[b0:b1, 0:b0]^T x_b
-                    add_gemm_tiles(P, gu, mk(0, f.blkptr + b0, nj, 1, 7, wofs(k) + b0, nj, 0,
-                                             7, wofs(k), nj, b0, nrhs, b1 - b0, -1.0, 1.0, 0), false);
-            }
-            push_gemm(P, S->bwd, gd);
-            push_gemm(P, S->bwd, gu);
-        }
-        std::vector<GemmTile> g;
+        auto nj_of = [&](int k) { return P->fi[k].nn + P->fi[k].na; };
+        push_list(P, S->bwd, K_SGATHER_B, row_tiles(large, nj_of));
+        std::vector<GemmTile> g1, g2;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
             const int nn = f.nn, na = f.na, nj = nn + na;
-            if (na > 0)  // x_N -= B21^T x_A
-                add_gemm_tiles(P, g, mk(0, f.blkptr + nn, nj, 1, 7, wofs(k) + nn, nj, 0, 7, wofs(k),
-                                        nj, nn, nrhs, na, -1.0, 1.0, 0), false);
+            // T = L11^{-T} x_N
+            add_gemm_tiles(P, g1, mk(6, xo[k], nn, 1, 7, wofs(k), nj, 0, 5, to[k], nn,
+                                     nn, nrhs, nn, 1.0, 0.0, 0, 2), false);
+            if (na > 0)  // T -= B21^T x_A
+                add_gemm_tiles(P, g2, mk(0, f.blkptr + nn, nj, 1, 7, wofs(k) + nn, nj, 0, 5, to[k],
+                                         nn, nn, nrhs, na, -1.0, 1.0, 0), false);
         }
-        push_gemm(P, S->bwd, g);
-        push_list(P, S->bwd, K_SSCATTER, large);
+        push_gemm(P, S->bwd, g1);
+        push_gemm(P, S->bwd, g2);
+        std::vector<STile> st = stiles(large);
+        Launch Lc{K_SSCATTER, (int)st.size(), 256, 0, P->pg->lists.add(st), 1};
+        S->bwd.push_back(Lc);
     }
     P->pg = &P->main;
     return S;
@@ -149,8 +133,10 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
     BufTab bt;
     for (int i = 0; i < 8; ++i) bt.p[i] = nullptr;
     bt.p[0] = const_cast<double*>(blkval);
-    bt.p[3] = P->d_sdinv;
+    bt.p[2] = S->d_w;
     bt.p[4] = S->d_splitp;
+    bt.p[5] = S->d_tmp;
+    bt.p[6] = S->d_inv;
     bt.p[7] = P->d_work;
     for (const Launch& L : seq) {
         const void* list = S->pg.d_lists + L.list;
@@ -166,20 +152,23 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
                                P->d_relidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
             break;
         case K_SGATHER_F:
-            hipLaunchKernelGGL(k_sgather_fwd, g, b, 0, s, (const int*)list, P->d_fi, P->d_snptr,
+            hipLaunchKernelGGL(k_sgather_fwd, g, b, 0, s, (const int2*)list, P->d_fi, P->d_snptr,
                                P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work,
                                P->d_wunit);
             break;
         case K_SGATHER_B:
-            hipLaunchKernelGGL(k_sgather_bwd, g, b, 0, s, (const int*)list, P->d_fi, P->d_snptr,
+            hipLaunchKernelGGL(k_sgather_bwd, g, b, 0, s, (const int2*)list, P->d_fi, P->d_snptr,
                                P->d_relidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
             break;
         case K_SSCATTER:
-            hipLaunchKernelGGL(k_sscatter, g, b, 0, s, (const int*)list, P->d_fi, P->d_snptr,
-                               P->d_perm, B, ldb, nrhs, P->d_work, P->d_wunit);
+            hipLaunchKernelGGL(k_sscatter, g, b, 0, s, (const STile*)list, P->d_fi, P->d_snptr,
+                               P->d_perm, B, ldb, nrhs, P->d_work, P->d_wunit, S->d_tmp, L.iarg);
             break;
         case K_DIAG:
             hipLaunchKernelGGL(k_diag_tasks, g, b, 0, s, (const DiagTask*)list, bt, P->d_status);
+            break;
+        case K_MATOP:
+            hipLaunchKernelGGL(k_matop, g, b, 0, s, (const MatOp*)list, bt);
             break;
         case K_GEMM:
             hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, S->pg.d_tasks, bt);
@@ -217,26 +206,73 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         P->solves.push_back(S);
         if (sync_program(S->pg)) return -2;
         if (alloc_doubles(&S->d_splitp, S->pg.splitp_len)) return -2;
+        if (alloc_doubles(&S->d_inv, S->inv_len)) return -2;
+        if (alloc_doubles(&S->d_w, S->w_len)) return -2;
+        if (alloc_doubles(&S->d_tmp, S->tmp_len)) return -2;
     }
     const i64 need = P->solve_len * nrhs;
+    // captured solves hold the addresses of the work buffers: drop them all
+    // before either buffer moves
+    auto drop_graphs = [&]() {
+        for (SolvePlan* q : P->solves) {
+            for (auto& g : q->graphs) (void)hipGraphExecDestroy(g.exec);
+            q->graphs.clear();
+            q->last = SolvePlan::Key();
+        }
+    };
     if (need > P->work_len) {
-        HIP_CHECK(hipStreamSynchronize(s));
+        HIP_CHECK(hipDeviceSynchronize());
+        drop_graphs();
+    }
+    if (need > P->work_len) {
         if (P->d_work) HIP_CHECK(hipFree(P->d_work));
         P->d_work = nullptr;
         HIP_CHECK(hipMalloc((void**)&P->d_work, need * sizeof(double)));
         P->work_len = need;
     }
-    if (P->sdinv_cap > P->dinv_len_solve) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        if (P->d_sdinv) HIP_CHECK(hipFree(P->d_sdinv));
-        P->d_sdinv = nullptr;
-        if (alloc_doubles(&P->d_sdinv, P->sdinv_cap)) return -2;
-        P->dinv_len_solve = P->sdinv_cap;
-    }
     const std::vector<Launch>& seq = trans == 'N' ? S->fwd : S->bwd;
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-    int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, s);
-    if (rc) return rc;
+    SolvePlan::Key key;
+    key.blk = blkval; key.B = B; key.ldb = ldb; key.trans = trans; key.alpha = alpha;
+    SolvePlan::Graph* G = nullptr;
+    for (auto& g : S->graphs) if (g.key == key) G = &g;
+    if (P->use_graphs && !P->prof && (G || S->last == key)) {
+        if (!P->gstream) {
+            HIP_CHECK(hipStreamCreateWithFlags(&P->gstream, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreateWithFlags(&P->gev_in, hipEventDisableTiming));
+            HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
+        }
+        if (!G) {
+            if (S->graphs.size() >= 4) {
+                HIP_CHECK(hipStreamSynchronize(s));
+                (void)hipGraphExecDestroy(S->graphs.front().exec);
+                S->graphs.erase(S->graphs.begin());
+            }
+            hipGraph_t graph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
+            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
+            int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, P->gstream);
+            hipError_t e = hipStreamEndCapture(P->gstream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
+            SolvePlan::Graph ng;
+            ng.key = key;
+            e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { chp_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return -2; }
+            S->graphs.push_back(ng);
+            G = &S->graphs.back();
+        }
+        HIP_CHECK(hipEventRecord(P->gev_in, s));
+        HIP_CHECK(hipStreamWaitEvent(P->gstream, P->gev_in, 0));
+        HIP_CHECK(hipGraphLaunch(G->exec, P->gstream));
+        HIP_CHECK(hipEventRecord(P->gev_out, P->gstream));
+        HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
+    } else {
+        S->last = key;
+        hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+        int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, s);
+        if (rc) return rc;
+    }
     i64 info[2];
     if (read_status(P, s, info) != 0) {
         chp_set_error("trsm: singular diagonal block in supernode " + std::to_string(info[0]));

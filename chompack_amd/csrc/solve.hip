@@ -107,12 +107,18 @@ __global__ __launch_bounds__(TPB) void k_solve_bwd(
             W[nn + i + (i64)j * nj] = Wp[r[i] + (i64)j * njp];
         }
         __syncthreads();
-        // x_N -= B21^T x_A
-        for (int e = tid; e < nn * nrhs; e += TPB) {
-            int m = e % nn, j = e / nn;
-            double s = W[m + (i64)j * nj];
-            for (int i = 0; i < na; ++i) s -= L[nn + i + (i64)m * nj] * W[nn + i + (i64)j * nj];
-            W[m + (i64)j * nj] = s;
+        // x_N -= B21^T x_A: one wave per output (m, j), lanes over the
+        // separator rows (coalesced column reads of B21 and x_A)
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int o = wv; o < nn * nrhs; o += TPB / 64) {
+            const int m = o % nn, j = o / nn;
+            const double* lc = L + nn + (i64)m * nj;
+            const double* xa = W + nn + (i64)j * nj;
+            double acc = 0.0;
+            for (int i = lane; i < na; i += 64) acc += lc[i] * xa[i];
+#pragma unroll
+            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
+            if (lane == 0) W[m + (i64)j * nj] -= acc;
         }
     }
     __syncthreads();
@@ -122,81 +128,100 @@ __global__ __launch_bounds__(TPB) void k_solve_bwd(
     }
 }
 
-// Large-front solve steps (the dense products run as GEMM tasks between them).
+// Large-front solve steps (the dense products run as GEMM tasks between
+// them).  Each workgroup owns rows [r0, r0 + 256) of one front's slab,
+// tiles[i] = (front, r0).
 // Forward gather: W[:nn] = alpha B[perm], W[nn:] = 0, then the children's
-// update rows added in reverse child order (trsm.c:42-60).
+// update rows added in reverse child order (trsm.c:42-60); a child's rows that
+// fall in the tile are found by binary search (relative indices increase).
 __global__ __launch_bounds__(TPB) void k_sgather_fwd(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const int* __restrict__ relidx,
     const int* __restrict__ chidx, const i64* __restrict__ perm, const double* __restrict__ B,
     int ldb, int nrhs, double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
-    const int k = fronts[blockIdx.x];
+    const int2 t = tiles[blockIdx.x];
+    const int k = t.x, r0 = t.y;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
+    const int r1 = min(r0 + 256, nj), rows = r1 - r0;
     const int tid = threadIdx.x;
     double* W = work + wptr[k] * nrhs;
     const i64* pk = perm + snptr[k];
-    for (i64 e = tid; e < (i64)nj * nrhs; e += TPB) {
-        const int i = (int)(e % nj), j = (int)(e / nj);
-        W[e] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
+    for (int e = tid; e < rows * nrhs; e += TPB) {
+        const int i = r0 + e % rows, j = e / rows;
+        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
     }
-    __syncthreads();
+    __shared__ int s_ja, s_jb;
     for (int l = f.chend - 1; l >= f.chbeg; --l) {
         const int c = chidx[l];
         const FrontInfo g = fi[c];
         const int N = g.na, njc = g.nn + g.na;
         const int* r = relidx + g.relptr;
-        const double* Wc = work + wptr[c] * nrhs + g.nn;
-        for (i64 e = tid; e < (i64)N * nrhs; e += TPB) {
-            const int i = (int)(e % N), j = (int)(e / N);
-            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
+        __syncthreads();
+        if (tid == 0) {
+            int lo = 0, hi = N;
+            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < r0) lo = m + 1; else hi = m; }
+            s_ja = lo;
+            hi = N;
+            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < r1) lo = m + 1; else hi = m; }
+            s_jb = lo;
         }
         __syncthreads();
+        const int ja = s_ja, cnt = s_jb - s_ja;
+        const double* Wc = work + wptr[c] * nrhs + g.nn;
+        for (int e = tid; e < cnt * nrhs; e += TPB) {
+            const int i = ja + e % cnt, j = e / cnt;
+            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
+        }
     }
 }
 
 // Backward gather: W[:nn] = alpha B[perm], W[nn:] = the parent's values of
 // the separator rows (trsm.c:97-115).
 __global__ __launch_bounds__(TPB) void k_sgather_bwd(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const int* __restrict__ relidx,
     const i64* __restrict__ perm, const double* __restrict__ B, int ldb, int nrhs,
     double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
-    const int k = fronts[blockIdx.x];
+    const int2 t = tiles[blockIdx.x];
+    const int k = t.x, r0 = t.y;
     const FrontInfo f = fi[k];
-    const int nn = f.nn, na = f.na, nj = nn + na;
-    const int tid = threadIdx.x;
+    const int nn = f.nn, nj = nn + f.na;
+    const int r1 = min(r0 + 256, nj), rows = r1 - r0;
     double* W = work + wptr[k] * nrhs;
     const i64* pk = perm + snptr[k];
-    for (i64 e = tid; e < (i64)nn * nrhs; e += TPB) {
-        const int i = (int)(e % nn), j = (int)(e / nn);
-        W[i + (i64)j * nj] = alpha * B[pk[i] + (i64)j * ldb];
-    }
-    if (na > 0) {
-        const FrontInfo p = fi[f.parent];
-        const int njp = p.nn + p.na;
-        const double* Wp = work + wptr[f.parent] * nrhs;
-        const int* r = relidx + f.relptr;
-        for (i64 e = tid; e < (i64)na * nrhs; e += TPB) {
-            const int i = (int)(e % na), j = (int)(e / na);
-            W[nn + i + (i64)j * nj] = Wp[r[i] + (i64)j * njp];
-        }
+    const FrontInfo p = fi[f.parent];
+    const int njp = p.nn + p.na;
+    const double* Wp = work + wptr[f.parent] * nrhs;
+    const int* r = relidx + f.relptr;
+    for (int e = threadIdx.x; e < rows * nrhs; e += TPB) {
+        const int i = r0 + e % rows, j = e / rows;
+        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb]
+                                      : Wp[r[i - nn] + (i64)j * njp];
     }
 }
 
-// B[perm] = W[:nn] for the listed fronts.
+// B[perm] = T[:nn] for the listed tiles (rows r0 .. r0 + 255 below nn), T the
+// front's solved block (column-major, ld nn, at tmp + t.tofs); with copyback
+// also W[:nn] = T (the backward sweep's children read it from the slab).
 __global__ __launch_bounds__(TPB) void k_sscatter(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+    const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const i64* __restrict__ perm, double* __restrict__ B,
-    int ldb, int nrhs, const double* __restrict__ work, const i64* __restrict__ wptr) {
-    const int k = fronts[blockIdx.x];
+    int ldb, int nrhs, double* __restrict__ work, const i64* __restrict__ wptr,
+    const double* __restrict__ tmp, int copyback) {
+    const STile t = tiles[blockIdx.x];
+    const int k = t.k, r0 = t.r0;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
-    const double* W = work + wptr[k] * nrhs;
+    const int r1 = min(r0 + 256, nn), rows = r1 - r0;
+    double* W = work + wptr[k] * nrhs;
+    const double* T = tmp + t.tofs;
     const i64* pk = perm + snptr[k];
-    for (i64 e = threadIdx.x; e < (i64)nn * nrhs; e += TPB) {
-        const int i = (int)(e % nn), j = (int)(e / nn);
-        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+    for (int e = threadIdx.x; e < rows * nrhs; e += TPB) {
+        const int i = r0 + e % rows, j = e / rows;
+        const double v = T[i + (i64)j * nn];
+        B[pk[i] + (i64)j * ldb] = v;
+        if (copyback) W[i + (i64)j * nj] = v;
     }
 }
 

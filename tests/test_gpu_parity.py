@@ -279,3 +279,29 @@ def test_grid40_full_size_vs_reference():
     e = _rel(Yg[m], L[m])
     print("grid40 projected_inverse rel err vs reference %.3e" % e, flush=True)
     assert e <= TOL
+
+
+def test_solve_graph_replays_match_eager():
+    """Repeated solves with the same buffers replay a captured HIP graph; each
+    replay must equal the eager solve, also after the work buffers grow."""
+    import torch
+    A = problems.grid3d(10, seed=6)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    cp.cholesky(X)
+    rng = np.random.default_rng(3)
+    B0 = torch.as_tensor(np.asfortranarray(rng.standard_normal((symb.n, 4)))).to("cuda:0")
+    B0 = B0.t().contiguous().t()
+    outs = []
+    B = B0.clone()
+    for rep in range(5):
+        B.copy_(B0)
+        cp.trsm(X, B)
+        cp.trsm(X, B, trans="T")
+        outs.append(B.cpu().numpy().copy())
+        _ = torch.ones(100, device="cuda:0").sum().item()
+        if rep == 2:  # a wider solve grows the work buffer: graphs must be rebuilt
+            W = torch.as_tensor(rng.standard_normal((symb.n, 64))).to("cuda:0").t().contiguous().t()
+            cp.trsm(X, W)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
