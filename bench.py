@@ -111,6 +111,32 @@ def main():
     print("# timed %d steps: %.3f ms/step" % (args.steps, ms), file=sys.stderr, flush=True)
     gpu_result = X.blkval.cpu().numpy()
 
+    # solve with 16 N(0,1) right-hand sides (L^{-1} then L^{-T}) on the factor
+    rng = np.random.default_rng(0)
+    Bh = rng.standard_normal((symb.n, 16))
+    B0 = torch.as_tensor(np.asfortranarray(Bh)).to(dev).t().contiguous().t()
+    Lf = cp.cspmatrix(symb, blkval=pristine.clone())
+    cp.cholesky(Lf)
+    Bs = B0.clone()
+    for _ in range(2):
+        Bs.copy_(B0)
+        cp.trsm(Lf, Bs)
+        cp.trsm(Lf, Bs, trans="T")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        Bs.copy_(B0)
+        cp.trsm(Lf, Bs)
+        cp.trsm(Lf, Bs, trans="T")
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t0) * 1e3 / reps
+    xs = Bs.cpu().numpy()
+    As = cp.symmetrize(A).tocsr()
+    resid = float(np.max(np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)))
+    solve = {"nrhs": 16, "ms": round(solve_ms, 3), "what": "trsm N then T with the factor",
+             "max_rel_residual": resid}
+
     # instrumented step: HIP events around every launch of both sweeps
     plan = symb._plan(dev)
     L = _native.lib()
@@ -177,6 +203,7 @@ def main():
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,
         "hbm_kernels": hbm,
+        "solve_16rhs": solve,
         "gemm_by_size": gemm_classes,
         "gemm_top": [{"tiles": int(r[1]), "gflop": round(r[2] / 1e9, 3), "ms": round(r[3], 4),
                       "tflops": round(r[2] / (r[3] * 1e-3) / 1e12, 2),
@@ -209,6 +236,19 @@ def main():
                                              (args.side, threads, args.cpu_runs)}
             out["speedup_vs_cpu"] = round(cpu_ms / ms, 2)
             out["parity_rel_err_vs_reference"] = err
+            Lr = b0.copy()
+            ref.cholesky(symb, Lr)
+            st = []
+            for it in range(args.cpu_runs + 1):
+                Br = np.asfortranarray(Bh.copy())
+                t0 = time.perf_counter()
+                ref.trsm(symb, Lr, Br, trans="N")
+                ref.trsm(symb, Lr, Br, trans="T")
+                if it > 0:
+                    st.append(time.perf_counter() - t0)
+            solve["cpu_reference_ms"] = round(statistics.median(st) * 1e3, 3)
+            solve["speedup_vs_cpu"] = round(solve["cpu_reference_ms"] / solve_ms, 2)
+            solve["rel_err_vs_reference"] = float(np.abs(xs - Br).max() / np.abs(Br).max())
         else:
             out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -857,6 +857,16 @@ int chp_logdet(chp_plan* P, const double* lblkval, double* out, void* stream) {
     return std::isfinite(v) ? 0 : 1;
 }
 
+int chp_logdiag(chp_plan* P, const double* lblkval, double* out, void* stream) {
+    if (!P || !lblkval || !out) { chp_set_error("null argument"); return -1; }
+    if (P->n == 0) return 0;
+    hipLaunchKernelGGL(k_logdiag, dim3((unsigned)((P->n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, P->d_colidx, (int)P->n, P->d_fi, P->d_colfront,
+                       lblkval, out);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int chp_frontal_add_update(double* F, i64 N, const i64* relidx, const double* U,
                            const i64* strides, double alpha, void* stream) {
     if (N <= 0) return 0;

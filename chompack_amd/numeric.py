@@ -15,7 +15,7 @@ from . import _native
 from .symbolic import cspmatrix
 
 __all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian",
-           "trsm", "dot", "logdet"]
+           "trsm", "dot", "logdet", "logdiag"]
 
 SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
 
@@ -267,3 +267,16 @@ def logdet(L):
     if rc == 1:
         raise ArithmeticError("factor has a non-positive diagonal entry")
     return out.value
+
+
+def logdiag(L):
+    """log L(j, j) for every column j of a factor, in the reordered numbering,
+    as a float64 GPU tensor (log det X = 2 * logdiag(L).sum())."""
+    torch = _torch()
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("L must be a cspmatrix factor")
+    plan, b = _plan(L)
+    out = torch.empty(L.symb.n, dtype=torch.float64, device=b.device)
+    rc = _native.lib().chp_logdiag(plan.handle, b.data_ptr(), out.data_ptr(), _stream(b))
+    _raise_rc(rc, "logdiag")
+    return out

@@ -140,6 +140,10 @@ int chp_dot(chp_plan* plan, const double* x, const double* y, double* out, void*
  * diagonal entry is not positive.  The reference computes it from
  * cspmatrix.diag() of the factor (symbolic.py:1113-1128); no kernel there. */
 int chp_logdet(chp_plan* plan, const double* lblkval, double* out, void* stream);
+/* out[j] = log L(j, j) for every (reordered) column j, out a device array of
+ * length n; per-instance log-determinants of a block-diagonal batch are
+ * segment sums of it (no reference counterpart; batching support). */
+int chp_logdiag(chp_plan* plan, const double* lblkval, double* out, void* stream);
 
 /* F[r[i], r[j]] += alpha U[i, j] for i >= j, with element (a, b) of F at
  * F + a*strides[0] + b*strides[1] and of U at U + a*strides[2] + b*strides[3];

@@ -305,3 +305,150 @@ def test_solve_graph_replays_match_eager():
             cp.trsm(X, W)
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+def test_batched_512_grids_logdet_and_solve():
+    """512 independent 2D grids, side ~ U{16..96} (seed 2), as one batched
+    factorization: per-instance logdet against the dense slogdet (the 24
+    smallest instances) and against single-instance factorizations; the joint
+    solve's residual per instance."""
+    import torch
+    As = problems.batch_grids(512, 16, 96, seed=2)
+    bs = cp.batch.BatchSymbolic(As)
+    X = bs.assemble()
+    cp.cholesky(X)
+    ld = bs.logdet(X)
+    order = np.argsort([A.shape[0] for A in As])
+    for i in order[:24]:
+        sign, ref = np.linalg.slogdet(cp.symmetrize(As[i]).toarray())
+        assert sign > 0 and abs(ld[i] - ref) <= 1e-12 * abs(ref), i
+    for i in order[-3:]:
+        s = cp.symbolic(As[i], p=cp.amd.order)
+        Xi = cp.cspmatrix(s, device="cuda:0") + As[i]
+        cp.cholesky(Xi)
+        li = cp.logdet(Xi)
+        assert abs(li - ld[i]) <= 1e-12 * abs(li), i
+    rng = np.random.default_rng(8)
+    bh = rng.standard_normal(bs.symb.n)
+    b = torch.as_tensor(bh, device="cuda:0")
+    cp.trsm(X, b)
+    cp.trsm(X, b, trans="T")
+    x = b.cpu().numpy()
+    for A, xi, bi in zip(As, bs.split(x), bs.split(bh)):
+        r = np.linalg.norm(cp.symmetrize(A) @ xi - bi) / np.linalg.norm(bi)
+        assert r <= 1e-12
+
+
+def test_grid32_gradient_and_hessian_directions():
+    """-logdet on the 32^3 7-point grid (n = 32768, seed 3): the autograd
+    gradient (projected-inverse backward) against central differences with
+    step 1e-6 (relative error <= 1e-6) and against the reference's projected
+    inverse; the full Hessian map on 32 random directions: symmetric under
+    the trace inner product, equal to the derivative of the projected inverse
+    (central differences) and to the reference's hessian()."""
+    import time
+    import torch
+    from oracle import ref
+    A = problems.grid3d(32, seed=3)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    m = torch.as_tensor(_lower_mask(symb), device="cuda:0")
+    w = torch.where(m, 1.0, 0.0).double()
+    rng = np.random.default_rng(9)
+    Xg = cp.cspmatrix(symb, blkval=X.blkval.clone().requires_grad_(True))
+    f = -cp.autograd.logdet(Xg)
+    f.backward()
+    G = Xg.blkval.grad
+    U = torch.as_tensor(rng.standard_normal(len(m)), device="cuda:0") * w
+    dd = float((G * U).sum())
+    h = 1e-6
+
+    def logdiag_at(t):
+        Lt = cp.cspmatrix(symb, blkval=X.blkval + t * U)
+        cp.cholesky(Lt)
+        return cp.numeric.logdiag(Lt)
+    # f(X + hU) - f(X - hU) summed column by column: the per-column log
+    # differences are O(h), so the central difference keeps its digits
+    # (the totals are ~1e5 and would cancel to ~1e-5 relative accuracy)
+    fd = -2.0 * float((logdiag_at(h) - logdiag_at(-h)).sum()) / (2 * h)
+    print("grid32 directional derivative %.10e fd %.10e" % (dd, fd), flush=True)
+    assert abs(dd - fd) <= 1e-6 * abs(dd)
+    # factor and projected inverse for the Hessian
+    L = X.copy()
+    cp.cholesky(L)
+    Y = L.copy()
+    cp.projected_inverse(Y)
+    # gradient of -logdet is -Y mapped to blkval coordinates
+    wgt = cp.autograd._grad_weights(symb, G.device)
+    assert float((G + Y.blkval * wgt).abs().max()) <= 1e-12 * float(Y.blkval.abs().max())
+    Us = [cp.cspmatrix(symb, blkval=torch.as_tensor(rng.standard_normal(len(m)), device="cuda:0") * w)
+          for _ in range(32)]
+    U0 = [u.blkval.clone() for u in Us]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cp.hessian(L, Y, Us, adj=None)
+    torch.cuda.synchronize()
+    print("grid32 hessian on 32 directions: %.1f ms" % ((time.perf_counter() - t0) * 1e3), flush=True)
+    Hs = [u.blkval for u in Us]
+    d = lambda a, b: cp.dot(cp.cspmatrix(symb, blkval=a), cp.cspmatrix(symb, blkval=b))
+    for i, j in [(0, 1), (2, 7), (30, 31)]:
+        a, b = d(Hs[i], U0[j]), d(U0[i], Hs[j])
+        assert abs(a - b) <= 1e-11 * max(abs(a), 1.0)
+    for i in (0, 5):
+        def Yat(t):
+            Xt = cp.cspmatrix(symb, blkval=X.blkval + t * U0[i])
+            cp.cholesky(Xt)
+            cp.projected_inverse(Xt)
+            return Xt.blkval
+        dY = (Yat(h) - Yat(-h)) / (2 * h)
+        e = float(((dY + Hs[i]) * w).abs().max() / (Hs[i] * w).abs().max())
+        print("grid32 hessian vs FD of Y: %.3e" % e, flush=True)
+        assert e <= 1e-6
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    b0 = X.blkval.cpu().numpy().copy()
+    Lr = b0.copy()
+    assert ref.cholesky(symb, Lr) == 0
+    Yr = Lr.copy()
+    assert ref.projected_inverse(symb, Yr) == 0
+    mm = _lower_mask(symb)
+    assert _rel(Y.blkval.cpu().numpy()[mm], Yr[mm]) <= TOL
+    urs = [u.cpu().numpy().copy() for u in U0[:4]]
+    ref.hessian(symb, Lr, Yr, urs, adj=None)
+    for a, r in zip(Hs[:4], urs):
+        e = _rel(a.cpu().numpy()[mm], r[mm])
+        print("grid32 hessian rel err vs reference %.3e" % e, flush=True)
+        assert e <= TOL
+
+
+def test_band_arrow_20000_completion():
+    """Maximum-determinant completion on the chordal band-plus-arrow pattern
+    (n = 20000, half-bandwidth 8, 50 arrow columns, seed 4, natural order: an
+    elimination-tree chain of 19942 supernodes): completion(P(A^{-1})) is the
+    factor of A, and equals the reference's completion."""
+    import time
+    import torch
+    from oracle import ref
+    A = problems.band_arrow(20000, 8, 50, seed=4)
+    symb = cp.symbolic(A)
+    assert symb.fill == (0, 0)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    cp.cholesky(X)
+    Lg = X.blkval.cpu().numpy().copy()
+    cp.projected_inverse(X)
+    Yg = X.blkval.cpu().numpy().copy()
+    for factored in (True, False):
+        Z = cp.cspmatrix(symb, blkval=torch.as_tensor(Yg, device="cuda:0"))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cp.completion(Z, factored_updates=factored)
+        torch.cuda.synchronize()
+        print("band_arrow completion factored=%s: %.1f ms" % (factored, (time.perf_counter() - t0) * 1e3),
+              flush=True)
+        mm = _lower_mask(symb)
+        got = Z.blkval.cpu().numpy()
+        assert _rel(got[mm], Lg[mm]) <= 1e-11
+        if ref.available():
+            r = Yg.copy()
+            assert ref.completion(symb, r, factored) == 0
+            assert _rel(got[mm], r[mm]) <= TOL
