@@ -452,3 +452,33 @@ def test_band_arrow_20000_completion():
             r = Yg.copy()
             assert ref.completion(symb, r, factored) == 0
             assert _rel(got[mm], r[mm]) <= TOL
+
+
+def test_rgg_200k_cholesky_and_64rhs_solve():
+    """Random geometric graph, n = 200000 points in the unit cube, mean degree
+    12 (seed 5), AMD: cholesky against the reference's kernels, and the solve
+    with 64 right-hand sides: residual ||A x - b|| / ||b|| per column."""
+    import torch
+    from oracle import ref
+    A = problems.rgg(200000, 12.0, seed=5)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    b0 = X.blkval.cpu().numpy()
+    cp.cholesky(X)
+    rng = np.random.default_rng(0)
+    Bh = rng.standard_normal((symb.n, 64))
+    B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+    cp.trsm(X, B)
+    cp.trsm(X, B, trans="T")
+    As = cp.symmetrize(A).tocsr()
+    res = np.linalg.norm(As @ B.cpu().numpy() - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
+    print("rgg residual %.3e" % res.max(), flush=True)
+    assert res.max() <= 1e-12
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    L = b0.copy()
+    assert ref.cholesky(symb, L) == 0
+    m = _lower_mask(symb)
+    e = _rel(X.blkval.cpu().numpy()[m], L[m])
+    print("rgg cholesky rel err vs reference %.3e" % e, flush=True)
+    assert e <= TOL
