@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
-KINDS = ["chol_small", "extadd", "potrf_diag", "unused", "gemm", "extract",
+KINDS = ["chol_small", "extadd", "potrf_diag", "panel_factor", "gemm", "extract",
          "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
          "copy_top", "splitk_reduce", "diag", "matop", "update_factor", "sgather_fwd",
          "sgather_bwd", "sscatter"]

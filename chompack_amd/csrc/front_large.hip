@@ -104,7 +104,7 @@ __device__ inline double prsqrt(double d) {
 // on a non-positive pivot, whose column goes to *bad.  rdiag[c] = 1/L(c,c).
 __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)[5],
                                     double* rdiag, int w, int l, int* bad) {
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < NB / 4; ++s) {
         const int r0 = 4 * s;
         double v = 0.0;
@@ -156,6 +156,70 @@ __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)
             const double q0 = Q[j][0], q1 = Q[j][1], q2 = Q[j][2], q3 = Q[j][3];
             const double m = (j >= r0 + 4 && l >= j) ? 1.0 : 0.0;
             a[q] -= m * (x0 * q0 + x1 * q1 + x2 * q2 + x3 * q3);
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// block_potrf4 with a second set of rows: b[] holds 64 rows below the block
+// (thread (w, l) owns panel row l, columns w + 4q), eliminated alongside the
+// block's own rows, so they end as the panel rows A_panel L^{-T} of the
+// factor.  P needs 2 * NB rows.
+__device__ inline bool block_potrf4_panel(double (&a)[DQ], double (&b)[DQ], double (*P)[5],
+                                          double (*Q)[5], int w, int l, int* bad) {
+#pragma unroll
+    for (int s = 0; s < NB / 4; ++s) {
+        const int r0 = 4 * s;
+        P[l][w] = a[s];
+        P[NB + l][w] = b[s];
+        __syncthreads();
+        const double d00 = P[r0][0], d10 = P[r0 + 1][0], d20 = P[r0 + 2][0], d30 = P[r0 + 3][0];
+        const double d11 = P[r0 + 1][1], d21 = P[r0 + 2][1], d31 = P[r0 + 3][1];
+        const double d22 = P[r0 + 2][2], d32 = P[r0 + 3][2], d33 = P[r0 + 3][3];
+        const double i0 = prsqrt(d00);
+        const double L10 = d10 * i0, L20 = d20 * i0, L30 = d30 * i0;
+        const double e11 = d11 - L10 * L10;
+        const double i1 = prsqrt(e11);
+        const double L21 = (d21 - L20 * L10) * i1, L31 = (d31 - L30 * L10) * i1;
+        const double e22 = d22 - L20 * L20 - L21 * L21;
+        const double i2 = prsqrt(e22);
+        const double L32 = (d32 - L30 * L20 - L31 * L21) * i2;
+        const double e33 = d33 - L30 * L30 - L31 * L31 - L32 * L32;
+        const double i3 = prsqrt(e33);
+        const int fail = !(d00 > 0.0) ? 0 : (!(e11 > 0.0) ? 1 : (!(e22 > 0.0) ? 2 : (!(e33 > 0.0) ? 3 : -1)));
+        if (fail >= 0) { *bad = r0 + fail; return false; }
+        double x0, x1, x2, x3;
+        if (l >= r0 + 4) {
+            const double p0 = P[l][0], p1 = P[l][1], p2 = P[l][2], p3 = P[l][3];
+            x0 = p0 * i0;
+            x1 = (p1 - x0 * L10) * i1;
+            x2 = (p2 - x0 * L20 - x1 * L21) * i2;
+            x3 = (p3 - x0 * L30 - x1 * L31 - x2 * L32) * i3;
+        } else {
+            const int r = l - r0;
+            x0 = r == 0 ? d00 * i0 : (r == 1 ? L10 : (r == 2 ? L20 : (r == 3 ? L30 : 0.0)));
+            x1 = r == 1 ? e11 * i1 : (r == 2 ? L21 : (r == 3 ? L31 : 0.0));
+            x2 = r == 2 ? e22 * i2 : (r == 3 ? L32 : 0.0);
+            x3 = r == 3 ? e33 * i3 : 0.0;
+        }
+        // the panel row: always below the block
+        const double q0 = P[NB + l][0], q1 = P[NB + l][1], q2 = P[NB + l][2], q3 = P[NB + l][3];
+        const double y0 = q0 * i0;
+        const double y1 = (q1 - y0 * L10) * i1;
+        const double y2 = (q2 - y0 * L20 - y1 * L21) * i2;
+        const double y3 = (q3 - y0 * L30 - y1 * L31 - y2 * L32) * i3;
+        a[s] = w == 0 ? x0 : (w == 1 ? x1 : (w == 2 ? x2 : x3));
+        b[s] = w == 0 ? y0 : (w == 1 ? y1 : (w == 2 ? y2 : y3));
+        if (w == 0) { Q[l][0] = x0; Q[l][1] = x1; Q[l][2] = x2; Q[l][3] = x3; }
+        __syncthreads();
+#pragma unroll
+        for (int q = s + 1; q < DQ; ++q) {
+            const int j = w + 4 * q;
+            const double c0 = Q[j][0], c1 = Q[j][1], c2 = Q[j][2], c3 = Q[j][3];
+            const double m = (l >= j) ? 1.0 : 0.0;
+            a[q] -= m * (x0 * c0 + x1 * c1 + x2 * c2 + x3 * c3);
+            b[q] -= y0 * c0 + y1 * c1 + y2 * c2 + y3 * c3;
         }
     }
     __syncthreads();
@@ -263,6 +327,49 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
     for (int q = 0; q < DQ; ++q) {
         const int r = w + 4 * q;  // row r, column l of Dinv
         inv[l + r * NB] = (r < nb && l < nb) ? X[r][l] : 0.0;
+    }
+}
+
+// One inner step of the large-front Cholesky: tasks[i] = (front, c0, r0).
+// Every workgroup factors the 64 x 64 diagonal block at (c0, c0) from the
+// front's current values (redundantly: the chain of dependent pivots is the
+// same length for all of them) together with the 64 rows r0 .. r0 + 63 of the
+// panel below it, and writes those rows of the factor; the workgroup with
+// r0 < 0 writes the diagonal block and reports a failed pivot.
+__global__ __launch_bounds__(TPB) void k_panel_factor(
+    const int4* __restrict__ tasks, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, unsigned long long* status) {
+    __shared__ double P[2 * NB][5], Q[NB][5];
+    const int4 t = tasks[blockIdx.x];
+    const int k = t.x, c0 = t.y, r0 = t.z;
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, nj = nn + f.na;
+    const int nb = min(NB, nn - c0);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double* blk = blkval + f.blkptr;
+    double* dg = blk + c0 + (i64)c0 * nj;
+    double a[DQ], b[DQ];
+    const bool prow = r0 >= 0 && r0 + l < nj;
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int j = w + 4 * q;
+        a[q] = (l < nb && j < nb && l >= j) ? dg[l + (i64)j * nj] : (l == j ? 1.0 : 0.0);
+        b[q] = (prow && j < nb) ? blk[(r0 + l) + (i64)(c0 + j) * nj] : 0.0;
+    }
+    int bad = 0;
+    if (!block_potrf4_panel(a, b, P, Q, w, l, &bad)) {
+        if (r0 < 0 && threadIdx.x == 0) chp_record_failure(status, k, c0 + bad + 1);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int j = w + 4 * q;
+        if (j >= nb) continue;
+        if (r0 < 0) {
+            if (l < nb && l >= j) dg[l + (i64)j * nj] = a[q];
+        } else if (prow) {
+            blk[(r0 + l) + (i64)(c0 + j) * nj] = b[q];
+        }
     }
 }
 

@@ -10,7 +10,7 @@
 namespace chpi {
 
 enum Kid {
-    K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_MUL, K_GEMM, K_EXTRACT,
+    K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_FACTOR, K_GEMM, K_EXTRACT,
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
     K_SGATHER_F, K_SGATHER_B, K_SSCATTER

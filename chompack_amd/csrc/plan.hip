@@ -170,20 +170,17 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
     for (int p0 = 0; p0 < maxnn; p0 += NB2) {
         for (int c0 = p0; c0 < std::min(p0 + NB2, maxnn); c0 += NB) {
-            std::vector<int> act;
-            for (int k : large) if (P->fi[k].nn > c0) act.push_back(k);
-            P->chol.push_back({K_POTRF_DIAG, (int)act.size(), 256, 0, P->pg->lists.add(act), c0});
-            std::vector<GemmTile> pt, it;
-            for (int k : act) {
+            // diagonal block and the panel below it, one launch: a workgroup
+            // per 64 panel rows (plus one writing the diagonal block)
+            std::vector<int4> pf;
+            std::vector<GemmTile> it;
+            for (int k : large) {
                 const FrontInfo& f = P->fi[k];
                 int nn = f.nn, nj = nn + f.na;
+                if (nn <= c0) continue;
                 int c1 = std::min(c0 + NB, nn), p1 = std::min(p0 + NB2, nn);
-                // panel solve, in place: L[c1:, c0:c1] = F[c1:, c0:c1] Dinv^T
-                // (Dinv row-major, so Dinv^T(m, j) = dinv[m + j*NB])
-                add_gemm_tiles(P, pt, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
-                                         3, P->h_dinvptr[k] + (i64)(c0 / NB) * NB * NB, NB, 0,
-                                         0, f.blkptr + c1 + (i64)c0 * nj, nj,
-                                         nj - c1, c1 - c0, c1 - c0, 1.0, 0.0, 0), false);
+                pf.push_back(make_int4(k, c0, -1, 0));
+                for (int r0 = c1; r0 < nj; r0 += NB) pf.push_back(make_int4(k, c0, r0, 0));
                 // update inside the outer panel: F[c1:, c1:p1] -= L[c1:, c0:c1] L[c1:p1, c0:c1]^T
                 if (c1 < p1)
                     add_gemm_tiles(P, it, mk(0, f.blkptr + c1 + (i64)c0 * nj, nj, 0,
@@ -191,7 +188,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                              0, f.blkptr + c1 + (i64)c1 * nj, nj,
                                              nj - c1, p1 - c1, c1 - c0, -1.0, 1.0, 1), true);
             }
-            push_gemm(P, P->chol, pt);
+            P->chol.push_back({K_PANEL_FACTOR, (int)pf.size(), 256, 0, P->pg->lists.add(pf), 0});
             push_gemm(P, P->chol, it);
         }
         // trailing update right of the outer panel, K = p1 - p0
@@ -217,6 +214,23 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // from the right; inside a group block by block (X_b = R_b Dinv_b, update
     // of the group's remaining columns), then one update of everything left
     // of the group with K = the group width.
+    std::vector<DiagTask> inv;
+    for (int k : large) {
+        const FrontInfo& f = P->fi[k];
+        if (f.na == 0) continue;
+        const int nj = f.nn + f.na;
+        for (int b = 0; b * NB < f.nn; ++b) {
+            DiagTask t;
+            std::memset(&t, 0, sizeof(t));
+            t.off = f.blkptr + (i64)b * NB * (1 + (i64)nj);
+            t.outoff = P->h_dinvptr[k] + (i64)b * NB * NB;
+            t.ld = nj; t.ldo = NB; t.nb = std::min(NB, f.nn - b * NB); t.key = k; t.col = b * NB;
+            t.buf = 0; t.obuf = 3; t.mode = 1;
+            inv.push_back(t);
+        }
+    }
+    if (!inv.empty())
+        P->chol.push_back({K_DIAG, (int)inv.size(), 256, 0, P->pg->lists.add(inv), 0});
     const int GB = NB2 / NB;
     int maxgrp = 0;
     for (int k : large)
@@ -234,9 +248,9 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                 int b = gb1 - 1 - sb;
                 if (b < gb0) continue;
                 int b0 = b * NB, b1 = std::min(b0 + NB, nn), q0 = gb0 * NB;
-                // X_b = R_b Dinv_b in place (Dinv(m, j) = dinv[j + m*NB])
+                // X_b = R_b L_bb^{-1} in place (inverse column-major, ld NB)
                 add_gemm_tiles(P, pt, mk(0, f.blkptr + nn + (i64)b0 * nj, nj, 0,
-                                         3, P->h_dinvptr[k] + (i64)b * NB * NB, NB, 1,
+                                         3, P->h_dinvptr[k] + (i64)b * NB * NB, NB, 0,
                                          0, f.blkptr + nn + (i64)b0 * nj, nj,
                                          na, b1 - b0, b1 - b0, 1.0, 0.0, 0), false);
                 // R[:, q0:b0] -= X_b L[b0:b1, q0:b0]
@@ -492,6 +506,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_POTRF_DIAG:
             CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, P->d_fi,
                        blkval, P->d_dinv, P->d_dinvptr, P->d_status);
+            break;
+        case K_PANEL_FACTOR:
+            CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, P->d_fi, blkval, P->d_status);
             break;
         case K_GEMM:
             CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
