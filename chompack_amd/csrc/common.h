@@ -87,11 +87,14 @@ struct BufTab {
 //           p[obuf][outoff + j + i*64] (64 x 64, zero padded).
 //   mode 1: inverse of the lower-triangular block, written column-major to
 //           p[obuf][outoff + i + j*ldo] (full nb x nb, zero above the diagonal).
+//   mode 2: as mode 1, and the block's lower triangle is also copied to
+//           p[buf2][off2 + i + j*ld2] (the output may overwrite the input).
 // A failure records (key, col + LAPACK-style column) in the status word.
 struct DiagTask {
-    i64 off, outoff;
-    int ld, ldo, nb, key, col;
-    unsigned char buf, obuf, mode, pad;
+    i64 off, outoff, off2;
+    int ld, ldo, nb, key, col, ld2;
+    unsigned char buf, obuf, mode, buf2;
+    int pad;
 };
 
 // Element-wise operation on one 64 x 64 tile (origin i0, j0) of an m x n

@@ -334,11 +334,15 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
 // Every workgroup factors the 64 x 64 diagonal block at (c0, c0) from the
 // front's current values (redundantly: the chain of dependent pivots is the
 // same length for all of them) together with the 64 rows r0 .. r0 + 63 of the
-// panel below it, and writes those rows of the factor; the workgroup with
-// r0 < 0 writes the diagonal block and reports a failed pivot.
+// panel below it, and writes those rows of the factor.  The workgroup with
+// r0 < 0 reports a failed pivot and stages the factored diagonal block in the
+// front's inverse area (dinv + dinvptr[k] + (c0 / 64) * 4096, ld 64): writing
+// it into blkval here would race with workgroups of the same launch that
+// have not read the block yet.  A later launch moves it into blkval.
 __global__ __launch_bounds__(TPB) void k_panel_factor(
     const int4* __restrict__ tasks, const FrontInfo* __restrict__ fi,
-    double* __restrict__ blkval, unsigned long long* status) {
+    double* __restrict__ blkval, double* __restrict__ dinv, const i64* __restrict__ dinvptr,
+    unsigned long long* status) {
     __shared__ double P[2 * NB][5], Q[NB][5];
     const int4 t = tasks[blockIdx.x];
     const int k = t.x, c0 = t.y, r0 = t.z;
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(TPB) void k_panel_factor(
         const int j = w + 4 * q;
         if (j >= nb) continue;
         if (r0 < 0) {
-            if (l < nb && l >= j) dg[l + (i64)j * nj] = a[q];
+            if (l < nb) dinv[dinvptr[k] + (i64)(c0 / NB) * NB * NB + l + (i64)j * NB] = a[q];
         } else if (prow) {
             blk[(r0 + l) + (i64)(c0 + j) * nj] = b[q];
         }
@@ -472,6 +476,14 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
         Ls[l][j] = (l < nb && j < nb && l >= j) ? blk[l + (i64)j * ld] : (l == j ? 1.0 : 0.0);
     }
     __syncthreads();
+    if (t.mode == 2) {
+        double* cp2 = bt.p[t.buf2] + t.off2;
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int j = w + 4 * q;
+            if (l < nb && j < nb && l >= j) cp2[l + (i64)j * t.ld2] = Ls[l][j];
+        }
+    }
     if (w == 0) {
         if (l < nb && Ls[l][l] == 0.0) atomicMin(&zero_at, l);
         rdiag[l] = 1.0 / Ls[l][l];
@@ -682,7 +694,7 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
         kbase = tl.ks * T.kchunk;
         kend = min(T.K, kbase + T.kchunk);
         Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
-        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // reduce handles it
+        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // the reduction handles it
     }
 
     // triangular operands: chunks where opA or opB is identically zero are

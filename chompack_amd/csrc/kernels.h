@@ -27,7 +27,7 @@ __global__ void k_copy_top(const int2* tiles, const FrontInfo* fi, const double*
 __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, double* blkval,
                              double* dinv, const i64* dinvptr, unsigned long long* status);
 __global__ void k_panel_factor(const int4* tasks, const FrontInfo* fi, double* blkval,
-                               unsigned long long* status);
+                               double* dinv, const i64* dinvptr, unsigned long long* status);
 __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,
                              double* dinv, const i64* dinvptr, double* X, const i64* xptr,
                              unsigned long long* status);

@@ -49,6 +49,7 @@ struct Program {
     size_t up_lists = 0, up_tasks = 0;  // sizes of the device copies
     i64 splitp_cur = 0, splitp_len = 0;
     std::vector<GemmTile> split_out;    // output tiles of split tasks, current launch
+    i64 split_doubles() const { return splitp_len + 1; }
 };
 
 // Solve schedule for one right-hand-side count (GEMM shapes and work offsets

@@ -214,18 +214,23 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // from the right; inside a group block by block (X_b = R_b Dinv_b, update
     // of the group's remaining columns), then one update of everything left
     // of the group with K = the group width.
+    // the diagonal blocks were staged in the inverse area by k_panel_factor
+    // (writing them into blkval in that launch would race with workgroups
+    // still reading the unfactored block): copy them into blkval and replace
+    // the staged copies by their inverses (stored convention below)
     std::vector<DiagTask> inv;
     for (int k : large) {
         const FrontInfo& f = P->fi[k];
-        if (f.na == 0) continue;
         const int nj = f.nn + f.na;
         for (int b = 0; b * NB < f.nn; ++b) {
             DiagTask t;
             std::memset(&t, 0, sizeof(t));
-            t.off = f.blkptr + (i64)b * NB * (1 + (i64)nj);
-            t.outoff = P->h_dinvptr[k] + (i64)b * NB * NB;
-            t.ld = nj; t.ldo = NB; t.nb = std::min(NB, f.nn - b * NB); t.key = k; t.col = b * NB;
-            t.buf = 0; t.obuf = 3; t.mode = 1;
+            t.off = P->h_dinvptr[k] + (i64)b * NB * NB;
+            t.outoff = t.off;
+            t.ld = NB; t.ldo = NB; t.nb = std::min(NB, f.nn - b * NB); t.key = k; t.col = b * NB;
+            t.buf = 3; t.obuf = 3; t.mode = 2;
+            t.off2 = f.blkptr + (i64)b * NB * (1 + (i64)nj);
+            t.ld2 = nj; t.buf2 = 0;
             inv.push_back(t);
         }
     }
@@ -508,7 +513,8 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        blkval, P->d_dinv, P->d_dinvptr, P->d_status);
             break;
         case K_PANEL_FACTOR:
-            CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, P->d_fi, blkval, P->d_status);
+            CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, P->d_fi, blkval, P->d_dinv,
+                       P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
             CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
@@ -719,7 +725,8 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= alloc_doubles(&P->d_dinv, P->dinv_len);
     rc |= alloc_doubles(&P->d_scratch, P->scratch_len);
     rc |= alloc_doubles(&P->d_partial, n);
-    rc |= alloc_doubles(&P->d_splitp, P->main.splitp_len);
+    rc |= alloc_doubles(&P->d_splitp, P->main.split_doubles());
+    if (!rc && hipMemset(P->d_splitp, 0, P->main.split_doubles() * sizeof(double)) != hipSuccess) rc = -1;
     if (hipMalloc((void**)&P->d_status, sizeof(unsigned long long)) != hipSuccess) rc = -1;
     if (rc) {
         std::string e = g_err;

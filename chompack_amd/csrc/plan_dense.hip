@@ -407,7 +407,7 @@ int ensure_dense(chp_plan* P, hipStream_t s) {
     struct { double** d; i64* have; i64 need; } bufs[] = {
         {&P->d_dscratch, &P->dscratch_cap, P->dscratch_len},
         {&P->d_ddinv, &P->ddinv_cap, P->ddinv_len},
-        {&P->d_dsplitp, &P->dsplitp_cap, P->dense.splitp_len},
+        {&P->d_dsplitp, &P->dsplitp_cap, P->dense.split_doubles()},
     };
     for (auto& b : bufs) {
         if (*b.d && b.need <= *b.have) continue;
@@ -416,6 +416,8 @@ int ensure_dense(chp_plan* P, hipStream_t s) {
         *b.d = nullptr;
         if (alloc_doubles(b.d, b.need)) return -2;
         *b.have = std::max<i64>(b.need, 1);
+        if (b.d == &P->d_dsplitp)  // arrival counters start at zero
+            HIP_CHECK(hipMemset(P->d_dsplitp, 0, std::max<i64>(b.need, 1) * sizeof(double)));
     }
     return 0;
 }

@@ -205,7 +205,8 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         S = build_solve(P, nrhs);
         P->solves.push_back(S);
         if (sync_program(S->pg)) return -2;
-        if (alloc_doubles(&S->d_splitp, S->pg.splitp_len)) return -2;
+        if (alloc_doubles(&S->d_splitp, S->pg.split_doubles())) return -2;
+        HIP_CHECK(hipMemset(S->d_splitp, 0, S->pg.split_doubles() * sizeof(double)));
         if (alloc_doubles(&S->d_inv, S->inv_len)) return -2;
         if (alloc_doubles(&S->d_w, S->w_len)) return -2;
         if (alloc_doubles(&S->d_tmp, S->tmp_len)) return -2;
