@@ -41,6 +41,20 @@ def algorithmic_flops(symb):
     return chol, pinv
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
+    None when the summary is absent."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic_grid40_cholesky_projinv.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def memory_bytes(symb, small_max=64):
     """Algorithmic HBM bytes of the memory-bound front kernels of one step:
     extend-add into large fronts (12 N (N + 1) B per child of N rows: read the
@@ -72,6 +86,7 @@ def main():
     ap.add_argument("--side", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-solve", action="store_true", help="skip the 16-RHS solve section")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path runs on a single MI355X "
@@ -95,7 +110,7 @@ def main():
           (symb.n, symb.Nsn, symb.nnz, symb.clique_number, t_symb), file=sys.stderr, flush=True)
 
     def step():
-        X.blkval.copy_(pristine)
+        cp.numeric.copy_blkval(X.blkval, pristine)
         X.is_factor = False
         cp.cholesky(X)
         cp.projected_inverse(X)
@@ -112,30 +127,32 @@ def main():
     gpu_result = X.blkval.cpu().numpy()
 
     # solve with 16 N(0,1) right-hand sides (L^{-1} then L^{-T}) on the factor
+    solve = None
     rng = np.random.default_rng(0)
-    Bh = rng.standard_normal((symb.n, 16))
-    B0 = torch.as_tensor(np.asfortranarray(Bh)).to(dev).t().contiguous().t()
-    Lf = cp.cspmatrix(symb, blkval=pristine.clone())
-    cp.cholesky(Lf)
-    Bs = B0.clone()
-    for _ in range(2):
-        Bs.copy_(B0)
-        cp.trsm(Lf, Bs)
-        cp.trsm(Lf, Bs, trans="T")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = 5
-    for _ in range(reps):
-        Bs.copy_(B0)
-        cp.trsm(Lf, Bs)
-        cp.trsm(Lf, Bs, trans="T")
-    torch.cuda.synchronize()
-    solve_ms = (time.perf_counter() - t0) * 1e3 / reps
-    xs = Bs.cpu().numpy()
-    As = cp.symmetrize(A).tocsr()
-    resid = float(np.max(np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)))
-    solve = {"nrhs": 16, "ms": round(solve_ms, 3), "what": "trsm N then T with the factor",
-             "max_rel_residual": resid}
+    if not args.no_solve:
+        Bh = rng.standard_normal((symb.n, 16))
+        B0 = torch.as_tensor(np.asfortranarray(Bh)).to(dev).t().contiguous().t()
+        Lf = cp.cspmatrix(symb, blkval=pristine.clone())
+        cp.cholesky(Lf)
+        Bs = B0.clone()
+        for _ in range(2):
+            Bs.copy_(B0)
+            cp.trsm(Lf, Bs)
+            cp.trsm(Lf, Bs, trans="T")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            Bs.copy_(B0)
+            cp.trsm(Lf, Bs)
+            cp.trsm(Lf, Bs, trans="T")
+        torch.cuda.synchronize()
+        solve_ms = (time.perf_counter() - t0) * 1e3 / reps
+        xs = Bs.cpu().numpy()
+        As = cp.symmetrize(A).tocsr()
+        resid = float(np.max(np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)))
+        solve = {"nrhs": 16, "ms": round(solve_ms, 3), "what": "trsm N then T with the factor",
+                 "max_rel_residual": resid}
 
     # instrumented step: HIP events around every launch of both sweeps
     plan = symb._plan(dev)
@@ -171,6 +188,12 @@ def main():
             "avg_launch_us": round(float(kms[4]) * 1e3 / max(int(kcnt[4]), 1), 3),
             "algorithmic_flops_per_step": float(kfl[4]),
             "dominant_kind_by_time": KINDS[dom]}
+    gb = np.zeros(4)
+    L.chp_plan_gemm_bytes(plan.handle, _native.ptr(gb))
+    if args.side == 40:
+        roof["traffic"] = profiled_traffic("k_gemm_tiles")
+        roof["traffic_source"] = "profiles/hbm_traffic_grid40_cholesky_projinv.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-kernel calibrated)"
+    roof["algorithmic_bytes_per_launch"] = float((gb[0] + gb[1]) / max(gb[2] + gb[3], 1))
 
     fl_chol, fl_pinv = algorithmic_flops(symb)
     ext_b, xtr_b = memory_bytes(symb)
@@ -236,19 +259,23 @@ def main():
                                              (args.side, threads, args.cpu_runs)}
             out["speedup_vs_cpu"] = round(cpu_ms / ms, 2)
             out["parity_rel_err_vs_reference"] = err
-            Lr = b0.copy()
-            ref.cholesky(symb, Lr)
+            if solve is None:
+                out["cpu_baseline"]["sample"] += "; solve section skipped"
+            Lr = b0.copy() if solve is not None else None
+            if Lr is not None:
+                ref.cholesky(symb, Lr)
             st = []
-            for it in range(args.cpu_runs + 1):
+            for it in range(args.cpu_runs + 1 if solve is not None else 0):
                 Br = np.asfortranarray(Bh.copy())
                 t0 = time.perf_counter()
                 ref.trsm(symb, Lr, Br, trans="N")
                 ref.trsm(symb, Lr, Br, trans="T")
                 if it > 0:
                     st.append(time.perf_counter() - t0)
-            solve["cpu_reference_ms"] = round(statistics.median(st) * 1e3, 3)
-            solve["speedup_vs_cpu"] = round(solve["cpu_reference_ms"] / solve_ms, 2)
-            solve["rel_err_vs_reference"] = float(np.abs(xs - Br).max() / np.abs(Br).max())
+            if solve is not None:
+                solve["cpu_reference_ms"] = round(statistics.median(st) * 1e3, 3)
+                solve["speedup_vs_cpu"] = round(solve["cpu_reference_ms"] / solve["ms"], 2)
+                solve["rel_err_vs_reference"] = float(np.abs(xs - Br).max() / np.abs(Br).max())
         else:
             out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

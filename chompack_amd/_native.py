@@ -69,6 +69,7 @@ def _declare(L):
     _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)
     _opt(L, "chp_logdet", [vp, vp, vp, vp], i32)
     _opt(L, "chp_logdiag", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_copy", [vp, vp, i64, vp], i32)
     _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
     _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
     _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)
@@ -77,6 +78,7 @@ def _declare(L):
     _opt(L, "chp_set_graphs", [vp, i32], i32)
     _opt(L, "chp_profile_read", [vp, vp, vp, vp], i32)
     _opt(L, "chp_plan_flops", [vp, vp], i32)
+    _opt(L, "chp_plan_gemm_bytes", [vp, vp], i32)
     _opt(L, "chp_profile_launches", [vp, vp, i64], i64)
     _opt(L, "chp_last_error", [], ctypes.c_char_p)
 

@@ -66,6 +66,7 @@ __global__ void k_sscatter(const STile* tiles, const FrontInfo* fi, const i64* s
 __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,
                               const int* colfront, const double* x, const double* y,
                               double* partial);
+__global__ void k_copy(double* dst, const double* src, i64 n);
 __global__ void k_logdiag(const int* cols, int ncols, const FrontInfo* fi, const int* colfront,
                           const double* x, double* partial);
 __global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,

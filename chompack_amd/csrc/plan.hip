@@ -814,6 +814,34 @@ int chp_set_graphs(chp_plan* P, int on) {
     return 0;
 }
 
+// Algorithmic GEMM bytes and launch counts: out[0] / out[1] bytes of the
+// cholesky / projected-inverse GEMM launches (each task reads A, B and, with
+// beta != 0, C once and writes C once), out[2] / out[3] their launch counts.
+int chp_plan_gemm_bytes(chp_plan* P, double* out) {
+    if (!P || !out) return -1;
+    const std::vector<Launch>* seqs[2] = {&P->chol, &P->projinv};
+    for (int q = 0; q < 2; ++q) {
+        double bytes = 0.0, n = 0.0;
+        for (const Launch& L : *seqs[q]) {
+            if (L.kid != K_GEMM) continue;
+            n += 1.0;
+            std::vector<GemmTile> tl(L.grid);
+            std::memcpy(tl.data(), P->main.lists.bytes.data() + L.list, L.grid * sizeof(GemmTile));
+            int last = -1;
+            for (const GemmTile& g : tl) {
+                if (g.task == last) continue;
+                last = g.task;
+                const GemmTask& t = P->main.tasks[g.task];
+                bytes += 8.0 * ((double)t.M * t.K + (double)t.K * t.N +
+                                (double)t.M * t.N * (t.beta != 0.0 ? 2.0 : 1.0));
+            }
+        }
+        out[q] = bytes;
+        out[2 + q] = n;
+    }
+    return 0;
+}
+
 // Algorithmic GEMM flops of one sweep: out[0] cholesky, out[1] projected inverse.
 int chp_plan_flops(chp_plan* P, double* out) {
     if (!P) return -1;
@@ -879,6 +907,19 @@ int chp_logdet(chp_plan* P, const double* lblkval, double* out, void* stream) {
     for (i64 j = 0; j < P->n; ++j) v += h[j];
     *out = 2.0 * v;
     return std::isfinite(v) ? 0 : 1;
+}
+
+int chp_copy(double* dst, const double* src, i64 n, void* stream) {
+    if (n <= 0) return 0;
+    if (!dst || !src || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) {
+        chp_set_error("chp_copy: null or misaligned (16 bytes) pointer");
+        return -1;
+    }
+    const i64 blocks = std::min<i64>((n / 2 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)std::max<i64>(blocks, 1)), dim3(256), 0,
+                       (hipStream_t)stream, dst, src, n);
+    HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 int chp_logdiag(chp_plan* P, const double* lblkval, double* out, void* stream) {

@@ -265,6 +265,17 @@ __global__ __launch_bounds__(TPB) void k_logdiag(const int* __restrict__ cols, i
     partial[j] = log(x[f.blkptr + jl + (i64)jl * (f.nn + f.na)]);
 }
 
+// dst[0:n] = src[0:n], 16-byte vector accesses, grid-stride.
+__global__ __launch_bounds__(TPB) void k_copy(double* __restrict__ dst,
+                                              const double* __restrict__ src, i64 n) {
+    const i64 n2 = n / 2;
+    const i64 stride = (i64)gridDim.x * TPB;
+    const double2* s2 = reinterpret_cast<const double2*>(src);
+    double2* d2 = reinterpret_cast<double2*>(dst);
+    for (i64 e = (i64)blockIdx.x * TPB + threadIdx.x; e < n2; e += stride) d2[e] = s2[e];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
+}
+
 // F[r[i], r[j]] += alpha U[i, j] (i >= j), general strides (cbase.c:11-34).
 __global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,
                               i64 fr, i64 fc, i64 ur, i64 uc, double alpha) {

@@ -15,7 +15,7 @@ from . import _native
 from .symbolic import cspmatrix
 
 __all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian",
-           "trsm", "dot", "logdet", "logdiag"]
+           "trsm", "dot", "logdet", "logdiag", "copy_blkval"]
 
 SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
 
@@ -280,3 +280,17 @@ def logdiag(L):
     rc = _native.lib().chp_logdiag(plan.handle, b.data_ptr(), out.data_ptr(), _stream(b))
     _raise_rc(rc, "logdiag")
     return out
+
+
+def copy_blkval(dst, src):
+    """dst[:] = src for two float64 GPU tensors of equal length (one vectorized
+    device copy on the current stream)."""
+    _torch()
+    if dst.numel() != src.numel():
+        raise ValueError("blkval arrays differ in length")
+    torch = _torch()
+    for t in (dst, src):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise TypeError("blkval must be a contiguous float64 GPU tensor")
+    rc = _native.lib().chp_copy(dst.data_ptr(), src.data_ptr(), dst.numel(), _stream(dst))
+    _raise_rc(rc, "copy")

@@ -507,7 +507,14 @@ class cspmatrix(object):
 
     def copy(self):
         """New cspmatrix sharing ``symb`` with a copy of ``blkval``."""
-        return cspmatrix(self.symb, blkval=self.blkval.clone(), factor=self.is_factor)
+        b = self.blkval
+        if getattr(b, "is_cuda", False) and str(b.dtype) == "torch.float64" \
+                and b.is_contiguous() and b.data_ptr() % 16 == 0:
+            from . import numeric
+            c = b.new_empty(b.shape)
+            numeric.copy_blkval(c, b)
+            return cspmatrix(self.symb, blkval=c, factor=self.is_factor)
+        return cspmatrix(self.symb, blkval=b.clone(), factor=self.is_factor)
 
     # ------------------------------------------------------ assembly/export
     def _entry_offsets(self, I, J):

@@ -91,6 +91,9 @@ int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);
  * sweeps: copies min(rows, max) rows into out[4*max], returns the row count. */
 int64_t chp_profile_launches(chp_plan* plan, double* out, int64_t max);
 int chp_plan_flops(chp_plan* plan, double* out);
+/* out[0..3]: algorithmic bytes of the cholesky / projected-inverse GEMM
+ * launches (A, B, C read once, C written once) and their launch counts. */
+int chp_plan_gemm_bytes(chp_plan* plan, double* out);
 
 /* ---------------------------------------------------------------------------
  * Numeric sweeps, in place on a device blkval array of length blkptr[nsn].
@@ -144,6 +147,10 @@ int chp_logdet(chp_plan* plan, const double* lblkval, double* out, void* stream)
  * length n; per-instance log-determinants of a block-diagonal batch are
  * segment sums of it (no reference counterpart; batching support). */
 int chp_logdiag(chp_plan* plan, const double* lblkval, double* out, void* stream);
+
+/* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
+ * restore a blkval array from a copy (no reference counterpart). */
+int chp_copy(double* dst, const double* src, int64_t n, void* stream);
 
 /* F[r[i], r[j]] += alpha U[i, j] for i >= j, with element (a, b) of F at
  * F + a*strides[0] + b*strides[1] and of U at U + a*strides[2] + b*strides[3];
