@@ -16,7 +16,7 @@ try:
     from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, dot, logdet
 except ImportError:  # numeric module not built into this checkout yet
     pass
-from . import autograd, batch
+from . import autograd, batch, mixed
 
 __version__ = "0.1.0"
 

@@ -746,6 +746,156 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same tiles with FP32 arithmetic (mixed-precision Schur updates): FP64
+// operands are rounded to FP32 as they are staged in LDS, multiplied on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation, and C = alpha acc + beta C is
+// formed in FP64.  Accumulator layout (16 x 16 f32): lane l, element v holds
+// row 4 (l / 16) + v, column l % 16.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <int OPA, int OPB, bool CHK>
+__device__ inline void gemm_tile32(const GemmTask& T, const double* __restrict__ A,
+                                   const double* __restrict__ B, double* __restrict__ C,
+                                   int row0, int col0, int kbase, int kend, double* Pp,
+                                   float (*As)[TILE + 4], float (*Bs)[TILE + 4]) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    constexpr bool AK = (OPA == 1);
+    constexpr bool BK_ = (OPB != 1);
+    const int ai = AK ? (tid >> 5) : (tid & 63);
+    const int ak = AK ? (tid & 31) : (tid >> 6);
+    const int bj = BK_ ? (tid >> 5) : (tid & 63);
+    const int bk = BK_ ? (tid & 31) : (tid >> 6);
+    double ra[8], rb[8];
+    const bool tile_in = row0 + TILE <= T.M && col0 + TILE <= T.N;
+    auto chunk_plain = [&](int kb) -> bool {
+        if (!CHK) return true;
+        if (!tile_in || kb + BK > T.K) return false;
+        if (T.triA == 1 && !(kb + BK - 1 <= row0)) return false;
+        if (T.triA == 2 && !(kb >= row0 + TILE - 1)) return false;
+        if (T.pad == 1 && !(kb >= col0 + TILE - 1)) return false;
+        if (T.pad == 2 && !(kb + BK - 1 <= col0)) return false;
+        return OPA != 2 && (OPB != 2 || kb >= col0 + TILE - 1);
+    };
+    auto fetch = [&](int kb) {
+        const bool plain = chunk_plain(kb);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+            const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+            if (plain) {
+                ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+                rb[q] = (OPB == 1) ? B[gj + (i64)gk2 * T.ldb] : B[gk2 + (i64)gj * T.ldb];
+            } else {
+                ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+                rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+            }
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            As[ak + (AK ? 0 : 4 * q)][ai + (AK ? 8 * q : 0)] = (float)ra[q];
+            Bs[bk + (BK_ ? 0 : 4 * q)][bj + (BK_ ? 8 * q : 0)] = (float)rb[q];
+        }
+    };
+    flt4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (flt4){0.f, 0.f, 0.f, 0.f};
+    if (kbase < kend) fetch(kbase);
+    for (int kb = kbase; kb < kend; kb += BK) {
+        stash();
+        __syncthreads();
+        if (kb + BK < kend) fetch(kb + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kr = ks * 4 + (lane >> 4);
+            const float a0 = As[kr][wr * 32 + (lane & 15)];
+            const float a1 = As[kr][wr * 32 + 16 + (lane & 15)];
+            const float q0 = Bs[kr][wc * 32 + (lane & 15)];
+            const float q1 = Bs[kr][wc * 32 + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, q0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, q1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, q0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, q1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row0 + wr * 32 + a * 16 + 4 * (lane >> 4) + v;
+                const int j = col0 + wc * 32 + b * 16 + (lane & 15);
+                if (CHK && (i >= T.M || j >= T.N)) continue;
+                const double r = (double)acc[a][b][v];
+                if (Pp) { Pp[i + (i64)j * T.M] = alpha * r; continue; }
+                double* c = C + i + (i64)j * T.ldc;
+                if (T.tri == 0 || i >= j) *c = alpha * r + (beta != 0.0 ? beta * (*c) : 0.0);
+                else if (T.tri == 2) *c = 0.0;
+            }
+}
+
+__global__ __launch_bounds__(TPB) void k_gemm_tiles32(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
+    __shared__ float As[BK][TILE + 4];
+    __shared__ float Bs[BK][TILE + 4];
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    const double* A = bt.p[T.bufA] + T.offA;
+    const double* B = bt.p[T.bufB] + T.offB;
+    double* C = bt.p[T.bufC] + T.offC;
+    const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
+    int kbase = 0, kend = T.K;
+    double* Pp = nullptr;
+    if (T.ksplit > 1) {
+        kbase = tl.ks * T.kchunk;
+        kend = min(T.K, kbase + T.kchunk);
+        Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
+        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // the reduction handles it
+    }
+    if (T.triA == 1) kend = min(kend, row0 + TILE);
+    if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
+    if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
+    if (T.pad == 2) kend = min(kend, col0 + TILE);
+    if (T.tri == 2 && row0 + TILE - 1 < col0) {
+        for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
+            const int i = row0 + e % TILE, j = col0 + e / TILE;
+            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
+        }
+        return;
+    }
+    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && ((kend - kbase) % BK) == 0 &&
+                      T.triA == 0 && T.pad == 0 && T.opA < 2 && T.opB < 2;
+    const int key = T.opA * 3 + T.opB;
+    if (full) {
+        switch (key) {
+        case 0: gemm_tile32<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile32<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile32<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile32<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        }
+    } else {
+        switch (key) {
+        case 0: gemm_tile32<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile32<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 2: gemm_tile32<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile32<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 4: gemm_tile32<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 5: gemm_tile32<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 6: gemm_tile32<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 7: gemm_tile32<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile32<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        }
+    }
+}
+
 // blk[nn + i, j] = -S[i + j*na] for the listed (front, 16-column) tiles:
 // F21 = -V_k B21 written back after F11 has consumed B21
 // (projected_inverse.c:51-56).

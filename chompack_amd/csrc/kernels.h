@@ -33,6 +33,7 @@ __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const doub
                              unsigned long long* status);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+__global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 // descriptor-addressed dense operations (completion, Hessian)
 __global__ void k_diag_tasks(const DiagTask* tasks, BufTab bt, unsigned long long* status);
 __global__ void k_matop(const MatOp* ops, BufTab bt);
@@ -67,6 +68,9 @@ __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,
                               const int* colfront, const double* x, const double* y,
                               double* partial);
 __global__ void k_copy(double* dst, const double* src, i64 n);
+__global__ void k_spmv_residual(i64 n, const i64* rowptr, const int* colind, const double* vals,
+                                const double* X, int ldx, const double* B, int ldb, double* R,
+                                int ldr, int nrhs);
 __global__ void k_logdiag(const int* cols, int ncols, const FrontInfo* fi, const int* colfront,
                           const double* x, double* partial);
 __global__ void k_frontal_add(double* F, i64 N, const i64* r, const double* U,

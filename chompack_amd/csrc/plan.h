@@ -138,7 +138,8 @@ struct chp_plan {
     hipStream_t gstream = nullptr;
     hipEvent_t gev_in = nullptr, gev_out = nullptr;
     struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
-    Cached g_chol, g_projinv, g_llt;
+    Cached g_chol, g_projinv, g_llt, g_chol32;
+    bool schur32 = false;  // the cholesky sweep in progress runs Schur updates in FP32
     // profiling: per-kernel-kind time from HIP events around every launch
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -155,7 +156,9 @@ GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, i
             int tri, int triA = 0, int triB = 0);
 double task_flops(const GemmTask& t);
 void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool skip_upper);
-void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles);
+// schur = 1 marks a Schur-complement update (FP32 arithmetic in the mixed sweep)
+void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles,
+               int schur = 0);
 // Uploads a program's lists and tasks if they grew since the last upload.
 int sync_program(Program& pg);
 // Replays `seq` of program `pg` on stream s with buffer table bt.

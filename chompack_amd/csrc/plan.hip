@@ -120,10 +120,11 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
         }
 }
 
-void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles) {
+void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles,
+               int schur) {
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
-    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->pg->lists.add(tiles), 0};
+    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->pg->lists.add(tiles), schur};
     double fl = 0.0;
     int last = -1;
     for (const GemmTile& g : tiles)
@@ -189,7 +190,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                              nj - c1, p1 - c1, c1 - c0, -1.0, 1.0, 1), true);
             }
             P->chol.push_back({K_PANEL_FACTOR, (int)pf.size(), 256, 0, P->pg->lists.add(pf), 0});
-            push_gemm(P, P->chol, it);
+            push_gemm(P, P->chol, it, 1);
         }
         // trailing update right of the outer panel, K = p1 - p0
         std::vector<GemmTile> gt;
@@ -208,7 +209,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                          0, f.blkptr + nn + (i64)p0 * nj, nj, 1,
                                          1, f.updptr, na, na, na, p1 - p0, -1.0, 1.0, 1), true);
         }
-        push_gemm(P, P->chol, gt);
+        push_gemm(P, P->chol, gt, 1);
     }
     // stored convention B21 = L21 L11^{-1}: groups of NB2/NB column blocks
     // from the right; inside a group block by block (X_b = R_b Dinv_b, update
@@ -517,7 +518,10 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            if (L.iarg == 1 && P->schur32)  // Schur update in FP32 arithmetic
+                CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            else
+                CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_SPLITK_REDUCE:
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
@@ -742,7 +746,7 @@ void chp_plan_destroy(chp_plan* P) {
     (void)hipDeviceSynchronize();  // no launch of this plan may still be in flight
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
-    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt})
+    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32})
         if (G->exec) (void)hipGraphExecDestroy(G->exec);
     if (P->gev_in) (void)hipEventDestroy(P->gev_in);
     if (P->gev_out) (void)hipEventDestroy(P->gev_out);
@@ -859,6 +863,16 @@ int chp_cholesky(chp_plan* P, double* blkval, void* stream, i64* info) {
     return read_status(P, s, info);
 }
 
+int chp_cholesky_mixed(chp_plan* P, double* blkval, void* stream, i64* info) {
+    if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    P->schur32 = true;
+    int rc = run_sweep(P, P->chol, P->g_chol32, blkval, s, true);
+    P->schur32 = false;
+    if (rc) return rc;
+    return read_status(P, s, info);
+}
+
 int chp_projected_inverse(chp_plan* P, double* blkval, void* stream, i64* info) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
     hipStream_t s = (hipStream_t)stream;
@@ -907,6 +921,19 @@ int chp_logdet(chp_plan* P, const double* lblkval, double* out, void* stream) {
     for (i64 j = 0; j < P->n; ++j) v += h[j];
     *out = 2.0 * v;
     return std::isfinite(v) ? 0 : 1;
+}
+
+int chp_spmv_residual(i64 n, const i64* rowptr, const int* colind, const double* vals,
+                      const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
+                      int nrhs, void* stream) {
+    if (n <= 0 || nrhs <= 0) return 0;
+    if (!rowptr || !colind || !vals || !X || !B || !R) { chp_set_error("null argument"); return -1; }
+    const i64 total = n * (i64)nrhs;
+    const i64 blocks = std::min<i64>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_spmv_residual, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       n, rowptr, colind, vals, X, ldx, B, ldb, R, ldr, nrhs);
+    HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 int chp_copy(double* dst, const double* src, i64 n, void* stream) {

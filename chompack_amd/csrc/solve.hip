@@ -265,6 +265,29 @@ __global__ __launch_bounds__(TPB) void k_logdiag(const int* __restrict__ cols, i
     partial[j] = log(x[f.blkptr + jl + (i64)jl * (f.nn + f.na)]);
 }
 
+// R = B - A X for a CSR matrix A (n x n, full symmetric pattern) and
+// column-major n x nrhs X, B, R; one thread per (row, column), rows fastest.
+__global__ __launch_bounds__(TPB) void k_spmv_residual(
+    i64 n, const i64* __restrict__ rowptr, const int* __restrict__ colind,
+    const double* __restrict__ vals, const double* __restrict__ X, int ldx,
+    const double* __restrict__ B, int ldb, double* __restrict__ R, int ldr, int nrhs) {
+    const i64 total = n * (i64)nrhs;
+    for (i64 e = (i64)blockIdx.x * TPB + threadIdx.x; e < total; e += (i64)gridDim.x * TPB) {
+        const i64 i = e % n;
+        const int j = (int)(e / n);
+        const double* x = X + (i64)j * ldx;
+        double s0 = 0.0, s1 = 0.0;
+        i64 p = rowptr[i];
+        const i64 p1 = rowptr[i + 1];
+        for (; p + 1 < p1; p += 2) {
+            s0 += vals[p] * x[colind[p]];
+            s1 += vals[p + 1] * x[colind[p + 1]];
+        }
+        if (p < p1) s0 += vals[p] * x[colind[p]];
+        R[i + (i64)j * ldr] = B[i + (i64)j * ldb] - (s0 + s1);
+    }
+}
+
 // dst[0:n] = src[0:n], 16-byte vector accesses, grid-stride.
 __global__ __launch_bounds__(TPB) void k_copy(double* __restrict__ dst,
                                               const double* __restrict__ src, i64 n) {

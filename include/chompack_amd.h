@@ -107,6 +107,13 @@ int chp_cholesky(chp_plan* plan, double* blkval, void* stream, int64_t* info);
 /* Chordal product X = L L^T; replaces llt() (llt.c:3-60, cbase.c:364-368). */
 int chp_llt(chp_plan* plan, double* blkval, void* stream);
 
+/* Cholesky with the Schur-complement updates (the trailing and inner-panel
+ * updates of large fronts) in FP32 arithmetic on MFMA; diagonal blocks,
+ * panels, the stored-convention solves and small fronts stay FP64.  The
+ * factor is accurate to about FP32 precision; pair it with iterative
+ * refinement (chp_spmv_residual + chp_trsm).  No reference counterpart. */
+int chp_cholesky_mixed(chp_plan* plan, double* blkval, void* stream, int64_t* info);
+
 /* Projected inverse Y = P(X^{-1}) from the factor; replaces
  * projected_inverse() (projected_inverse.c:3-75, cbase.c:470-474). */
 int chp_projected_inverse(chp_plan* plan, double* blkval, void* stream, int64_t* info);
@@ -147,6 +154,14 @@ int chp_logdet(chp_plan* plan, const double* lblkval, double* out, void* stream)
  * length n; per-instance log-determinants of a block-diagonal batch are
  * segment sums of it (no reference counterpart; batching support). */
 int chp_logdiag(chp_plan* plan, const double* lblkval, double* out, void* stream);
+
+/* R = B - A X for a device CSR matrix (rowptr int64[n+1], colind int32,
+ * vals float64; the full symmetric pattern) and column-major device arrays
+ * X, B, R with nrhs columns: the FP64 residual of iterative refinement
+ * (no reference counterpart). */
+int chp_spmv_residual(int64_t n, const int64_t* rowptr, const int* colind, const double* vals,
+                      const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
+                      int nrhs, void* stream);
 
 /* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
  * restore a blkval array from a copy (no reference counterpart). */

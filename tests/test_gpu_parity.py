@@ -337,6 +337,13 @@ def test_batched_512_grids_logdet_and_solve():
     for A, xi, bi in zip(As, bs.split(x), bs.split(bh)):
         r = np.linalg.norm(cp.symmetrize(A) @ xi - bi) / np.linalg.norm(bi)
         assert r <= 1e-12
+    # the same batch with FP32 Schur updates: logdet error against FP64
+    Xm = bs.assemble()
+    cp.mixed.cholesky(Xm)
+    ldm = bs.logdet(Xm)
+    err = np.abs(ldm - ld) / np.abs(ld)
+    print("batched mixed logdet rel err max %.3e" % err.max(), flush=True)
+    assert err.max() <= 1e-6
 
 
 def test_grid32_gradient_and_hessian_directions():
@@ -482,3 +489,36 @@ def test_rgg_200k_cholesky_and_64rhs_solve():
     e = _rel(X.blkval.cpu().numpy()[m], L[m])
     print("rgg cholesky rel err vs reference %.3e" % e, flush=True)
     assert e <= TOL
+
+
+def test_mixed_precision_refined_solve():
+    """FP32 Schur updates + FP64 iterative refinement: the refined solve
+    reaches max_j ||b_j - A x_j|| / ||b_j|| <= 1e-10; the mixed factor's
+    logdet is within 1e-6 relative of the FP64 one, its factor within 1e-5
+    relative (FP32 rounding of the updates, each element accumulated in FP32)."""
+    import torch
+    A = problems.grid3d(16, seed=1)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    symb._plans["cuda:0"] = numeric.Plan(symb, "cuda:0", small_max=16)  # exercise large fronts
+    X0 = cp.cspmatrix(symb, device="cuda:0") + A
+    L64 = X0.copy()
+    cp.cholesky(L64)
+    L32 = X0.copy()
+    cp.mixed.cholesky(L32)
+    assert L32.is_factor is True
+    e = float((L32.blkval - L64.blkval).abs().max() / L64.blkval.abs().max())
+    print("mixed factor rel diff %.3e" % e, flush=True)
+    assert 0.0 < e <= 1e-5   # FP32 arithmetic was used, and only at FP32 level
+    l64, l32 = cp.logdet(L64), cp.logdet(L32)
+    assert abs(l32 - l64) <= 1e-6 * abs(l64)
+    Am = cp.mixed.SpMat(A)
+    rng = np.random.default_rng(2)
+    Bh = rng.standard_normal((symb.n, 4))
+    B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0")
+    x, hist = cp.mixed.solve(Am, L32, B, tol=1e-10)
+    print("refinement", hist, flush=True)
+    assert hist[-1] <= 1e-10
+    assert np.abs(B.cpu().numpy() - Bh).max() == 0.0   # B untouched
+    As = cp.symmetrize(A).tocsr()
+    res = np.linalg.norm(As @ x.cpu().numpy() - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
+    assert res.max() <= 1e-10
