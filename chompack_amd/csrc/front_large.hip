@@ -909,9 +909,22 @@ __global__ __launch_bounds__(TPB) void k_neg_copy(
     const int nn = f.nn, na = f.na, nj = nn + na;
     double* blk = blkval + f.blkptr + nn;
     const double* S = scratch + sptr[k];
-    const int c1 = min(c0 + 16, nn);
-    for (int j = c0; j < c1; ++j)
-        for (int i = threadIdx.x; i < na; i += TPB) blk[i + (i64)j * nj] = -S[i + (i64)j * na];
+    const int nc = min(c0 + 16, nn) - c0;
+    const i64 total = (i64)nc * na;
+    // 16 columns x na rows over all threads, rows fastest, 4 in flight
+    for (i64 e0 = threadIdx.x; e0 < total; e0 += 4 * TPB) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const i64 e = e0 + (i64)u * TPB;
+            v[u] = (e < total) ? S[(e % na) + (i64)(c0 + e / na) * na] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const i64 e = e0 + (i64)u * TPB;
+            if (e < total) blk[(e % na) + (i64)(c0 + e / na) * nj] = -v[u];
+        }
+    }
 }
 
 // scratch[sptr[k] + i + j*nj] = blk[i + j*nj] for the rows i < nn of the
@@ -936,13 +949,16 @@ __global__ __launch_bounds__(TPB) void k_copy_top(
 // slice order (deterministic), honouring the task's triangular output mode.
 __global__ __launch_bounds__(TPB) void k_splitk_reduce(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
-    const GemmTile tl = tiles[blockIdx.x];
+    // tiles[i / 8] is the output tile; workgroup i % 8 reduces its 8 columns
+    // i % 8 * 8 .. +7 (slices summed in slice order)
+    const GemmTile tl = tiles[blockIdx.x >> 3];
+    const int q = blockIdx.x & 7;
     const GemmTask T = tasks[tl.task];
     double* C = bt.p[T.bufC] + T.offC;
     const double* Pp = bt.p[4] + T.offP;
     const i64 mn = (i64)T.M * T.N;
-    for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
-        const int i = tl.tm * TILE + (e & 63), j = tl.tn * TILE + (e >> 6);
+    for (int e = threadIdx.x; e < TILE * 8; e += TPB) {
+        const int i = tl.tm * TILE + (e & 63), j = tl.tn * TILE + q * 8 + (e >> 6);
         if (i >= T.M || j >= T.N) continue;
         double* c = C + i + (i64)j * T.ldc;
         if (T.tri != 0 && i < j) {

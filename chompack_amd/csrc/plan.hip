@@ -140,8 +140,8 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
         (mine ? red : keep).push_back(g);
     }
     P->pg->split_out.swap(keep);
-    if (!red.empty())
-        seq.push_back({K_SPLITK_REDUCE, (int)red.size(), 256, 0, P->pg->lists.add(red), 0});
+    if (!red.empty())  // eight workgroups per output tile
+        seq.push_back({K_SPLITK_REDUCE, 8 * (int)red.size(), 256, 0, P->pg->lists.add(red), 0});
     if (P->pg->split_out.empty()) P->pg->splitp_cur = 0;
 }
 
