@@ -13,9 +13,9 @@ enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_FACTOR, K_GEMM, K_EXTRACT,
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
-    K_SGATHER_F, K_SGATHER_B, K_SSCATTER
+    K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_FORK, K_JOIN
 };
-const int NKIDS = 20;
+const int NKIDS = 20;  // kernel kinds (K_FORK / K_JOIN are stream markers)
 
 struct Launch {
     Kid kid;
@@ -26,6 +26,7 @@ struct Launch {
     int iarg;
     double flops = 0.0;  // algorithmic flops of a GEMM launch
     int bsel = 0;        // buffer (BufTab index) used as blkval by front kernels
+    int side = 0;        // 0: the sweep's stream; 1, 2: side stream (after a K_FORK of it)
 };
 
 struct ListBuf {
@@ -50,6 +51,18 @@ struct Program {
     i64 splitp_cur = 0, splitp_len = 0;
     std::vector<GemmTile> split_out;    // output tiles of split tasks, current launch
     i64 split_doubles() const { return splitp_len + 1; }
+    // GEMMs built while split_side is set run on side stream 2: their split-K
+    // partials take a region of their own, placed after the main stream's by
+    // place_side_splits() once the program is complete
+    bool split_side = false;
+    i64 side_cur = 0, side_len = 0;
+    std::vector<int> side_tasks;
+    void place_side_splits() {
+        for (int id : side_tasks) tasks[id].offP += splitp_len;
+        splitp_len += side_len;
+        side_tasks.clear();
+        side_len = 0;
+    }
 };
 
 // Solve schedule for one right-hand-side count (GEMM shapes and work offsets
@@ -137,6 +150,11 @@ struct chp_plan {
     bool use_graphs = true;
     hipStream_t gstream = nullptr;
     hipEvent_t gev_in = nullptr, gev_out = nullptr;
+    // side streams for independent work (K_FORK / K_JOIN with iarg = 1 or 2):
+    // 1 carries a level's small fronts, 2 the stored-convention conversion
+    hipStream_t sstream[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t sev_fork[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t sev_join[3] = {nullptr, nullptr, nullptr};
     struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
     Cached g_chol, g_projinv, g_llt, g_chol32;
     bool schur32 = false;  // the cholesky sweep in progress runs Schur updates in FP32

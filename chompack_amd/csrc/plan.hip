@@ -107,9 +107,17 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
         ks = (t.K + chunk - 1) / chunk;
         t.ksplit = ks;
         t.kchunk = chunk;
-        t.offP = P->pg->splitp_cur;
-        P->pg->splitp_cur += (i64)ks * t.M * t.N;
-        P->pg->splitp_len = std::max(P->pg->splitp_len, P->pg->splitp_cur);
+        Program& pg = *P->pg;
+        if (pg.split_side) {
+            t.offP = pg.side_cur;
+            pg.side_cur += (i64)ks * t.M * t.N;
+            pg.side_len = std::max(pg.side_len, pg.side_cur);
+            pg.side_tasks.push_back(id);
+        } else {
+            t.offP = pg.splitp_cur;
+            pg.splitp_cur += (i64)ks * t.M * t.N;
+            pg.splitp_len = std::max(pg.splitp_len, pg.splitp_cur);
+        }
     }
     P->pg->tasks.push_back(t);
     for (int tn = 0; tn < nt; ++tn)
@@ -142,7 +150,7 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     P->pg->split_out.swap(keep);
     if (!red.empty())  // eight workgroups per output tile
         seq.push_back({K_SPLITK_REDUCE, 8 * (int)red.size(), 256, 0, P->pg->lists.add(red), 0});
-    if (P->pg->split_out.empty()) P->pg->splitp_cur = 0;
+    if (P->pg->split_out.empty()) P->pg->splitp_cur = P->pg->side_cur = 0;
 }
 
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
@@ -154,8 +162,15 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         if (chol_is_small(P, f)) { small.push_back(k); lds = std::max(lds, chol_small_lds(f)); }
         else large.push_back(k);
     }
-    if (!small.empty())
-        P->chol.push_back({K_CHOL_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
+    // small fronts (LDS kernel) and large fronts of one level write disjoint
+    // front and update storage: the small-front launch runs on the side stream
+    const bool fork = !small.empty() && !large.empty();
+    if (fork) P->chol.push_back({K_FORK, 0, 0, 0, 0, 1});
+    if (!small.empty()) {
+        Launch Ls{K_CHOL_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0};
+        Ls.side = fork ? 1 : 0;
+        P->chol.push_back(Ls);
+    }
     if (large.empty()) return;
     P->n_large_chol += (int)large.size();
     std::vector<int2> et;
@@ -219,6 +234,12 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // (writing them into blkval in that launch would race with workgroups
     // still reading the unfactored block): copy them into blkval and replace
     // the staged copies by their inverses (stored convention below)
+    // Nothing later in the factorization reads these blocks (parents read only
+    // update matrices), so the conversion runs on side stream 2, overlapping
+    // the next levels; the sweep joins it at its end.
+    const size_t conv0 = P->chol.size();
+    P->chol.push_back({K_FORK, 0, 0, 0, 0, 2});
+    P->pg->split_side = true;
     std::vector<DiagTask> inv;
     for (int k : large) {
         const FrontInfo& f = P->fi[k];
@@ -285,15 +306,25 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
         }
         push_gemm(P, P->chol, gt);
     }
+    P->pg->split_side = false;
+    if (P->chol.size() == conv0 + 1)
+        P->chol.pop_back();  // nothing to overlap
+    else
+        for (size_t i = conv0 + 1; i < P->chol.size(); ++i) P->chol[i].side = 2;
+    if (fork) P->chol.push_back({K_JOIN, 0, 0, 0, 0, 1});
 }
 
 // Projected-inverse launches for one level (projected_inverse.c:25-73).
-void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need) {
+// X = L11^{-1} of a large front reads only the factor, so it is computed on
+// side stream 2, which runs ahead through the levels; every large front has
+// its own X at h_xptr[k] in [0, wbase).  W (inversion temporary, side stream
+// 2 only) is per level from wbase on, T = V_k B21 (main stream only) per level
+// from tbase on: each stream reuses its own region level after level.
+void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64 tbase,
+                         i64& xcur, i64& scratch_need) {
     std::vector<int2> xt;
     for (int k : fr)
         for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
-    if (!xt.empty())
-        P->projinv.push_back({K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0});
     std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
@@ -301,65 +332,90 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_n
         if (projinv_is_small(P, f)) { small.push_back(k); lds = std::max(lds, projinv_small_lds(f)); }
         else large.push_back(k);
     }
-    if (!small.empty())
-        P->projinv.push_back({K_PROJINV_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
-    if (large.empty()) return;
-    P->n_large_projinv += (int)large.size();
-    // per-front scratch: X = L11^{-1} (nn x nn), W (nn x nn), T = V B21 (na x nn)
-    i64 off = 0;
-    std::vector<i64> woff(large.size());
-    for (size_t q = 0; q < large.size(); ++q) {
-        const FrontInfo& f = P->fi[large[q]];
-        P->h_xptr[large[q]] = off; off += (i64)f.nn * f.nn;
-        woff[q] = off; off += (i64)f.nn * f.nn;
-        P->h_tptr[large[q]] = off; off += (i64)f.na * f.nn;
-        off = (off + 31) / 32 * 32;
-    }
-    scratch_need = std::max(scratch_need, off);
-    std::vector<int2> bt;
-    for (int k : large) {
-        int nbk = (P->fi[k].nn + NB - 1) / NB;
-        for (int b = 0; b < nbk; ++b) bt.push_back(make_int2(k, b));
-    }
-    P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 256, 0, P->pg->lists.add(bt), 0});
-    // X = L11^{-1} by recursive halving of the block range [a, b) at m:
-    //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
-    // all nodes of one height (over all large fronts) form one launch pair.
-    std::vector<std::vector<std::array<int, 4>>> byh;  // (q, a, m, b)
-    std::function<int(int, int, int)> tree = [&](int q, int a, int b) -> int {
-        if (b - a <= 1) return 0;
-        int m = (a + b) / 2;
-        int h = 1 + std::max(tree(q, a, m), tree(q, m, b));
-        if ((int)byh.size() < h) byh.resize(h);
-        byh[h - 1].push_back({q, a, m, b});
-        return h;
-    };
-    for (size_t q = 0; q < large.size(); ++q)
-        tree((int)q, 0, (P->fi[large[q]].nn + NB - 1) / NB);
-    for (auto& level_nodes : byh) {
-        std::vector<GemmTile> g1, g2;
-        for (auto& nd : level_nodes) {
-            int q = nd[0];
-            int k = large[q];
-            const FrontInfo& f = P->fi[k];
-            int nn = f.nn, nj = nn + f.na;
-            int r1 = nd[1] * NB, r2 = nd[2] * NB, r3 = std::min(nd[3] * NB, nn);
-            int m1 = r2 - r1, m2 = r3 - r2;
-            i64 xo = P->h_xptr[k], wo = woff[q];
-            // W[r2:r3, r1:r2] = L[r2:r3, r1:r2] X[r1:r2, r1:r2]   (X lower)
-            add_gemm_tiles(P, g1, mk(0, f.blkptr + r2 + (i64)r1 * nj, nj, 0,
-                                     2, xo + r1 + (i64)r1 * nn, nn, 0,
-                                     2, wo + r2 + (i64)r1 * nn, nn,
-                                     m2, m1, m1, 1.0, 0.0, 0, 0, 1), false);
-            // X[r2:r3, r1:r2] = -X[r2:r3, r2:r3] W[r2:r3, r1:r2]   (X lower)
-            add_gemm_tiles(P, g2, mk(2, xo + r2 + (i64)r2 * nn, nn, 0,
-                                     2, wo + r2 + (i64)r1 * nn, nn, 0,
-                                     2, xo + r2 + (i64)r1 * nn, nn,
-                                     m2, m1, m2, -1.0, 0.0, 0, 1, 0), false);
+    if (!large.empty()) {
+        P->n_large_projinv += (int)large.size();
+        i64 woff_end = wbase, toff_end = tbase;
+        std::vector<i64> woff(large.size());
+        for (size_t q = 0; q < large.size(); ++q) {
+            const FrontInfo& f = P->fi[large[q]];
+            P->h_xptr[large[q]] = xcur;
+            xcur += ((i64)f.nn * f.nn + 31) / 32 * 32;
+            woff[q] = woff_end; woff_end += ((i64)f.nn * f.nn + 31) / 32 * 32;
         }
-        push_gemm(P, P->projinv, g1);
-        push_gemm(P, P->projinv, g2);
+        for (size_t q = 0; q < large.size(); ++q) {
+            const FrontInfo& f = P->fi[large[q]];
+            P->h_tptr[large[q]] = toff_end;
+            toff_end += ((i64)f.na * f.nn + 31) / 32 * 32;
+        }
+        scratch_need = std::max(scratch_need, toff_end);  // W ends at or below tbase
+        // side stream 2: diagonal block inverses, then recursive halving of
+        // the block range [a, b) at m:
+        //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
+        // all nodes of one height (over all large fronts) form one launch pair.
+        const size_t s0 = P->projinv.size();
+        P->pg->split_side = true;
+        std::vector<int2> bt;
+        for (int k : large) {
+            int nbk = (P->fi[k].nn + NB - 1) / NB;
+            for (int b = 0; b < nbk; ++b) bt.push_back(make_int2(k, b));
+        }
+        P->projinv.push_back({K_TRTRI_DIAG, (int)bt.size(), 256, 0, P->pg->lists.add(bt), 0});
+        std::vector<std::vector<std::array<int, 4>>> byh;  // (q, a, m, b)
+        std::function<int(int, int, int)> tree = [&](int q, int a, int b) -> int {
+            if (b - a <= 1) return 0;
+            int m = (a + b) / 2;
+            int h = 1 + std::max(tree(q, a, m), tree(q, m, b));
+            if ((int)byh.size() < h) byh.resize(h);
+            byh[h - 1].push_back({q, a, m, b});
+            return h;
+        };
+        for (size_t q = 0; q < large.size(); ++q)
+            tree((int)q, 0, (P->fi[large[q]].nn + NB - 1) / NB);
+        for (auto& level_nodes : byh) {
+            std::vector<GemmTile> g1, g2;
+            for (auto& nd : level_nodes) {
+                int q = nd[0];
+                int k = large[q];
+                const FrontInfo& f = P->fi[k];
+                int nn = f.nn, nj = nn + f.na;
+                int r1 = nd[1] * NB, r2 = nd[2] * NB, r3 = std::min(nd[3] * NB, nn);
+                int m1 = r2 - r1, m2 = r3 - r2;
+                i64 xo = P->h_xptr[k], wo = woff[q];
+                // W[r2:r3, r1:r2] = L[r2:r3, r1:r2] X[r1:r2, r1:r2]   (X lower)
+                add_gemm_tiles(P, g1, mk(0, f.blkptr + r2 + (i64)r1 * nj, nj, 0,
+                                         2, xo + r1 + (i64)r1 * nn, nn, 0,
+                                         2, wo + r2 + (i64)r1 * nn, nn,
+                                         m2, m1, m1, 1.0, 0.0, 0, 0, 1), false);
+                // X[r2:r3, r1:r2] = -X[r2:r3, r2:r3] W[r2:r3, r1:r2]   (X lower)
+                add_gemm_tiles(P, g2, mk(2, xo + r2 + (i64)r2 * nn, nn, 0,
+                                         2, wo + r2 + (i64)r1 * nn, nn, 0,
+                                         2, xo + r2 + (i64)r1 * nn, nn,
+                                         m2, m1, m2, -1.0, 0.0, 0, 1, 0), false);
+            }
+            push_gemm(P, P->projinv, g1);
+            push_gemm(P, P->projinv, g2);
+        }
+        P->pg->split_side = false;
+        for (size_t i = s0; i < P->projinv.size(); ++i) P->projinv[i].side = 2;
     }
+    // side stream 1: V_k for every front of the level (needs the parents'
+    // finished projected inverse, so it forks from the main stream here) and
+    // the small fronts
+    const bool fork = !large.empty() && (!xt.empty() || !small.empty());
+    if (fork) P->projinv.push_back({K_FORK, 0, 0, 0, 0, 1});
+    if (!xt.empty()) {
+        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0};
+        Lx.side = fork ? 1 : 0;
+        P->projinv.push_back(Lx);
+    }
+    if (!small.empty()) {
+        Launch Ls{K_PROJINV_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0};
+        Ls.side = fork ? 1 : 0;
+        P->projinv.push_back(Ls);
+    }
+    if (large.empty()) return;
+    P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
+    if (fork) P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 1});
     // F11 = X^T X (lower, strict upper zeroed) and T = V B21
     std::vector<GemmTile> g3, g4;
     for (int k : large) {
@@ -490,9 +546,34 @@ int sync_program(Program& pg) {
     return 0;
 }
 
+static int ensure_side_stream(chp_plan* P) {
+    for (int i = 1; i < 3; ++i) {
+        if (P->sstream[i]) continue;
+        HIP_CHECK(hipStreamCreateWithFlags(&P->sstream[i], hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&P->sev_fork[i], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&P->sev_join[i], hipEventDisableTiming));
+    }
+    return 0;
+}
+
 int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
-            hipStream_t s) {
+            hipStream_t s0) {
+    const bool two = !P->prof;  // profiling serializes everything on one stream
+    if (two && ensure_side_stream(P)) return -2;
     for (const Launch& L : seq) {
+        if (L.kid == K_FORK || L.kid == K_JOIN) {
+            if (!two) continue;
+            const int i = L.iarg;
+            if (L.kid == K_FORK) {
+                HIP_CHECK(hipEventRecord(P->sev_fork[i], s0));
+                HIP_CHECK(hipStreamWaitEvent(P->sstream[i], P->sev_fork[i], 0));
+            } else {
+                HIP_CHECK(hipEventRecord(P->sev_join[i], P->sstream[i]));
+                HIP_CHECK(hipStreamWaitEvent(s0, P->sev_join[i], 0));
+            }
+            continue;
+        }
+        const hipStream_t s = (two && L.side) ? P->sstream[L.side] : s0;
         const void* list = pg.d_lists + L.list;
         double* blkval = bt.p[L.bsel];
         dim3 g(L.grid), b(L.block);
@@ -589,6 +670,7 @@ int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, 
         HIP_CHECK(hipEventCreateWithFlags(&P->gev_in, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
     }
+    if (ensure_side_stream(P)) return -2;  // created outside any capture
     if (G.key != (const void*)blkval || !G.exec) {
         if (G.exec) { HIP_CHECK(hipGraphExecDestroy(G.exec)); G.exec = nullptr; }
         hipGraph_t graph = nullptr;
@@ -694,8 +776,28 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     P->dinv_len = dinv;
     P->solve_len = wun;
     for (int L = 0; L < H; ++L) build_chol_level(P, P->bylevel[L]);
-    i64 scratch = 0;
-    for (int L = H - 1; L >= 0; --L) build_projinv_level(P, P->bylevel[L], scratch);
+    P->chol.push_back({K_JOIN, 0, 0, 0, 0, 2});
+    // scratch of the projected inverse: X = L11^{-1} of every large front,
+    // then the largest level's W, then the largest level's T
+    i64 wbase = 0, wmax = 0;
+    for (int L = 0; L < H; ++L) {
+        i64 w = 0;
+        for (int k : P->bylevel[L]) {
+            const FrontInfo& f = P->fi[k];
+            if (projinv_is_small(P, f)) continue;
+            const i64 sq = ((i64)f.nn * f.nn + 31) / 32 * 32;
+            wbase += sq;
+            w += sq;
+        }
+        wmax = std::max(wmax, w);
+    }
+    i64 scratch = wbase + wmax, xcur = 0;
+    // side stream 2 (the inversions) starts from the sweep's stream
+    P->projinv.push_back({K_FORK, 0, 0, 0, 0, 2});
+    for (int L = H - 1; L >= 0; --L)
+        build_projinv_level(P, P->bylevel[L], wbase, wbase + wmax, xcur, scratch);
+    P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
+    if (xcur != wbase) return fail("projected-inverse scratch layout");
     for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
     // device copies
@@ -724,6 +826,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= upload(&P->d_lltptr, P->h_lltptr);
     rc |= upload(&P->d_colfront, colfront);
     rc |= upload(&P->d_colidx, colidx);
+    P->main.place_side_splits();
     rc |= sync_program(P->main);
     rc |= alloc_doubles(&P->d_upd, P->upd_len);
     rc |= alloc_doubles(&P->d_dinv, P->dinv_len);
@@ -751,6 +854,11 @@ void chp_plan_destroy(chp_plan* P) {
     if (P->gev_in) (void)hipEventDestroy(P->gev_in);
     if (P->gev_out) (void)hipEventDestroy(P->gev_out);
     if (P->gstream) (void)hipStreamDestroy(P->gstream);
+    for (int i = 1; i < 3; ++i) {
+        if (P->sev_fork[i]) (void)hipEventDestroy(P->sev_fork[i]);
+        if (P->sev_join[i]) (void)hipEventDestroy(P->sev_join[i]);
+        if (P->sstream[i]) (void)hipStreamDestroy(P->sstream[i]);
+    }
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
