@@ -18,6 +18,7 @@
 // The schedule depends on the number of right-hand sides and is cached per
 // count.
 #include "plan.h"
+#include <hip/hip_ext.h>
 #include "../../include/chompack_amd.h"
 
 #include <algorithm>
@@ -138,49 +139,66 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
     bt.p[5] = S->d_tmp;
     bt.p[6] = S->d_inv;
     bt.p[7] = P->d_work;
+#define CHP_SLAUNCH(K, ...)                                                        \
+    do {                                                                       \
+        if (P->prof)                                                           \
+            hipExtLaunchKernelGGL(K, g, b, 0, s, P->ev0, P->ev1, 0, __VA_ARGS__); \
+        else                                                                   \
+            hipLaunchKernelGGL(K, g, b, 0, s, __VA_ARGS__);                    \
+    } while (0)
     for (const Launch& L : seq) {
         const void* list = S->pg.d_lists + L.list;
         const dim3 g(L.grid), b(L.block);
         switch (L.kid) {
         case K_SOLVE_FWD:
-            hipLaunchKernelGGL(k_solve_fwd, g, b, 0, s, (const int*)list, P->d_fi, blkval, P->d_snptr,
+            CHP_SLAUNCH(k_solve_fwd, (const int*)list, P->d_fi, blkval, P->d_snptr,
                                P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work,
                                P->d_wunit);
             break;
         case K_SOLVE_BWD:
-            hipLaunchKernelGGL(k_solve_bwd, g, b, 0, s, (const int*)list, P->d_fi, blkval, P->d_snptr,
+            CHP_SLAUNCH(k_solve_bwd, (const int*)list, P->d_fi, blkval, P->d_snptr,
                                P->d_relidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
             break;
         case K_SGATHER_F:
-            hipLaunchKernelGGL(k_sgather_fwd, g, b, 0, s, (const int2*)list, P->d_fi, P->d_snptr,
+            CHP_SLAUNCH(k_sgather_fwd, (const int2*)list, P->d_fi, P->d_snptr,
                                P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work,
                                P->d_wunit);
             break;
         case K_SGATHER_B:
-            hipLaunchKernelGGL(k_sgather_bwd, g, b, 0, s, (const int2*)list, P->d_fi, P->d_snptr,
+            CHP_SLAUNCH(k_sgather_bwd, (const int2*)list, P->d_fi, P->d_snptr,
                                P->d_relidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
             break;
         case K_SSCATTER:
-            hipLaunchKernelGGL(k_sscatter, g, b, 0, s, (const STile*)list, P->d_fi, P->d_snptr,
+            CHP_SLAUNCH(k_sscatter, (const STile*)list, P->d_fi, P->d_snptr,
                                P->d_perm, B, ldb, nrhs, P->d_work, P->d_wunit, S->d_tmp, L.iarg);
             break;
         case K_DIAG:
-            hipLaunchKernelGGL(k_diag_tasks, g, b, 0, s, (const DiagTask*)list, bt, P->d_status);
+            CHP_SLAUNCH(k_diag_tasks, (const DiagTask*)list, bt, P->d_status);
             break;
         case K_MATOP:
-            hipLaunchKernelGGL(k_matop, g, b, 0, s, (const MatOp*)list, bt);
+            CHP_SLAUNCH(k_matop, (const MatOp*)list, bt);
             break;
         case K_GEMM:
-            hipLaunchKernelGGL(k_gemm_tiles, g, b, 0, s, (const GemmTile*)list, S->pg.d_tasks, bt);
+            CHP_SLAUNCH(k_gemm_tiles, (const GemmTile*)list, S->pg.d_tasks, bt);
             break;
         case K_SPLITK_REDUCE:
-            hipLaunchKernelGGL(k_splitk_reduce, g, b, 0, s, (const GemmTile*)list, S->pg.d_tasks, bt);
+            CHP_SLAUNCH(k_splitk_reduce, (const GemmTile*)list, S->pg.d_tasks, bt);
             break;
         default:
             chp_set_error("internal: unexpected launch kind in solve");
             return -3;
         }
+        if (P->prof) {  // per-kind times, as for the factorization sweeps
+            HIP_CHECK(hipEventSynchronize(P->ev1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+            P->prof_ms[L.kid] += ms;
+            P->prof_cnt[L.kid] += 1;
+            P->prof_flops[L.kid] += L.flops;
+            P->prof_rows.insert(P->prof_rows.end(), {(double)L.kid, (double)L.grid, L.flops, (double)ms});
+        }
     }
+#undef CHP_SLAUNCH
     HIP_CHECK(hipGetLastError());
     return 0;
 }
