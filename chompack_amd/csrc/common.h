@@ -123,9 +123,20 @@ struct UFTask {
     unsigned char fbuf, pad[7];
 };
 
-// Row tile of a front in the solve's scatter: rows r0 .. r0 + 255 of front k,
+// Row tile of a front in the solve's scatter: rows r0 .. r0 + 63 of front k,
 // whose solved block sits at offset tofs of the solve's block buffer.
 struct STile {
     int k, r0;
     i64 tofs;
+};
+
+// Forward-solve gather map.  Slab row s = wunit[k] + i (row i of front k's
+// slab) receives the update rows gent[gptr[s] .. gptr[s + 1]) of k's children,
+// listed in reverse child order (the order in which the reference adds them,
+// trsm.c:46-58).  An entry addresses row `row` of a child slab that starts at
+// wunit `wu` and has leading dimension `ld`: element (row, j) of it is
+// work[wu * nrhs + row + j * ld].
+struct GEnt {
+    i64 wu;
+    int ld, row;
 };

@@ -43,25 +43,27 @@ __global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkva
                            const double* scratch, const i64* sptr);
 
 // triangular solve with the chordal factor
-__global__ void k_solve_fwd(const int* fronts, const FrontInfo* fi, const double* blkval,
-                            const i64* snptr, const int* relidx, const int* chidx,
-                            const i64* perm, double* B, int ldb, int nrhs, double alpha,
-                            double* work, const i64* wptr);
-__global__ void k_solve_bwd(const int* fronts, const FrontInfo* fi, const double* blkval,
+__global__ void k_solve_fwd(const int2* items, const FrontInfo* fi, const double* blkval,
+                            const i64* snptr, const i64* perm, double* B, int ldb, int nrhs,
+                            double alpha, double* work, const i64* wptr, const int* gptr,
+                            const GEnt* gent);
+__global__ void k_solve_bwd(const int2* items, const FrontInfo* fi, const double* blkval,
                             const i64* snptr, const int* relidx, const i64* perm,
                             double* B, int ldb, int nrhs, double alpha, double* work,
                             const i64* wptr);
+__global__ void k_l11_check(const STile* tiles, const FrontInfo* fi, const double* blkval,
+                            double* copy, int* changed);
 
 __global__ void k_sgather_fwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
-                              const int* relidx, const int* chidx, const i64* perm,
-                              const double* B, int ldb, int nrhs, double alpha, double* work,
-                              const i64* wptr);
+                              const i64* perm, const double* B, int ldb, int nrhs, double alpha,
+                              double* work, const i64* wptr, const int* gptr, const GEnt* gent);
 __global__ void k_sgather_bwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
                               const int* relidx, const i64* perm, const double* B, int ldb,
                               int nrhs, double alpha, double* work, const i64* wptr);
 __global__ void k_sscatter(const STile* tiles, const FrontInfo* fi, const i64* snptr,
                            const i64* perm, double* B, int ldb, int nrhs, double* work,
-                           const i64* wptr, const double* tmp, int copyback);
+                           const i64* wptr, const double* tmp, const double* tmp2,
+                           int copyback);
 
 // utilities
 __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,

@@ -70,7 +70,13 @@ struct Program {
 struct SolvePlan {
     int nrhs = 0;
     Program pg;
-    std::vector<Launch> fwd, bwd;
+    std::vector<Launch> inv, fwd, bwd;  // X_k = L11^{-1}; the two sweeps
+    i64 t2base = 0;                     // offset of T2 = B21^T x_A in d_tmp
+    std::vector<STile> chk;             // (front, 16-column tile, X offset) of the L11 check
+    STile* d_chk = nullptr;
+    int* d_changed = nullptr;           // set by k_l11_check when L11 differs from its copy
+    double* d_l11 = nullptr;            // copy of the L11 the inverses were formed from
+    bool inv_valid = false;             // d_inv holds the inverses of d_l11
     double* d_splitp = nullptr;
     // inverses of the large diagonal blocks, their workspace, solved blocks
     double* d_inv = nullptr;
@@ -114,6 +120,8 @@ struct chp_plan {
     i64* d_xptr = nullptr;
     i64* d_tptr = nullptr;
     i64* d_wunit = nullptr;
+    int* d_gptr = nullptr;   // forward-solve gather map (GEnt)
+    GEnt* d_gent = nullptr;
     int* d_colfront = nullptr;
     int* d_colidx = nullptr;
     double* d_upd = nullptr;
@@ -177,12 +185,23 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
 // schur = 1 marks a Schur-complement update (FP32 arithmetic in the mixed sweep)
 void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles,
                int schur = 0);
+// Device copy of a host vector (at least one element allocated).
+template <class T>
+int upload(T** dptr, const std::vector<T>& v) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIP_CHECK(hipMalloc((void**)dptr, bytes));
+    if (!v.empty()) HIP_CHECK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
 // Uploads a program's lists and tasks if they grew since the last upload.
 int sync_program(Program& pg);
 // Replays `seq` of program `pg` on stream s with buffer table bt.
 int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
             hipStream_t s);
 int read_status(chp_plan* P, hipStream_t s, i64* info);
+// Creates the side streams and their fork / join events (once per plan).
+int ensure_side_stream(chp_plan* P);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
 

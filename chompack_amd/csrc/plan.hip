@@ -500,14 +500,6 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
     P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0});
 }
 
-template <class T>
-int upload(T** dptr, const std::vector<T>& v) {
-    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
-    HIP_CHECK(hipMalloc((void**)dptr, bytes));
-    if (!v.empty()) HIP_CHECK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return 0;
-}
-
 int alloc_doubles(double** d, i64 len) {
     HIP_CHECK(hipMalloc((void**)d, (size_t)std::max<i64>(len, 1) * sizeof(double)));
     return 0;
@@ -546,7 +538,7 @@ int sync_program(Program& pg) {
     return 0;
 }
 
-static int ensure_side_stream(chp_plan* P) {
+int ensure_side_stream(chp_plan* P) {
     for (int i = 1; i < 3; ++i) {
         if (P->sstream[i]) continue;
         HIP_CHECK(hipStreamCreateWithFlags(&P->sstream[i], hipStreamNonBlocking));
@@ -823,6 +815,34 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     rc |= upload(&P->d_xptr, P->h_xptr);
     rc |= upload(&P->d_tptr, P->h_tptr);
     rc |= upload(&P->d_wunit, P->h_wunit);
+    {
+        // forward-solve gather map: per slab row, the children's update rows
+        // that land there, children in reverse order
+        std::vector<int> gptr(P->solve_len + 1, 0);
+        for (i64 k = 0; k < nsn; ++k)
+            for (int l = P->fi[k].chend - 1; l >= P->fi[k].chbeg; --l) {
+                const FrontInfo& g = P->fi[P->h_chidx[l]];
+                for (int q = 0; q < g.na; ++q)
+                    ++gptr[P->h_wunit[k] + P->h_relidx[g.relptr + q] + 1];
+            }
+        for (i64 s = 0; s < P->solve_len; ++s) gptr[s + 1] += gptr[s];
+        std::vector<GEnt> gent(gptr[P->solve_len]);
+        std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+        for (i64 k = 0; k < nsn; ++k)
+            for (int l = P->fi[k].chend - 1; l >= P->fi[k].chbeg; --l) {
+                const int c = P->h_chidx[l];
+                const FrontInfo& g = P->fi[c];
+                for (int q = 0; q < g.na; ++q) {
+                    GEnt e;
+                    e.wu = P->h_wunit[c];
+                    e.ld = g.nn + g.na;
+                    e.row = g.nn + q;
+                    gent[fill[P->h_wunit[k] + P->h_relidx[g.relptr + q]]++] = e;
+                }
+            }
+        rc |= upload(&P->d_gptr, gptr);
+        rc |= upload(&P->d_gent, gent);
+    }
     rc |= upload(&P->d_lltptr, P->h_lltptr);
     rc |= upload(&P->d_colfront, colfront);
     rc |= upload(&P->d_colidx, colidx);
@@ -860,7 +880,7 @@ void chp_plan_destroy(chp_plan* P) {
         if (P->sstream[i]) (void)hipStreamDestroy(P->sstream[i]);
     }
     void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
-                    P->d_xptr, P->d_tptr, P->d_wunit, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
+                    P->d_xptr, P->d_tptr, P->d_wunit, P->d_gptr, P->d_gent, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
                     P->d_dscratch, P->d_ddinv, P->d_dsplitp};
@@ -868,8 +888,10 @@ void chp_plan_destroy(chp_plan* P) {
         for (auto& g : S->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (S->pg.d_lists) (void)hipFree(S->pg.d_lists);
         if (S->pg.d_tasks) (void)hipFree(S->pg.d_tasks);
-        for (double* d : {S->d_splitp, S->d_inv, S->d_w, S->d_tmp})
+        for (double* d : {S->d_splitp, S->d_inv, S->d_w, S->d_tmp, S->d_l11})
             if (d) (void)hipFree(d);
+        if (S->d_chk) (void)hipFree(S->d_chk);
+        if (S->d_changed) (void)hipFree(S->d_changed);
         delete S;
     }
     for (void* p : ptrs)

@@ -26,23 +26,37 @@
 namespace chpi {
 namespace {
 
+// Fronts that one wave solves (k_solve_fwd / k_solve_bwd): at most 16
+// columns and 64 rows.  Everything else takes the batched gather / GEMM /
+// scatter path, whose launches spread a front's rows over many workgroups.
 bool solve_large(const FrontInfo& f, int nrhs) {
-    return f.nn > 16 || (i64)f.na * f.nn * nrhs > 32 * 1024;
+    (void)nrhs;
+    return f.nn > 16 || f.nn + f.na > 64;
 }
 
-// (front, first row) tiles of 256 rows over rows [0, nrow(front)).
+// (front, first row) tiles of `step` rows over rows [0, nrow(front)).
 template <class F>
-std::vector<int2> row_tiles(const std::vector<int>& fr, F nrow) {
+std::vector<int2> row_tiles(const std::vector<int>& fr, F nrow, int step = 256) {
     std::vector<int2> t;
     for (int k : fr)
-        for (int r0 = 0; r0 < nrow(k); r0 += 256) t.push_back(make_int2(k, r0));
+        for (int r0 = 0; r0 < nrow(k); r0 += step) t.push_back(make_int2(k, r0));
+    return t;
+}
+
+// (front, first right-hand side) items of the small-front solve kernels,
+// 16 right-hand sides per wave.
+std::vector<int2> rhs_items(const std::vector<int>& fr, int nrhs) {
+    std::vector<int2> t;
+    for (int k : fr)
+        for (int j0 = 0; j0 < nrhs; j0 += 16) t.push_back(make_int2(k, j0));
     return t;
 }
 
 template <class T>
-void push_list(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vector<T>& v) {
+void push_list(chp_plan* P, std::vector<Launch>& seq, Kid kid, const std::vector<T>& v,
+               int block = 256) {
     if (v.empty()) return;
-    seq.push_back({kid, (int)v.size(), 256, 0, P->pg->lists.add(v), 0});
+    seq.push_back({kid, (int)v.size(), block, 0, P->pg->lists.add(v), 0});
 }
 
 SolvePlan* build_solve(chp_plan* P, int nrhs) {
@@ -51,7 +65,8 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
     P->pg = &S->pg;
     const int H = P->nlevels;
     // X_k = L11^{-1} of every large front, one batched recursive-halving
-    // inverse at the start of each solve (buffer 6, ld nn; workspace buffer 2)
+    // inverse (buffer 6, ld nn; workspace buffer 2), formed again only when
+    // L11 differs from the copy it was formed from (k_l11_check)
     std::vector<i64> xo(P->fi.size(), -1), to(P->fi.size(), -1);
     std::vector<Mat> Ls, Xs;
     std::vector<i64> Ws;
@@ -66,26 +81,34 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
         Xs.push_back({6, xo[k], f.nn, f.nn});
         Ws.push_back(bw.take((i64)f.nn * f.nn));
         keys.push_back((int)k);
+        for (int c0 = 0; c0 < f.nn; c0 += 16) S->chk.push_back({(int)k, c0, xo[k]});
     }
     S->inv_len = bx.cur;
     S->w_len = bw.cur;
-    S->tmp_len = bt.cur;
-    if (!Ls.empty()) emit_trtri(P, S->fwd, Ls, Xs, Ws, keys);
-    S->bwd = S->fwd;
+    // T = X_k x_N (or X_k^T x_N) at to[k]; the backward sweep's
+    // T2 = B21^T x_A at to[k] + t2base, subtracted by the scatter
+    S->t2base = bt.cur;
+    S->tmp_len = 2 * bt.cur;
+    if (!Ls.empty()) emit_trtri(P, S->inv, Ls, Xs, Ws, keys);
     auto wofs = [&](int k) { return P->h_wunit[k] * nrhs; };
     auto stiles = [&](const std::vector<int>& large) {
         std::vector<STile> t;
         for (int k : large)
-            for (int r0 = 0; r0 < P->fi[k].nn; r0 += 256) t.push_back({k, r0, to[k]});
+            for (int r0 = 0; r0 < P->fi[k].nn; r0 += 64) t.push_back({k, r0, to[k]});
         return t;
     };
     for (int L = 0; L < H; ++L) {
         std::vector<int> small, large;
         for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
-        push_list(P, S->fwd, K_SOLVE_FWD, small);
+        // small and large fronts of a level touch disjoint slabs and rows of
+        // B: the small fronts run on side stream 1
+        const bool fork = !small.empty() && !large.empty();
+        if (fork) S->fwd.push_back({K_FORK, 0, 0, 0, 0, 1});
+        push_list(P, S->fwd, K_SOLVE_FWD, rhs_items(small, nrhs), 64);
+        if (fork) S->fwd.back().side = 1;
         if (large.empty()) continue;
         auto nj_of = [&](int k) { return P->fi[k].nn + P->fi[k].na; };
-        push_list(P, S->fwd, K_SGATHER_F, row_tiles(large, nj_of));
+        push_list(P, S->fwd, K_SGATHER_F, row_tiles(large, nj_of, 64));
         std::vector<GemmTile> g;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
@@ -100,30 +123,34 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
         push_gemm(P, S->fwd, g);
         std::vector<STile> st = stiles(large);
         push_list(P, S->fwd, K_SSCATTER, st);
+        if (fork) S->fwd.push_back({K_JOIN, 0, 0, 0, 0, 1});
     }
     for (int L = H - 1; L >= 0; --L) {
         std::vector<int> small, large;
         for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
-        push_list(P, S->bwd, K_SOLVE_BWD, small);
+        const bool fork = !small.empty() && !large.empty();
+        if (fork) S->bwd.push_back({K_FORK, 0, 0, 0, 0, 1});
+        push_list(P, S->bwd, K_SOLVE_BWD, rhs_items(small, nrhs), 64);
+        if (fork) S->bwd.back().side = 1;
         if (large.empty()) continue;
         auto nj_of = [&](int k) { return P->fi[k].nn + P->fi[k].na; };
-        push_list(P, S->bwd, K_SGATHER_B, row_tiles(large, nj_of));
-        std::vector<GemmTile> g1, g2;
+        push_list(P, S->bwd, K_SGATHER_B, row_tiles(large, nj_of, 64));
+        std::vector<GemmTile> g1;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
             const int nn = f.nn, na = f.na, nj = nn + na;
             // T = L11^{-T} x_N
             add_gemm_tiles(P, g1, mk(6, xo[k], nn, 1, 7, wofs(k), nj, 0, 5, to[k], nn,
                                      nn, nrhs, nn, 1.0, 0.0, 0, 2), false);
-            if (na > 0)  // T -= B21^T x_A
-                add_gemm_tiles(P, g2, mk(0, f.blkptr + nn, nj, 1, 7, wofs(k) + nn, nj, 0, 5, to[k],
-                                         nn, nn, nrhs, na, -1.0, 1.0, 0), false);
+            if (na > 0)  // T2 = B21^T x_A (the scatter writes T - T2)
+                add_gemm_tiles(P, g1, mk(0, f.blkptr + nn, nj, 1, 7, wofs(k) + nn, nj, 0, 5,
+                                         to[k] + S->t2base, nn, nn, nrhs, na, 1.0, 0.0, 0), false);
         }
         push_gemm(P, S->bwd, g1);
-        push_gemm(P, S->bwd, g2);
         std::vector<STile> st = stiles(large);
         Launch Lc{K_SSCATTER, (int)st.size(), 256, 0, P->pg->lists.add(st), 1};
         S->bwd.push_back(Lc);
+        if (fork) S->bwd.push_back({K_JOIN, 0, 0, 0, 0, 1});
     }
     P->pg = &P->main;
     return S;
@@ -146,23 +173,38 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
         else                                                                   \
             hipLaunchKernelGGL(K, g, b, 0, s, __VA_ARGS__);                    \
     } while (0)
+    const bool two = !P->prof;  // profiling serializes everything on one stream
+    if (two && ensure_side_stream(P)) return -2;
+    const hipStream_t s0 = s;
     for (const Launch& L : seq) {
+        if (L.kid == K_FORK || L.kid == K_JOIN) {
+            if (!two) continue;
+            const int i = L.iarg;
+            if (L.kid == K_FORK) {
+                HIP_CHECK(hipEventRecord(P->sev_fork[i], s0));
+                HIP_CHECK(hipStreamWaitEvent(P->sstream[i], P->sev_fork[i], 0));
+            } else {
+                HIP_CHECK(hipEventRecord(P->sev_join[i], P->sstream[i]));
+                HIP_CHECK(hipStreamWaitEvent(s0, P->sev_join[i], 0));
+            }
+            continue;
+        }
+        s = (two && L.side) ? P->sstream[L.side] : s0;
         const void* list = S->pg.d_lists + L.list;
         const dim3 g(L.grid), b(L.block);
         switch (L.kid) {
         case K_SOLVE_FWD:
-            CHP_SLAUNCH(k_solve_fwd, (const int*)list, P->d_fi, blkval, P->d_snptr,
-                               P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work,
-                               P->d_wunit);
+            CHP_SLAUNCH(k_solve_fwd, (const int2*)list, P->d_fi, blkval, P->d_snptr,
+                        P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit, P->d_gptr,
+                        P->d_gent);
             break;
         case K_SOLVE_BWD:
-            CHP_SLAUNCH(k_solve_bwd, (const int*)list, P->d_fi, blkval, P->d_snptr,
+            CHP_SLAUNCH(k_solve_bwd, (const int2*)list, P->d_fi, blkval, P->d_snptr,
                                P->d_relidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
             break;
         case K_SGATHER_F:
-            CHP_SLAUNCH(k_sgather_fwd, (const int2*)list, P->d_fi, P->d_snptr,
-                               P->d_relidx, P->d_chidx, P->d_perm, B, ldb, nrhs, alpha, P->d_work,
-                               P->d_wunit);
+            CHP_SLAUNCH(k_sgather_fwd, (const int2*)list, P->d_fi, P->d_snptr, P->d_perm, B,
+                        ldb, nrhs, alpha, P->d_work, P->d_wunit, P->d_gptr, P->d_gent);
             break;
         case K_SGATHER_B:
             CHP_SLAUNCH(k_sgather_bwd, (const int2*)list, P->d_fi, P->d_snptr,
@@ -170,7 +212,8 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
             break;
         case K_SSCATTER:
             CHP_SLAUNCH(k_sscatter, (const STile*)list, P->d_fi, P->d_snptr,
-                               P->d_perm, B, ldb, nrhs, P->d_work, P->d_wunit, S->d_tmp, L.iarg);
+                        P->d_perm, B, ldb, nrhs, P->d_work, P->d_wunit, S->d_tmp,
+                        L.iarg ? S->d_tmp + S->t2base : nullptr, L.iarg);
             break;
         case K_DIAG:
             CHP_SLAUNCH(k_diag_tasks, (const DiagTask*)list, bt, P->d_status);
@@ -228,6 +271,9 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         if (alloc_doubles(&S->d_inv, S->inv_len)) return -2;
         if (alloc_doubles(&S->d_w, S->w_len)) return -2;
         if (alloc_doubles(&S->d_tmp, S->tmp_len)) return -2;
+        if (alloc_doubles(&S->d_l11, S->inv_len)) return -2;
+        if (upload(&S->d_chk, S->chk)) return -2;
+        HIP_CHECK(hipMalloc((void**)&S->d_changed, sizeof(int)));
     }
     const i64 need = P->solve_len * nrhs;
     // captured solves hold the addresses of the work buffers: drop them all
@@ -249,6 +295,23 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         HIP_CHECK(hipMalloc((void**)&P->d_work, need * sizeof(double)));
         P->work_len = need;
     }
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    // the inverses X_k: formed again when L11 of a large front is not the one
+    // they were formed from (an exact comparison with a copy)
+    if (!S->chk.empty()) {
+        int changed = 0;
+        HIP_CHECK(hipMemsetAsync(S->d_changed, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_l11_check, dim3((unsigned)S->chk.size()), dim3(256), 0, s, S->d_chk,
+                           P->d_fi, blkval, S->d_l11, S->d_changed);
+        HIP_CHECK(hipMemcpyAsync(&changed, S->d_changed, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (changed || !S->inv_valid) {
+            S->inv_valid = false;
+            int rc = run_solve(P, S, S->inv, blkval, B, ldb, nrhs, alpha, s);
+            if (rc) return rc;
+            S->inv_valid = true;
+        }
+    }
     const std::vector<Launch>& seq = trans == 'N' ? S->fwd : S->bwd;
     SolvePlan::Key key;
     key.blk = blkval; key.B = B; key.ldb = ldb; key.trans = trans; key.alpha = alpha;
@@ -260,6 +323,7 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
             HIP_CHECK(hipEventCreateWithFlags(&P->gev_in, hipEventDisableTiming));
             HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
         }
+        if (ensure_side_stream(P)) return -2;  // created outside any capture
         if (!G) {
             if (S->graphs.size() >= 4) {
                 HIP_CHECK(hipStreamSynchronize(s));
@@ -268,7 +332,6 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
             }
             hipGraph_t graph = nullptr;
             HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
-            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
             int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, P->gstream);
             hipError_t e = hipStreamEndCapture(P->gstream, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -288,12 +351,12 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
     } else {
         S->last = key;
-        hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
         int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, s);
         if (rc) return rc;
     }
     i64 info[2];
     if (read_status(P, s, info) != 0) {
+        S->inv_valid = false;
         chp_set_error("trsm: singular diagonal block in supernode " + std::to_string(info[0]));
         return -4;
     }

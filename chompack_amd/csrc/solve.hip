@@ -11,216 +11,313 @@
 
 #define TPB 256
 
-// B := alpha L^{-1} B, leaves to root (trsm.c:36-81).
-__global__ __launch_bounds__(TPB) void k_solve_fwd(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+// Small fronts (nn <= 16) of the solve: one wave per (front, chunk of up to
+// 16 right-hand sides), items[i] = (front, first rhs).  L11 and the chunk of
+// x_N live in LDS; each of the first rc lanes solves its right-hand side's
+// triangular system in registers.  Rows of the slab are spread over the
+// lanes with all right-hand sides of a row in registers, so that a lane's
+// loads (B, gather map, children's slabs, a row of B21) are independent.
+#define SRC 16
+typedef double sdbl4 __attribute__((ext_vector_type(4)));
+
+// B := alpha L^{-1} B, leaves to root (trsm.c:36-81).  Row i of the front's
+// slab is alpha B[p[i]] (i < nn) or 0, plus the children's update rows
+// (gather map, reverse child order).  x_A -= B21 x_N uses x_N before its
+// solve (B21 = L21 L11^{-1}); then x_N := L11^{-1} x_N.  The slab keeps x_A
+// for the parent; x_N goes to B.
+__global__ __launch_bounds__(64) void k_solve_fwd(
+    const int2* __restrict__ items, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, const i64* __restrict__ snptr,
-    const int* __restrict__ relidx, const int* __restrict__ chidx,
     const i64* __restrict__ perm, double* __restrict__ B, int ldb, int nrhs,
-    double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
-    const int k = fronts[blockIdx.x];
+    double alpha, double* __restrict__ work, const i64* __restrict__ wptr,
+    const int* __restrict__ gptr, const GEnt* __restrict__ gent) {
+    __shared__ double Ls[16][17];
+    __shared__ double xs[16][17];
+    const int2 it = items[blockIdx.x];
+    const int k = it.x, j0 = it.y;
+    const int rc = min(SRC, nrhs - j0);
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
     const double* L = blkval + f.blkptr;
-    double* W = work + wptr[k] * nrhs;
-    const i64* pk = perm + snptr[k];  // original rows of this supernode's columns
-    for (int e = tid; e < nj * nrhs; e += TPB) {
-        int i = e % nj, j = e / nj;
-        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
+    const i64 wk = wptr[k];
+    double* W = work + wk * nrhs;
+    const i64* pk = perm + snptr[k];
+#pragma unroll
+    for (int e = lane; e < 256; e += 64) {
+        const int i = e & 15, c = e >> 4;
+        if (i < nn && c <= i) Ls[i][c] = L[i + (i64)c * nj];
+    }
+    if (lane < nn) {  // gathered right-hand side rows of x_N
+        const int i = lane;
+        const i64 bi = pk[i];
+        double g[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) g[j] = (j < rc) ? alpha * B[bi + (i64)(j0 + j) * ldb] : 0.0;
+        const int e1 = gptr[wk + i + 1];
+        for (int q = gptr[wk + i]; q < e1; ++q) {
+            const GEnt t = gent[q];
+            const double* src = work + t.wu * nrhs + t.row + (i64)j0 * t.ld;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < rc) g[j] += src[(i64)j * t.ld];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) xs[i][j] = g[j];
     }
     __syncthreads();
-    for (int l = f.chend - 1; l >= f.chbeg; --l) {
-        const int c = chidx[l];
-        const FrontInfo g = fi[c];
-        const int N = g.na, njc = g.nn + g.na;
-        const int* r = relidx + g.relptr;
-        const double* Wc = work + wptr[c] * nrhs + g.nn;
-        for (int e = tid; e < N * nrhs; e += TPB) {
-            int i = e % N, j = e / N;
-            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
+    // x_A = gathered rows - B21 x_N, one row per lane
+    for (int i = nn + lane; i < nj; i += 64) {
+        double g[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) g[j] = 0.0;
+        const int e1 = gptr[wk + i + 1];
+        for (int q = gptr[wk + i]; q < e1; ++q) {
+            const GEnt t = gent[q];
+            const double* src = work + t.wu * nrhs + t.row + (i64)j0 * t.ld;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < rc) g[j] += src[(i64)j * t.ld];
         }
-        __syncthreads();
-    }
-    // x_A -= B21 x_N  (uses x_N before the solve: B21 = L21 L11^-1)
-    for (int e = tid; e < na * nrhs; e += TPB) {
-        int i = e % na, j = e / na;
-        double s = W[nn + i + (i64)j * nj];
-        for (int m = 0; m < nn; ++m) s -= L[nn + i + (i64)m * nj] * W[m + (i64)j * nj];
-        W[nn + i + (i64)j * nj] = s;
-    }
-    // x_N := L11^{-1} x_N, column oriented
-    for (int c = 0; c < nn; ++c) {
-        __syncthreads();
-        const double d = L[c + (i64)c * nj];
-        for (int j = tid; j < nrhs; j += TPB) W[c + (i64)j * nj] /= d;
-        __syncthreads();
-        const int m = nn - c - 1;
-        for (int e = tid; e < m * nrhs; e += TPB) {
-            int i = c + 1 + e % m, j = e / m;
-            W[i + (i64)j * nj] -= L[i + (i64)c * nj] * W[c + (i64)j * nj];
+#pragma unroll 1
+        for (int m = 0; m < nn; ++m) {
+            const double lv = L[i + (i64)m * nj];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) g[j] -= lv * xs[m][j];
         }
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < rc) W[i + (i64)(j0 + j) * nj] = g[j];
     }
-    __syncthreads();
-    for (int e = tid; e < nn * nrhs; e += TPB) {
-        int i = e % nn, j = e / nn;
-        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+    if (lane < rc) {
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = (i < nn) ? xs[i][lane] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            if (c < nn) {
+                x[c] /= Ls[c][c];
+#pragma unroll
+                for (int i = c + 1; i < 16; ++i)
+                    if (i < nn) x[i] -= Ls[i][c] * x[c];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < nn) B[pk[i] + (i64)(j0 + lane) * ldb] = x[i];
     }
 }
 
-// B := alpha L^{-T} B, root to leaves (trsm.c:84-135).
-__global__ __launch_bounds__(TPB) void k_solve_bwd(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+// B := alpha L^{-T} B, root to leaves (trsm.c:84-135).  With the stored
+// B21 = L21 L11^{-1}: x_N = L11^{-T} (alpha B[p]) - B21^T x_A, x_A from the
+// parent's slab (kept in this front's slab for the children); x_N goes to B
+// and to the slab.  B21^T x_A runs on v_mfma_f64_16x16x4f64 over 16-row
+// chunks of x_A and B21 staged in LDS (operands: lane l gives A(l % 16, k)
+// and B(k, l % 16) with k = l / 16; accumulator element v of lane l is row
+// l / 16 + 4 v, column l % 16).
+__global__ __launch_bounds__(64) void k_solve_bwd(
+    const int2* __restrict__ items, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, const i64* __restrict__ snptr,
     const int* __restrict__ relidx, const i64* __restrict__ perm,
     double* __restrict__ B, int ldb, int nrhs, double alpha,
     double* __restrict__ work, const i64* __restrict__ wptr) {
-    const int k = fronts[blockIdx.x];
+    __shared__ double Ls[16][17];
+    __shared__ double xs[16][17];
+    __shared__ double xa[16][17];
+    __shared__ double lb[16][17];
+    const int2 it = items[blockIdx.x];
+    const int k = it.x, j0 = it.y;
+    const int rc = min(SRC, nrhs - j0);
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x;
     const double* L = blkval + f.blkptr;
     double* W = work + wptr[k] * nrhs;
     const i64* pk = perm + snptr[k];
-    for (int e = tid; e < nn * nrhs; e += TPB) {
-        int i = e % nn, j = e / nn;
-        W[i + (i64)j * nj] = alpha * B[pk[i] + (i64)j * ldb];
+#pragma unroll
+    for (int e = lane; e < 256; e += 64) {
+        const int i = e & 15, c = e >> 4;
+        if (i < nn && c <= i) Ls[i][c] = L[i + (i64)c * nj];
+        if (i < nn && c < rc) xs[i][c] = alpha * B[pk[i] + (i64)(j0 + c) * ldb];
     }
-    // x_N := L11^{-T} x_N (backward substitution, axpy form)
-    for (int c = nn - 1; c >= 0; --c) {
-        __syncthreads();
-        const double d = L[c + (i64)c * nj];
-        for (int j = tid; j < nrhs; j += TPB) W[c + (i64)j * nj] /= d;
-        __syncthreads();
-        for (int e = tid; e < c * nrhs; e += TPB) {
-            int i = e % c, j = e / c;
-            W[i + (i64)j * nj] -= L[c + (i64)i * nj] * W[c + (i64)j * nj];
+    __syncthreads();
+    // x_N := L11^{-T} x_N, one right-hand side per lane, in registers
+    if (lane < rc) {
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = (i < nn) ? xs[i][lane] : 0.0;
+#pragma unroll
+        for (int c = 15; c >= 0; --c) {
+            if (c < nn) {
+                x[c] /= Ls[c][c];
+#pragma unroll
+                for (int i = 0; i < c; ++i) x[i] -= Ls[c][i] * x[c];
+            }
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < nn) xs[i][lane] = x[i];
     }
+    const int col = lane & 15, kq = lane >> 4;
     if (na > 0) {
-        const FrontInfo p = fi[f.parent];
-        const int njp = p.nn + p.na;
+        const FrontInfo pf = fi[f.parent];
+        const int njp = pf.nn + pf.na;
         const double* Wp = work + wptr[f.parent] * nrhs;
         const int* r = relidx + f.relptr;
-        for (int e = tid; e < na * nrhs; e += TPB) {
-            int i = e % na, j = e / na;
-            W[nn + i + (i64)j * nj] = Wp[r[i] + (i64)j * njp];
-        }
-        __syncthreads();
-        // x_N -= B21^T x_A: one wave per output (m, j), lanes over the
-        // separator rows (coalesced column reads of B21 and x_A)
-        const int lane = tid & 63, wv = tid >> 6;
-        for (int o = wv; o < nn * nrhs; o += TPB / 64) {
-            const int m = o % nn, j = o / nn;
-            const double* lc = L + nn + (i64)m * nj;
-            const double* xa = W + nn + (i64)j * nj;
-            double acc = 0.0;
-            for (int i = lane; i < na; i += 64) acc += lc[i] * xa[i];
+        sdbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int i0 = 0; i0 < na; i0 += 16) {
+            // stage rows i0 .. i0 + 15 of x_A (also kept in the slab) and B21
+            const int ii = lane & 15;
+            const int i = i0 + ii;
+            const bool ok = i < na;
+            const int ri = ok ? r[i] : 0;
 #pragma unroll
-            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
-            if (lane == 0) W[m + (i64)j * nj] -= acc;
+            for (int h = 0; h < 4; ++h) {
+                const int c = (lane >> 4) + 4 * h;  // column 0..15
+                double v = 0.0, lv = 0.0;
+                if (ok && c < rc) v = Wp[ri + (i64)(j0 + c) * njp];
+                if (ok && c < nn) lv = L[nn + i + (i64)c * nj];
+                xa[ii][c] = v;
+                lb[ii][c] = lv;
+                if (ok && c < rc) W[nn + i + (i64)(j0 + c) * nj] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int kk = 4 * s4 + kq;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(lb[kk][col], xa[kk][col], acc, 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = kq + 4 * v;
+            if (m < nn && col < rc) xs[m][col] -= acc[v];
         }
     }
     __syncthreads();
-    for (int e = tid; e < nn * nrhs; e += TPB) {
-        int i = e % nn, j = e / nn;
-        B[pk[i] + (i64)j * ldb] = W[i + (i64)j * nj];
+#pragma unroll
+    for (int e = lane; e < 256; e += 64) {
+        const int i = e & 15, c = e >> 4;
+        if (i < nn && c < rc) {
+            const double v = xs[i][c];
+            B[pk[i] + (i64)(j0 + c) * ldb] = v;
+            W[i + (i64)(j0 + c) * nj] = v;
+        }
     }
 }
 
+// Compares the lower triangle of L11 of the listed fronts (16-column tiles,
+// t.r0 = first column) with the copy the solve's inverses were formed from
+// (at t.tofs, ld nn); refreshes the copy and sets *changed where they differ.
+__global__ __launch_bounds__(TPB) void k_l11_check(
+    const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, double* __restrict__ copy, int* __restrict__ changed) {
+    const STile t = tiles[blockIdx.x];
+    const FrontInfo f = fi[t.k];
+    const int nn = f.nn, nj = nn + f.na;
+    const int c1 = min(t.r0 + 16, nn);
+    bool diff = false;
+    for (int c = t.r0; c < c1; ++c) {
+        const double* src = blkval + f.blkptr + (i64)c * nj;
+        double* dst = copy + t.tofs + (i64)c * nn;
+        for (int i = c + threadIdx.x; i < nn; i += TPB) {
+            const double v = src[i];
+            if (__double_as_longlong(v) != __double_as_longlong(dst[i])) { dst[i] = v; diff = true; }
+        }
+    }
+    if (diff) *changed = 1;
+}
+
 // Large-front solve steps (the dense products run as GEMM tasks between
-// them).  Each workgroup owns rows [r0, r0 + 256) of one front's slab,
+// them).  Each workgroup owns rows [r0, r0 + 64) of one front's slab,
 // tiles[i] = (front, r0).
-// Forward gather: W[:nn] = alpha B[perm], W[nn:] = 0, then the children's
-// update rows added in reverse child order (trsm.c:42-60); a child's rows that
-// fall in the tile are found by binary search (relative indices increase).
+// Forward gather: row i of the slab = alpha B[p[i]] (i < nn) or 0, plus the
+// children's update rows that land on it (gather map, reverse child order as
+// in trsm.c:46-58).  tiles[i] = (front, r0): rows r0 .. r0 + 63, thread
+// (row, g) handles right-hand sides g, g + 4, ...
 __global__ __launch_bounds__(TPB) void k_sgather_fwd(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
-    const i64* __restrict__ snptr, const int* __restrict__ relidx,
-    const int* __restrict__ chidx, const i64* __restrict__ perm, const double* __restrict__ B,
-    int ldb, int nrhs, double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
+    const i64* __restrict__ snptr, const i64* __restrict__ perm, const double* __restrict__ B,
+    int ldb, int nrhs, double alpha, double* __restrict__ work, const i64* __restrict__ wptr,
+    const int* __restrict__ gptr, const GEnt* __restrict__ gent) {
     const int2 t = tiles[blockIdx.x];
-    const int k = t.x, r0 = t.y;
+    const int k = t.x;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
-    const int r1 = min(r0 + 256, nj), rows = r1 - r0;
-    const int tid = threadIdx.x;
-    double* W = work + wptr[k] * nrhs;
-    const i64* pk = perm + snptr[k];
-    for (int e = tid; e < rows * nrhs; e += TPB) {
-        const int i = r0 + e % rows, j = e / rows;
-        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb] : 0.0;
-    }
-    __shared__ int s_ja, s_jb;
-    for (int l = f.chend - 1; l >= f.chbeg; --l) {
-        const int c = chidx[l];
-        const FrontInfo g = fi[c];
-        const int N = g.na, njc = g.nn + g.na;
-        const int* r = relidx + g.relptr;
-        __syncthreads();
-        if (tid == 0) {
-            int lo = 0, hi = N;
-            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < r0) lo = m + 1; else hi = m; }
-            s_ja = lo;
-            hi = N;
-            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < r1) lo = m + 1; else hi = m; }
-            s_jb = lo;
+    const int i = t.y + (int)(threadIdx.x & 63);
+    if (i >= nj) return;
+    const i64 wk = wptr[k];
+    double* W = work + wk * nrhs;
+    const int e0 = gptr[wk + i], e1 = gptr[wk + i + 1];
+    const i64 bi = (i < nn) ? perm[snptr[k] + i] : 0;
+    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
+        double g = (i < nn) ? alpha * B[bi + (i64)j * ldb] : 0.0;
+        for (int e = e0; e < e1; ++e) {
+            const GEnt q = gent[e];
+            g += work[q.wu * nrhs + q.row + (i64)j * q.ld];
         }
-        __syncthreads();
-        const int ja = s_ja, cnt = s_jb - s_ja;
-        const double* Wc = work + wptr[c] * nrhs + g.nn;
-        for (int e = tid; e < cnt * nrhs; e += TPB) {
-            const int i = ja + e % cnt, j = e / cnt;
-            W[r[i] + (i64)j * nj] += Wc[i + (i64)j * njc];
-        }
+        W[i + (i64)j * nj] = g;
     }
 }
 
 // Backward gather: W[:nn] = alpha B[perm], W[nn:] = the parent's values of
-// the separator rows (trsm.c:97-115).
+// the separator rows (trsm.c:97-115).  tiles[i] = (front, r0): rows r0 ..
+// r0 + 63, thread (row, g) handles right-hand sides g, g + 4, ...
 __global__ __launch_bounds__(TPB) void k_sgather_bwd(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const int* __restrict__ relidx,
     const i64* __restrict__ perm, const double* __restrict__ B, int ldb, int nrhs,
     double alpha, double* __restrict__ work, const i64* __restrict__ wptr) {
     const int2 t = tiles[blockIdx.x];
-    const int k = t.x, r0 = t.y;
+    const int k = t.x;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
-    const int r1 = min(r0 + 256, nj), rows = r1 - r0;
+    const int i = t.y + (int)(threadIdx.x & 63);
+    if (i >= nj) return;
     double* W = work + wptr[k] * nrhs;
-    const i64* pk = perm + snptr[k];
-    const FrontInfo p = fi[f.parent];
-    const int njp = p.nn + p.na;
-    const double* Wp = work + wptr[f.parent] * nrhs;
-    const int* r = relidx + f.relptr;
-    for (int e = threadIdx.x; e < rows * nrhs; e += TPB) {
-        const int i = r0 + e % rows, j = e / rows;
-        W[i + (i64)j * nj] = (i < nn) ? alpha * B[pk[i] + (i64)j * ldb]
-                                      : Wp[r[i - nn] + (i64)j * njp];
+    const double* src;
+    i64 ld;
+    double scale;
+    if (i < nn) {
+        src = B + perm[snptr[k] + i];
+        ld = ldb;
+        scale = alpha;
+    } else {
+        const FrontInfo p = fi[f.parent];
+        src = work + wptr[f.parent] * nrhs + relidx[f.relptr + i - nn];
+        ld = p.nn + p.na;
+        scale = 1.0;
     }
+    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64)
+        W[i + (i64)j * nj] = scale * src[(i64)j * ld];
 }
 
-// B[perm] = T[:nn] for the listed tiles (rows r0 .. r0 + 255 below nn), T the
-// front's solved block (column-major, ld nn, at tmp + t.tofs); with copyback
-// also W[:nn] = T (the backward sweep's children read it from the slab).
+// B[perm] = T[:nn] for the listed tiles (rows r0 .. r0 + 63 below nn), T the
+// front's solved block (column-major, ld nn, at tmp + t.tofs); with tmp2 (the
+// backward sweep) the value is T - T2, T2 at tmp2 + t.tofs, for fronts with a
+// separator; with copyback also W[:nn] = the value (the backward sweep's
+// children read it from the slab).  Thread (row, g) handles right-hand sides
+// g, g + 4, ...
 __global__ __launch_bounds__(TPB) void k_sscatter(
     const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const i64* __restrict__ perm, double* __restrict__ B,
     int ldb, int nrhs, double* __restrict__ work, const i64* __restrict__ wptr,
-    const double* __restrict__ tmp, int copyback) {
+    const double* __restrict__ tmp, const double* __restrict__ tmp2, int copyback) {
     const STile t = tiles[blockIdx.x];
-    const int k = t.k, r0 = t.r0;
+    const int k = t.k;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
-    const int r1 = min(r0 + 256, nn), rows = r1 - r0;
+    const int i = t.r0 + (int)(threadIdx.x & 63);
+    if (i >= nn) return;
     double* W = work + wptr[k] * nrhs;
     const double* T = tmp + t.tofs;
-    const i64* pk = perm + snptr[k];
-    for (int e = threadIdx.x; e < rows * nrhs; e += TPB) {
-        const int i = r0 + e % rows, j = e / rows;
-        const double v = T[i + (i64)j * nn];
-        B[pk[i] + (i64)j * ldb] = v;
+    const double* T2 = (tmp2 && f.na > 0) ? tmp2 + t.tofs : nullptr;
+    const i64 bi = perm[snptr[k] + i];
+    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
+        double v = T[i + (i64)j * nn];
+        if (T2) v -= T2[i + (i64)j * nn];
+        B[bi + (i64)j * ldb] = v;
         if (copyback) W[i + (i64)j * nj] = v;
     }
 }

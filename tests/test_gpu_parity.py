@@ -307,6 +307,50 @@ def test_solve_graph_replays_match_eager():
         assert np.array_equal(o, outs[0])
 
 
+def test_solve_after_refactor_in_same_buffer():
+    """The solve keeps L11^{-1} of its large fronts between calls; a new
+    factor written into the same blkval (a refactorization with other values,
+    and a copy of a third factor) must give that factor's solution, with 20
+    right-hand sides (a 16-wide chunk and a partial one)."""
+    import torch
+    A1 = problems.grid3d(12, seed=6)
+    symb = cp.symbolic(A1, p=cp.amd.order)
+    A2 = A1.copy()
+    A2.data = A2.data * 1.5
+    A2.setdiag(A2.diagonal() + 1.0)
+    As = [cp.symmetrize(A).tocsr() for A in (A1, A2)]
+    X = cp.cspmatrix(symb, device="cuda:0") + A1
+    X2 = cp.cspmatrix(symb, device="cuda:0") + A2
+    cp.cholesky(X2)
+    rng = np.random.default_rng(5)
+    Bh = rng.standard_normal((symb.n, 20))
+    B0 = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+
+    def solve_and_check(which):
+        B = B0.clone().t().contiguous().t()
+        cp.trsm(X, B)
+        cp.trsm(X, B, trans="T")
+        x = B.cpu().numpy()
+        r = np.linalg.norm(As[which] @ x - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
+        assert r.max() <= 1e-12, (which, r.max())
+
+    cp.cholesky(X)
+    solve_and_check(0)
+    solve_and_check(0)
+    X.blkval.zero_()
+    X.is_factor = False
+    X += A2
+    cp.cholesky(X)
+    solve_and_check(1)
+    X.blkval.copy_(X2.blkval)  # values changed behind the library's back
+    solve_and_check(1)
+    X.blkval.zero_()
+    X.is_factor = False
+    X += A1
+    cp.cholesky(X)
+    solve_and_check(0)
+
+
 def test_batched_512_grids_logdet_and_solve():
     """512 independent 2D grids, side ~ U{16..96} (seed 2), as one batched
     factorization: per-instance logdet against the dense slogdet (the 24
