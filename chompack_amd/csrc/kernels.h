@@ -70,6 +70,10 @@ __global__ void k_dot_partial(const int* cols, int ncols, const FrontInfo* fi,
                               const int* colfront, const double* x, const double* y,
                               double* partial);
 __global__ void k_copy(double* dst, const double* src, i64 n);
+__global__ void k_to_rowmajor(i64 n, int nrhs, const double* X, int ldx, double* Xt);
+__global__ void k_spmv_residual_rm(i64 n, const i64* rowptr, const int* colind,
+                                   const double* vals, const double* Xt, int nrhs,
+                                   const double* B, int ldb, double* R, int ldr);
 __global__ void k_spmv_residual(i64 n, const i64* rowptr, const int* colind, const double* vals,
                                 const double* X, int ldx, const double* B, int ldb, double* R,
                                 int ldr, int nrhs);

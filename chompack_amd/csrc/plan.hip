@@ -1055,9 +1055,19 @@ int chp_logdet(chp_plan* P, const double* lblkval, double* out, void* stream) {
 
 int chp_spmv_residual(i64 n, const i64* rowptr, const int* colind, const double* vals,
                       const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
-                      int nrhs, void* stream) {
+                      int nrhs, double* work, void* stream) {
     if (n <= 0 || nrhs <= 0) return 0;
     if (!rowptr || !colind || !vals || !X || !B || !R) { chp_set_error("null argument"); return -1; }
+    if (work && nrhs > 1) {
+        // X to row-major in work (n x nrhs), then the row-tiled product
+        const dim3 g((unsigned)((n + 63) / 64), (unsigned)((nrhs + 63) / 64));
+        hipLaunchKernelGGL(k_to_rowmajor, g, dim3(256), 0, (hipStream_t)stream, n, nrhs, X, ldx,
+                           work);
+        hipLaunchKernelGGL(k_spmv_residual_rm, g, dim3(256), 0, (hipStream_t)stream, n, rowptr,
+                           colind, vals, (const double*)work, nrhs, B, ldb, R, ldr);
+        HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     const i64 total = n * (i64)nrhs;
     const i64 blocks = std::min<i64>((total + 255) / 256, 1 << 20);
     hipLaunchKernelGGL(k_spmv_residual, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,

@@ -385,6 +385,66 @@ __global__ __launch_bounds__(TPB) void k_spmv_residual(
     }
 }
 
+// Xt = X as a row-major n x nrhs array (Xt[i * nrhs + j] = X[i + j * ldx]):
+// 64 x 64 tiles through LDS, grid (ceil(n / 64), ceil(nrhs / 64)).
+__global__ __launch_bounds__(TPB) void k_to_rowmajor(i64 n, int nrhs,
+                                                     const double* __restrict__ X, int ldx,
+                                                     double* __restrict__ Xt) {
+    __shared__ double t[64][65];
+    const i64 r0 = (i64)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = w; c < 64; c += 4) {
+        const i64 i = r0 + l;
+        t[l][c] = (i < n && c0 + c < nrhs) ? X[i + (i64)(c0 + c) * ldx] : 0.0;
+    }
+    __syncthreads();
+    for (int r = w; r < 64; r += 4) {
+        const i64 i = r0 + r;
+        if (i < n && c0 + l < nrhs) Xt[i * nrhs + c0 + l] = t[r][l];
+    }
+}
+
+// R = B - A X for a CSR matrix A (full symmetric pattern) with X given
+// row-major (Xt, n x nrhs).  A workgroup owns 64 rows and up to 64
+// right-hand sides (blockIdx.y); B and R pass through an LDS tile so that
+// their column-major accesses are coalesced; in the product, lpr lanes share
+// a row (one right-hand side each), so each nonzero reads a contiguous piece
+// of a row of Xt.  Each row sums its nonzeros in CSR order.
+__global__ __launch_bounds__(TPB) void k_spmv_residual_rm(
+    i64 n, const i64* __restrict__ rowptr, const int* __restrict__ colind,
+    const double* __restrict__ vals, const double* __restrict__ Xt, int nrhs,
+    const double* __restrict__ B, int ldb, double* __restrict__ R, int ldr) {
+    __shared__ double t[64][65];
+    const i64 r0 = (i64)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int cw = min(64, nrhs - c0);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = w; c < 64; c += 4) {
+        const i64 i = r0 + l;
+        t[l][c] = (i < n && c < cw) ? B[i + (i64)(c0 + c) * ldb] : 0.0;
+    }
+    __syncthreads();
+    int lpr = 1;
+    while (lpr < cw) lpr <<= 1;
+    const int rpw = 64 / lpr;            // rows per wave and pass
+    const int sub = l / lpr, col = l % lpr;
+    for (int r = w * rpw + sub; r < 64; r += 4 * rpw) {
+        const i64 i = r0 + r;
+        if (i >= n || col >= cw) continue;
+        double s = 0.0;
+        const i64 p1 = rowptr[i + 1];
+        for (i64 p = rowptr[i]; p < p1; ++p)
+            s += vals[p] * Xt[(i64)colind[p] * nrhs + c0 + col];
+        t[r][col] -= s;
+    }
+    __syncthreads();
+    for (int c = w; c < 64; c += 4) {
+        const i64 i = r0 + l;
+        if (i < n && c < cw) R[i + (i64)(c0 + c) * ldr] = t[l][c];
+    }
+}
+
 // dst[0:n] = src[0:n], 16-byte vector accesses, grid-stride.
 __global__ __launch_bounds__(TPB) void k_copy(double* __restrict__ dst,
                                               const double* __restrict__ src, i64 n) {

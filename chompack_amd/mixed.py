@@ -31,18 +31,25 @@ class SpMat(object):
         self.rowptr = torch.as_tensor(S.indptr.astype(np.int64), device=device)
         self.colind = torch.as_tensor(S.indices.astype(np.int32), device=device)
         self.vals = torch.as_tensor(S.data.astype(np.float64), device=device)
+        self._work = None  # row-major copy of X for several right-hand sides
 
     def residual(self, X, B, R):
         """R = B - A X for column-major (n, k) float64 GPU tensors."""
+        import torch
         k = 1 if X.dim() == 1 else X.shape[1]
         ld = lambda T: T.shape[0] if T.dim() == 1 or k == 1 else T.stride(1)
         for T in (X, B, R):
             if T.dim() == 2 and k > 1 and T.stride(0) != 1:
                 raise TypeError("X, B and R must be column-major")
+        work = None
+        if k > 1:
+            if self._work is None or self._work.numel() < self.n * k:
+                self._work = torch.empty(self.n * k, dtype=torch.float64, device=X.device)
+            work = self._work.data_ptr()
         rc = _native.lib().chp_spmv_residual(self.n, self.rowptr.data_ptr(), self.colind.data_ptr(),
                                              self.vals.data_ptr(), X.data_ptr(), ld(X),
                                              B.data_ptr(), ld(B), R.data_ptr(), ld(R), k,
-                                             numeric._stream(X))
+                                             work, numeric._stream(X))
         numeric._raise_rc(rc, "spmv")
 
 

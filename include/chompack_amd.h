@@ -158,10 +158,12 @@ int chp_logdiag(chp_plan* plan, const double* lblkval, double* out, void* stream
 /* R = B - A X for a device CSR matrix (rowptr int64[n+1], colind int32,
  * vals float64; the full symmetric pattern) and column-major device arrays
  * X, B, R with nrhs columns: the FP64 residual of iterative refinement
- * (no reference counterpart). */
+ * (no reference counterpart).  work: NULL, or a device workspace of n * nrhs
+ * doubles that receives a row-major copy of X, so that every nonzero reads
+ * a contiguous row of right-hand sides (used when nrhs > 1). */
 int chp_spmv_residual(int64_t n, const int64_t* rowptr, const int* colind, const double* vals,
                       const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
-                      int nrhs, void* stream);
+                      int nrhs, double* work, void* stream);
 
 /* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
  * restore a blkval array from a copy (no reference counterpart). */

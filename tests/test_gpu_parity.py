@@ -566,3 +566,26 @@ def test_mixed_precision_refined_solve():
     As = cp.symmetrize(A).tocsr()
     res = np.linalg.norm(As @ x.cpu().numpy() - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
     assert res.max() <= 1e-10
+
+
+@pytest.mark.parametrize("nrhs", [1, 5, 70])
+def test_spmv_residual(nrhs):
+    """R = B - A X on the GPU (device CSR, row-major copy of X for several
+    right-hand sides, 64-column chunks) against scipy, on the random
+    geometric graph's pattern (no locality in the column indices)."""
+    import torch
+    A = problems.rgg(3000, 12.0, seed=5)
+    Am = cp.mixed.SpMat(A)
+    As = cp.symmetrize(A).tocsr()
+    rng = np.random.default_rng(nrhs)
+    Xh = rng.standard_normal((A.shape[0], nrhs))
+    Bh = rng.standard_normal((A.shape[0], nrhs))
+    col = lambda M: torch.as_tensor(np.asfortranarray(M)).to("cuda:0").t().contiguous().t()
+    X, B = col(Xh), col(Bh)
+    if nrhs == 1:
+        X, B = X[:, 0].contiguous(), B[:, 0].contiguous()
+    R = torch.empty_like(X)
+    Am.residual(X, B, R)
+    ref = (Bh - As @ Xh).reshape(R.shape)
+    err = np.abs(R.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err <= 1e-14, err
