@@ -146,15 +146,6 @@ struct chp_plan {
         std::vector<std::vector<chpi::Launch>> shared, dir;  // per level, top-down
     };
     Seq comp[2], y2k[2], m2t[2];
-    // captured completion sweeps: valid for the buffer table and the device
-    // copies of the dense program they were captured with
-    struct SeqGraph {
-        BufTab key;
-        const void* lists = nullptr;
-        const void* tasks = nullptr;
-        std::vector<hipGraphExec_t> execs;
-    };
-    SeqGraph g_comp[2];
     LevelSeq scale[8];
     std::vector<chpi::SolvePlan*> solves;  // cached per nrhs
     double* d_dscratch = nullptr;
@@ -211,11 +202,6 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
 int read_status(chp_plan* P, hipStream_t s, i64* info);
 // Creates the side streams and their fork / join events (once per plan).
 int ensure_side_stream(chp_plan* P);
-// Runs `seq` on stream s through HIP graphs (chunks of at most 4096
-// launches, captured on first use and again whenever the buffer table or the
-// program's device lists moved); eagerly when graphs are off or profiling.
-int run_graphed(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
-                chp_plan::SeqGraph& G, hipStream_t s);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
 

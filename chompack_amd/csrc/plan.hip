@@ -686,49 +686,6 @@ int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, 
     return 0;
 }
 
-int run_graphed(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, const BufTab& bt,
-                chp_plan::SeqGraph& G, hipStream_t s) {
-    if (!P->use_graphs || P->prof) return run_seq(P, pg, seq, bt, s);
-    if (!P->gstream) {
-        HIP_CHECK(hipStreamCreateWithFlags(&P->gstream, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&P->gev_in, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
-    }
-    if (ensure_side_stream(P)) return -2;  // created outside any capture
-    const bool same = !G.execs.empty() && std::memcmp(&G.key, &bt, sizeof(bt)) == 0 &&
-                      G.lists == pg.d_lists && G.tasks == pg.d_tasks;
-    if (!same) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        for (hipGraphExec_t e : G.execs) (void)hipGraphExecDestroy(e);
-        G.execs.clear();
-        const size_t CH = 4096;
-        for (size_t c0 = 0; c0 < seq.size(); c0 += CH) {
-            const std::vector<Launch> part(seq.begin() + c0,
-                                           seq.begin() + std::min(seq.size(), c0 + CH));
-            hipGraph_t graph = nullptr;
-            HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
-            int rc = run_seq(P, pg, part, bt, P->gstream);
-            hipError_t e = hipStreamEndCapture(P->gstream, &graph);
-            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-            if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
-            hipGraphExec_t ex = nullptr;
-            e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) { chp_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return -2; }
-            G.execs.push_back(ex);
-        }
-        G.key = bt;
-        G.lists = pg.d_lists;
-        G.tasks = pg.d_tasks;
-    }
-    HIP_CHECK(hipEventRecord(P->gev_in, s));
-    HIP_CHECK(hipStreamWaitEvent(P->gstream, P->gev_in, 0));
-    for (hipGraphExec_t ex : G.execs) HIP_CHECK(hipGraphLaunch(ex, P->gstream));
-    HIP_CHECK(hipEventRecord(P->gev_out, P->gstream));
-    HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
-    return 0;
-}
-
 int read_status(chp_plan* P, hipStream_t s, i64* info) {
     unsigned long long h = CHP_STATUS_OK;
     HIP_CHECK(hipMemcpyAsync(&h, P->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -914,8 +871,6 @@ void chp_plan_destroy(chp_plan* P) {
     if (P->ev1) (void)hipEventDestroy(P->ev1);
     for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32})
         if (G->exec) (void)hipGraphExecDestroy(G->exec);
-    for (chp_plan::SeqGraph& G : P->g_comp)
-        for (hipGraphExec_t e : G.execs) (void)hipGraphExecDestroy(e);
     if (P->gev_in) (void)hipEventDestroy(P->gev_in);
     if (P->gev_out) (void)hipEventDestroy(P->gev_out);
     if (P->gstream) (void)hipStreamDestroy(P->gstream);

@@ -521,8 +521,7 @@ int chp_completion(chp_plan* P, double* blkval, int factored, void* stream, i64*
     if (!P->comp[fu].built) build_completion(P, fu, sc, dv);
     if (prepare(P, sc, dv, s)) return -2;
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-    int rc = run_graphed(P, P->dense, P->comp[fu].seq, dense_tab(P, blkval, nullptr, nullptr),
-                         P->g_comp[fu], s);
+    int rc = run_seq(P, P->dense, P->comp[fu].seq, dense_tab(P, blkval, nullptr, nullptr), s);
     if (rc) return rc;
     return read_status(P, s, info);
 }
