@@ -3,7 +3,7 @@
 Drop-in for the numeric hot path of CHOMPACK (src/python/__init__.py:24-44):
 ``symbolic``/``cspmatrix`` objects with the reference's block storage, and
 ``cholesky``, ``llt``, ``projected_inverse``, ``completion``, ``hessian``,
-``trsm`` and ``dot`` running as hand-written HIP kernels on gfx950 through the
+``trsm``, ``trmm`` and ``dot`` running as hand-written HIP kernels on gfx950 through the
 C-ABI in include/chompack_amd.h.  Symbolic analysis (AMD, elimination tree,
 supernodes, relative indices) runs on the host and is integer bit-exact.
 """
@@ -12,15 +12,13 @@ from .misc import tril, triu, symmetrize, perm, eye, lmerge, \
     frontal_add_update, frontal_get_update
 from . import amd
 
-try:
-    from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, dot, logdet
-except ImportError:  # numeric module not built into this checkout yet
-    pass
+from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, trmm, dot, \
+    logdet
 from . import autograd, batch, mixed
 
 __version__ = "0.1.0"
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
-           "completion", "projected_inverse", "hessian", "trsm", "dot", "logdet", "autograd",
+           "completion", "projected_inverse", "hessian", "trsm", "trmm", "dot", "logdet", "autograd",
            "tril", "triu",
            "symmetrize", "perm", "eye", "lmerge", "amd"]

@@ -56,7 +56,19 @@ __global__ void k_l11_check(const STile* tiles, const FrontInfo* fi, const doubl
 
 __global__ void k_sgather_fwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
                               const i64* perm, const double* B, int ldb, int nrhs, double alpha,
-                              double* work, const i64* wptr, const int* gptr, const GEnt* gent);
+                              double* work, const i64* wptr, const int* gptr, const GEnt* gent,
+                              int bonly);
+__global__ void k_trmm_fwd(const int2* items, const FrontInfo* fi, const double* blkval,
+                           const i64* snptr, const i64* perm, double* B, int ldb, int nrhs,
+                           double alpha, double* work, const i64* wptr, const int* gptr,
+                           const GEnt* gent);
+__global__ void k_trmm_bwd(const int2* items, const FrontInfo* fi, const double* blkval,
+                           const i64* snptr, const int* relidx, const i64* perm, double* B,
+                           int ldb, int nrhs, double alpha, double* work, const i64* wptr);
+__global__ void k_trmm_finish(const STile* tiles, const FrontInfo* fi, const i64* snptr,
+                              const i64* perm, double* B, int ldb, int nrhs, double* work,
+                              const i64* wptr, const int* gptr, const GEnt* gent,
+                              const double* tmp);
 __global__ void k_sgather_bwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,
                               const int* relidx, const i64* perm, const double* B, int ldb,
                               int nrhs, double alpha, double* work, const i64* wptr);

@@ -13,9 +13,9 @@ enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_FACTOR, K_GEMM, K_EXTRACT,
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
-    K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_FORK, K_JOIN
+    K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_TRMM_FWD, K_TRMM_BWD, K_TRMM_FIN, K_FORK, K_JOIN
 };
-const int NKIDS = 20;  // kernel kinds (K_FORK / K_JOIN are stream markers)
+const int NKIDS = 23;  // kernel kinds (K_FORK / K_JOIN are stream markers)
 
 struct Launch {
     Kid kid;
@@ -71,6 +71,8 @@ struct SolvePlan {
     int nrhs = 0;
     Program pg;
     std::vector<Launch> inv, fwd, bwd;  // X_k = L11^{-1}; the two sweeps
+    std::vector<Launch> mfwd, mbwd;     // products with L ('N') and L^T ('T')
+    bool mbuilt = false;
     i64 t2base = 0;                     // offset of T2 = B21^T x_A in d_tmp
     std::vector<STile> chk;             // (front, 16-column tile, X offset) of the L11 check
     STile* d_chk = nullptr;

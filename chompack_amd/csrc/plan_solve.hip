@@ -156,6 +156,91 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
     return S;
 }
 
+// Schedules of B := alpha L B ('N', leaves to root) and alpha L^T B ('T',
+// root to leaves) on the solve plan's level schedule, slabs and per-front
+// blocks T (to[k]) and T2 (to[k] + t2base), pybase/trmm.py:44-114.
+void build_trmm(chp_plan* P, SolvePlan* S) {
+    const int nrhs = S->nrhs;
+    P->pg = &S->pg;
+    const int H = P->nlevels;
+    std::vector<i64> to(P->fi.size(), -1);
+    {
+        Bump bt;  // the same offsets build_solve gave the large fronts
+        for (size_t k = 0; k < P->fi.size(); ++k)
+            if (solve_large(P->fi[k], nrhs)) to[k] = bt.take((i64)P->fi[k].nn * nrhs);
+    }
+    auto wofs = [&](int k) { return P->h_wunit[k] * nrhs; };
+    auto nj_of = [&](int k) { return P->fi[k].nn + P->fi[k].na; };
+    for (int L = 0; L < H; ++L) {
+        std::vector<int> small, large;
+        for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
+        const bool fork = !small.empty() && !large.empty();
+        if (fork) S->mfwd.push_back({K_FORK, 0, 0, 0, 0, 1});
+        push_list(P, S->mfwd, K_TRMM_FWD, rhs_items(small, nrhs), 64);
+        if (fork) S->mfwd.back().side = 1;
+        if (large.empty()) continue;
+        Launch Lg{K_SGATHER_F, 0, 256, 0, 0, 1};  // iarg 1: alpha B rows only
+        {
+            std::vector<int2> t = row_tiles(large, [&](int k) { return P->fi[k].nn; }, 64);
+            Lg.grid = (int)t.size();
+            Lg.list = P->pg->lists.add(t);
+        }
+        S->mfwd.push_back(Lg);
+        std::vector<GemmTile> g1, g2;
+        for (int k : large) {
+            const FrontInfo& f = P->fi[k];
+            const int nn = f.nn, na = f.na, nj = nn + na;
+            // T = tril(L11) (alpha b_N)
+            add_gemm_tiles(P, g1, mk(0, f.blkptr, nj, 0, 7, wofs(k), nj, 0, 5, to[k], nn,
+                                     nn, nrhs, nn, 1.0, 0.0, 0, 1, 0), false);
+            if (na > 0)  // slab rows >= nn = B21 T
+                add_gemm_tiles(P, g2, mk(0, f.blkptr + nn, nj, 0, 5, to[k], nn, 0, 7,
+                                         wofs(k) + nn, nj, na, nrhs, nn, 1.0, 0.0, 0, 0, 0), false);
+        }
+        push_gemm(P, S->mfwd, g1);
+        push_gemm(P, S->mfwd, g2);
+        std::vector<STile> ft;
+        for (int k : large)
+            for (int r0 = 0; r0 < nj_of(k); r0 += 64) ft.push_back({k, r0, to[k]});
+        push_list(P, S->mfwd, K_TRMM_FIN, ft);
+        if (fork) S->mfwd.push_back({K_JOIN, 0, 0, 0, 0, 1});
+    }
+    for (int L = H - 1; L >= 0; --L) {
+        std::vector<int> small, large;
+        for (int k : P->bylevel[L]) (solve_large(P->fi[k], nrhs) ? large : small).push_back(k);
+        const bool fork = !small.empty() && !large.empty();
+        if (fork) S->mbwd.push_back({K_FORK, 0, 0, 0, 0, 1});
+        push_list(P, S->mbwd, K_TRMM_BWD, rhs_items(small, nrhs), 64);
+        if (fork) S->mbwd.back().side = 1;
+        if (large.empty()) continue;
+        push_list(P, S->mbwd, K_SGATHER_B, row_tiles(large, nj_of, 64));
+        std::vector<GemmTile> g1, g2;
+        for (int k : large) {
+            const FrontInfo& f = P->fi[k];
+            const int nn = f.nn, na = f.na, nj = nn + na;
+            // T = tril(L11)^T u_N; T2 = B21^T u_A; then T += tril(L11)^T T2
+            add_gemm_tiles(P, g1, mk(0, f.blkptr, nj, 1, 7, wofs(k), nj, 0, 5, to[k], nn,
+                                     nn, nrhs, nn, 1.0, 0.0, 0, 2, 0), false);
+            if (na > 0) {
+                add_gemm_tiles(P, g1, mk(0, f.blkptr + nn, nj, 1, 7, wofs(k) + nn, nj, 0, 5,
+                                         to[k] + S->t2base, nn, nn, nrhs, na, 1.0, 0.0, 0, 0, 0),
+                               false);
+                add_gemm_tiles(P, g2, mk(0, f.blkptr, nj, 1, 5, to[k] + S->t2base, nn, 0, 5, to[k],
+                                         nn, nn, nrhs, nn, 1.0, 1.0, 0, 2, 0), false);
+            }
+        }
+        push_gemm(P, S->mbwd, g1);
+        push_gemm(P, S->mbwd, g2);
+        std::vector<STile> st;
+        for (int k : large)
+            for (int r0 = 0; r0 < P->fi[k].nn; r0 += 64) st.push_back({k, r0, to[k]});
+        push_list(P, S->mbwd, K_SSCATTER, st);  // iarg 0: B only, no T2
+        if (fork) S->mbwd.push_back({K_JOIN, 0, 0, 0, 0, 1});
+    }
+    S->mbuilt = true;
+    P->pg = &P->main;
+}
+
 int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
               const double* blkval, double* B, int ldb, int nrhs, double alpha, hipStream_t s) {
     BufTab bt;
@@ -204,7 +289,19 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
             break;
         case K_SGATHER_F:
             CHP_SLAUNCH(k_sgather_fwd, (const int2*)list, P->d_fi, P->d_snptr, P->d_perm, B,
+                        ldb, nrhs, alpha, P->d_work, P->d_wunit, P->d_gptr, P->d_gent, L.iarg);
+            break;
+        case K_TRMM_FWD:
+            CHP_SLAUNCH(k_trmm_fwd, (const int2*)list, P->d_fi, blkval, P->d_snptr, P->d_perm, B,
                         ldb, nrhs, alpha, P->d_work, P->d_wunit, P->d_gptr, P->d_gent);
+            break;
+        case K_TRMM_BWD:
+            CHP_SLAUNCH(k_trmm_bwd, (const int2*)list, P->d_fi, blkval, P->d_snptr, P->d_relidx,
+                        P->d_perm, B, ldb, nrhs, alpha, P->d_work, P->d_wunit);
+            break;
+        case K_TRMM_FIN:
+            CHP_SLAUNCH(k_trmm_finish, (const STile*)list, P->d_fi, P->d_snptr, P->d_perm, B, ldb,
+                        nrhs, P->d_work, P->d_wunit, P->d_gptr, P->d_gent, S->d_tmp);
             break;
         case K_SGATHER_B:
             CHP_SLAUNCH(k_sgather_bwd, (const int2*)list, P->d_fi, P->d_snptr,
@@ -251,10 +348,12 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
 
 using namespace chpi;
 
-extern "C" {
+namespace {
 
-int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
-             int trans, void* stream) {
+// B := alpha L^{-1} B / alpha L^{-T} B (product = false) or alpha L B /
+// alpha L^T B (product = true) on the cached schedule for nrhs.
+int triangular_op(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
+                  int trans, bool product, void* stream) {
     if (!P || !blkval || !B) { chp_set_error("null argument"); return -1; }
     if (trans != 'N' && trans != 'T') { chp_set_error("trans must be 'N' or 'T'"); return -1; }
     if (nrhs <= 0) return 0;
@@ -274,6 +373,23 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         if (alloc_doubles(&S->d_l11, S->inv_len)) return -2;
         if (upload(&S->d_chk, S->chk)) return -2;
         HIP_CHECK(hipMalloc((void**)&S->d_changed, sizeof(int)));
+    }
+    if (product && !S->mbuilt) {
+        // the product schedules extend the plan's lists and tasks, which may
+        // move them and the split-K partials: the plan's graphs go first
+        HIP_CHECK(hipDeviceSynchronize());
+        for (auto& g : S->graphs) (void)hipGraphExecDestroy(g.exec);
+        S->graphs.clear();
+        S->last = SolvePlan::Key();
+        const i64 before = S->pg.split_doubles();
+        build_trmm(P, S);
+        if (sync_program(S->pg)) return -2;
+        if (S->pg.split_doubles() > before) {
+            HIP_CHECK(hipFree(S->d_splitp));
+            S->d_splitp = nullptr;
+            if (alloc_doubles(&S->d_splitp, S->pg.split_doubles())) return -2;
+            HIP_CHECK(hipMemset(S->d_splitp, 0, S->pg.split_doubles() * sizeof(double)));
+        }
     }
     const i64 need = P->solve_len * nrhs;
     // captured solves hold the addresses of the work buffers: drop them all
@@ -298,7 +414,7 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
     // the inverses X_k: formed again when L11 of a large front is not the one
     // they were formed from (an exact comparison with a copy)
-    if (!S->chk.empty()) {
+    if (!product && !S->chk.empty()) {
         int changed = 0;
         HIP_CHECK(hipMemsetAsync(S->d_changed, 0, sizeof(int), s));
         hipLaunchKernelGGL(k_l11_check, dim3((unsigned)S->chk.size()), dim3(256), 0, s, S->d_chk,
@@ -312,9 +428,11 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
             S->inv_valid = true;
         }
     }
-    const std::vector<Launch>& seq = trans == 'N' ? S->fwd : S->bwd;
+    const std::vector<Launch>& seq = product ? (trans == 'N' ? S->mfwd : S->mbwd)
+                                             : (trans == 'N' ? S->fwd : S->bwd);
     SolvePlan::Key key;
-    key.blk = blkval; key.B = B; key.ldb = ldb; key.trans = trans; key.alpha = alpha;
+    key.blk = blkval; key.B = B; key.ldb = ldb; key.trans = trans + (product ? 256 : 0);
+    key.alpha = alpha;
     SolvePlan::Graph* G = nullptr;
     for (auto& g : S->graphs) if (g.key == key) G = &g;
     if (P->use_graphs && !P->prof && (G || S->last == key)) {
@@ -361,6 +479,20 @@ int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, do
         return -4;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chp_trsm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
+             int trans, void* stream) {
+    return triangular_op(P, blkval, B, ldb, nrhs, alpha, trans, false, stream);
+}
+
+int chp_trmm(chp_plan* P, const double* blkval, double* B, int ldb, int nrhs, double alpha,
+             int trans, void* stream) {
+    return triangular_op(P, blkval, B, ldb, nrhs, alpha, trans, true, stream);
 }
 
 }  // extern "C"

@@ -234,22 +234,24 @@ __global__ __launch_bounds__(TPB) void k_l11_check(
 // tiles[i] = (front, r0).
 // Forward gather: row i of the slab = alpha B[p[i]] (i < nn) or 0, plus the
 // children's update rows that land on it (gather map, reverse child order as
-// in trsm.c:46-58).  tiles[i] = (front, r0): rows r0 .. r0 + 63, thread
-// (row, g) handles right-hand sides g, g + 4, ...
+// in trsm.c:46-58); with bonly, only rows i < nn and only alpha B[p[i]] (the
+// product with L adds the children's rows after the front's own product).
+// tiles[i] = (front, r0): rows r0 .. r0 + 63, thread (row, g) handles
+// right-hand sides g, g + 4, ...
 __global__ __launch_bounds__(TPB) void k_sgather_fwd(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const i64* __restrict__ perm, const double* __restrict__ B,
     int ldb, int nrhs, double alpha, double* __restrict__ work, const i64* __restrict__ wptr,
-    const int* __restrict__ gptr, const GEnt* __restrict__ gent) {
+    const int* __restrict__ gptr, const GEnt* __restrict__ gent, int bonly) {
     const int2 t = tiles[blockIdx.x];
     const int k = t.x;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
     const int i = t.y + (int)(threadIdx.x & 63);
-    if (i >= nj) return;
+    if (i >= (bonly ? nn : nj)) return;
     const i64 wk = wptr[k];
     double* W = work + wk * nrhs;
-    const int e0 = gptr[wk + i], e1 = gptr[wk + i + 1];
+    const int e0 = gptr[wk + i], e1 = bonly ? e0 : gptr[wk + i + 1];
     const i64 bi = (i < nn) ? perm[snptr[k] + i] : 0;
     for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
         double g = (i < nn) ? alpha * B[bi + (i64)j * ldb] : 0.0;
@@ -319,6 +321,215 @@ __global__ __launch_bounds__(TPB) void k_sscatter(
         if (T2) v -= T2[i + (i64)j * nn];
         B[bi + (i64)j * ldb] = v;
         if (copyback) W[i + (i64)j * nj] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Products with the chordal factor, B := alpha L B or alpha L^T B
+// (pybase/trmm.py:44-114), on the solve's level schedule, slabs and gather
+// map.  With the stored B21 = L21 L11^{-1}:
+//   'N', leaves to root: y_N = L11 (alpha b_N), y_A = B21 y_N (= L21 alpha
+//        b_N); then the children's update rows are added (reverse child
+//        order); x_N goes to B, the slab keeps y_A + children's rows for the
+//        parent.
+//   'T', root to leaves: u = [alpha b_N; the parent's u at the separator
+//        rows] stays in the slab for the children (u before any product);
+//        x_N = L11^T (u_N + B21^T u_A) goes to B.
+
+// Small fronts, 'N': one wave per (front, chunk of up to 16 right-hand sides).
+__global__ __launch_bounds__(64) void k_trmm_fwd(
+    const int2* __restrict__ items, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, const i64* __restrict__ snptr,
+    const i64* __restrict__ perm, double* __restrict__ B, int ldb, int nrhs,
+    double alpha, double* __restrict__ work, const i64* __restrict__ wptr,
+    const int* __restrict__ gptr, const GEnt* __restrict__ gent) {
+    __shared__ double Ls[16][17];
+    __shared__ double ys[16][17];
+    const int2 it = items[blockIdx.x];
+    const int k = it.x, j0 = it.y;
+    const int rc = min(SRC, nrhs - j0);
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int lane = threadIdx.x;
+    const double* L = blkval + f.blkptr;
+    const i64 wk = wptr[k];
+    double* W = work + wk * nrhs;
+    const i64* pk = perm + snptr[k];
+#pragma unroll
+    for (int e = lane; e < 256; e += 64) {
+        const int i = e & 15, c = e >> 4;
+        if (i < nn && c <= i) Ls[i][c] = L[i + (i64)c * nj];
+    }
+    __syncthreads();
+    if (lane < rc) {  // y_N = L11 (alpha b_N), one right-hand side per lane
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = (i < nn) ? alpha * B[pk[i] + (i64)(j0 + lane) * ldb] : 0.0;
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+            if (i < nn) {
+                double y = 0.0;
+#pragma unroll
+                for (int c = 0; c <= i; ++c) y += Ls[i][c] * x[c];
+                x[i] = y;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ys[i][lane] = (i < nn) ? x[i] : 0.0;
+    }
+    __syncthreads();
+    // every row: own product, then the children's rows in map order
+    for (int i = lane; i < nj; i += 64) {
+        double g[16];
+        if (i < nn) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) g[j] = ys[i][j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) g[j] = 0.0;
+#pragma unroll 1
+            for (int m = 0; m < nn; ++m) {
+                const double lv = L[i + (i64)m * nj];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) g[j] += lv * ys[m][j];
+            }
+        }
+        const int e1 = gptr[wk + i + 1];
+        for (int q = gptr[wk + i]; q < e1; ++q) {
+            const GEnt t = gent[q];
+            const double* src = work + t.wu * nrhs + t.row + (i64)j0 * t.ld;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < rc) g[j] += src[(i64)j * t.ld];
+        }
+        if (i < nn) {
+            const i64 bi = pk[i];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < rc) B[bi + (i64)(j0 + j) * ldb] = g[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < rc) W[i + (i64)(j0 + j) * nj] = g[j];
+        }
+    }
+}
+
+// Small fronts, 'T'.  B21^T u_A on v_mfma_f64_16x16x4f64 over 16-row chunks
+// staged in LDS (operand and accumulator layout as in k_solve_bwd).
+__global__ __launch_bounds__(64) void k_trmm_bwd(
+    const int2* __restrict__ items, const FrontInfo* __restrict__ fi,
+    const double* __restrict__ blkval, const i64* __restrict__ snptr,
+    const int* __restrict__ relidx, const i64* __restrict__ perm,
+    double* __restrict__ B, int ldb, int nrhs, double alpha,
+    double* __restrict__ work, const i64* __restrict__ wptr) {
+    __shared__ double Ls[16][17];
+    __shared__ double zs[16][17];
+    __shared__ double ua[16][17];
+    __shared__ double lb[16][17];
+    const int2 it = items[blockIdx.x];
+    const int k = it.x, j0 = it.y;
+    const int rc = min(SRC, nrhs - j0);
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, na = f.na, nj = nn + na;
+    const int lane = threadIdx.x;
+    const double* L = blkval + f.blkptr;
+    double* W = work + wptr[k] * nrhs;
+    const i64* pk = perm + snptr[k];
+#pragma unroll
+    for (int e = lane; e < 256; e += 64) {
+        const int i = e & 15, c = e >> 4;
+        if (i < nn && c <= i) Ls[i][c] = L[i + (i64)c * nj];
+        double u = 0.0;
+        if (i < nn && c < rc) {
+            u = alpha * B[pk[i] + (i64)(j0 + c) * ldb];
+            W[i + (i64)(j0 + c) * nj] = u;  // the children's view of this front
+        }
+        zs[i][c] = u;
+    }
+    const int col = lane & 15, kq = lane >> 4;
+    if (na > 0) {
+        const FrontInfo pf = fi[f.parent];
+        const int njp = pf.nn + pf.na;
+        const double* Wp = work + wptr[f.parent] * nrhs;
+        const int* r = relidx + f.relptr;
+        sdbl4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int i0 = 0; i0 < na; i0 += 16) {
+            const int ii = lane & 15;
+            const int i = i0 + ii;
+            const bool ok = i < na;
+            const int ri = ok ? r[i] : 0;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int c = (lane >> 4) + 4 * h;
+                double v = 0.0, lv = 0.0;
+                if (ok && c < rc) v = Wp[ri + (i64)(j0 + c) * njp];
+                if (ok && c < nn) lv = L[nn + i + (i64)c * nj];
+                ua[ii][c] = v;
+                lb[ii][c] = lv;
+                if (ok && c < rc) W[nn + i + (i64)(j0 + c) * nj] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int kk = 4 * s4 + kq;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(lb[kk][col], ua[kk][col], acc, 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = kq + 4 * v;
+            if (m < nn && col < rc) zs[m][col] += acc[v];
+        }
+    }
+    __syncthreads();
+    if (lane < rc) {  // x_N = L11^T z
+        double z[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) z[i] = (i < nn) ? zs[i][lane] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < nn) {
+                double y = 0.0;
+#pragma unroll
+                for (int c = i; c < 16; ++c)
+                    if (c < nn) y += Ls[c][i] * z[c];
+                B[pk[i] + (i64)(j0 + lane) * ldb] = y;
+            }
+        }
+    }
+}
+
+// Large fronts, 'N', after the level's products (T = L11 (alpha b_N) at
+// tmp + t.tofs, ld nn; slab rows >= nn = B21 T): the children's rows are
+// added in map order; x_N = T + children's rows goes to B, the slab's rows
+// >= nn keep their sums for the parent.  tiles[i] = (front, r0, T offset):
+// rows r0 .. r0 + 63 of the slab.
+__global__ __launch_bounds__(TPB) void k_trmm_finish(
+    const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const i64* __restrict__ snptr, const i64* __restrict__ perm, double* __restrict__ B,
+    int ldb, int nrhs, double* __restrict__ work, const i64* __restrict__ wptr,
+    const int* __restrict__ gptr, const GEnt* __restrict__ gent, const double* __restrict__ tmp) {
+    const STile t = tiles[blockIdx.x];
+    const int k = t.k;
+    const FrontInfo f = fi[k];
+    const int nn = f.nn, nj = nn + f.na;
+    const int i = t.r0 + (int)(threadIdx.x & 63);
+    if (i >= nj) return;
+    const i64 wk = wptr[k];
+    double* W = work + wk * nrhs;
+    const double* T = tmp + t.tofs;
+    const int e0 = gptr[wk + i], e1 = gptr[wk + i + 1];
+    const i64 bi = (i < nn) ? perm[snptr[k] + i] : 0;
+    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
+        double g = (i < nn) ? T[i + (i64)j * nn] : W[i + (i64)j * nj];
+        for (int e = e0; e < e1; ++e) {
+            const GEnt q = gent[e];
+            g += work[q.wu * nrhs + q.row + (i64)j * q.ld];
+        }
+        if (i < nn) B[bi + (i64)j * ldb] = g;
+        else W[i + (i64)j * nj] = g;
     }
 }
 

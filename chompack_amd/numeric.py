@@ -14,7 +14,7 @@ import numpy as np
 from . import _native
 from .symbolic import cspmatrix
 
-__all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian",
+__all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian", "trmm",
            "trsm", "dot", "logdet", "logdiag", "copy_blkval"]
 
 SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
@@ -196,14 +196,8 @@ def hessian(L, Y, U, adj=False, inv=False, factored_updates=True):
     del torch
 
 
-def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):
-    """B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B ('T') with the chordal
-    factor L, B in the original ordering (cbase.c:965-1077).
-
-    B is a float64 CUDA tensor: a vector of length n, or an (n, k) matrix.
-    As in the reference, ``nrhs``, ``offsetB`` and ``ldB`` address B's
-    column-major storage; a row-major (n, k) tensor is solved through a
-    column-major copy that is written back."""
+def _triangular(fn, what, L, B, alpha, trans, nrhs, offsetB, ldB):
+    """Shared argument handling of trsm and trmm (cbase.c:965-1041)."""
     torch = _torch()
     if not (isinstance(L, cspmatrix) and L.is_factor is True):
         raise ValueError("L must be a cspmatrix factor")
@@ -219,22 +213,41 @@ def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):
         cols = B.shape[1]
         if B.stride(0) == 1:          # column-major storage
             work, ld = B, (B.stride(1) if cols > 1 else B.shape[0])
-        else:                         # row-major: solve a column-major copy
+        else:                         # row-major: work on a column-major copy
             work, ld = B.t().contiguous(), B.shape[0]
     else:
         raise TypeError("B must be a vector or a matrix")
-    if ldB != -1:
+    if ldB != -1 and ldB is not None:
         ld = int(ldB)
-    k = cols if nrhs == -1 else int(nrhs)
+    k = cols if (nrhs == -1 or nrhs is None) else int(nrhs)
     if ld < n:
         raise ValueError("ldB must be at least n")
     if k > 0 and int(offsetB) + (k - 1) * ld + n > work.numel():
         raise ValueError("B is too small for n, nrhs, offsetB and ldB")
-    rc = _native.lib().chp_trsm(plan.handle, b.data_ptr(), work.data_ptr() + 8 * int(offsetB),
-                                int(ld), int(k), float(alpha), ord(trans), _stream(b))
-    _raise_rc(rc, "trsm")
+    rc = fn(plan.handle, b.data_ptr(), work.data_ptr() + 8 * int(offsetB),
+            int(ld), int(k), float(alpha), ord(trans), _stream(b))
+    _raise_rc(rc, what)
     if work is not B:
         B.copy_(work.t())
+
+
+def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):
+    """B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B ('T') with the chordal
+    factor L, B in the original ordering (cbase.c:965-1077).
+
+    B is a float64 CUDA tensor: a vector of length n, or an (n, k) matrix.
+    As in the reference, ``nrhs``, ``offsetB`` and ``ldB`` address B's
+    column-major storage; a row-major (n, k) tensor is solved through a
+    column-major copy that is written back."""
+    _triangular(_native.lib().chp_trsm, "trsm", L, B, alpha, trans, nrhs, offsetB, ldB)
+
+
+def trmm(L, B, alpha=1.0, trans='N', nrhs=None, offsetB=0, ldB=None):
+    """B := alpha L B (trans 'N') or alpha L^T B ('T') with the chordal factor
+    L, B in the original ordering (pybase/trmm.py:4-114; the reference's
+    defaults nrhs = None and ldB = None mean B's column count and row
+    count).  B as for ``trsm``."""
+    _triangular(_native.lib().chp_trmm, "trmm", L, B, alpha, trans, nrhs, offsetB, ldB)
 
 
 def dot(X, Y):

@@ -78,13 +78,15 @@ int chp_set_graphs(chp_plan* plan, int on);
 
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
- * (synchronizing after each launch) and clears the counters;
- * chp_profile_read fills ms/cnt/flops[20] per kernel kind (0 chol_small,
- * 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm, 5 extract, 6 projinv_small,
- * 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd, 11 solve_bwd,
- * 12 copy_top, 13 splitk_reduce, 14 diag, 15 matop, 16 update_factor,
- * 17 sgather_fwd, 18 sgather_bwd, 19 sscatter) and returns 20.  chp_plan_flops: algorithmic GEMM flops of one cholesky
- * (out[0]) and one projected-inverse sweep (out[1]). */
+ * (synchronizing after each launch, every launch on that stream) and clears
+ * the counters; chp_profile_read fills ms/cnt/flops[23] per kernel kind
+ * (0 chol_small, 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm, 5 extract,
+ * 6 projinv_small, 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd,
+ * 11 solve_bwd, 12 copy_top, 13 splitk_reduce, 14 diag, 15 matop,
+ * 16 update_factor, 17 sgather_fwd, 18 sgather_bwd, 19 sscatter,
+ * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish) and returns 23.
+ * chp_plan_flops: algorithmic GEMM flops of one cholesky (out[0]) and one
+ * projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
 int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);
 /* Per-launch rows (kind, workgroups, algorithmic flops, ms) of the profiled
@@ -140,6 +142,12 @@ int chp_hessian(chp_plan* plan, const double* lblkval, const double* yblkval,
  * column-major n x nrhs array with leading dimension ldb in the original
  * ordering; replaces trsm() (trsm.c:3-138, cbase.c:1042-1049). */
 int chp_trsm(chp_plan* plan, const double* blkval, double* B, int ldb, int nrhs,
+             double alpha, int trans, void* stream);
+
+/* B := alpha L B (trans 'N') or alpha L^T B (trans 'T'), B as for chp_trsm;
+ * replaces trmm() (pybase/trmm.py:4-114, the reference's only
+ * implementation; chompack.trmm is that function). */
+int chp_trmm(chp_plan* plan, const double* blkval, double* B, int ldb, int nrhs,
              double alpha, int trans, void* stream);
 
 /* *out = tr(X Y) for two chordal matrices on the plan's pattern; replaces

@@ -21,7 +21,7 @@ import numpy as np
 import scipy.linalg as sla
 
 __all__ = ["cholesky", "llt", "projected_inverse", "completion", "update_factor", "hessian",
-           "trsm", "dot", "logdet"]
+           "trsm", "trmm", "dot", "logdet"]
 
 
 class _S(object):
@@ -194,6 +194,50 @@ def trsm(symb, lblkval, B, alpha=1.0, trans="N"):
             for c in S.children(k):
                 stack.append(x[S.rel(c)].copy())
             B2[rows] = x[:nn]
+    else:
+        raise ValueError("trans must be 'N' or 'T'")
+
+
+def trmm(symb, lblkval, B, alpha=1.0, trans="N"):
+    """B := alpha L B ('N', leaves to root) or alpha L^T B ('T', root to
+    leaves) with the chordal factor L and B in the original ordering
+    (pybase/trmm.py:44-114).  With the stored B21 = L21 L11^{-1}, 'N' forms
+    [L11 b; B21 (L11 b)] and then adds the children's contributions (popped,
+    i.e. in reverse child order); 'T' receives the parent's pre-product rows,
+    adds B21^T u_A and multiplies by L11^T.  B is an (n, nrhs) float64 array,
+    modified in place."""
+    S = _S(symb)
+    B2 = B.reshape((S.n, -1))
+    stack = []
+    if trans == "N":
+        for k in S.snpost:
+            nn, na, nj = S.dims(k)
+            L = _blk(lblkval, S, k)
+            rows = S.p[S.snptr[k]:S.snptr[k + 1]]
+            u = np.zeros((nj, B2.shape[1]))
+            u[:nn] = np.tril(L[:nn]) @ (alpha * B2[rows])
+            if na > 0:
+                u[nn:] = L[nn:, :] @ u[:nn]
+            for c in reversed(S.children(k)):
+                u[S.rel(c)] += stack.pop()
+            if na > 0:
+                stack.append(u[nn:].copy())
+            B2[rows] = u[:nn]
+    elif trans == "T":
+        for k in reversed(S.snpost):
+            nn, na, nj = S.dims(k)
+            L = _blk(lblkval, S, k)
+            rows = S.p[S.snptr[k]:S.snptr[k + 1]]
+            u = np.zeros((nj, B2.shape[1]))
+            u[:nn] = alpha * B2[rows]
+            if na > 0:
+                u[nn:] = stack.pop()
+            for c in S.children(k):
+                stack.append(u[S.rel(c)].copy())
+            x = u[:nn].copy()
+            if na > 0:
+                x += L[nn:, :].T @ u[nn:]
+            B2[rows] = np.tril(L[:nn]).T @ x
     else:
         raise ValueError("trans must be 'N' or 'T'")
 

@@ -74,6 +74,21 @@ def test_cholesky_projinv_solve(name, small_max):
     res = np.abs(As @ B.cpu().numpy() - Bh).max() / np.abs(Bh).max()
     assert res <= 1e-12
 
+    # products with the factor (trmm N / T, alpha) against the oracle; 20
+    # right-hand sides (a 16-wide chunk and a partial one in the small-front
+    # kernels)
+    Mh = rng.standard_normal((symb.n, 20))
+    for tr in "NT":
+        M = torch.as_tensor(np.asfortranarray(Mh)).to("cuda:0").t().contiguous().t()
+        Mref = np.asfortranarray(Mh.copy())
+        on.trmm(symb, ref, Mref, alpha=0.75, trans=tr)
+        cp.trmm(X, M, alpha=0.75, trans=tr)
+        e = _rel(M.cpu().numpy(), Mref)
+        print("trmm %s rel err %.3e" % (tr, e), flush=True)
+        assert e <= TOL
+        cp.trsm(X, M, alpha=1.0 / 0.75, trans=tr)   # undone by the solve
+        assert _rel(M.cpu().numpy(), Mh) <= 1e-12
+
     on.projected_inverse(symb, ref)
     cp.projected_inverse(X)
     assert X.is_factor is False

@@ -114,6 +114,15 @@ def test_oracle_closed_forms(maker):
     on.trsm(symb, L.blkval.numpy(), BT, trans="T")
     ref_T = np.linalg.solve(Ld.T, B[symb.p])[ip]
     assert np.abs(BT - ref_T).max() < 1e-12 * np.abs(ref_T).max()
+    # products: alpha L b and alpha L^T b in the original ordering, and the
+    # product undone by the solve
+    for tr, Lop in (("N", Ld), ("T", Ld.T)):
+        BM = B.copy()
+        on.trmm(symb, L.blkval.numpy(), BM, alpha=1.5, trans=tr)
+        ref_M = (1.5 * Lop @ B[symb.p])[ip]
+        assert np.abs(BM - ref_M).max() < 1e-12 * np.abs(ref_M).max()
+        on.trsm(symb, L.blkval.numpy(), BM, alpha=1.0 / 1.5, trans=tr)
+        assert np.abs(BM - B).max() < 1e-12 * np.abs(B).max()
     # llt(cholesky(A)) == A and <P(A^-1), A> = n
     P = L.blkval.numpy().copy()
     on.llt(symb, P)
