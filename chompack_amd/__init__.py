@@ -15,10 +15,12 @@ from . import amd
 from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, trmm, dot, \
     logdet
 from . import autograd, batch, mixed
+from .pfcholesky import pfcholesky
 
 __version__ = "0.1.0"
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
            "completion", "projected_inverse", "hessian", "trsm", "trmm", "dot", "logdet", "autograd",
+           "pfcholesky",
            "tril", "triu",
            "symmetrize", "perm", "eye", "lmerge", "amd"]

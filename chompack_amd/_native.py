@@ -67,6 +67,9 @@ def _declare(L):
     _opt(L, "chp_projected_inverse", [vp, vp, vp, vp], i32)
     _opt(L, "chp_trsm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
     _opt(L, "chp_trmm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
+    _opt(L, "chp_pf_factor", [i64, i32, vp, vp, i32, vp, i32, vp, i32, vp], i32)
+    _opt(L, "chp_pf_solve", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)
+    _opt(L, "chp_pf_product", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)
     _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)
     _opt(L, "chp_logdet", [vp, vp, vp, vp], i32)
     _opt(L, "chp_logdiag", [vp, vp, vp, vp], i32)

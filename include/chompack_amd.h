@@ -173,6 +173,29 @@ int chp_spmv_residual(int64_t n, const int64_t* rowptr, const int* colind, const
                       const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
                       int nrhs, double* work, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Product-form Cholesky factors of I + V diag(a) V^T (pfcholesky.c), V, L, B
+ * device column-major n x m arrays, a a host array of m doubles.  Each factor
+ * L_i = diag(l_i) + strictly-lower(v_i b_i^T); the sequential row loops of
+ * the reference run as scans, one workgroup per column.
+ * ------------------------------------------------------------------------- */
+/* Factorization in place (V's columns i+1.. are overwritten as in the
+ * reference); replaces dpftrf (pfcholesky.c:52-87) as called by cbase.c
+ * pfchol (cbase.c:862-880).  Returns 1 where the reference fails
+ * ("factorization failed"). */
+int chp_pf_factor(int64_t n, int m, const double* a, double* V, int ldv, double* L, int ldl,
+                  double* B, int ldb, void* stream);
+/* U := (L_1 ... L_m)^{-1} U (trans 'N') or (L_1 ... L_m)^{-T} U ('T') for nu
+ * device columns of U; replaces dpfsv (pfcholesky.c:89-112) as called by
+ * cbase.c pftrsm (cbase.c:885-911). */
+int chp_pf_solve(int64_t n, int m, const double* V, int ldv, const double* L, int ldl,
+                 const double* B, int ldb, double* U, int ldu, int nu, int trans, void* stream);
+/* U := L_1 ... L_m U ('N') or (L_1 ... L_m)^T U ('T'); replaces dpfmv
+ * (pfcholesky.c:114-136) as called by cbase.c pftrmm (cbase.c:916-942). */
+int chp_pf_product(int64_t n, int m, const double* V, int ldv, const double* L, int ldl,
+                   const double* B, int ldb, double* U, int ldu, int nu, int trans,
+                   void* stream);
+
 /* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
  * restore a blkval array from a copy (no reference counterpart). */
 int chp_copy(double* dst, const double* src, int64_t n, void* stream);

@@ -21,7 +21,7 @@ import numpy as np
 import scipy.linalg as sla
 
 __all__ = ["cholesky", "llt", "projected_inverse", "completion", "update_factor", "hessian",
-           "trsm", "trmm", "dot", "logdet"]
+           "trsm", "trmm", "pf_factor", "pf_solve", "pf_product", "dot", "logdet"]
 
 
 class _S(object):
@@ -240,6 +240,76 @@ def trmm(symb, lblkval, B, alpha=1.0, trans="N"):
             B2[rows] = np.tril(L[:nn]).T @ x
     else:
         raise ValueError("trans must be 'N' or 'T'")
+
+
+def _ddrsv(trans, v, l, b, x):
+    """x := L^{-1} x ('N') or L^{-T} x ('T'), L = diag(l) + strictly-lower
+    (v b^T) (pfcholesky.c:4-25)."""
+    s = 0.0
+    n = len(x)
+    if trans == "N":
+        for k in range(n):
+            x[k] = (x[k] - v[k] * s) / l[k]
+            s += x[k] * b[k]
+    else:
+        for k in range(n - 1, -1, -1):
+            x[k] = (x[k] - b[k] * s) / l[k]
+            s += x[k] * v[k]
+
+
+def _ddrmv(trans, v, l, b, x):
+    """x := L x ('N') or L^T x ('T') (pfcholesky.c:27-50)."""
+    s = 0.0
+    n = len(x)
+    if trans == "N":
+        for k in range(n):
+            tmp = x[k] * b[k]
+            x[k] = l[k] * x[k] + v[k] * s
+            s += tmp
+    else:
+        for k in range(n - 1, -1, -1):
+            tmp = x[k] * v[k]
+            x[k] = l[k] * x[k] + b[k] * s
+            s += tmp
+
+
+def pf_factor(a, V, L, B):
+    """Product-form factorization of I + V diag(a) V^T in place
+    (dpftrf, pfcholesky.c:52-87); returns 0, or -1 where the reference
+    fails."""
+    n, m = V.shape
+    for i in range(m):
+        v, l, b = V[:, i], L[:, i], B[:, i]
+        alpha = -float(a[i])
+        for j in range(n):
+            tmp = 1.0 - alpha * v[j] * v[j]
+            if tmp <= 0:
+                return -1
+            l[j] = np.sqrt(tmp)
+            b[j] = -alpha * v[j] / l[j]
+            alpha += b[j] * b[j]
+        for j in range(i + 1, m):
+            _ddrsv("N", v, l, b, V[:, j])
+    return 0
+
+
+def pf_solve(V, L, B, U, trans="N"):
+    """U := (L_1 ... L_m)^{-1} U or its transpose, column by column
+    (dpfsv, pfcholesky.c:89-112)."""
+    m = V.shape[1]
+    order = range(m) if trans == "N" else range(m - 1, -1, -1)
+    for c in range(U.shape[1]):
+        for i in order:
+            _ddrsv(trans, V[:, i], L[:, i], B[:, i], U[:, c])
+
+
+def pf_product(V, L, B, U, trans="N"):
+    """U := L_1 ... L_m U or its transpose (dpfmv, pfcholesky.c:114-136)."""
+    m = V.shape[1]
+    order = range(m - 1, -1, -1) if trans == "N" else range(m)
+    for c in range(U.shape[1]):
+        for i in order:
+            _ddrmv(trans, V[:, i], L[:, i], B[:, i], U[:, c])
 
 
 def dot(symb, x, y):

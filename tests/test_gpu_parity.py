@@ -604,3 +604,51 @@ def test_spmv_residual(nrhs):
     ref = (Bh - As @ Xh).reshape(R.shape)
     err = np.abs(R.cpu().numpy() - ref).max() / np.abs(ref).max()
     assert err <= 1e-14, err
+
+
+def test_pfcholesky():
+    """Product-form Cholesky: the scan kernels against the oracle's
+    sequential restatement of pfcholesky.c (factor, solve, product, both
+    transposes), the factored operator against X + V diag(a) V^T (solve
+    residual, product round trip), and the reference's failure on an
+    indefinite downdate."""
+    import torch
+    from chompack_amd.pfcholesky import pfcholesky, pfchol, pftrsm, pftrmm
+    rng = np.random.default_rng(12)
+    n, m = 2500, 6
+    V0 = rng.standard_normal((n, m))
+    a = rng.uniform(0.5, 2.0, m)
+    a[1] = -1e-4   # a downdate that stays positive definite (1 / 1e-4 > |v|^2)
+    col = lambda M: torch.as_tensor(np.asfortranarray(M)).to("cuda:0").t().contiguous().t()
+    V, L, B = col(V0), col(np.zeros((n, m))), col(np.zeros((n, m)))
+    pfchol(a, V, L, B)
+    Vr, Lr, Br = V0.copy(), np.zeros((n, m)), np.zeros((n, m))
+    assert on.pf_factor(a, Vr, Lr, Br) == 0
+    for got, want in ((V, Vr), (L, Lr), (B, Br)):
+        assert _rel(got.cpu().numpy(), want) <= TOL
+    U0 = rng.standard_normal((n, 5))
+    for tr in "NT":
+        for gfun, ofun in ((pftrsm, on.pf_solve), (pftrmm, on.pf_product)):
+            U = col(U0)
+            gfun(V, L, B, U, trans=tr)
+            Ur = U0.copy()
+            ofun(Vr, Lr, Br, Ur, trans=tr)
+            assert _rel(U.cpu().numpy(), Ur) <= TOL, (gfun.__name__, tr)
+    # the whole factorization of X + V diag(a) V^T on a chordal X
+    A = problems.grid2d(50, seed=2)
+    W0 = rng.standard_normal((A.shape[0], 4))
+    aw = np.array([1.0, 0.5, 2.0, 0.25])
+    F = pfcholesky(A, W0, a=aw, p=cp.amd.order)
+    M = cp.symmetrize(A).toarray() + W0 @ np.diag(aw) @ W0.T
+    bh = rng.standard_normal((A.shape[0], 3))
+    x = col(bh)
+    F.trsm(x)
+    F.trsm(x, trans="T")
+    r = np.linalg.norm(M @ x.cpu().numpy() - bh, axis=0) / np.linalg.norm(bh, axis=0)
+    assert r.max() <= 1e-12
+    y = col(bh)
+    F.trmm(y, trans="T")
+    F.trmm(y)
+    assert _rel(y.cpu().numpy(), M @ bh) <= 1e-12
+    with pytest.raises(ArithmeticError, match="factorization failed"):
+        pfcholesky(A, W0, a=-10.0 * np.ones(4), p=cp.amd.order)

@@ -287,3 +287,33 @@ def test_factored_hessian_sign_quirk():
     ref.hessian(symb, L, Y, hr, adj=None, factored_updates=True)
     on.hessian(symb, L, Y, ho, adj=None, factored_updates=True)
     assert _rel(ho[0], hr[0]) < 1e-12
+
+
+def test_pf_oracle_closed_form():
+    """The product-form factors reproduce I + V diag(a) V^T: the dense
+    product (L_1 ... L_m)(L_1 ... L_m)^T with L_i = diag(l_i) +
+    strictly-lower(v_i b_i^T), and the solve / product routines invert each
+    other and agree with the dense factor."""
+    rng = np.random.default_rng(4)
+    n, m = 40, 5
+    V0 = rng.standard_normal((n, m))
+    a = rng.uniform(0.5, 2.0, m)
+    a[2] = -0.01   # a small downdate stays positive definite
+    V, L, B = V0.copy(), np.zeros((n, m)), np.zeros((n, m))
+    assert on.pf_factor(a, V, L, B) == 0
+    F = np.eye(n)
+    for i in range(m):
+        Li = np.diag(L[:, i]) + np.tril(np.outer(V[:, i], B[:, i]), -1)
+        F = F @ Li
+    M = np.eye(n) + V0 @ np.diag(a) @ V0.T
+    assert np.abs(F @ F.T - M).max() < 1e-12 * np.abs(M).max()
+    U0 = rng.standard_normal((n, 3))
+    for tr, op in (("N", F), ("T", F.T)):
+        U = U0.copy()
+        on.pf_product(V, L, B, U, trans=tr)
+        assert np.abs(U - op @ U0).max() < 1e-12 * np.abs(op @ U0).max()
+        on.pf_solve(V, L, B, U, trans=tr)
+        assert np.abs(U - U0).max() < 1e-12 * np.abs(U0).max()
+    # a downdate that makes I + V diag(a) V^T indefinite fails
+    V2 = V0.copy()
+    assert on.pf_factor(np.array([-10.0] * m), V2, np.zeros((n, m)), np.zeros((n, m))) == -1
