@@ -16,11 +16,12 @@ from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm
     logdet
 from . import autograd, batch, mixed
 from .pfcholesky import pfcholesky
+from .orderings import maxcardsearch, maxchord
 
 __version__ = "0.1.0"
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
            "completion", "projected_inverse", "hessian", "trsm", "trmm", "dot", "logdet", "autograd",
-           "pfcholesky",
+           "pfcholesky", "maxcardsearch", "maxchord",
            "tril", "triu",
            "symmetrize", "perm", "eye", "lmerge", "amd"]

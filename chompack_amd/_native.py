@@ -67,6 +67,8 @@ def _declare(L):
     _opt(L, "chp_projected_inverse", [vp, vp, vp, vp], i32)
     _opt(L, "chp_trsm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
     _opt(L, "chp_trmm", [vp, vp, vp, i32, i32, dbl, i32, vp], i32)
+    _opt(L, "chp_maxcardsearch", [i64, vp, vp, i64, vp], None)
+    _opt(L, "chp_maxchord", [i64, vp, vp, i64, vp, vp, vp, vp], i64)
     _opt(L, "chp_pf_factor", [i64, i32, vp, vp, i32, vp, i32, vp, i32, vp], i32)
     _opt(L, "chp_pf_solve", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)
     _opt(L, "chp_pf_product", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)

@@ -255,4 +255,107 @@ i64 chp_relative_idx(i64 nsn, const i64* snptr, const i64* snpar,
     return 0;
 }
 
+// Maximum cardinality search (mcs.py:4-57): vertices are numbered from
+// n - 1 down to 0, each time taking the unnumbered vertex with the most
+// numbered neighbours.  Ties follow the reference exactly: the candidates of
+// each weight form a stack (bucket lists S[w]), a vertex is pushed again each
+// time its weight grows (stale entries are skipped when popped), and the
+// initial stack S[0] holds 0 .. n-1 without ve, then ve on top.  cp/ri: the
+// full symmetric pattern in compressed-column form, rows ascending in each
+// column (the CCS of symmetrize(A)).  Writes the ordering p[0..n-1].
+void chp_maxcardsearch(i64 n, const i64* cp, const i64* ri, i64 ve, i64* p) {
+    std::vector<i64> w(n, 0);
+    std::vector<std::vector<i64>> S(std::max<i64>(n, 1));
+    for (i64 i = 0; i < n; ++i)
+        if (i != ve) S[0].push_back(i);
+    S[0].push_back(ve);
+    i64 max_w = 0;
+    for (i64 i = n - 1; i >= 0; --i) {
+        i64 v = -1;
+        for (;;) {
+            if (!S[max_w].empty()) {
+                v = S[max_w].back();
+                S[max_w].pop_back();
+                if (w[v] >= 0) break;
+            } else {
+                --max_w;
+            }
+        }
+        p[i] = v;
+        w[v] = -1;
+        for (i64 q = cp[v]; q < cp[v + 1]; ++q) {
+            const i64 r = ri[q];
+            if (w[r] >= 0) {
+                ++w[r];
+                S[w[r]].push_back(r);
+                max_w = std::max(max_w, w[r]);
+            }
+        }
+    }
+}
+
+// Maximal chordal subgraph (maxchord.py:5-83, Dearing, Shier and Warner):
+// the maximum cardinality search in which an unnumbered neighbour u of the
+// vertex v just numbered gains v (and the edge {u, v}) only when u's set of
+// chordal numbered neighbours C[u] is a subset of C[v].  Outputs the kept
+// entries of the lower triangle in the reference's order (for each column j
+// ascending, the rows in the order they were kept; a diagonal entry is kept
+// when met), their positions in vals (so the caller copies the values), and
+// the ordering p.  Returns the number of kept entries (at most cp[n]).
+i64 chp_maxchord(i64 n, const i64* cp, const i64* ri, i64 ve, i64* p, i64* rows, i64* cols,
+                 i64* src) {
+    std::vector<i64> w(n, 0);
+    std::vector<std::vector<i64>> S(std::max<i64>(n, 1));
+    for (i64 i = 0; i < n; ++i)
+        if (i != ve) S[0].push_back(i);
+    S[0].push_back(ve);
+    std::vector<std::vector<i64>> C(n);       // chordal numbered neighbours
+    std::vector<std::vector<i64>> E(n), Q(n);  // kept rows and value positions per column
+    std::vector<i64> mark(n, -1);
+    i64 max_w = 0;
+    for (i64 i = n - 1; i >= 0; --i) {
+        i64 v = -1;
+        for (;;) {
+            if (!S[max_w].empty()) {
+                v = S[max_w].back();
+                S[max_w].pop_back();
+                if (w[v] >= 0) break;
+            } else {
+                --max_w;
+            }
+        }
+        p[i] = v;
+        w[v] = -1;
+        for (i64 x : C[v]) mark[x] = v;  // C[v] as a marked set; fixed while v's neighbours are visited
+        for (i64 q = cp[v]; q < cp[v + 1]; ++q) {
+            const i64 u = ri[q];
+            if (w[u] >= 0) {
+                bool subset = true;
+                for (i64 x : C[u])
+                    if (mark[x] != v) { subset = false; break; }
+                if (subset) {
+                    C[u].push_back(v);
+                    ++w[u];
+                    S[w[u]].push_back(u);
+                    max_w = std::max(max_w, w[u]);
+                    E[std::min(u, v)].push_back(std::max(u, v));
+                    Q[std::min(u, v)].push_back(q);
+                }
+            } else if (u == v) {
+                E[u].push_back(u);
+                Q[u].push_back(q);
+            }
+        }
+    }
+    i64 cnt = 0;
+    for (i64 j = 0; j < n; ++j)
+        for (size_t t = 0; t < E[j].size(); ++t) {
+            rows[cnt] = E[j][t];
+            cols[cnt] = j;
+            src[cnt] = Q[j][t];
+            ++cnt;
+        }
+    return cnt;
+}
+
 }  // extern "C"

@@ -49,6 +49,16 @@ int64_t chp_embed(int64_t nsn, const int64_t* cpl, const int64_t* ril, const int
 int64_t chp_relative_idx(int64_t nsn, const int64_t* snptr, const int64_t* snpar,
                          const int64_t* sncolptr, const int64_t* snrowidx,
                          const int64_t* relptr, int64_t* relidx);
+/* Maximum cardinality search ordering (mcs.py:4-57); cp/ri: the full
+ * symmetric pattern (CCS of symmetrize(A), rows ascending per column); ve:
+ * the vertex eliminated last.  Writes p[0..n-1]. */
+void chp_maxcardsearch(int64_t n, const int64_t* cp, const int64_t* ri, int64_t ve, int64_t* p);
+/* Maximal chordal subgraph (maxchord.py:5-83): writes the ordering p and the
+ * kept lower-triangular entries (rows[i], cols[i]; src[i] = position of the
+ * entry in ri, so the caller copies its value); returns their number
+ * (at most cp[n]). */
+int64_t chp_maxchord(int64_t n, const int64_t* cp, const int64_t* ri, int64_t ve, int64_t* p,
+                     int64_t* rows, int64_t* cols, int64_t* src);
 /* Approximate minimum degree ordering; replaces cvxopt.amd.order as called
  * through symbolic(A, p=amd.order) (symbolic.py:540-546). */
 int chp_amd_order(int64_t n, const int64_t* cp, const int64_t* ri, int64_t* perm);
