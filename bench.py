@@ -154,6 +154,52 @@ def main():
         solve = {"nrhs": 16, "ms": round(solve_ms, 3), "what": "trsm N then T with the factor",
                  "max_rel_residual": resid}
 
+    # products with the factor (trmm N then T, 16 right-hand sides) and the
+    # product-form Cholesky factor of I + V diag(a) V^T (n = 40^3, m = 16):
+    # factor, then a solve with 16 right-hand sides (N then T)
+    trmm = pfsec = None
+    if not args.no_solve:
+        Ms = B0.clone()
+        for _ in range(2):
+            Ms.copy_(B0)
+            cp.trmm(Lf, Ms)
+            cp.trmm(Lf, Ms, trans="T")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            Ms.copy_(B0)
+            cp.trmm(Lf, Ms)
+            cp.trmm(Lf, Ms, trans="T")
+        torch.cuda.synchronize()
+        trmm = {"nrhs": 16, "ms": round((time.perf_counter() - t0) * 1e3 / 5, 3),
+                "what": "trmm N then T with the factor (B := L^T L B)"}
+        trmm_gpu = Ms.cpu().numpy()
+        from chompack_amd.pfcholesky import pfchol, pftrsm
+        m_pf = 16
+        Vh = np.asfortranarray(rng.standard_normal((symb.n, m_pf)) / np.sqrt(symb.n))
+        a_pf = rng.uniform(0.5, 2.0, m_pf)
+        V0g = torch.as_tensor(Vh).to(dev).t().contiguous().t()
+        Vg, Lg, Bg = V0g.clone(), torch.zeros_like(V0g), torch.zeros_like(V0g)
+        Ug = B0.clone()
+
+        def pf_step():
+            Vg.copy_(V0g)
+            pfchol(a_pf, Vg, Lg, Bg)
+            Ug.copy_(B0)
+            pftrsm(Vg, Lg, Bg, Ug, trans="N")
+            pftrsm(Vg, Lg, Bg, Ug, trans="T")
+
+        pf_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            pf_step()
+        torch.cuda.synchronize()
+        pfsec = {"n": symb.n, "m": m_pf, "nrhs": 16,
+                 "ms": round((time.perf_counter() - t0) * 1e3 / 5, 3),
+                 "what": "pfchol of I + V diag(a) V^T, then pftrsm N and T on 16 right-hand sides"}
+        pf_gpu = Ug.cpu().numpy()
+
     # mixed precision: FP32 Schur updates + FP64 iterative refinement of the
     # 16-RHS solve to a relative residual <= 1e-10, against the FP64 path
     # (FP64 factorization + the same solve)
@@ -268,6 +314,8 @@ def main():
         "components_instrumented_ms": comps,
         "hbm_kernels": hbm,
         "solve_16rhs": solve,
+        "trmm_16rhs": trmm,
+        "pfcholesky": pfsec,
         "mixed_vs_fp64": mixed,
         "gemm_by_size": gemm_classes,
         "gemm_top": [{"tiles": int(r[1]), "gflop": round(r[2] / 1e9, 3), "ms": round(r[3], 4),
@@ -318,6 +366,34 @@ def main():
                 solve["cpu_reference_ms"] = round(statistics.median(st) * 1e3, 3)
                 solve["speedup_vs_cpu"] = round(solve["cpu_reference_ms"] / solve["ms"], 2)
                 solve["rel_err_vs_reference"] = float(np.abs(xs - Br).max() / np.abs(Br).max())
+            if trmm is not None:
+                # the reference's trmm is Python (pybase/trmm.py): its NumPy
+                # restatement is the CPU path, one run
+                from oracle import numeric as on
+                Mr = np.asfortranarray(Bh.copy())
+                t0 = time.perf_counter()
+                on.trmm(symb, Lr, Mr, trans="N")
+                on.trmm(symb, Lr, Mr, trans="T")
+                trmm["cpu_port_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+                trmm["speedup_vs_cpu"] = round(trmm["cpu_port_ms"] / trmm["ms"], 2)
+                trmm["rel_err_vs_port"] = float(np.abs(trmm_gpu - Mr).max() / np.abs(Mr).max())
+            if pfsec is not None:
+                pt = []
+                for it in range(args.cpu_runs + 1):
+                    Vr = Vh.copy(order="F")
+                    Lp = np.zeros_like(Vr, order="F")
+                    Bp = np.zeros_like(Vr, order="F")
+                    Ur = np.asfortranarray(Bh.copy())
+                    t0 = time.perf_counter()
+                    assert ref.pf_factor(a_pf, Vr, Lp, Bp) == 0
+                    ref.pf_solve(Vr, Lp, Bp, Ur, trans="N")
+                    ref.pf_solve(Vr, Lp, Bp, Ur, trans="T")
+                    if it > 0:
+                        pt.append(time.perf_counter() - t0)
+                pfsec["cpu_reference_ms"] = round(statistics.median(pt) * 1e3, 3)
+                pfsec["cpu_kind"] = "reference (pfcholesky.c, 1 thread: sequential row loops)"
+                pfsec["speedup_vs_cpu"] = round(pfsec["cpu_reference_ms"] / pfsec["ms"], 2)
+                pfsec["rel_err_vs_reference"] = float(np.abs(pf_gpu - Ur).max() / np.abs(Ur).max())
         else:
             out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -50,6 +50,12 @@ def lib():
         L.trsm.restype = None
         L.dot.argtypes = [P, P, P, P, P, P]
         L.dot.restype = ctypes.c_double
+        L.dpftrf.argtypes = [P, P, P, P, P, P, P, P, P]
+        L.dpftrf.restype = ctypes.c_int
+        L.dpfsv.argtypes = [P, P, P, P, P, P, P, P, P, P]
+        L.dpfsv.restype = None
+        L.dpfmv.argtypes = [P, P, P, P, P, P, P, P, P, P]
+        L.dpfmv.restype = None
         _lib = L
     return _lib
 
@@ -191,3 +197,41 @@ def dot(symb, x, y):
     nsn = ctypes.c_int64(s.nsn)
     return lib().dot(ctypes.addressof(nsn), _p(s.snptr), _p(s.sncolptr), _p(s.blkptr),
                      _p(_blk(x)), _p(_blk(y)))
+
+
+def _ints(*v):
+    return [ctypes.c_int(int(x)) for x in v]
+
+
+def pf_factor(a, V, L, B):
+    """dpftrf (pfcholesky.c:52-87) on column-major n x m arrays, in place;
+    returns 0 or -1."""
+    n, m = V.shape
+    for M in (V, L, B):
+        assert M.dtype == np.float64 and M.flags["F_CONTIGUOUS"]
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    cn, cm = _ints(n, m)
+    return lib().dpftrf(ctypes.addressof(cn), ctypes.addressof(cm), _p(a), V.ctypes.data,
+                        ctypes.addressof(cn), L.ctypes.data, ctypes.addressof(cn), B.ctypes.data,
+                        ctypes.addressof(cn))
+
+
+def _pf_apply(fn, V, L, B, U, trans):
+    n, m = V.shape
+    assert U.dtype == np.float64 and U.flags["F_CONTIGUOUS"] and U.shape[0] == n
+    cn, cm = _ints(n, m)
+    tr = ctypes.c_char(trans.encode())
+    for c in range(1 if U.ndim == 1 else U.shape[1]):
+        fn(ctypes.addressof(cn), ctypes.addressof(cm), ctypes.addressof(tr), V.ctypes.data,
+           ctypes.addressof(cn), L.ctypes.data, ctypes.addressof(cn), B.ctypes.data,
+           ctypes.addressof(cn), U.ctypes.data + 8 * n * c)
+
+
+def pf_solve(V, L, B, U, trans="N"):
+    """dpfsv (pfcholesky.c:89-112) for every column of U, as cbase.c pftrsm."""
+    _pf_apply(lib().dpfsv, V, L, B, U, trans)
+
+
+def pf_product(V, L, B, U, trans="N"):
+    """dpfmv (pfcholesky.c:114-136) for every column of U, as cbase.c pftrmm."""
+    _pf_apply(lib().dpfmv, V, L, B, U, trans)

@@ -317,3 +317,29 @@ def test_pf_oracle_closed_form():
     # a downdate that makes I + V diag(a) V^T indefinite fails
     V2 = V0.copy()
     assert on.pf_factor(np.array([-10.0] * m), V2, np.zeros((n, m)), np.zeros((n, m))) == -1
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
+def test_pf_oracle_vs_reference_kernels():
+    """The oracle's product-form routines against pfcholesky.c itself
+    (oracle/_ref): the same sequential arithmetic, so equal to the last bit
+    up to the compiler's contraction of a*b+c."""
+    rng = np.random.default_rng(9)
+    n, m = 300, 4
+    V0 = np.asfortranarray(rng.standard_normal((n, m)))
+    a = np.array([1.0, 0.5, -0.002, 2.0])
+    V1, L1, B1 = V0.copy(order="F"), np.zeros((n, m), order="F"), np.zeros((n, m), order="F")
+    V2, L2, B2 = V0.copy(order="F"), np.zeros((n, m), order="F"), np.zeros((n, m), order="F")
+    assert on.pf_factor(a, V1, L1, B1) == 0
+    assert ref.pf_factor(a, V2, L2, B2) == 0
+    for x, y in ((V1, V2), (L1, L2), (B1, B2)):
+        assert _rel(x, y) < 1e-14
+    U0 = np.asfortranarray(rng.standard_normal((n, 3)))
+    for tr in "NT":
+        for ofun, rfun in ((on.pf_solve, ref.pf_solve), (on.pf_product, ref.pf_product)):
+            Uo, Ur = U0.copy(order="F"), U0.copy(order="F")
+            ofun(V1, L1, B1, Uo, trans=tr)
+            rfun(V2, L2, B2, Ur, trans=tr)
+            assert _rel(Uo, Ur) < 1e-14
+    assert ref.pf_factor(np.full(m, -10.0), V0.copy(order="F"), np.zeros((n, m), order="F"),
+                         np.zeros((n, m), order="F")) == -1
