@@ -652,3 +652,78 @@ def test_pfcholesky():
     assert _rel(y.cpu().numpy(), M @ bh) <= 1e-12
     with pytest.raises(ArithmeticError, match="factorization failed"):
         pfcholesky(A, W0, a=-10.0 * np.ones(4), p=cp.amd.order)
+
+
+def _edge_problems():
+    import scipy.sparse as sp
+    rng = np.random.default_rng(21)
+    out = {}
+    out["n1"] = sp.csc_matrix(np.array([[3.0]]))
+    out["diagonal"] = sp.csc_matrix(sp.diags(rng.uniform(1.0, 2.0, 100)))
+    D = rng.standard_normal((200, 200))
+    out["dense200"] = sp.csc_matrix(np.tril(D @ D.T + 200.0 * np.eye(200)))
+    n = 300   # tridiagonal: an elimination-tree chain of singleton fronts
+    T = sp.diags([np.full(n - 1, -1.0), np.full(n, 3.0)], [-1, 0], format="csc")
+    out["chain"] = T
+    return out
+
+
+@pytest.mark.parametrize("name", ["n1", "diagonal", "dense200", "chain"])
+def test_edge_shapes(name):
+    """Degenerate patterns: one column, no off-diagonal (every front a root
+    with na = 0), one dense front (the large-front path end to end), a chain
+    of singleton fronts; cholesky / solve / trmm / projected inverse /
+    completion against the oracle and the closed forms."""
+    import torch
+    A = _edge_problems()[name]
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    ref = X.blkval.cpu().numpy().copy()
+    assert on.cholesky(symb, ref) == 0
+    cp.cholesky(X)
+    assert _rel(X.blkval.cpu().numpy(), ref) <= TOL
+    n = symb.n
+    rng = np.random.default_rng(3)
+    Bh = rng.standard_normal((n, 5))
+    B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+    cp.trsm(X, B)
+    cp.trsm(X, B, trans="T")
+    As = cp.symmetrize(A).toarray()
+    assert np.abs(As @ B.cpu().numpy() - Bh).max() <= 1e-12 * np.abs(Bh).max() * max(1, np.abs(As).max())
+    cp.trmm(X, B, trans="T")
+    cp.trmm(X, B)
+    assert _rel(B.cpu().numpy(), Bh) <= 1e-12
+    Y = X.copy()
+    cp.projected_inverse(Y)
+    yr = ref.copy()
+    on.projected_inverse(symb, yr)
+    assert _rel(Y.blkval.cpu().numpy(), yr) <= TOL
+    cp.completion(Y)
+    m = _lower_mask(symb)
+    assert _rel(Y.blkval.cpu().numpy()[m], ref[m]) <= 1e-11
+
+
+def test_trsm_offset_and_leading_dimension():
+    """trsm / trmm on a column block of a larger array (offsetB, ldB, nrhs as
+    cbase.c:1006-1041 reads them): only the addressed entries change."""
+    import torch
+    A = problems.grid2d(20, seed=5)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    cp.cholesky(X)
+    n = symb.n
+    ld = n + 7
+    rng = np.random.default_rng(1)
+    store = rng.standard_normal(ld * 6)
+    Bt = torch.as_tensor(store.copy(), device="cuda:0")
+    off = ld + 3   # column 1, row 3 of an ld x 6 array: n rows x 3 columns
+    cp.trsm(X, Bt, nrhs=3, offsetB=off, ldB=ld)
+    got = Bt.cpu().numpy()
+    blk = np.asfortranarray(np.stack([store[off + j * ld: off + j * ld + n] for j in range(3)], axis=1))
+    on.trsm(symb, X.blkval.cpu().numpy(), blk, trans="N")
+    for j in range(3):
+        assert _rel(got[off + j * ld: off + j * ld + n], blk[:, j]) <= TOL
+    mask = np.ones(ld * 6, dtype=bool)
+    for j in range(3):
+        mask[off + j * ld: off + j * ld + n] = False
+    assert np.array_equal(got[mask], store[mask])
