@@ -628,27 +628,15 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
-    // C is read early (beta != 0) so its latency hides behind the K loop;
-    // split-K slices write unscaled-by-beta partials instead
+    // C is read in the epilogue (registers for an early read would cost
+    // occupancy); split-K slices write unscaled-by-beta partials instead
     const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
-    dbl4 cin[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
-                const int j = col0 + wc * 32 + b * 16 + (lane & 15);
-                const bool in = !CHK || (i < T.M && j < T.N);
-                cin[a][b][v] = (beta != 0.0 && in && (T.tri == 0 || i >= j)) ? C[i + (i64)j * T.ldc] : 0.0;
-            }
     if (kbase < kend) fetch(kbase);
     for (int kb = kbase; kb < kend; kb += BK) {
         stash();
         __syncthreads();
         if (kb + BK < kend) fetch(kb + BK);
-#pragma unroll
+#pragma unroll 2
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
             const double a0 = As[kr][wr * 32 + (lane & 15)];
@@ -673,12 +661,15 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
                 if (CHK && (i >= T.M || j >= T.N)) continue;
                 if (Pp) { Pp[i + (i64)j * T.M] = alpha * acc[a][b][v]; continue; }
                 double* c = C + i + (i64)j * T.ldc;
-                if (T.tri == 0 || i >= j) *c = alpha * acc[a][b][v] + beta * cin[a][b][v];
+                if (T.tri == 0 || i >= j) *c = alpha * acc[a][b][v] + (beta != 0.0 ? beta * *c : 0.0);
                 else if (T.tri == 2) *c = 0.0;
             }
 }
 
-__global__ __launch_bounds__(TPB) void k_gemm_tiles(
+// Four waves per SIMD (the LDS of four workgroups fits a CU): the register
+// budget is capped at 128 so that the K loop's load and LDS latency overlap
+// with the other workgroups' MFMA work.
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gemm_tiles(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     __shared__ double As[BK][TILE + 2];
     __shared__ double Bs[BK][TILE + 2];
