@@ -9,68 +9,77 @@
 #define TPB 256
 
 // ---------------------------------------------------------------------------
-// Deterministic extend-add for large fronts.  One workgroup owns 16 target
-// columns [c0, c0+16) of front k: it zeroes the F22 part of those columns,
-// then adds every child's update in reverse child order (the summation order
-// of cholesky.c:41-52), so each target entry is updated by exactly one
-// workgroup, in the reference's order, without atomics.  (llt.c:36-46 adds
-// into an F22 already holding the product, so zeroing is optional.)
+// Deterministic extend-add for large fronts.  One workgroup owns the target
+// tile rows [r0, r0+256) x columns [c0, c0+16) of front k (lower part): it
+// zeroes the F22 entries of the tile, then adds every child's update in
+// reverse child order (the summation order of cholesky.c:41-52), so each
+// target entry is updated by exactly one workgroup, in the reference's order,
+// without atomics.  (llt.c:36-46 adds into an F22 already holding the
+// product, so zeroing is optional.)  Relative indices increase, so a child's
+// rows and columns landing in the tile are contiguous index ranges; each
+// thread owns one child row and keeps four of its columns in flight.
 // Target (row, col) with col < nn lives in the blkval block (ld nj); with
 // col >= nn in k's update slot (ld na).
 __global__ __launch_bounds__(TPB) void k_extadd_tiles(
-    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ upd,
     const int* __restrict__ relidx, const int* __restrict__ chidx, int zero22) {
-    const int2 t = tiles[blockIdx.x];
-    const int k = t.x, c0 = t.y;
+    const int4 t = tiles[blockIdx.x];
+    const int k = t.x, c0 = t.y, r0 = t.z;
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
-    const int c1 = min(c0 + 16, nj);
+    const int c1 = min(c0 + 16, nj), r1 = min(r0 + TPB, nj);
     const int tid = threadIdx.x;
     double* blk = blkval + f.blkptr;
     double* U = upd + f.updptr;
     if (zero22) {
-        for (int tc = max(c0, nn); tc < c1; ++tc)
-            for (int tr = tc + tid; tr < nj; tr += TPB)
+        const int tr = r0 + tid;
+        if (tr < r1)
+            for (int tc = max(c0, nn); tc < c1 && tc <= tr; ++tc)
                 U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
     }
     __syncthreads();
-    __shared__ int s_ja, s_jb;
+    __shared__ int s_ja, s_jb, s_ia, s_ib;
     for (int l = f.chend - 1; l >= f.chbeg; --l) {
         const FrontInfo g = fi[chidx[l]];
         const int N = g.na;
         const int* r = relidx + g.relptr;
         const double* Uc = upd + g.updptr;
         if (tid == 0) {
-            int lo = 0, hi = N;
-            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < c0) lo = m + 1; else hi = m; }
-            s_ja = lo;
-            hi = N;
-            while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < c1) lo = m + 1; else hi = m; }
-            s_jb = lo;
+            auto lower = [&](int v) {  // first child index with r >= v
+                int lo = 0, hi = N;
+                while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < v) lo = m + 1; else hi = m; }
+                return lo;
+            };
+            s_ja = lower(c0);
+            s_jb = lower(c1);
+            s_ia = lower(r0);
+            s_ib = lower(r1);
         }
         __syncthreads();
-        const int ja = s_ja, jb = s_jb;
-        for (int j = ja; j < jb; ++j) {
-            const int tc = r[j];
-            const double* col = Uc + (i64)j * N;
-            double* dst = (tc < nn) ? blk + (i64)tc * nj : U + (i64)(tc - nn) * na - nn;
-            // 4 rows per thread in flight: the loads of r, U and F are independent
-            for (int i0 = j + tid; i0 < N; i0 += 4 * TPB) {
-                int ri[4];
+        const int ja = s_ja, jb = s_jb, i = s_ia + tid;
+        if (i < s_ib) {
+            const int ri = r[i];
+            const int je = min(jb, i + 1);  // lower part: column j <= row i
+            for (int j0 = ja; j0 < je; j0 += 4) {
+                double* d[4];
                 double u[4], fv[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int i = i0 + t * TPB;
-                    ri[t] = (i < N) ? r[i] : 0;
-                    u[t] = (i < N) ? col[i] : 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    const int j = j0 + q;
+                    d[q] = nullptr;
+                    u[q] = 0.0;
+                    if (j < je) {
+                        const int tc = r[j];
+                        d[q] = (tc < nn) ? blk + (i64)tc * nj : U + (i64)(tc - nn) * na - nn;
+                        u[q] = Uc[i + (i64)j * N];
+                    }
                 }
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    fv[t] = (i0 + t * TPB < N) ? dst[ri[t]] : 0.0;
+                for (int q = 0; q < 4; ++q) fv[q] = d[q] ? d[q][ri] : 0.0;
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    if (i0 + t * TPB < N) dst[ri[t]] = fv[t] + u[t];
+                for (int q = 0; q < 4; ++q)
+                    if (d[q]) d[q][ri] = fv[q] + u[q];
             }
         }
         __syncthreads();

@@ -20,8 +20,8 @@ __global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkv
                             double* upd, const int* relidx, const int* chidx);
 
 // large fronts
-__global__ void k_extadd_tiles(const int2* tiles, const FrontInfo* fi, double* blkval,
-                               double* upd, const int* relidx, const int* chidx, int zero22);
+__global__ void k_extadd_tiles(const int4* tiles, const FrontInfo* fi, double* blkval, double* upd,
+                               const int* relidx, const int* chidx, int zero22);
 __global__ void k_copy_top(const int2* tiles, const FrontInfo* fi, const double* blkval,
                            double* scratch, const i64* sptr);
 __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, double* blkval,

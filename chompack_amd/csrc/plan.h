@@ -204,6 +204,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
 int read_status(chp_plan* P, hipStream_t s, i64* info);
 // Creates the side streams and their fork / join events (once per plan).
 int ensure_side_stream(chp_plan* P);
+// Extend-add work list of the listed fronts: (front, c0, r0) for the target
+// tiles of 16 columns x 256 rows that meet the lower triangle.
+std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
 

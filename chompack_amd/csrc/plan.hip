@@ -153,6 +153,16 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     if (P->pg->split_out.empty()) P->pg->splitp_cur = P->pg->side_cur = 0;
 }
 
+std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts) {
+    std::vector<int4> et;
+    for (int k : fronts) {
+        const int nj = P->fi[k].nn + P->fi[k].na;
+        for (int c0 = 0; c0 < nj; c0 += 16)
+            for (int r0 = c0 & ~255; r0 < nj; r0 += 256) et.push_back(make_int4(k, c0, r0, 0));
+    }
+    return et;
+}
+
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
 void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     std::vector<int> small, large;
@@ -173,11 +183,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     }
     if (large.empty()) return;
     P->n_large_chol += (int)large.size();
-    std::vector<int2> et;
-    for (int k : large) {
-        int nj = P->fi[k].nn + P->fi[k].na;
-        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
-    }
+    const std::vector<int4> et = extadd_tiles(P, large);
     P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1});
     // Two-level blocking: 64-wide inner panels (diagonal factor, panel solve,
     // update confined to the current outer panel), then one trailing update
@@ -492,11 +498,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
     P->llt.push_back({K_COPY_TOP, (int)ct.size(), 256, 0, P->pg->lists.add(ct), 0});
     push_gemm(P, P->llt, g1);
     push_gemm(P, P->llt, g2);
-    std::vector<int2> et;
-    for (int k : large) {
-        int nj = P->fi[k].nn + P->fi[k].na;
-        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
-    }
+    const std::vector<int4> et = extadd_tiles(P, large);
     P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0});
 }
 
@@ -575,7 +577,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
             break;
         case K_EXTADD:
-            CHP_LAUNCH(k_extadd_tiles, 0, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, P->d_fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
             break;
         case K_COPY_TOP:

@@ -328,11 +328,7 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
 // One level of _Y2K (hessian.c:3-101), leaves to root, for one direction.
 void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv) {
     const double alpha = inv ? 1.0 : -1.0;
-    std::vector<int2> et;
-    for (int k : fr) {
-        const int nj = P->fi[k].nn + P->fi[k].na;
-        for (int c0 = 0; c0 < nj; c0 += 16) et.push_back(make_int2(k, c0));
-    }
+    const std::vector<int4> et = extadd_tiles(P, fr);
     if (!inv) {
         Launch L{K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1};
         seq.push_back(L);
