@@ -104,15 +104,15 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
 // slot, both triangles (projected_inverse.c:60-70 / cbase.c:39-59, done by
 // the child).  The parent's front is its blkval block (columns < nn_p) and
 // its own V_p (columns >= nn_p, lower triangle read).  One workgroup per
-// (front, 16-column tile); rows go through LDS in chunks of 256 so that the
-// mirrored upper-triangle writes are coalesced too.
+// (front, 16-column tile, 256-row block); the block goes through LDS so that
+// the mirrored upper-triangle writes are coalesced too.
 __global__ __launch_bounds__(TPB) void k_extract(
-    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ upd,
     const int* __restrict__ relidx) {
     __shared__ double S[256][17];
-    const int2 t = tiles[blockIdx.x];
-    const int k = t.x, j0 = t.y;
+    const int4 t = tiles[blockIdx.x];
+    const int k = t.x, j0 = t.y, ib = t.z;
     const FrontInfo f = fi[k];
     const int na = f.na;
     const FrontInfo p = fi[f.parent];
@@ -123,9 +123,8 @@ __global__ __launch_bounds__(TPB) void k_extract(
     double* V = upd + f.updptr;
     const int j1 = min(j0 + 16, na);
     const int tid = threadIdx.x;
-    for (int i0 = j0; i0 < na; i0 += 256) {
-        const int i1 = min(i0 + 256, na);
-        __syncthreads();
+    {
+        const int i0 = max(ib, j0), i1 = min(ib + 256, na);
         const int i = i0 + tid;
         const int ri = (i < i1) ? r[i] : 0;
         // all 16 columns' loads issued before any store

@@ -207,6 +207,9 @@ int ensure_side_stream(chp_plan* P);
 // Extend-add work list of the listed fronts: (front, c0, r0) for the target
 // tiles of 16 columns x 256 rows that meet the lower triangle.
 std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts);
+// Extract work list of the listed fronts: (front, j0, i0) for the 16-column x
+// 256-row blocks of V_k that meet the lower triangle.
+std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& fronts);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
 

@@ -163,6 +163,16 @@ std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts
     return et;
 }
 
+std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& fronts) {
+    std::vector<int4> xt;
+    for (int k : fronts) {
+        const int na = P->fi[k].na;
+        for (int j0 = 0; j0 < na; j0 += 16)
+            for (int i0 = j0 & ~255; i0 < na; i0 += 256) xt.push_back(make_int4(k, j0, i0, 0));
+    }
+    return xt;
+}
+
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
 void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     std::vector<int> small, large;
@@ -328,9 +338,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
 // from tbase on: each stream reuses its own region level after level.
 void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64 tbase,
                          i64& xcur, i64& scratch_need) {
-    std::vector<int2> xt;
-    for (int k : fr)
-        for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
+    const std::vector<int4> xt = extract_tiles(P, fr);
     std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
@@ -602,7 +610,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_EXTRACT:
-            CHP_LAUNCH(k_extract, 0, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extract, 0, (const int4*)list, P->d_fi, blkval,
                        P->d_upd, P->d_relidx);
             break;
         case K_PROJINV_SMALL:

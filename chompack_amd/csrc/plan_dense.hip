@@ -177,9 +177,7 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
     const std::vector<int> ch = children_of(P, fr);
     if (ch.empty()) return;
     if (!factored) {
-        std::vector<int2> xt;
-        for (int c : ch)
-            for (int j0 = 0; j0 < P->fi[c].na; j0 += 16) xt.push_back(make_int2(c, j0));
+        const std::vector<int4> xt = extract_tiles(P, ch);
         push_tasks(P, seq, K_EXTRACT, xt, fbuf);
         return;
     }
@@ -372,10 +370,8 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     if (inv) {
         emit_child_updates(P, seq, fr, false, 0, sc);  // before this level's update
     } else {
-        std::vector<int2> xt;  // this level's V from the (finished) parents
-        for (int k : fr)
-            for (int j0 = 0; j0 < P->fi[k].na; j0 += 16) xt.push_back(make_int2(k, j0));
-        push_tasks(P, seq, K_EXTRACT, xt, 0);
+        // this level's V from the (finished) parents
+        push_tasks(P, seq, K_EXTRACT, extract_tiles(P, fr), 0);
     }
     std::vector<GemmTile> g1, g2, g3;
     for (int k : fr) {
