@@ -160,12 +160,26 @@ struct chp_plan {
     bool use_graphs = true;
     hipStream_t gstream = nullptr;
     hipEvent_t gev_in = nullptr, gev_out = nullptr;
-    // side streams for independent work (K_FORK / K_JOIN with iarg = 1 or 2):
-    // 1 carries a level's small fronts, 2 the stored-convention conversion
-    hipStream_t sstream[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t sev_fork[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t sev_join[3] = {nullptr, nullptr, nullptr};
-    struct Cached { const void* key = nullptr; hipGraphExec_t exec = nullptr; };
+    // side streams for independent work (K_FORK / K_JOIN with iarg = 1..3):
+    // 1 carries a level's small fronts, 2 the stored-convention conversion and
+    // the L11 inversions, 3 the projected inverse's F21 := -T copies
+    static const int NSIDE = 4;
+    hipStream_t sstream[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t sev_fork[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t sev_join[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+    // A sweep's captured graph, and the choice between replaying it and
+    // launching eagerly on the side streams (whichever measured faster: the
+    // graph saves host launch work, eager launches give every side stream
+    // its own hardware queue).  calls 1-2 run eagerly, 3-4 through the graph;
+    // calls 2 and 4 are timed with events; from call 5 on `mode` is fixed.
+    struct Cached {
+        const void* key = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int calls = 0, mode = 0;  // mode: 0 undecided, 1 eager, 2 graph
+        float t_eager = 0.f, t_graph = 0.f;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int pending = 0;          // mode of a timed call whose end event is unread
+    };
     Cached g_chol, g_projinv, g_llt, g_chol32;
     bool schur32 = false;  // the cholesky sweep in progress runs Schur updates in FP32
     // profiling: per-kernel-kind time from HIP events around every launch

@@ -334,9 +334,11 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
 // X = L11^{-1} of a large front reads only the factor, so it is computed on
 // side stream 2, which runs ahead through the levels; every large front has
 // its own X at h_xptr[k] in [0, wbase).  W (inversion temporary, side stream
-// 2 only) is per level from wbase on, T = V_k B21 (main stream only) per level
-// from tbase on: each stream reuses its own region level after level.
-void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64 tbase,
+// 2 only) is per level from wbase on.  T = V_k B21 has its own region per
+// large front (h_tptr[k], from tcur on): the children's extract reads the
+// parent's F21 = -T there, so that the F21 := -T copy into blkval (side
+// stream 3) stays off the main stream.
+void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64& tcur,
                          i64& xcur, i64& scratch_need) {
     const std::vector<int4> xt = extract_tiles(P, fr);
     std::vector<int> small, large;
@@ -348,7 +350,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     }
     if (!large.empty()) {
         P->n_large_projinv += (int)large.size();
-        i64 woff_end = wbase, toff_end = tbase;
+        i64 woff_end = wbase;
         std::vector<i64> woff(large.size());
         for (size_t q = 0; q < large.size(); ++q) {
             const FrontInfo& f = P->fi[large[q]];
@@ -358,10 +360,10 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
         }
         for (size_t q = 0; q < large.size(); ++q) {
             const FrontInfo& f = P->fi[large[q]];
-            P->h_tptr[large[q]] = toff_end;
-            toff_end += ((i64)f.na * f.nn + 31) / 32 * 32;
+            P->h_tptr[large[q]] = tcur;
+            tcur += ((i64)f.na * f.nn + 31) / 32 * 32;
         }
-        scratch_need = std::max(scratch_need, toff_end);  // W ends at or below tbase
+        scratch_need = std::max(scratch_need, tcur);  // W ends at or below the T regions
         // side stream 2: diagonal block inverses, then recursive halving of
         // the block range [a, b) at m:
         //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
@@ -417,8 +419,8 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     // the small fronts
     const bool fork = !large.empty() && (!xt.empty() || !small.empty());
     if (fork) P->projinv.push_back({K_FORK, 0, 0, 0, 0, 1});
-    if (!xt.empty()) {
-        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0};
+    if (!xt.empty()) {  // iarg 1: a large parent's F21 read as -T
+        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 1};
         Lx.side = fork ? 1 : 0;
         P->projinv.push_back(Lx);
     }
@@ -452,8 +454,12 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     for (int k : large)
         if (P->fi[k].na > 0)
             for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
-    if (!nt.empty())
-        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0});
+    if (!nt.empty()) {
+        P->projinv.push_back({K_FORK, 0, 0, 0, 0, 3});
+        Launch Ln{K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0};
+        Ln.side = 3;
+        P->projinv.push_back(Ln);
+    }
 }
 
 // llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
@@ -549,7 +555,7 @@ int sync_program(Program& pg) {
 }
 
 int ensure_side_stream(chp_plan* P) {
-    for (int i = 1; i < 3; ++i) {
+    for (int i = 1; i < chp_plan::NSIDE; ++i) {
         if (P->sstream[i]) continue;
         HIP_CHECK(hipStreamCreateWithFlags(&P->sstream[i], hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&P->sev_fork[i], hipEventDisableTiming));
@@ -611,7 +617,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, P->d_fi, blkval,
-                       P->d_upd, P->d_relidx);
+                       P->d_upd, P->d_relidx, L.iarg == 1 ? P->d_scratch : nullptr, P->d_tptr);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
@@ -657,8 +663,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
 }
 #undef CHP_LAUNCH
 
-// Runs `seq` (preceded by a status reset) on stream s, through a cached HIP
-// graph when enabled.
+// Runs `seq` (preceded by a status reset) on stream s, eagerly or through a
+// cached HIP graph: with graphs enabled, the first calls time both ways and
+// the faster one is kept (chp_plan::Cached).
 int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, double* blkval,
               hipStream_t s, bool reset_status) {
     const BufTab bt = main_tab(P, blkval);
@@ -673,26 +680,57 @@ int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, 
         HIP_CHECK(hipEventCreateWithFlags(&P->gev_out, hipEventDisableTiming));
     }
     if (ensure_side_stream(P)) return -2;  // created outside any capture
-    if (G.key != (const void*)blkval || !G.exec) {
-        if (G.exec) { HIP_CHECK(hipGraphExecDestroy(G.exec)); G.exec = nullptr; }
-        hipGraph_t graph = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
-        if (reset_status)
-            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
-        int rc = run_seq(P, P->main, seq, bt, P->gstream);
-        hipError_t e = hipStreamEndCapture(P->gstream, &graph);
-        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-        if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
-        e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) { chp_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return -2; }
-        G.key = blkval;
+    if (!G.e0) {
+        HIP_CHECK(hipEventCreate(&G.e0));
+        HIP_CHECK(hipEventCreate(&G.e1));
     }
-    HIP_CHECK(hipEventRecord(P->gev_in, s));
-    HIP_CHECK(hipStreamWaitEvent(P->gstream, P->gev_in, 0));
-    HIP_CHECK(hipGraphLaunch(G.exec, P->gstream));
-    HIP_CHECK(hipEventRecord(P->gev_out, P->gstream));
-    HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
+    if (G.pending) {  // the timed call before this one has been issued: read it
+        HIP_CHECK(hipEventSynchronize(G.e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, G.e0, G.e1));
+        (G.pending == 1 ? G.t_eager : G.t_graph) = ms;
+        G.pending = 0;
+    }
+    int mode = G.mode;
+    if (!mode) {
+        if (G.calls < 2) mode = 1;
+        else if (G.calls < 4) mode = 2;
+        else mode = G.mode = (G.t_graph <= G.t_eager ? 2 : 1);
+    }
+    const bool timed = !G.mode && (G.calls == 1 || G.calls == 3);
+    ++G.calls;
+    if (timed) HIP_CHECK(hipEventRecord(G.e0, s));
+    if (mode == 1) {
+        if (reset_status)
+            hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+        int rc = run_seq(P, P->main, seq, bt, s);
+        if (rc) return rc;
+    } else {
+        if (G.key != (const void*)blkval || !G.exec) {
+            if (G.exec) { HIP_CHECK(hipGraphExecDestroy(G.exec)); G.exec = nullptr; }
+            hipGraph_t graph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
+            if (reset_status)
+                hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, P->gstream, P->d_status);
+            int rc = run_seq(P, P->main, seq, bt, P->gstream);
+            hipError_t e = hipStreamEndCapture(P->gstream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
+            e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { chp_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return -2; }
+            G.key = blkval;
+        }
+        HIP_CHECK(hipEventRecord(P->gev_in, s));
+        HIP_CHECK(hipStreamWaitEvent(P->gstream, P->gev_in, 0));
+        HIP_CHECK(hipGraphLaunch(G.exec, P->gstream));
+        HIP_CHECK(hipEventRecord(P->gev_out, P->gstream));
+        HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
+    }
+    if (timed) {
+        HIP_CHECK(hipEventRecord(G.e1, s));
+        G.pending = mode;
+    }
     return 0;
 }
 
@@ -750,7 +788,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     i64 upd = 0, dinv = 0, wun = 0;
     P->h_dinvptr.assign(nsn, 0);
     P->h_xptr.assign(nsn, 0);
-    P->h_tptr.assign(nsn, 0);
+    P->h_tptr.assign(nsn, -1);  // -1: no T region (small fronts)
     P->h_wunit.assign(nsn, 0);
     P->h_lltptr.assign(nsn, 0);
     for (i64 k = 0; k < nsn; ++k) {
@@ -793,12 +831,13 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         }
         wmax = std::max(wmax, w);
     }
-    i64 scratch = wbase + wmax, xcur = 0;
+    i64 scratch = wbase + wmax, xcur = 0, tcur = wbase + wmax;
     // side stream 2 (the inversions) starts from the sweep's stream
     P->projinv.push_back({K_FORK, 0, 0, 0, 0, 2});
     for (int L = H - 1; L >= 0; --L)
-        build_projinv_level(P, P->bylevel[L], wbase, wbase + wmax, xcur, scratch);
+        build_projinv_level(P, P->bylevel[L], wbase, tcur, xcur, scratch);
     P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
+    P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 3});
     if (xcur != wbase) return fail("projected-inverse scratch layout");
     for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
@@ -879,12 +918,15 @@ void chp_plan_destroy(chp_plan* P) {
     (void)hipDeviceSynchronize();  // no launch of this plan may still be in flight
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
-    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32})
+    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32}) {
         if (G->exec) (void)hipGraphExecDestroy(G->exec);
+        if (G->e0) (void)hipEventDestroy(G->e0);
+        if (G->e1) (void)hipEventDestroy(G->e1);
+    }
     if (P->gev_in) (void)hipEventDestroy(P->gev_in);
     if (P->gev_out) (void)hipEventDestroy(P->gev_out);
     if (P->gstream) (void)hipStreamDestroy(P->gstream);
-    for (int i = 1; i < 3; ++i) {
+    for (int i = 1; i < chp_plan::NSIDE; ++i) {
         if (P->sev_fork[i]) (void)hipEventDestroy(P->sev_fork[i]);
         if (P->sev_join[i]) (void)hipEventDestroy(P->sev_join[i]);
         if (P->sstream[i]) (void)hipStreamDestroy(P->sstream[i]);

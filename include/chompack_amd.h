@@ -80,10 +80,12 @@ void chp_plan_destroy(chp_plan* plan);
  * GEMM tiles, update-arena doubles, diagonal-inverse doubles, scratch doubles,
  * cholesky launches. */
 int chp_plan_info(chp_plan* plan, int64_t* out);
-/* Replay the cholesky / projected-inverse / llt launch sequences through HIP
- * graphs (on != 0, the default) or launch them one by one (on == 0).  Graphs
- * are captured on first use per blkval pointer and run on a plan-owned stream
- * ordered after and before the caller's stream with events. */
+/* on != 0 (the default): each of the cholesky / projected-inverse / llt
+ * sweeps times its first calls launched one by one and replayed from a HIP
+ * graph, and keeps the faster way (calls 1-2 eager, 3-4 graph, decided from
+ * call 5 on); on == 0: always one by one.  Graphs are captured per blkval
+ * pointer and run on a plan-owned stream ordered after and before the
+ * caller's stream with events. */
 int chp_set_graphs(chp_plan* plan, int on);
 
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
