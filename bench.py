@@ -31,7 +31,7 @@ NK = len(KINDS)
 
 
 def algorithmic_flops(symb):
-    """Survey formulas per front: cholesky nn^3/3 + 2 na nn^2 + na^2 nn;
+    """Algorithmic flops per front: cholesky nn^3/3 + 2 na nn^2 + na^2 nn;
     projected inverse nn^3/3 + nn^3 + 2 na^2 nn + 2 nn^2 na."""
     import numpy as np
     nn = np.diff(symb.snptr).astype(float)
@@ -78,6 +78,96 @@ def memory_bytes(symb, small_max=64):
     return ext, xtr
 
 
+def _timed(f, reps):
+    """Median wall time (ms) of reps calls of f after one warm-up call."""
+    import torch
+    f()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        f()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(ts)
+
+
+def batched_section(dev):
+    """512 independent 2D 5-point grids, side ~ U{16..96} (seed 2), as one
+    block-diagonal factorization: cholesky + per-instance logdet in FP64 and
+    with FP32 Schur updates; the logdet error of the mixed path."""
+    import numpy as np
+    import torch
+    import chompack_amd as cp
+    from chompack_amd import problems
+    As = problems.batch_grids(512, 16, 96, seed=2)
+    bs = cp.batch.BatchSymbolic(As)
+    X0 = bs.assemble()
+    X = cp.cspmatrix(bs.symb, blkval=torch.empty_like(X0.blkval))
+    res = {}
+
+    def run(chol):
+        def f():
+            cp.numeric.copy_blkval(X.blkval, X0.blkval)
+            X.is_factor = False
+            chol(X)
+            res["ld"] = bs.logdet(X)
+        return f
+    t64 = _timed(run(cp.cholesky), 10)
+    ld64 = res["ld"]
+    t32 = _timed(run(cp.mixed.cholesky), 10)
+    err = float(np.max(np.abs(res["ld"] - ld64) / np.abs(ld64)))
+    return {"instances": 512, "n_total": int(bs.symb.n), "fp64_ms": round(t64, 3),
+            "mixed_ms": round(t32, 3), "mixed_vs_fp64_speedup": round(t64 / t32, 3),
+            "mixed_logdet_max_rel_err": err,
+            "what": "cholesky + per-instance logdet of the whole batch"}
+
+
+def rgg_section(dev):
+    """Random geometric graph, n = 200000 points in the unit cube, mean degree
+    12 (seed 5), AMD: cholesky and a 64-RHS solve in FP64 against FP32 Schur
+    updates with FP64 iterative refinement to a 1e-10 relative residual."""
+    import numpy as np
+    import torch
+    import chompack_amd as cp
+    from chompack_amd import problems
+    A = problems.rgg(200000, 12.0, seed=5)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X0 = cp.cspmatrix(symb, device=dev) + A
+    X = cp.cspmatrix(symb, blkval=torch.empty_like(X0.blkval))
+    rng = np.random.default_rng(0)
+    Bh = rng.standard_normal((symb.n, 64))
+    B0 = torch.as_tensor(np.asfortranarray(Bh)).to(dev).t().contiguous().t()
+    Bs = B0.clone()
+    Am = cp.mixed.SpMat(A, device=dev)
+    hist = []
+
+    def fp64():
+        cp.numeric.copy_blkval(X.blkval, X0.blkval)
+        X.is_factor = False
+        cp.cholesky(X)
+        Bs.copy_(B0)
+        cp.trsm(X, Bs)
+        cp.trsm(X, Bs, trans="T")
+
+    def mixed():
+        cp.numeric.copy_blkval(X.blkval, X0.blkval)
+        X.is_factor = False
+        cp.mixed.cholesky(X)
+        _, h = cp.mixed.solve(Am, X, B0, tol=1e-10)
+        hist[:] = h
+    t64 = _timed(fp64, 3)
+    xs = Bs.cpu().numpy()
+    As = cp.symmetrize(A).tocsr()
+    r64 = float(np.max(np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)))
+    t32 = _timed(mixed, 3)
+    return {"n": int(symb.n), "nnz_L": int(symb.nnz), "nrhs": 64,
+            "fp64_ms": round(t64, 3), "fp64_residual": r64,
+            "mixed_ms": round(t32, 3), "mixed_vs_fp64_speedup": round(t64 / t32, 3),
+            "mixed_residual_history": hist,
+            "what": "cholesky + solve of 64 N(0,1) right-hand sides, residual <= 1e-10"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +177,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true", help="skip the 16-RHS solve section")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the batched (512 grids) and random-geometric-graph sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path runs on a single MI355X "
@@ -314,6 +406,8 @@ def main():
         "components_instrumented_ms": comps,
         "hbm_kernels": hbm,
         "solve_16rhs": solve,
+        "batched_512_grids": None,
+        "rgg_200k": None,
         "trmm_16rhs": trmm,
         "pfcholesky": pfsec,
         "mixed_vs_fp64": mixed,
@@ -323,6 +417,14 @@ def main():
                       "k_equiv": int(r[2] / max(r[1], 1) / (2 * 64 * 64))}
                      for r in sorted(g.tolist(), key=lambda r: -r[3])[:15]],
     }
+
+    if not args.no_configs:
+        for key, fn in (("batched_512_grids", batched_section), ("rgg_200k", rgg_section)):
+            try:
+                out[key] = fn(dev)
+            except Exception as e:  # a failed side section must not lose the main line
+                out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            print("# %s done" % key, file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         from oracle import ref
