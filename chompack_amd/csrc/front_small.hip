@@ -103,16 +103,13 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
 // Gather V_k = F_p[r, r] from the parent's finished front into k's update
 // slot, both triangles (projected_inverse.c:60-70 / cbase.c:39-59, done by
 // the child).  The parent's front is its blkval block (columns < nn_p) and
-// its own V_p (columns >= nn_p, lower triangle read).  With tsc, a parent
-// that has a T = V_p B21 region (tptr[p] >= 0) supplies its F21 = -T from
-// there instead of the blkval block (which receives -T on another stream).
-// One workgroup per (front, 16-column tile, 256-row block); the block goes
-// through LDS so that the mirrored upper-triangle writes are coalesced too.
+// its own V_p (columns >= nn_p, lower triangle read).  One workgroup per
+// (front, 16-column tile, 256-row block); the block goes through LDS so that
+// the mirrored upper-triangle writes are coalesced too.
 __global__ __launch_bounds__(TPB) void k_extract(
     const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx, const double* __restrict__ tsc,
-    const i64* __restrict__ tptr) {
+    const int* __restrict__ relidx) {
     __shared__ double S[256][17];
     const int4 t = tiles[blockIdx.x];
     const int k = t.x, j0 = t.y, ib = t.z;
@@ -123,8 +120,6 @@ __global__ __launch_bounds__(TPB) void k_extract(
     const int* r = relidx + f.relptr;
     const double* pblk = blkval + p.blkptr;
     const double* pV = upd + p.updptr;
-    const i64 pt = tsc ? tptr[f.parent] : -1;
-    const double* pT = pt >= 0 ? tsc + pt : nullptr;  // -F21 of the parent, ld nap
     double* V = upd + f.updptr;
     const int j1 = min(j0 + 16, na);
     const int tid = threadIdx.x;
@@ -140,9 +135,8 @@ __global__ __launch_bounds__(TPB) void k_extract(
             v[jj] = 0.0;
             if (j < j1 && i < i1 && i >= j) {
                 const int rj = r[j];
-                if (rj >= nnp) v[jj] = pV[(ri - nnp) + (i64)(rj - nnp) * nap];
-                else if (pT && ri >= nnp) v[jj] = -pT[(ri - nnp) + (i64)rj * nap];
-                else v[jj] = pblk[ri + (i64)rj * njp];
+                v[jj] = (rj < nnp) ? pblk[ri + (i64)rj * njp]
+                                   : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
             }
         }
 #pragma unroll

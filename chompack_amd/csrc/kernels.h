@@ -13,7 +13,7 @@ __global__ void k_chol_small(const int* fronts, const FrontInfo* fi, double* blk
                              double* upd, const int* relidx, const int* chidx,
                              unsigned long long* status);
 __global__ void k_extract(const int4* tiles, const FrontInfo* fi, const double* blkval,
-                          double* upd, const int* relidx, const double* tsc, const i64* tptr);
+                          double* upd, const int* relidx);
 __global__ void k_projinv_small(const int* fronts, const FrontInfo* fi, double* blkval,
                                 const double* upd, unsigned long long* status);
 __global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkval,

@@ -160,13 +160,13 @@ struct chp_plan {
     bool use_graphs = true;
     hipStream_t gstream = nullptr;
     hipEvent_t gev_in = nullptr, gev_out = nullptr;
-    // side streams for independent work (K_FORK / K_JOIN with iarg = 1..3):
+    // side streams for independent work (K_FORK / K_JOIN with iarg = 1 or 2):
     // 1 carries a level's small fronts, 2 the stored-convention conversion and
-    // the L11 inversions, 3 the projected inverse's F21 := -T copies
-    static const int NSIDE = 4;
-    hipStream_t sstream[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t sev_fork[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t sev_join[NSIDE] = {nullptr, nullptr, nullptr, nullptr};
+    // the L11 inversions
+    static const int NSIDE = 3;
+    hipStream_t sstream[NSIDE] = {nullptr, nullptr, nullptr};
+    hipEvent_t sev_fork[NSIDE] = {nullptr, nullptr, nullptr};
+    hipEvent_t sev_join[NSIDE] = {nullptr, nullptr, nullptr};
     // A sweep's captured graph, and the choice between replaying it and
     // launching eagerly on the side streams (whichever measured faster: the
     // graph saves host launch work, eager launches give every side stream

@@ -334,11 +334,9 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
 // X = L11^{-1} of a large front reads only the factor, so it is computed on
 // side stream 2, which runs ahead through the levels; every large front has
 // its own X at h_xptr[k] in [0, wbase).  W (inversion temporary, side stream
-// 2 only) is per level from wbase on.  T = V_k B21 has its own region per
-// large front (h_tptr[k], from tcur on): the children's extract reads the
-// parent's F21 = -T there, so that the F21 := -T copy into blkval (side
-// stream 3) stays off the main stream.
-void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64& tcur,
+// 2 only) is per level from wbase on, T = V_k B21 (main stream only) per level
+// from tbase on: each stream reuses its own region level after level.
+void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64 tbase,
                          i64& xcur, i64& scratch_need) {
     const std::vector<int4> xt = extract_tiles(P, fr);
     std::vector<int> small, large;
@@ -350,7 +348,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     }
     if (!large.empty()) {
         P->n_large_projinv += (int)large.size();
-        i64 woff_end = wbase;
+        i64 woff_end = wbase, toff_end = tbase;
         std::vector<i64> woff(large.size());
         for (size_t q = 0; q < large.size(); ++q) {
             const FrontInfo& f = P->fi[large[q]];
@@ -360,10 +358,10 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
         }
         for (size_t q = 0; q < large.size(); ++q) {
             const FrontInfo& f = P->fi[large[q]];
-            P->h_tptr[large[q]] = tcur;
-            tcur += ((i64)f.na * f.nn + 31) / 32 * 32;
+            P->h_tptr[large[q]] = toff_end;
+            toff_end += ((i64)f.na * f.nn + 31) / 32 * 32;
         }
-        scratch_need = std::max(scratch_need, tcur);  // W ends at or below the T regions
+        scratch_need = std::max(scratch_need, toff_end);  // W ends at or below tbase
         // side stream 2: diagonal block inverses, then recursive halving of
         // the block range [a, b) at m:
         //   X[m:b, a:m] = -X[m:b, m:b] (L[m:b, a:m] X[a:m, a:m]);
@@ -419,8 +417,8 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     // the small fronts
     const bool fork = !large.empty() && (!xt.empty() || !small.empty());
     if (fork) P->projinv.push_back({K_FORK, 0, 0, 0, 0, 1});
-    if (!xt.empty()) {  // iarg 1: a large parent's F21 read as -T
-        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 1};
+    if (!xt.empty()) {
+        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0};
         Lx.side = fork ? 1 : 0;
         P->projinv.push_back(Lx);
     }
@@ -454,12 +452,8 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     for (int k : large)
         if (P->fi[k].na > 0)
             for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
-    if (!nt.empty()) {
-        P->projinv.push_back({K_FORK, 0, 0, 0, 0, 3});
-        Launch Ln{K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0};
-        Ln.side = 3;
-        P->projinv.push_back(Ln);
-    }
+    if (!nt.empty())
+        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0});
 }
 
 // llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
@@ -617,7 +611,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, P->d_fi, blkval,
-                       P->d_upd, P->d_relidx, L.iarg == 1 ? P->d_scratch : nullptr, P->d_tptr);
+                       P->d_upd, P->d_relidx);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
@@ -788,7 +782,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     i64 upd = 0, dinv = 0, wun = 0;
     P->h_dinvptr.assign(nsn, 0);
     P->h_xptr.assign(nsn, 0);
-    P->h_tptr.assign(nsn, -1);  // -1: no T region (small fronts)
+    P->h_tptr.assign(nsn, 0);
     P->h_wunit.assign(nsn, 0);
     P->h_lltptr.assign(nsn, 0);
     for (i64 k = 0; k < nsn; ++k) {
@@ -831,13 +825,12 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         }
         wmax = std::max(wmax, w);
     }
-    i64 scratch = wbase + wmax, xcur = 0, tcur = wbase + wmax;
+    i64 scratch = wbase + wmax, xcur = 0;
     // side stream 2 (the inversions) starts from the sweep's stream
     P->projinv.push_back({K_FORK, 0, 0, 0, 0, 2});
     for (int L = H - 1; L >= 0; --L)
-        build_projinv_level(P, P->bylevel[L], wbase, tcur, xcur, scratch);
+        build_projinv_level(P, P->bylevel[L], wbase, wbase + wmax, xcur, scratch);
     P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
-    P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 3});
     if (xcur != wbase) return fail("projected-inverse scratch layout");
     for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
