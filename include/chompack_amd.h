@@ -208,6 +208,33 @@ int chp_pf_product(int64_t n, int m, const double* V, int ldv, const double* L, 
                    const double* B, int ldb, double* U, int ldu, int nu, int trans,
                    void* stream);
 
+/* Maximum-determinant positive semidefinite completion; replaces
+ * psdcompletion (pybase/psdcompletion.py:4-70; exported by
+ * src/python/__init__.py:29,32).  X (device, n x n, ld ldx, zero on entry)
+ * receives the dense completion of the cspmatrix (snptr, sncolptr,
+ * snrowidx, blkptr: host symbolic arrays; blkval: device) in the reordered
+ * numbering; tol: the pseudo-inverse threshold of the eigendecomposition
+ * branch (the reference's kwargs tol, default 1e-15).  Cliques up to 2048
+ * rows.  Synchronizes the stream. */
+int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                      const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
+                      double* X, int64_t ldx, double tol, void* stream);
+/* Minimum-rank positive semidefinite completion; replaces mrcompletion
+ * (pybase/mrcompletion.py:6-108; src/python/__init__.py:29,32).  Y (device,
+ * n x maxr, ld ldy; maxr = the clique number) receives Y with
+ * P(Y Y^T) = A in its first *rank columns (reordered numbering; the other
+ * columns are zero); snpar: the supernodal parent array (root: snpar[k]
+ * == k).  Cliques up to 2048 rows.  Synchronizes the stream. */
+int chp_mrcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                     const int64_t* snrowidx, const int64_t* blkptr, const int64_t* snpar,
+                     const double* blkval, double* Y, int64_t ldy, int maxr, int* rank,
+                     void* stream);
+/* out[a, b] = X[ip[a], permute_cols ? ip[b] : b] for a < rows, b < cols
+ * (device arrays, column-major): the `reordered=False` return of both
+ * completions (psdcompletion.py:67-70, mrcompletion.py:176-179). */
+int chp_dense_permute(const double* X, int64_t ldx, const int64_t* ip, int64_t rows,
+                      int64_t cols, int permute_cols, double* out, int64_t ldo, void* stream);
+
 /* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
  * restore a blkval array from a copy (no reference counterpart). */
 int chp_copy(double* dst, const double* src, int64_t n, void* stream);
