@@ -168,6 +168,61 @@ def rgg_section(dev):
             "what": "cholesky + solve of 64 N(0,1) right-hand sides, residual <= 1e-10"}
 
 
+def dense_completions_section(dev, with_cpu):
+    """Dense completions on the 64 x 64 5-point grid (n = 4096, seed 0, AMD):
+    psdcompletion of A (dense 4096 x 4096 result) and mrcompletion of
+    P(U U^T), U 4096 x 4 N(0,1) (seed 1); the oracle's numpy restatement of
+    the reference timed beside it (one run each, 1 thread of numpy/LAPACK
+    calls per supernode)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    import chompack_amd as cp
+    from chompack_amd import problems
+    A = problems.grid2d(64, seed=0)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device=dev) + A
+    U = np.random.default_rng(1).standard_normal((symb.n, 4))
+    P = sp.coo_matrix(symb.sparsity_pattern(reordered=False, symmetric=False))
+    A2 = sp.csc_matrix((np.einsum("ij,ij->i", U[P.row], U[P.col]), (P.row, P.col)),
+                       shape=A.shape)
+    X2 = cp.cspmatrix(symb, device=dev) + A2
+    res = {}
+
+    def psd():
+        res["X"] = cp.psdcompletion(X)
+
+    def mr():
+        res["Y"] = cp.mrcompletion(X2)
+    t_psd = _timed(psd, 5)
+    t_mr = _timed(mr, 5)
+    Y = res["Y"].cpu().numpy()
+    out = {"n": int(symb.n), "nsn": int(symb.Nsn), "clique_number": int(symb.clique_number),
+           "psdcompletion_ms": round(t_psd, 3), "mrcompletion_ms": round(t_mr, 3),
+           "mrcompletion_rank": int(Y.shape[1])}
+    if with_cpu:
+        from oracle import completions as oc
+        blk = X.blkval.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oc.psdcompletion(symb, blk)
+        cpu_psd = (time.perf_counter() - t0) * 1e3
+        blk2 = X2.blkval.cpu().numpy()
+        t0 = time.perf_counter()
+        oc.mrcompletion(symb, blk2)
+        cpu_mr = (time.perf_counter() - t0) * 1e3
+        Xg = res["X"].cpu().numpy()
+        Ad2 = oc.dense_projection(symb, blk2)
+        mask = oc.dense_projection(symb, np.ones(blk2.size)) != 0
+        out.update({
+            "cpu_port_psd_ms": round(cpu_psd, 1), "cpu_port_mr_ms": round(cpu_mr, 1),
+            "psd_speedup_vs_cpu": round(cpu_psd / t_psd, 2),
+            "mr_speedup_vs_cpu": round(cpu_mr / t_mr, 2),
+            "psd_rel_err_vs_port": float(np.abs(Xg - ref).max() / np.abs(ref).max()),
+            "mr_pattern_residual": float(np.abs((Y @ Y.T - Ad2)[mask]).max()
+                                         / np.abs(Ad2).max())})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +233,7 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true", help="skip the 16-RHS solve section")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the batched (512 grids) and random-geometric-graph sections")
+                    help="skip the batched (512 grids), random-geometric-graph and dense completion sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path runs on a single MI355X "
@@ -425,6 +480,11 @@ def main():
             except Exception as e:  # a failed side section must not lose the main line
                 out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("# %s done" % key, file=sys.stderr, flush=True)
+        try:
+            out["dense_completions_grid64"] = dense_completions_section(dev, not args.no_cpu)
+        except Exception as e:
+            out["dense_completions_grid64"] = {"error": "%s: %s" % (type(e).__name__, e)}
+        print("# dense_completions_grid64 done", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         from oracle import ref
