@@ -18,12 +18,12 @@ from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm
 from . import autograd, batch, mixed
 from .pfcholesky import pfcholesky
 from .orderings import maxcardsearch, maxchord
-from .completions import psdcompletion, mrcompletion
+from .completions import psdcompletion, mrcompletion, edmcompletion
 
 __version__ = "0.1.0"
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "cholesky", "llt",
            "completion", "projected_inverse", "hessian", "trsm", "trmm", "dot", "logdet", "autograd",
-           "pfcholesky", "maxcardsearch", "maxchord", "psdcompletion", "mrcompletion",
+           "pfcholesky", "maxcardsearch", "maxchord", "psdcompletion", "mrcompletion", "edmcompletion",
            "tril", "triu",
            "symmetrize", "perm", "eye", "lmerge", "amd"]

@@ -73,6 +73,7 @@ def _declare(L):
     _opt(L, "chp_pf_solve", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)
     _opt(L, "chp_pf_product", [i64, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp], i32)
     _opt(L, "chp_psdcompletion", [i64, i64, vp, vp, vp, vp, vp, vp, i64, dbl, vp], i32)
+    _opt(L, "chp_edmcompletion", [i64, i64, vp, vp, vp, vp, vp, vp, i64, dbl, vp], i32)
     _opt(L, "chp_mrcompletion", [i64, i64, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, vp], i32)
     _opt(L, "chp_dense_permute", [vp, i64, vp, i64, i64, i32, vp, i64, vp], i32)
     _opt(L, "chp_dot", [vp, vp, vp, vp, vp], i32)

@@ -1,10 +1,11 @@
 """Dense completions of a chordal partial matrix on the GPU: the drop-in for
-``chompack.psdcompletion`` and ``chompack.mrcompletion``
-(src/python/__init__.py:29,32; pybase/psdcompletion.py, pybase/mrcompletion.py).
+``chompack.psdcompletion``, ``chompack.mrcompletion`` and
+``chompack.edmcompletion`` (src/python/__init__.py:29,32;
+pybase/psdcompletion.py, pybase/mrcompletion.py, pybase/edmcompletion.py).
 
 The results are float64 tensors on the cspmatrix's device (the reference
-returns cvxopt dense matrices): psdcompletion an n x n matrix, mrcompletion
-an n x r column-major matrix.  The work runs in the HIP kernels of
+returns cvxopt dense matrices): psdcompletion and edmcompletion an n x n
+matrix, mrcompletion an n x r column-major matrix.  The work runs in the HIP kernels of
 csrc/dense_completion.hip; there is no CPU fallback.
 """
 import numpy as np
@@ -12,7 +13,7 @@ import numpy as np
 from . import _native, numeric
 from .symbolic import cspmatrix
 
-__all__ = ["psdcompletion", "mrcompletion"]
+__all__ = ["psdcompletion", "mrcompletion", "edmcompletion"]
 
 
 def _arrays(symb):
@@ -33,12 +34,7 @@ def _unpermute(M, symb, cols, permute_cols):
     return out
 
 
-def psdcompletion(A, reordered=True, **kwargs):
-    """Maximum determinant positive semidefinite matrix completion
-    (pybase/psdcompletion.py:4-70): returns the dense X with P(X) = A, in the
-    reordered numbering if ``reordered`` (else X[ip, ip]).  Keyword ``tol``
-    (default 1e-15): eigenvalues <= tol * lambda_max are dropped from the
-    separator pseudo-inverse where a Cholesky factorization fails."""
+def _dense_sweep(fn, what, A, reordered, kwargs):
     if not (isinstance(A, cspmatrix) and A.is_factor is False):
         raise AssertionError("A must be a cspmatrix")
     torch = numeric._torch()
@@ -48,13 +44,29 @@ def psdcompletion(A, reordered=True, **kwargs):
     n = symb.n
     X = torch.zeros((n, n), dtype=torch.float64, device=blk.device)
     a = _arrays(symb)
-    rc = _native.lib().chp_psdcompletion(n, symb.Nsn, *[_native.ptr(x) for x in a],
-                                         blk.data_ptr(), X.data_ptr(), n, tol,
-                                         numeric._stream(blk))
-    numeric._raise_rc(rc, "psdcompletion")
+    rc = fn(n, symb.Nsn, *[_native.ptr(x) for x in a], blk.data_ptr(), X.data_ptr(), n, tol,
+            numeric._stream(blk))
+    numeric._raise_rc(rc, what)
     if reordered:
         return X
     return _unpermute(X, symb, n, True)
+
+
+def psdcompletion(A, reordered=True, **kwargs):
+    """Maximum determinant positive semidefinite matrix completion
+    (pybase/psdcompletion.py:4-70): returns the dense X with P(X) = A, in the
+    reordered numbering if ``reordered`` (else X[ip, ip]).  Keyword ``tol``
+    (default 1e-15): eigenvalues <= tol * lambda_max are dropped from the
+    separator pseudo-inverse where a Cholesky factorization fails."""
+    return _dense_sweep(_native.lib().chp_psdcompletion, "psdcompletion", A, reordered, kwargs)
+
+
+def edmcompletion(A, reordered=True, **kwargs):
+    """Euclidean distance matrix completion (pybase/edmcompletion.py:4-74):
+    returns the dense EDM X with P(X) = A of an EDM-completable A, in the
+    reordered numbering if ``reordered`` (else X[ip, ip]).  Keyword ``tol``
+    as in psdcompletion (every separator takes the pseudo-inverse)."""
+    return _dense_sweep(_native.lib().chp_edmcompletion, "edmcompletion", A, reordered, kwargs)
 
 
 def mrcompletion(A, reordered=True):

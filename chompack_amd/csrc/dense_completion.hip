@@ -262,26 +262,53 @@ __global__ __launch_bounds__(DC_TPB) void k_blk_to_dense(const i64* __restrict__
 //   w^+_i = 1 / w_i for w_i > tol * max(w), else 0.
 // Also marks the separator rows (mark[alpha_a] = k) for k_psd_update.
 // Work: S, Z (na x na), T (na x nn).
+//
+// edm != 0: the separator step of edmcompletion (edmcompletion.py:17-46),
+// on the Gram transform relative to the first separator row a0,
+//   Y[i, j] = -X[i, j] / 2 - X[a0, a0] / 2 + X[i, a0] / 2 + X[j, a0] / 2:
+// W := Yaa^+ Yan, always through the eigendecomposition, and E receives
+// the vector c_a = -X[a0, a0] / 2 + X[a0, alpha_a] / 2 (na) of the rows'
+// Yea and the diagonal Ynn[j, j] (nn) for k_psd_update.
 __global__ __launch_bounds__(DC_TPB) void k_psd_sep(int k, const double* __restrict__ X, i64 ldx,
                                                     const i64* __restrict__ alpha, int na,
-                                                    i64 c0, int nn, double tol,
+                                                    i64 c0, int nn, double tol, int edm,
                                                     double* __restrict__ S,
                                                     double* __restrict__ Z,
                                                     double* __restrict__ W,
                                                     double* __restrict__ T,
+                                                    double* __restrict__ E,
                                                     int* __restrict__ mark) {
     __shared__ JacobiLds sh;
     __shared__ int fail;
     const int t = threadIdx.x;
     const i64 aa = (i64)na * na, an = (i64)na * nn;
-    if (t == 0) fail = 0;
+    if (t == 0) fail = edm;
     for (int a = t; a < na; a += DC_TPB) mark[alpha[a]] = k;
-    for (i64 e = t; e < aa; e += DC_TPB) S[e] = X[alpha[e % na] + alpha[e / na] * ldx];
-    for (i64 e = t; e < an; e += DC_TPB) W[e] = X[alpha[e % na] + (c0 + e / na) * ldx];
-    __syncthreads();
+    if (edm) {
+        const i64 a0 = alpha[0];
+        const double h00 = 0.5 * X[a0 + a0 * ldx];
+        for (i64 e = t; e < aa; e += DC_TPB) {
+            const i64 ra = alpha[e % na], rb = alpha[e / na];
+            S[e] = -0.5 * X[ra + rb * ldx] - h00 + 0.5 * X[ra + a0 * ldx] + 0.5 * X[rb + a0 * ldx];
+        }
+        for (i64 e = t; e < an; e += DC_TPB) {
+            const i64 ra = alpha[e % na], cj = c0 + e / na;
+            W[e] = -0.5 * X[ra + cj * ldx] - h00 + 0.5 * X[a0 + cj * ldx] + 0.5 * X[ra + a0 * ldx];
+        }
+        for (int a = t; a < na; a += DC_TPB) E[a] = -h00 + 0.5 * X[a0 + alpha[a] * ldx];
+        for (int j = t; j < nn; j += DC_TPB) {
+            const i64 cj = c0 + j;
+            E[na + j] = -0.5 * X[cj + cj * ldx] - h00 + X[cj + a0 * ldx];
+        }
+        __syncthreads();
+    } else {
+        for (i64 e = t; e < aa; e += DC_TPB) S[e] = X[alpha[e % na] + alpha[e / na] * ldx];
+        for (i64 e = t; e < an; e += DC_TPB) W[e] = X[alpha[e % na] + (c0 + e / na) * ldx];
+        __syncthreads();
+    }
     // right-looking Cholesky, lower triangle (dpotf2's failure test: a
     // pivot <= 0 or NaN)
-    for (int j = 0; j < na; ++j) {
+    for (int j = 0; j < na && !edm; ++j) {
         const double d = S[j + (i64)j * na];
         if (!(d > 0.0)) {
             if (t == 0) fail = 1;
@@ -325,8 +352,9 @@ __global__ __launch_bounds__(DC_TPB) void k_psd_sep(int k, const double* __restr
         }
         return;
     }
-    // eigendecomposition pseudo-inverse
-    for (i64 e = t; e < aa; e += DC_TPB) S[e] = X[alpha[e % na] + alpha[e / na] * ldx];
+    // eigendecomposition pseudo-inverse (S still holds Yaa in the edm case)
+    if (!edm)
+        for (i64 e = t; e < aa; e += DC_TPB) S[e] = X[alpha[e % na] + alpha[e / na] * ldx];
     __syncthreads();
     wg_jacobi_evd(S, Z, na, sh);
     double lmax = -INFINITY;
@@ -355,11 +383,15 @@ constexpr int PSD_NB = 8;  // columns of nu per thread
 
 // X[i, nu] = X[i, alpha] W and X[nu, i] = its transpose for every row
 // i >= c0 + nn outside the separator (mark[i] != k): one thread per row,
-// PSD_NB columns of nu per blockIdx.y.
+// PSD_NB columns of nu per blockIdx.y.  edm != 0 (edmcompletion.py:32-48):
+// X[i, nu_j] = -2 Yea[i, :] W[:, j] + Ynn[j, j] + Yee[i, i] with
+// Yea[i, a] = -X[i, alpha_a] / 2 + c_a + X[i, a0] / 2 and
+// Yee[i, i] = -X[i, i] / 2 - X[a0, a0] / 2 + X[i, a0].
 __global__ __launch_bounds__(DC_TPB) void k_psd_update(int k, double* __restrict__ X, i64 ldx,
                                                        i64 n, const i64* __restrict__ alpha,
-                                                       int na, i64 c0, int nn,
+                                                       int na, i64 c0, int nn, int edm,
                                                        const double* __restrict__ W,
+                                                       const double* __restrict__ E,
                                                        const int* __restrict__ mark) {
     const i64 i = c0 + nn + (i64)blockIdx.x * DC_TPB + threadIdx.x;
     if (i >= n || mark[i] == k) return;
@@ -368,19 +400,24 @@ __global__ __launch_bounds__(DC_TPB) void k_psd_update(int k, double* __restrict
     double acc[PSD_NB];
 #pragma unroll
     for (int j = 0; j < PSD_NB; ++j) acc[j] = 0.0;
+    const i64 a0 = alpha[0];
+    const double xi0 = edm ? 0.5 * X[i + a0 * ldx] : 0.0;
     for (int a = 0; a < na; ++a) {
-        const double x = X[i + alpha[a] * ldx];
+        double x = X[i + alpha[a] * ldx];
+        if (edm) x = -0.5 * x + E[a] + xi0;
         const double* w = W + a + (i64)j0 * na;
 #pragma unroll
         for (int j = 0; j < PSD_NB; ++j)
             if (j < nb) acc[j] += x * w[(i64)j * na];
     }
+    const double yee = edm ? -0.5 * X[i + i * ldx] - 0.5 * X[a0 + a0 * ldx] + 2.0 * xi0 : 0.0;
 #pragma unroll
     for (int j = 0; j < PSD_NB; ++j) {
         if (j >= nb) break;
         const i64 c = c0 + j0 + j;
-        X[i + c * ldx] = acc[j];
-        X[c + i * ldx] = acc[j];
+        const double v = edm ? -2.0 * acc[j] + E[na + j0 + j] + yee : acc[j];
+        X[i + c * ldx] = v;
+        X[c + i * ldx] = v;
     }
 }
 
@@ -629,13 +666,10 @@ int check_structure(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t*
     return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
-                      const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
-                      double* X, int64_t ldx, double tol, void* stream) {
+// The sweep of psdcompletion (edm = 0) and edmcompletion (edm = 1).
+int dense_sweep(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
+                double* X, int64_t ldx, double tol, int edm, void* stream) {
     if (!snptr || !sncolptr || !snrowidx || !blkptr || !blkval || !X) {
         chp_set_error("null argument");
         return -1;
@@ -658,12 +692,14 @@ int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_
         abmax = std::max(abmax, na * nn);
     }
     const i64 aa = std::max<i64>(amax * amax, 1), an = std::max<i64>(abmax, 1);
-    HIP_CHECK(hipMalloc(&d_work.p, (2 * aa + 2 * an) * sizeof(double)));
+    const i64 ne = std::max<i64>(amax + (i64)(snptr[nsn] - snptr[0]), 1);  // na + nn <= n
+    HIP_CHECK(hipMalloc(&d_work.p, (2 * aa + 2 * an + ne) * sizeof(double)));
     HIP_CHECK(hipMalloc(&d_mark.p, n * sizeof(int)));
     double* S = (double*)d_work.p;
     double* Z = S + aa;
     double* W = Z + aa;
     double* T = W + an;
+    double* E = T + an;
     int* mark = (int*)d_mark.p;
     HIP_CHECK(hipMemsetAsync(mark, 0xff, n * sizeof(int), s));  // -1: no supernode
     hipLaunchKernelGGL(k_blk_to_dense, dim3((unsigned)nsn), dim3(DC_TPB), 0, s,
@@ -677,18 +713,34 @@ int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_
         if (na == 0) continue;
         const i64* alpha = rowidx + sncolptr[k] + nn;
         hipLaunchKernelGGL(k_psd_sep, dim3(1), dim3(DC_TPB), 0, s, (int)k, (const double*)X,
-                           (i64)ldx, alpha, na, c0, nn, tol, S, Z, W, T, mark);
+                           (i64)ldx, alpha, na, c0, nn, tol, edm, S, Z, W, T, E, mark);
         const i64 rows = n - (c0 + nn);
         if (rows > 0)
             hipLaunchKernelGGL(k_psd_update,
                                dim3((unsigned)((rows + DC_TPB - 1) / DC_TPB),
                                     (unsigned)((nn + PSD_NB - 1) / PSD_NB)),
                                dim3(DC_TPB), 0, s, (int)k, X, (i64)ldx, (i64)n, alpha, na, c0,
-                               nn, (const double*)W, (const int*)mark);
+                               nn, edm, (const double*)W, (const double*)E, (const int*)mark);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));  // the work arrays are freed on return
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                      const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
+                      double* X, int64_t ldx, double tol, void* stream) {
+    return dense_sweep(n, nsn, snptr, sncolptr, snrowidx, blkptr, blkval, X, ldx, tol, 0, stream);
+}
+
+int chp_edmcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                      const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
+                      double* X, int64_t ldx, double tol, void* stream) {
+    return dense_sweep(n, nsn, snptr, sncolptr, snrowidx, blkptr, blkval, X, ldx, tol, 1, stream);
 }
 
 int chp_mrcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,

@@ -219,6 +219,13 @@ int chp_pf_product(int64_t n, int m, const double* V, int ldv, const double* L, 
 int chp_psdcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
                       const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
                       double* X, int64_t ldx, double tol, void* stream);
+/* Euclidean distance matrix completion; replaces edmcompletion
+ * (pybase/edmcompletion.py:4-74; src/python/__init__.py:29,32).  Same
+ * arguments and conventions as chp_psdcompletion; every separator takes the
+ * eigendecomposition pseudo-inverse of the Gram transform. */
+int chp_edmcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t* sncolptr,
+                      const int64_t* snrowidx, const int64_t* blkptr, const double* blkval,
+                      double* X, int64_t ldx, double tol, void* stream);
 /* Minimum-rank positive semidefinite completion; replaces mrcompletion
  * (pybase/mrcompletion.py:6-108; src/python/__init__.py:29,32).  Y (device,
  * n x maxr, ld ldy; maxr = the clique number) receives Y with

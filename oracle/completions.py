@@ -7,31 +7,36 @@ CPU-baseline leg; the product (chompack_amd) never imports it.
   psdcompletion  follows pybase/psdcompletion.py:4-70 (maximum-determinant
                  positive semidefinite completion, dense n x n);
   mrcompletion   follows pybase/mrcompletion.py:6-108 (minimum-rank
-                 completion Y, n x r, with P(Y Y^T) = A).
+                 completion Y, n x r, with P(Y Y^T) = A);
+  edmcompletion  follows pybase/edmcompletion.py:4-74 (Euclidean distance
+                 matrix completion, dense n x n).
 
-Both work on the reordered matrix exactly as the reference does (a dense
-copy of P(A) for psdcompletion; the clique matrices F_k for mrcompletion),
+All three work on the reordered matrix exactly as the reference does (a
+dense copy of P(A) for psdcompletion and edmcompletion; the clique matrices
+F_k for mrcompletion),
 with numpy.linalg (LAPACK) in place of cvxopt.lapack: ``cholesky`` and
 ``solve_triangular`` for potrf/trtrs, ``eigh`` for syevr, ``svd`` for gesvd.
 
 Pinned in tests/test_completions.py:
   psdcompletion by (a) P(X) = A, (b) the optimality condition of the
   maximum-determinant completion (X^{-1} vanishes outside the chordal
-  pattern), (c) X^{-1} = L L^T with L from the oracle's sparse ``completion``
-  (completion.c), itself pinned against the reference C kernels, and (d) for
-  the pseudo-inverse branch, the completion of the same matrix with its zero
-  rows and columns removed;
+  pattern), (c) the closed form psdcompletion(P(A^{-1})) = A^{-1} for an SPD
+  A with the chordal pattern, and (d) for the pseudo-inverse branch, the
+  completion of the same matrix with its zero rows and columns removed;
   mrcompletion by the reference test's own check (tests/test_mrcompletion.py
   :18-36 of the reference: P(Y Y^T) = A on its 17 x 17 example, including the
   rank-2 case built with syr2) and by r = the largest numerical rank of a
   clique.  Y itself is not unique (any Y Q with Q orthogonal also completes
   A, and LAPACK's sign choices fix one), so Y is compared through Y Y^T on
-  the pattern, not entrywise.
+  the pattern, not entrywise;
+  edmcompletion by P(X) = A, a zero diagonal and -J X J / 2 positive
+  semidefinite (J the centering projection: X is a Euclidean distance
+  matrix), on distances of random points.
 """
 import numpy as np
 import scipy.linalg as sla
 
-__all__ = ["dense_projection", "psdcompletion", "mrcompletion", "clique_matrix"]
+__all__ = ["dense_projection", "psdcompletion", "mrcompletion", "edmcompletion", "clique_matrix"]
 
 
 def dense_projection(symb, blkval):
@@ -148,3 +153,40 @@ def mrcompletion(symb, blkval, reordered=True):
     if reordered:
         return Y[:, :r]
     return Y[np.asarray(symb.ip), :r]
+
+
+def edmcompletion(symb, blkval, reordered=True, tol=1e-15):
+    """pybase/edmcompletion.py:16-74."""
+    X = dense_projection(symb, blkval)
+    n = symb.n
+    snptr, sncolptr, snrowidx = (np.asarray(a) for a in (symb.snptr, symb.sncolptr, symb.snrowidx))
+    for k in range(symb.Nsn - 1, -1, -1):                       # :28
+        nn = snptr[k + 1] - snptr[k]
+        beta = snrowidx[sncolptr[k]:sncolptr[k + 1]]
+        nj = len(beta)
+        if nj - nn == 0:                                         # :33
+            continue
+        alpha = beta[nn:]
+        nu = beta[:nn]
+        inb = np.zeros(n, dtype=bool)
+        inb[beta] = True
+        eta = np.nonzero(~inb[beta[0] + 1:])[0] + beta[0] + 1     # :36
+        a0 = alpha[0]
+        h = 0.5 * X[a0, a0]
+        xa, xn, xe = X[alpha, a0], X[nu, a0], X[eta, a0]
+        Yaa = -0.5 * X[np.ix_(alpha, alpha)] - h + 0.5 * (xa[:, None] + xa[None, :])  # :40-41
+        Ynn = -0.5 * np.diag(X)[nu] - h + xn                     # :43-44, diagonal
+        Yee = -0.5 * np.diag(X)[eta] - h + xe                    # :46-47, diagonal
+        Yan = -0.5 * X[np.ix_(alpha, nu)] - h + 0.5 * X[a0, nu][None, :] + 0.5 * xa[:, None]  # :49-51
+        Yea = -0.5 * X[np.ix_(eta, alpha)] - h + 0.5 * X[a0, alpha][None, :] + 0.5 * xe[:, None]  # :53-55
+        w, Z = np.linalg.eigh(Yaa)                               # :58-60
+        lmax = w.max()
+        d = np.array([1.0 / wi if wi > lmax * tol else 0.0 for wi in w])
+        Yp = (Z * d) @ Z.T                                       # :63-64
+        tmp = -2.0 * (Yea @ Yp @ Yan) + Ynn[None, :] + Yee[:, None]  # :67
+        X[np.ix_(eta, nu)] = tmp
+        X[np.ix_(nu, eta)] = tmp.T
+    if reordered:
+        return X
+    ip = np.asarray(symb.ip)
+    return X[np.ix_(ip, ip)]

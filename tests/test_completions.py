@@ -2,15 +2,18 @@
 
 CPU tests pin the oracle (oracle/completions.py) by what the mathematics
 fixes: the maximum-determinant completion X reproduces A on the pattern and
-its inverse vanishes off the pattern; with a zero row and column (the
-pseudo-inverse branch) the rest is the maximum-determinant completion of the
-reduced matrix; the minimum-rank completion reproduces A through Y Y^T with
-r = the largest clique rank (the reference test's own check,
-tests/test_mrcompletion.py:18-36 of the reference, on its 17 x 17 example).
+its inverse vanishes off the pattern, and psdcompletion(P(A^{-1})) = A^{-1};
+with a zero row and column (the pseudo-inverse branch) the rest is the
+maximum-determinant completion of the reduced matrix; the minimum-rank
+completion reproduces A through Y Y^T with r = the largest clique rank (the
+reference test's own check, tests/test_mrcompletion.py:18-36 of the
+reference, on its 17 x 17 example); the EDM completion of squared distances
+reproduces them on the pattern and is a Euclidean distance matrix.
 
 GPU tests compare the HIP path with the oracle: psdcompletion entrywise
 (max |gpu - oracle| / max |oracle| <= 1e-12 on the Cholesky branch, 1e-11
-on the pseudo-inverse branch, where the eigensolvers differ), mrcompletion
+on the pseudo-inverse branch, where the eigensolvers differ), edmcompletion
+entrywise (1e-11, pseudo-inverses throughout), mrcompletion
 through P(Y Y^T) = A (1e-12 relative) and the rank (Y itself is only
 determined up to an orthogonal factor).
 """
@@ -144,6 +147,38 @@ def test_mr_oracle_reference_example():
     assert np.abs((Y2o @ Y2o.T - U @ U.T)[Po]).max() <= 1e-12 * np.abs(U @ U.T).max()
 
 
+def _edm_problem(side, seed):
+    """Squared distances of random points on the filled pattern of a 2D
+    grid; the dimension exceeds every separator, so each Gram matrix of a
+    separator has exactly one zero eigenvalue (its first row)."""
+    A = problems.grid2d(side, seed=seed)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    d = int(max(np.diff(symb.sncolptr) - np.diff(symb.snptr))) + 4
+    pts = np.random.default_rng(seed).standard_normal((symb.n, d))
+    D = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    return symb, _proj_lower(symb, D)
+
+
+def _is_edm(X):
+    n = X.shape[0]
+    J = np.eye(n) - np.ones((n, n)) / n
+    G = -0.5 * J @ X @ J
+    return np.abs(np.diag(X)).max() == 0.0 and np.linalg.eigvalsh(G).min() >= -1e-10 * np.abs(G).max()
+
+
+@pytest.mark.parametrize("side", [6, 9])
+def test_edm_oracle(side):
+    symb, A = _edm_problem(side, side)
+    blk = _blkval(symb, A)
+    X = oc.edmcompletion(symb, blk)
+    mask = _pattern_re(symb)
+    Ad = oc.dense_projection(symb, blk)
+    assert np.abs((X - Ad)[mask]).max() <= 1e-14 * np.abs(Ad).max()
+    assert np.abs(X - X.T).max() == 0.0
+    assert _is_edm(X)
+    assert (X[~mask] > 0).all()
+
+
 def test_completions_reject_factor():
     A = problems.grid2d(4, seed=0)
     symb = cp.symbolic(A, p=cp.amd.order)
@@ -153,6 +188,8 @@ def test_completions_reject_factor():
         cp.psdcompletion(X)
     with pytest.raises(AssertionError, match="A must be a cspmatrix"):
         cp.mrcompletion(X)
+    with pytest.raises(AssertionError, match="A must be a cspmatrix"):
+        cp.edmcompletion(X)
 
 
 # ---------------------------------------------------------------- GPU parity
@@ -271,3 +308,20 @@ def test_mr_gpu_low_rank_grid(side, rank):
     e = np.abs((Y @ Y.T - Ad)[mask]).max() / np.abs(Ad).max()
     print("mrcompletion P(YY^T) - A rel %.3e" % e, flush=True)
     assert e <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side", [8, 16])
+def test_edm_gpu(side):
+    print("case edmcompletion grid", side, flush=True)
+    symb, A = _edm_problem(side, side + 1)
+    X = _gpu(symb, A)
+    blk = X.blkval.cpu().numpy()
+    ref = oc.edmcompletion(symb, blk)
+    got = cp.edmcompletion(X).cpu().numpy()
+    e = _rel(got, ref)
+    print("edmcompletion rel err %.3e" % e, flush=True)
+    assert e <= 1e-11
+    assert _is_edm(got)
+    ip = np.asarray(symb.ip)
+    assert _rel(cp.edmcompletion(X, reordered=False).cpu().numpy(), ref[np.ix_(ip, ip)]) <= 1e-11
