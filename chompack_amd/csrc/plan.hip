@@ -100,6 +100,12 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
     if (ntiles < 128 && t.K >= 512) {
         ks = std::max(1, std::min(t.K / 256, (512 + ntiles - 1) / ntiles));
         ks = std::min(ks, 16);
+    } else if (ntiles < 1024 && t.K >= 768) {
+        // a few tiles per CU: split up to 4 ways so that the launch holds
+        // about 2048 units of K >= 384 (the tail of a 2-3 tile-per-CU launch
+        // otherwise leaves CUs idle)
+        ks = std::max(1, std::min(t.K / 384, (2048 + ntiles - 1) / ntiles));
+        ks = std::min(ks, 4);
     }
     if (ks > 1) {
         int chunk = (t.K + ks - 1) / ks;
