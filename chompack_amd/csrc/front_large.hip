@@ -174,9 +174,14 @@ __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)
 // block_potrf4 with a second set of rows: b[] holds 64 rows below the block
 // (thread (w, l) owns panel row l, columns w + 4q), eliminated alongside the
 // block's own rows, so they end as the panel rows A_panel L^{-T} of the
-// factor.  P needs 2 * NB rows.
+// factor.  P needs 2 * NB rows; Q is 16-byte aligned (rows read as pairs).
+// The step is latency-bound (dependent FP64 operations cost tens of cycles
+// each on one wave): the block-row and panel-row substitutions run side by
+// side without a branch, and the rank-4 update takes four columns at a time
+// with their chains interleaved, so the scheduler does not serialize them
+// through one temporary.
 __device__ inline bool block_potrf4_panel(double (&a)[DQ], double (&b)[DQ], double (*P)[5],
-                                          double (*Q)[5], int w, int l, int* bad) {
+                                          double (*Q)[4], int w, int l, int* bad) {
 #pragma unroll
     for (int s = 0; s < NB / 4; ++s) {
         const int r0 = 4 * s;
@@ -186,6 +191,8 @@ __device__ inline bool block_potrf4_panel(double (&a)[DQ], double (&b)[DQ], doub
         const double d00 = P[r0][0], d10 = P[r0 + 1][0], d20 = P[r0 + 2][0], d30 = P[r0 + 3][0];
         const double d11 = P[r0 + 1][1], d21 = P[r0 + 2][1], d31 = P[r0 + 3][1];
         const double d22 = P[r0 + 2][2], d32 = P[r0 + 3][2], d33 = P[r0 + 3][3];
+        const double p0 = P[l][0], p1 = P[l][1], p2 = P[l][2], p3 = P[l][3];
+        const double q0 = P[NB + l][0], q1 = P[NB + l][1], q2 = P[NB + l][2], q3 = P[NB + l][3];
         const double i0 = prsqrt(d00);
         const double L10 = d10 * i0, L20 = d20 * i0, L30 = d30 * i0;
         const double e11 = d11 - L10 * L10;
@@ -198,37 +205,53 @@ __device__ inline bool block_potrf4_panel(double (&a)[DQ], double (&b)[DQ], doub
         const double i3 = prsqrt(e33);
         const int fail = !(d00 > 0.0) ? 0 : (!(e11 > 0.0) ? 1 : (!(e22 > 0.0) ? 2 : (!(e33 > 0.0) ? 3 : -1)));
         if (fail >= 0) { *bad = r0 + fail; return false; }
-        double x0, x1, x2, x3;
-        if (l >= r0 + 4) {
-            const double p0 = P[l][0], p1 = P[l][1], p2 = P[l][2], p3 = P[l][3];
-            x0 = p0 * i0;
-            x1 = (p1 - x0 * L10) * i1;
-            x2 = (p2 - x0 * L20 - x1 * L21) * i2;
-            x3 = (p3 - x0 * L30 - x1 * L31 - x2 * L32) * i3;
-        } else {
-            const int r = l - r0;
+        double x0 = p0 * i0, y0 = q0 * i0;
+        double x1 = (p1 - x0 * L10) * i1, y1 = (q1 - y0 * L10) * i1;
+        double x2 = (p2 - x0 * L20 - x1 * L21) * i2, y2 = (q2 - y0 * L20 - y1 * L21) * i2;
+        double x3 = (p3 - x0 * L30 - x1 * L31 - x2 * L32) * i3;
+        const double y3 = (q3 - y0 * L30 - y1 * L31 - y2 * L32) * i3;
+        // rows of the pivot block and above it: the factor's own entries (zero above)
+        const int r = l - r0;
+        if (r < 4) {
             x0 = r == 0 ? d00 * i0 : (r == 1 ? L10 : (r == 2 ? L20 : (r == 3 ? L30 : 0.0)));
             x1 = r == 1 ? e11 * i1 : (r == 2 ? L21 : (r == 3 ? L31 : 0.0));
             x2 = r == 2 ? e22 * i2 : (r == 3 ? L32 : 0.0);
             x3 = r == 3 ? e33 * i3 : 0.0;
         }
-        // the panel row: always below the block
-        const double q0 = P[NB + l][0], q1 = P[NB + l][1], q2 = P[NB + l][2], q3 = P[NB + l][3];
-        const double y0 = q0 * i0;
-        const double y1 = (q1 - y0 * L10) * i1;
-        const double y2 = (q2 - y0 * L20 - y1 * L21) * i2;
-        const double y3 = (q3 - y0 * L30 - y1 * L31 - y2 * L32) * i3;
         a[s] = w == 0 ? x0 : (w == 1 ? x1 : (w == 2 ? x2 : x3));
         b[s] = w == 0 ? y0 : (w == 1 ? y1 : (w == 2 ? y2 : y3));
         if (w == 0) { Q[l][0] = x0; Q[l][1] = x1; Q[l][2] = x2; Q[l][3] = x3; }
         __syncthreads();
+        // rank-4 update of owned a(l, j), j = w + 4q > r0 + 3, l >= j, and of b
 #pragma unroll
-        for (int q = s + 1; q < DQ; ++q) {
-            const int j = w + 4 * q;
-            const double c0 = Q[j][0], c1 = Q[j][1], c2 = Q[j][2], c3 = Q[j][3];
-            const double m = (l >= j) ? 1.0 : 0.0;
-            a[q] -= m * (x0 * c0 + x1 * c1 + x2 * c2 + x3 * c3);
-            b[q] -= y0 * c0 + y1 * c1 + y2 * c2 + y3 * c3;
+        for (int g = s + 1; g < DQ; g += 4) {
+            double c[4][4], ta[4], tb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (g + u >= DQ) continue;
+                const int j = w + 4 * (g + u);
+                c[u][0] = Q[j][0]; c[u][1] = Q[j][1]; c[u][2] = Q[j][2]; c[u][3] = Q[j][3];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g + u < DQ) { ta[u] = x0 * c[u][0]; tb[u] = y0 * c[u][0]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g + u < DQ) { ta[u] = fma(x1, c[u][1], ta[u]); tb[u] = fma(y1, c[u][1], tb[u]); }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g + u < DQ) { ta[u] = fma(x2, c[u][2], ta[u]); tb[u] = fma(y2, c[u][2], tb[u]); }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g + u < DQ) { ta[u] = fma(x3, c[u][3], ta[u]); tb[u] = fma(y3, c[u][3], tb[u]); }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = g + u;
+                if (q >= DQ) continue;
+                const double m = (l >= w + 4 * q) ? 1.0 : 0.0;
+                a[q] -= m * ta[u];
+                b[q] -= tb[u];
+            }
         }
     }
     __syncthreads();
@@ -352,7 +375,8 @@ __global__ __launch_bounds__(TPB) void k_panel_factor(
     const int4* __restrict__ tasks, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ dinv, const i64* __restrict__ dinvptr,
     unsigned long long* status) {
-    __shared__ double P[2 * NB][5], Q[NB][5];
+    __shared__ double P[2 * NB][5];
+    __shared__ __attribute__((aligned(16))) double Q[NB][4];
     const int4 t = tasks[blockIdx.x];
     const int k = t.x, c0 = t.y, r0 = t.z;
     const FrontInfo f = fi[k];
