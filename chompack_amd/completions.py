@@ -20,11 +20,11 @@ def _arrays(symb):
     return [_native.as_i64(a) for a in (symb.snptr, symb.sncolptr, symb.snrowidx, symb.blkptr)]
 
 
-def _unpermute(M, symb, cols, permute_cols):
+def _unpermute(M, symb, cols, permute_cols, perm=None):
     """M[ip, ip] (permute_cols) or M[ip, :cols]; M is column-major with
-    leading dimension n (or symmetric)."""
+    leading dimension n (or symmetric).  ``perm`` replaces ip."""
     torch = numeric._torch()
-    ip = torch.as_tensor(_native.as_i64(symb.ip), device=M.device)
+    ip = torch.as_tensor(_native.as_i64(symb.ip if perm is None else perm), device=M.device)
     out = torch.empty_strided((symb.n, cols), (1, symb.n), dtype=torch.float64, device=M.device)
     rc = _native.lib().chp_dense_permute(M.data_ptr(), symb.n,
                                          ip.data_ptr(), symb.n, cols, int(permute_cols),
@@ -52,12 +52,46 @@ def _dense_sweep(fn, what, A, reordered, kwargs):
     return _unpermute(X, symb, n, True)
 
 
+def _psd_multifrontal(A, reordered):
+    """The maximum-determinant completion of a positive definite completable
+    A is (L L^T)^{-1} with L from the sparse completion (completion.c): the
+    multifrontal completion, then trsm N and T on the identity give it in
+    the original numbering.  Returns None when the completion fails (a clique
+    that is not positive definite)."""
+    torch = numeric._torch()
+    L = A.copy()
+    try:
+        numeric.completion(L)
+    except ArithmeticError:
+        return None
+    n = A.symb.n
+    X = torch.eye(n, dtype=torch.float64, device=L.blkval.device).t()  # column-major
+    numeric.trsm(L, X)
+    numeric.trsm(L, X, trans='T')
+    if not reordered:
+        return X.t()
+    return _unpermute(X, A.symb, n, True, perm=A.symb.p).t()
+
+
 def psdcompletion(A, reordered=True, **kwargs):
     """Maximum determinant positive semidefinite matrix completion
     (pybase/psdcompletion.py:4-70): returns the dense X with P(X) = A, in the
-    reordered numbering if ``reordered`` (else X[ip, ip]).  Keyword ``tol``
-    (default 1e-15): eigenvalues <= tol * lambda_max are dropped from the
-    separator pseudo-inverse where a Cholesky factorization fails."""
+    reordered numbering if ``reordered`` (else X[ip, ip]).
+
+    A positive definite completable A takes the multifrontal route,
+    X = (L L^T)^{-1} with L from ``completion``.  Otherwise (a clique that is
+    not positive definite) the reference's sweep runs: per supernode,
+    X[eta, nu] = X[eta, alpha] X[alpha, alpha]^+ X[alpha, nu] with the
+    Cholesky factor of the separator block, or its eigendecomposition
+    pseudo-inverse where that factorization fails (keyword ``tol``, default
+    1e-15: eigenvalues <= tol * lambda_max are dropped).  Keyword
+    ``sweep=True`` takes the sweep in every case."""
+    if not (isinstance(A, cspmatrix) and A.is_factor is False):
+        raise AssertionError("A must be a cspmatrix")
+    if not kwargs.get('sweep', False):
+        X = _psd_multifrontal(A, reordered)
+        if X is not None:
+            return X
     return _dense_sweep(_native.lib().chp_psdcompletion, "psdcompletion", A, reordered, kwargs)
 
 

@@ -10,7 +10,8 @@ reference test's own check, tests/test_mrcompletion.py:18-36 of the
 reference, on its 17 x 17 example); the EDM completion of squared distances
 reproduces them on the pattern and is a Euclidean distance matrix.
 
-GPU tests compare the HIP path with the oracle: psdcompletion entrywise
+GPU tests compare the HIP path with the oracle: psdcompletion entrywise,
+through the multifrontal route and through the reference's sweep
 (max |gpu - oracle| / max |oracle| <= 1e-12 on the Cholesky branch, 1e-11
 on the pseudo-inverse branch, where the eigensolvers differ), edmcompletion
 entrywise (1e-11, pseudo-inverses throughout), mrcompletion
@@ -229,6 +230,12 @@ def test_psd_gpu(name):
     got_o = cp.psdcompletion(X, reordered=False).cpu().numpy()
     ip = np.asarray(symb.ip)
     assert _rel(got_o, ref[np.ix_(ip, ip)]) <= 1e-12
+    # the reference's sweep (separator Cholesky branch) on the same input
+    e = _rel(cp.psdcompletion(X, sweep=True).cpu().numpy(), ref)
+    print("psdcompletion sweep rel err %.3e" % e, flush=True)
+    assert e <= 1e-12
+    assert _rel(cp.psdcompletion(X, reordered=False, sweep=True).cpu().numpy(),
+                ref[np.ix_(ip, ip)]) <= 1e-12
 
 
 @pytest.mark.gpu
