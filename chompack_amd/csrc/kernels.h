@@ -6,6 +6,7 @@
 #define NB 64          // inner panel width / diagonal block order of the large-front algorithms
 #define NB2 256        // outer panel width (K of the big trailing updates)
 #define TILE 64        // GEMM output tile (TILE x TILE)
+#define SOLVE_RHS_CHUNK 64  // right-hand sides per workgroup of the slab gather/scatter (grid y)
 
 __global__ void k_status_init(unsigned long long* status);
 

@@ -276,7 +276,10 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
         }
         s = (two && L.side) ? P->sstream[L.side] : s0;
         const void* list = S->pg.d_lists + L.list;
-        const dim3 g(L.grid), b(L.block);
+        // the slab gather/scatter kernels take the right-hand sides in chunks
+        // along the grid's y dimension
+        const bool rhs_y = L.kid == K_SGATHER_F || L.kid == K_SGATHER_B || L.kid == K_SSCATTER;
+        const dim3 g(L.grid, rhs_y ? (nrhs + SOLVE_RHS_CHUNK - 1) / SOLVE_RHS_CHUNK : 1), b(L.block);
         switch (L.kid) {
         case K_SOLVE_FWD:
             CHP_SLAUNCH(k_solve_fwd, (const int2*)list, P->d_fi, blkval, P->d_snptr,

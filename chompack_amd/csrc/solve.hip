@@ -237,7 +237,8 @@ __global__ __launch_bounds__(TPB) void k_l11_check(
 // in trsm.c:46-58); with bonly, only rows i < nn and only alpha B[p[i]] (the
 // product with L adds the children's rows after the front's own product).
 // tiles[i] = (front, r0): rows r0 .. r0 + 63, thread (row, g) handles
-// right-hand sides g, g + 4, ...
+// right-hand sides j0 + g, j0 + g + 4, ... of the chunk j0 = blockIdx.y *
+// SOLVE_RHS_CHUNK (many right-hand sides spread over workgroups).
 __global__ __launch_bounds__(TPB) void k_sgather_fwd(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const i64* __restrict__ perm, const double* __restrict__ B,
@@ -253,7 +254,8 @@ __global__ __launch_bounds__(TPB) void k_sgather_fwd(
     double* W = work + wk * nrhs;
     const int e0 = gptr[wk + i], e1 = bonly ? e0 : gptr[wk + i + 1];
     const i64 bi = (i < nn) ? perm[snptr[k] + i] : 0;
-    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
+    const int j0 = blockIdx.y * SOLVE_RHS_CHUNK, j1 = min(nrhs, j0 + SOLVE_RHS_CHUNK);
+    for (int j = j0 + (int)(threadIdx.x >> 6); j < j1; j += TPB / 64) {
         double g = (i < nn) ? alpha * B[bi + (i64)j * ldb] : 0.0;
         for (int e = e0; e < e1; ++e) {
             const GEnt q = gent[e];
@@ -265,7 +267,8 @@ __global__ __launch_bounds__(TPB) void k_sgather_fwd(
 
 // Backward gather: W[:nn] = alpha B[perm], W[nn:] = the parent's values of
 // the separator rows (trsm.c:97-115).  tiles[i] = (front, r0): rows r0 ..
-// r0 + 63, thread (row, g) handles right-hand sides g, g + 4, ...
+// r0 + 63, thread (row, g) handles right-hand sides j0 + g, j0 + g + 4, ...
+// of the chunk blockIdx.y.
 __global__ __launch_bounds__(TPB) void k_sgather_bwd(
     const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const int* __restrict__ relidx,
@@ -291,7 +294,8 @@ __global__ __launch_bounds__(TPB) void k_sgather_bwd(
         ld = p.nn + p.na;
         scale = 1.0;
     }
-    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64)
+    const int j0 = blockIdx.y * SOLVE_RHS_CHUNK, j1 = min(nrhs, j0 + SOLVE_RHS_CHUNK);
+    for (int j = j0 + (int)(threadIdx.x >> 6); j < j1; j += TPB / 64)
         W[i + (i64)j * nj] = scale * src[(i64)j * ld];
 }
 
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(TPB) void k_sgather_bwd(
 // backward sweep) the value is T - T2, T2 at tmp2 + t.tofs, for fronts with a
 // separator; with copyback also W[:nn] = the value (the backward sweep's
 // children read it from the slab).  Thread (row, g) handles right-hand sides
-// g, g + 4, ...
+// j0 + g, j0 + g + 4, ... of the chunk blockIdx.y.
 __global__ __launch_bounds__(TPB) void k_sscatter(
     const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const i64* __restrict__ snptr, const i64* __restrict__ perm, double* __restrict__ B,
@@ -316,7 +320,8 @@ __global__ __launch_bounds__(TPB) void k_sscatter(
     const double* T = tmp + t.tofs;
     const double* T2 = (tmp2 && f.na > 0) ? tmp2 + t.tofs : nullptr;
     const i64 bi = perm[snptr[k] + i];
-    for (int j = (int)(threadIdx.x >> 6); j < nrhs; j += TPB / 64) {
+    const int j0 = blockIdx.y * SOLVE_RHS_CHUNK, j1 = min(nrhs, j0 + SOLVE_RHS_CHUNK);
+    for (int j = j0 + (int)(threadIdx.x >> 6); j < j1; j += TPB / 64) {
         double v = T[i + (i64)j * nn];
         if (T2) v -= T2[i + (i64)j * nn];
         B[bi + (i64)j * ldb] = v;
