@@ -38,35 +38,44 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
             for (int tc = max(c0, nn); tc < c1 && tc <= tr; ++tc)
                 U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
     }
-    __syncthreads();
-    __shared__ int s_ja, s_jb, s_ia, s_ib;
-    for (int l = f.chend - 1; l >= f.chbeg; --l) {
-        const FrontInfo g = fi[chidx[l]];
-        const int N = g.na;
-        const int* r = relidx + g.relptr;
-        const double* Uc = upd + g.updptr;
-        if (tid == 0) {
-            auto lower = [&](int v) {  // first child index with r >= v
-                int lo = 0, hi = N;
-                while (lo < hi) { int m = (lo + hi) >> 1; if (r[m] < v) lo = m + 1; else hi = m; }
-                return lo;
-            };
-            s_ja = lower(c0);
-            s_jb = lower(c1);
-            s_ia = lower(r0);
-            s_ib = lower(r1);
+    // The child index ranges landing in the tile (first child index with
+    // r >= c0, c1, r0, r1), for up to TPB / 4 children at a time: one thread
+    // per (child, bound), so the dependent binary-search loads of all
+    // children overlap instead of running one after another.
+    __shared__ int s_bnd[TPB / 4][4];
+    const int nch = f.chend - f.chbeg;
+    for (int b0 = 0; b0 < nch; b0 += TPB / 4) {
+        __syncthreads();
+        {
+            const int cb = tid >> 2, which = tid & 3;
+            if (b0 + cb < nch) {
+                const FrontInfo g = fi[chidx[f.chend - 1 - (b0 + cb)]];
+                const int* r = relidx + g.relptr;
+                const int v = which == 0 ? c0 : (which == 1 ? c1 : (which == 2 ? r0 : r1));
+                int lo = 0, hi = g.na;
+                while (lo < hi) { const int m = (lo + hi) >> 1; if (r[m] < v) lo = m + 1; else hi = m; }
+                s_bnd[cb][which] = lo;
+            }
         }
         __syncthreads();
-        const int ja = s_ja, jb = s_jb, i = s_ia + tid;
-        if (i < s_ib) {
-            const int ri = r[i];
-            const int je = min(jb, i + 1);  // lower part: column j <= row i
-            for (int j0 = ja; j0 < je; j0 += 4) {
-                double* d[4];
-                double u[4], fv[4];
+        for (int cb = 0; cb < min(TPB / 4, nch - b0); ++cb) {
+            const int l = f.chend - 1 - (b0 + cb);  // reverse child order
+            const FrontInfo g = fi[chidx[l]];
+            const int N = g.na;
+            const int* r = relidx + g.relptr;
+            const double* Uc = upd + g.updptr;
+            const int ja = s_bnd[cb][0], jb = s_bnd[cb][1], i = s_bnd[cb][2] + tid;
+            if (i < s_bnd[cb][3]) {
+                // at most 16 child columns land in the 16 target columns
+                // (relative indices increase): all loads first, then the
+                // stores, so the row's memory round trips overlap
+                const int ri = r[i];
+                const int je = min(jb, i + 1);  // lower part: column j <= row i
+                double* d[16];
+                double u[16], fv[16];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int j = j0 + q;
+                for (int q = 0; q < 16; ++q) {
+                    const int j = ja + q;
                     d[q] = nullptr;
                     u[q] = 0.0;
                     if (j < je) {
@@ -76,13 +85,13 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) fv[q] = d[q] ? d[q][ri] : 0.0;
+                for (int q = 0; q < 16; ++q) fv[q] = d[q] ? d[q][ri] : 0.0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < 16; ++q)
                     if (d[q]) d[q][ri] = fv[q] + u[q];
             }
+            __syncthreads();  // the next child may add into the same entries
         }
-        __syncthreads();
     }
 }
 
