@@ -575,12 +575,26 @@ class cspmatrix(object):
         factor, the (2,1) blocks are multiplied back by L11 so the result is
         the lower-triangular factor L of P A P^T."""
         symb = self.symb
-        vals = self.blkval.detach().cpu().numpy().copy()
+        b = self.blkval
         if self.is_factor:
             if symmetric:
                 raise ValueError("'symmetric = True' not implemented for Cholesky factors")
             if not reordered:
                 raise ValueError("'reordered = False' not implemented for Cholesky factors")
+        if self.is_factor and getattr(b, "is_cuda", False):
+            # L21 = B21 L11 for every supernode on the GPU (k_b21_l11)
+            from . import numeric, _native
+            blk = numeric._check_blkval(self)
+            out = blk.new_empty(blk.shape)
+            plan = symb._plan(blk.device)
+            rc = _native.lib().chp_factor_export(plan.handle, blk.data_ptr(), out.data_ptr(),
+                                                 numeric._stream(blk))
+            numeric._raise_rc(rc, "spmatrix")
+            vals = out.cpu().numpy()
+        else:
+            vals = b.detach().cpu().numpy().copy() if hasattr(b, "detach") else np.array(b, dtype=float)
+        if self.is_factor and not getattr(b, "is_cuda", False):
+            # a host cspmatrix: the same products with numpy, per supernode
             nnk = np.diff(symb.snptr)
             njk = np.diff(symb.sncolptr)
             for k in range(symb.Nsn):

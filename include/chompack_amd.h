@@ -242,6 +242,14 @@ int chp_mrcompletion(int64_t n, int64_t nsn, const int64_t* snptr, const int64_t
 int chp_dense_permute(const double* X, int64_t ldx, const int64_t* ip, int64_t rows,
                       int64_t cols, int permute_cols, double* out, int64_t ldo, void* stream);
 
+/* out := the factor's blkval with every (2,1) block B21 = L21 L11^{-1}
+ * replaced by L21 = B21 L11, i.e. the blocks of the lower-triangular L of
+ * P A P^T; replaces the per-supernode trmm of a factor's export in
+ * cspmatrix.spmatrix() (symbolic.py:1052-1068).  lblkval and out are device
+ * arrays of the plan's blkval length (16-byte aligned).  Synchronizes the
+ * stream. */
+int chp_factor_export(chp_plan* plan, const double* lblkval, double* out, void* stream);
+
 /* dst[0:n] = src[0:n] for device float64 arrays (16-byte aligned); used to
  * restore a blkval array from a copy (no reference counterpart). */
 int chp_copy(double* dst, const double* src, int64_t n, void* stream);

@@ -727,3 +727,22 @@ def test_trsm_offset_and_leading_dimension():
     for j in range(3):
         mask[off + j * ld: off + j * ld + n] = False
     assert np.array_equal(got[mask], store[mask])
+
+
+@pytest.mark.parametrize("name", ["grid2d_64", "grid3d_12", "ref31"])
+def test_factor_export_on_gpu(name):
+    """spmatrix() of a GPU factor (L21 = B21 L11 on the GPU) against the same
+    export done on the host from a copy of the blocks, and against the dense
+    Cholesky factor of P A P^T."""
+    print("case factor export", name, flush=True)
+    A, symb, X = _setup(name, 64)
+    cp.cholesky(X)
+    L_gpu = X.spmatrix().toarray()
+    Xh = cp.cspmatrix(symb, blkval=X.blkval.cpu(), factor=True)
+    L_host = Xh.spmatrix().toarray()
+    assert _rel(L_gpu, L_host) <= 1e-14
+    p = np.asarray(symb.p)
+    Ad = cp.symmetrize(A).toarray()[np.ix_(p, p)]
+    e = _rel(L_gpu, np.linalg.cholesky(Ad))
+    print("factor export vs dense Cholesky %.3e" % e, flush=True)
+    assert e <= 1e-12
