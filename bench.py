@@ -223,6 +223,99 @@ def dense_completions_section(dev, with_cpu):
     return out
 
 
+def differentiation_section(dev, with_cpu):
+    """-logdet on the 32^3 7-point grid (n = 32768, seed 3): gradient through
+    the autograd function (cholesky forward, projected-inverse backward) and
+    its central-difference error (step 1e-6) along one random direction; the
+    Hessian map (adj=None) on 32 random directions; and the maximum-
+    determinant completion of the band-plus-arrow pattern (n = 20000,
+    bandwidth 8, 50 arrow columns, seed 4).  The reference's C kernels time
+    the Hessian on 4 of the directions (scaled to 32) and the completion."""
+    import numpy as np
+    import torch
+    import chompack_amd as cp
+    from chompack_amd import problems
+    A = problems.grid3d(32, seed=3)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device=dev) + A
+    wgt = cp.autograd._grad_weights(symb, X.blkval.device)
+    w = torch.where(wgt != 0, 1.0, 0.0).double()
+    rng = np.random.default_rng(9)
+    res = {}
+
+    def grad():
+        Xg = cp.cspmatrix(symb, blkval=X.blkval.clone().requires_grad_(True))
+        f = -cp.autograd.logdet(Xg)
+        f.backward()
+        res["G"] = Xg.blkval.grad
+    t_grad = _timed(grad, 3)
+    U = torch.as_tensor(rng.standard_normal(X.blkval.numel()), device=dev) * w
+    dd = float((res["G"] * U).sum())
+    h = 1e-6
+
+    def logdiag_at(t):
+        Lt = cp.cspmatrix(symb, blkval=X.blkval + t * U)
+        cp.cholesky(Lt)
+        return cp.numeric.logdiag(Lt)
+    fd = -2.0 * float((logdiag_at(h) - logdiag_at(-h)).sum()) / (2 * h)
+    L = X.copy()
+    cp.cholesky(L)
+    Y = L.copy()
+    cp.projected_inverse(Y)
+    U0 = [torch.as_tensor(rng.standard_normal(X.blkval.numel()), device=dev) * w for _ in range(32)]
+    Us = [cp.cspmatrix(symb, blkval=u.clone()) for u in U0]
+
+    def hess():
+        for u, u0 in zip(Us, U0):
+            u.blkval.copy_(u0)
+        cp.hessian(L, Y, Us, adj=None)
+    t_hess = _timed(hess, 3)
+    B = problems.band_arrow(20000, 8, 50, seed=4)
+    bsymb = cp.symbolic(B, p=None)
+    Xb0 = cp.cspmatrix(bsymb, device=dev) + B
+    Xb = Xb0.copy()
+
+    def compl():
+        cp.numeric.copy_blkval(Xb.blkval, Xb0.blkval)
+        Xb.is_factor = False
+        cp.completion(Xb)
+    t_comp = _timed(compl, 1)
+    out = {"grid32_n": int(symb.n), "gradient_ms": round(t_grad, 3),
+           "gradient_fd_rel_err": abs(dd - fd) / abs(dd), "hessian_32_directions_ms": round(t_hess, 2),
+           "band_arrow_n": int(bsymb.n), "band_arrow_supernodes": int(bsymb.Nsn),
+           "band_arrow_completion_ms": round(t_comp, 1),
+           "what": "-logdet gradient (cholesky + projected-inverse backward) with its central "
+                   "difference error; hessian(adj=None) on 32 directions; max-det completion "
+                   "(factored updates) of the band-plus-arrow chain"}
+    if with_cpu:
+        from oracle import ref
+        if ref.available():
+            threads = min(16, os.cpu_count() or 1)
+            ref.set_threads(threads)
+            Lh, Yh = L.blkval.cpu().numpy().copy(), Y.blkval.cpu().numpy().copy()
+            Uh = [u.cpu().numpy().copy() for u in U0[:4]]
+            t0 = time.perf_counter()
+            ref.hessian(symb, Lh, Yh, Uh, adj=None)
+            t_hc = (time.perf_counter() - t0) * 1e3 * 8
+            Hg = Us[0].blkval.cpu().numpy()
+            bh = Xb0.blkval.cpu().numpy().copy()
+            t0 = time.perf_counter()
+            ref.completion(bsymb, bh)
+            t_cc = (time.perf_counter() - t0) * 1e3
+            gc = Xb.blkval.cpu().numpy()
+            out.update({"cpu_reference_threads": threads,
+                        "cpu_hessian_32_directions_ms": round(t_hc, 1),
+                        "hessian_speedup_vs_cpu": round(t_hc / t_hess, 2),
+                        "hessian_rel_err_vs_reference": float(np.abs(Hg - Uh[0]).max() / np.abs(Uh[0]).max()),
+                        "cpu_band_arrow_completion_ms": round(t_cc, 1),
+                        "band_arrow_completion_speedup_vs_cpu": round(t_cc / t_comp, 3),
+                        "band_arrow_completion_rel_err_vs_reference":
+                            float(np.abs(gc - bh).max() / np.abs(bh).max()),
+                        "cpu_sample": "reference C kernels, %d OpenBLAS threads; Hessian timed on 4 "
+                                      "of the 32 directions and scaled by 8" % threads})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -480,6 +573,11 @@ def main():
             except Exception as e:  # a failed side section must not lose the main line
                 out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
             print("# %s done" % key, file=sys.stderr, flush=True)
+        try:
+            out["differentiation_grid32_band_arrow"] = differentiation_section(dev, not args.no_cpu)
+        except Exception as e:
+            out["differentiation_grid32_band_arrow"] = {"error": "%s: %s" % (type(e).__name__, e)}
+        print("# differentiation_grid32_band_arrow done", file=sys.stderr, flush=True)
         try:
             out["dense_completions_grid64"] = dense_completions_section(dev, not args.no_cpu)
         except Exception as e:

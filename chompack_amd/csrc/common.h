@@ -120,7 +120,11 @@ struct MatOp {
 struct UFTask {
     i64 foff, lkoff, outoff, woff, relpos;
     int nn, na, N, key;
-    unsigned char fbuf, pad[7];
+    // defer_solve: stop once Li22 is in place (Li21 and the (1,1) block then
+    // come from multi-workgroup triangular inverse and GEMM launches);
+    // defer_rchol: leave A - Li21^T Li21 in Li11 (its reversed Cholesky runs
+    // afterwards as a blocked multi-workgroup potrf)
+    unsigned char fbuf, defer_solve, defer_rchol, pad[5];
 };
 
 // Row tile of a front in the solve's scatter: rows r0 .. r0 + 63 of front k,

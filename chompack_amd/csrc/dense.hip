@@ -1,4 +1,4 @@
-// update_factor on the GPU: one 256-thread workgroup per child update
+// update_factor on the GPU: one 1024-thread workgroup per child update
 // (update_factor.c:3-108).  The child's update V = F[r, r] is produced in
 // factored form V = Li^T Li from the parent's frontal matrix whose (2,2)
 // block holds the parent's factor Lk (V_k = Lk^T Lk):
@@ -13,11 +13,17 @@
 //   * Li11 is the reversed Cholesky factor of A - Li21^T Li21
 //     (A = Li11^T Li11, eliminated from the last row up).
 // All work is in global memory (workspace C in the scratch buffer); the
-// sequential steps are separated by workgroup barriers.
+// sequential steps are separated by workgroup barriers.  The steps are
+// latency-bound chains (a dependent FP64 operation costs tens of cycles on
+// one wave), so the workgroup has 16 waves: a reflection updates 16 columns
+// at once, the dot products keep two partial sums, and the triangular solve
+// runs row by row over all columns at once instead of one column per thread.
 #include "common.h"
 #include "kernels.h"
 
-#define TPB 256
+#define UF_TPB 1024
+#define UF_WAVES (UF_TPB / 64)
+#define UF_ROWMAX 4096  // rows staged in LDS (longer rows are read from global memory)
 #define PIVOT_NOISE 1e-13
 
 __device__ inline double wave_sum(double v) {
@@ -26,7 +32,7 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict__ tasks,
+__global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restrict__ tasks,
                                                        BufTab bt, const int* __restrict__ relidx,
                                                        unsigned long long* status) {
     const UFTask t = tasks[blockIdx.x];
@@ -38,8 +44,9 @@ __global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict_
     double* C = bt.p[2] + t.woff;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     __shared__ int s_n1;
-    __shared__ double red[4];
+    __shared__ double red[UF_WAVES];
     __shared__ double s_tau, s_beta, s_scal, s_piv;
+    __shared__ double rowbuf[UF_ROWMAX];
     if (tid == 0) {
         int c = 0;
         while (c < N && r[c] < nn) ++c;
@@ -47,14 +54,16 @@ __global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict_
     }
     __syncthreads();
     const int N1 = s_n1, N2 = N - N1;
-    for (i64 e = tid; e < (i64)N * N; e += TPB) {
-        const int i = (int)(e % N), j = (int)(e / N);
-        Li[e] = (j < N1 && i >= j) ? F[r[i] + (i64)r[j] * nj] : 0.0;
+    // element loops: one column per wave, its rows over the lanes
+    for (int j = wv; j < N; j += UF_WAVES) {
+        const i64 fj = (i64)r[j] * nj;
+        for (int i = lane; i < N; i += 64)
+            Li[i + (i64)j * N] = (j < N1 && i >= j) ? F[r[i] + fj] : 0.0;
     }
-    for (i64 e = tid; e < (i64)na * N2; e += TPB) {
-        const int i = (int)(e % na), j = (int)(e / na);
+    for (int j = wv; j < N2; j += UF_WAVES) {
         const int cj = r[N1 + j] - nn;
-        C[e] = (i >= cj) ? Lk[i + (i64)cj * na] : 0.0;
+        for (int i = lane; i < na; i += 64)
+            C[i + (i64)j * na] = (i >= cj) ? Lk[i + (i64)cj * na] : 0.0;
     }
     __syncthreads();
     if (N2 > 0) {
@@ -65,12 +74,14 @@ __global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict_
             const int piv = na - 1 - js;
             double* cc = C + (i64)col * na;
             double s = 0.0;
-            for (int i = jj + tid; i < piv; i += TPB) s += cc[i] * cc[i];
+            for (int i = jj + tid; i < piv; i += UF_TPB) s += cc[i] * cc[i];
             s = wave_sum(s);
             if (lane == 0) red[wv] = s;
             __syncthreads();
             if (tid == 0) {
-                const double xn = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+                double ss = 0.0;
+                for (int w = 0; w < UF_WAVES; ++w) ss += red[w];
+                const double xn = sqrt(ss);
                 double alpha = cc[piv];
                 if (fabs(alpha) <= PIVOT_NOISE * xn) alpha = 0.0;
                 if (xn == 0.0) {
@@ -86,47 +97,126 @@ __global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict_
             const double tau = s_tau;
             if (tau == 0.0) continue;
             const double scal = s_scal;
-            for (int i = jj + tid; i < piv; i += TPB) cc[i] *= scal;
+            for (int i = jj + tid; i < piv; i += UF_TPB) cc[i] *= scal;
             if (tid == 0) cc[piv] = s_beta;
             __syncthreads();
-            // H = I - tau v v^T, v = [cc[jj:piv]; 1], on rows jj..piv of columns < col
-            for (int c = wv; c < col; c += 4) {
-                double* cd = C + (i64)c * na;
-                double d = 0.0;
-                for (int i = jj + lane; i < piv; i += 64) d += cc[i] * cd[i];
-                d = wave_sum(d) + cd[piv];
-                const double f = tau * d;
-                for (int i = jj + lane; i < piv; i += 64) cd[i] -= f * cc[i];
-                if (lane == 0) cd[piv] -= f;
+            // H = I - tau v v^T, v = [cc[jj:piv]; 1], on rows jj..piv of columns < col.
+            // A short band (most reflections of a nearly triangular C zero one
+            // or two entries) takes one thread per column, so the columns'
+            // memory round trips overlap; a long one one wave per column.
+            if (piv - jj < 32) {
+                for (int c = tid; c < col; c += UF_TPB) {
+                    double* cd = C + (i64)c * na;
+                    double d = cd[piv];
+                    for (int i = jj; i < piv; ++i) d += cc[i] * cd[i];
+                    const double f = tau * d;
+                    for (int i = jj; i < piv; ++i) cd[i] -= f * cc[i];
+                    cd[piv] -= f;
+                }
+            } else if (piv - jj <= UF_ROWMAX) {
+                // v staged in LDS; a wave updates two columns at once so that
+                // twice the loads are in flight
+                for (int i = tid; i < piv - jj; i += UF_TPB) rowbuf[i] = cc[jj + i];
+                __syncthreads();
+                const double* v = rowbuf - jj;
+                for (int c = wv; c < col; c += 2 * UF_WAVES) {
+                    const int c2 = c + UF_WAVES;
+                    const bool two = c2 < col;
+                    double* ca = C + (i64)c * na;
+                    double* cb = C + (i64)(two ? c2 : c) * na;
+                    double da = 0.0, db = 0.0;
+#pragma unroll 4
+                    for (int i = jj + lane; i < piv; i += 64) {
+                        const double vi = v[i];
+                        da += vi * ca[i];
+                        db += vi * cb[i];
+                    }
+                    const double fa = tau * (wave_sum(da) + ca[piv]);
+                    const double fb = tau * (wave_sum(db) + cb[piv]);
+#pragma unroll 4
+                    for (int i = jj + lane; i < piv; i += 64) {
+                        const double vi = v[i];
+                        ca[i] -= fa * vi;
+                        if (two) cb[i] -= fb * vi;
+                    }
+                    if (lane == 0) {
+                        ca[piv] -= fa;
+                        if (two) cb[piv] -= fb;
+                    }
+                }
+            } else {
+                for (int c = wv; c < col; c += UF_WAVES) {
+                    double* cd = C + (i64)c * na;
+                    double d0 = 0.0, d1 = 0.0;
+                    int i = jj + lane;
+                    for (; i + 64 < piv; i += 128) {
+                        d0 += cc[i] * cd[i];
+                        d1 += cc[i + 64] * cd[i + 64];
+                    }
+                    if (i < piv) d0 += cc[i] * cd[i];
+                    const double d = wave_sum(d0 + d1) + cd[piv];
+                    const double f = tau * d;
+                    for (int i = jj + lane; i < piv; i += 64) cd[i] -= f * cc[i];
+                    if (lane == 0) cd[piv] -= f;
+                }
             }
             __syncthreads();
         }
-        for (i64 e = tid; e < (i64)N2 * N2; e += TPB) {
-            const int i = (int)(e % N2), j = (int)(e / N2);
-            if (i >= j) Li[(N1 + i) + (i64)(N1 + j) * N] = C[(na - N2 + i) + (i64)j * na];
-        }
+        for (int j = wv; j < N2; j += UF_WAVES)
+            for (int i = j + lane; i < N2; i += 64)
+                Li[(N1 + i) + (i64)(N1 + j) * N] = C[(na - N2 + i) + (i64)j * na];
+        if (t.defer_solve) return;  // Li21 = Li22^{-T} B and the rest follow as launches
         __syncthreads();
+        // Li21 from Li22^T Li21 = B: row i of every column once rows > i are
+        // done, then its contribution to rows < i
         const double* L22 = Li + N1 + (i64)N1 * N;
-        for (int c = tid; c < N1; c += TPB) {
-            double* x = Li + N1 + (i64)c * N;
-            for (int i = N2 - 1; i >= 0; --i) {
-                double v = x[i];
-                for (int k = i + 1; k < N2; ++k) v -= L22[k + (i64)i * N] * x[k];
-                x[i] = v / L22[i + (i64)i * N];
+        double* X = Li + N1;  // column c at X + c N
+        if (N1 < 32) {
+            // few columns: each thread substitutes its own column
+            for (int c = tid; c < N1; c += UF_TPB) {
+                double* x = X + (i64)c * N;
+                for (int i = N2 - 1; i >= 0; --i) {
+                    double v = x[i];
+                    for (int k = i + 1; k < N2; ++k) v -= L22[k + (i64)i * N] * x[k];
+                    x[i] = v / L22[i + (i64)i * N];
+                }
             }
+            __syncthreads();
+        } else
+        for (int i = N2 - 1; i >= 0; --i) {
+            const double dinv = 1.0 / L22[i + (i64)i * N];
+            const bool st = i <= UF_ROWMAX;  // row i of L22 staged in LDS
+            for (int c = tid; c < N1; c += UF_TPB) X[i + (i64)c * N] *= dinv;
+            if (st)
+                for (int k = tid; k < i; k += UF_TPB) rowbuf[k] = L22[i + (i64)k * N];
+            __syncthreads();
+            for (int c = wv; c < N1; c += UF_WAVES) {
+                double* xc = X + (i64)c * N;
+                const double xi = xc[i];
+                for (int k = lane; k < i; k += 64)
+                    xc[k] -= (st ? rowbuf[k] : L22[i + (i64)k * N]) * xi;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (i64 e = tid; e < (i64)N1 * N1; e += TPB) {
-            const int i = (int)(e % N1), j = (int)(e / N1);
-            if (i < j) continue;
-            double s0 = 0.0;
-            const double* xi = Li + N1 + (i64)i * N;
-            const double* xj = Li + N1 + (i64)j * N;
-            for (int k = 0; k < N2; ++k) s0 += xi[k] * xj[k];
-            Li[i + (i64)j * N] -= s0;
+        for (int j = wv; j < N1; j += UF_WAVES) {
+            for (int i = j + lane; i < N1; i += 64) {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                const double* xi = Li + N1 + (i64)i * N;
+                const double* xj = Li + N1 + (i64)j * N;
+                int k = 0;
+                for (; k + 3 < N2; k += 4) {
+                    s0 += xi[k] * xj[k];
+                    s1 += xi[k + 1] * xj[k + 1];
+                    s2 += xi[k + 2] * xj[k + 2];
+                    s3 += xi[k + 3] * xj[k + 3];
+                }
+                for (; k < N2; ++k) s0 += xi[k] * xj[k];
+                Li[i + (i64)j * N] -= (s0 + s1) + (s2 + s3);
+            }
         }
         __syncthreads();
     }
+    if (t.defer_rchol) return;  // a blocked potrf on the reversed block follows
     // reversed Cholesky of the (1,1) block: A = Lr^T Lr, last row first
     for (int p = N1 - 1; p >= 0; --p) {
         if (tid == 0) s_piv = Li[p + (i64)p * N];
@@ -137,13 +227,131 @@ __global__ __launch_bounds__(TPB) void k_update_factor(const UFTask* __restrict_
             return;
         }
         const double sd = sqrt(d), rs = 1.0 / sd;
-        for (int j = tid; j < p; j += TPB) Li[p + (i64)j * N] *= rs;
+        const bool st = p <= UF_ROWMAX;  // row p staged in LDS
+        for (int j = tid; j < p; j += UF_TPB) {
+            const double v = Li[p + (i64)j * N] * rs;
+            Li[p + (i64)j * N] = v;
+            if (st) rowbuf[j] = v;
+        }
         if (tid == 0) Li[p + (i64)p * N] = sd;
         __syncthreads();
-        for (i64 e = tid; e < (i64)p * p; e += TPB) {
-            const int i = (int)(e % p), j = (int)(e / p);
-            if (i >= j) Li[i + (i64)j * N] -= Li[p + (i64)i * N] * Li[p + (i64)j * N];
+        // rank-1 update of the lower triangle of the leading p x p block
+        for (int j = wv; j < p; j += UF_WAVES) {
+            const double rj = st ? rowbuf[j] : Li[p + (i64)j * N];
+            double* cj = Li + (i64)j * N;
+            for (int i = j + lane; i < p; i += 64)
+                cj[i] -= (st ? rowbuf[i] : Li[p + (i64)i * N]) * rj;
         }
         __syncthreads();
     }
+}
+
+// The same update for small tasks (N <= 64 and na <= 64): one wave, with C
+// and Li held in LDS; lanes walk rows for the reductions and the copies and
+// own a column each in the reflections, the solve and the rank-1 updates.
+// The arithmetic of every entry follows k_update_factor's (the thread-per-
+// column reflection path, the serial column solve, the four partial sums).
+__global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __restrict__ tasks,
+                                                            BufTab bt,
+                                                            const int* __restrict__ relidx,
+                                                            unsigned long long* status) {
+    const UFTask t = tasks[blockIdx.x];
+    const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
+    const int l = threadIdx.x;
+    if (N > 64 || na > 64) {  // the planner sends only small tasks here
+        if (l == 0) chp_record_failure(status, t.key, 1);
+        return;
+    }
+    const int* r = relidx + t.relpos;
+    const double* F = bt.p[t.fbuf] + t.foff;
+    const double* Lk = bt.p[1] + t.lkoff;
+    double* Li = bt.p[1] + t.outoff;
+    __shared__ double Cs[64][65];  // C(i, j)
+    __shared__ double Ls[64][65];  // Li(i, j)
+    __shared__ int s_r[64];
+    const int rl = l < N ? r[l] : 0;
+    s_r[l] = rl;
+    const int N1 = __popcll(__ballot(l < N && rl < nn)), N2 = N - N1;
+    __syncthreads();
+    for (int j = 0; j < N; ++j)
+        Ls[l][j] = (l < N && j < N1 && l >= j) ? F[rl + (i64)s_r[j] * nj] : 0.0;
+    for (int j = 0; j < N2; ++j) {
+        const int cj = s_r[N1 + j] - nn;
+        Cs[l][j] = (l < na && l >= cj) ? Lk[l + (i64)cj * na] : 0.0;
+    }
+    __syncthreads();
+    for (int js = 0; js < N2; ++js) {
+        const int col = N2 - 1 - js;
+        const int jj = s_r[N - 1 - js] - nn;
+        if (na - jj - (js + 1) == 0) continue;
+        const int piv = na - 1 - js;
+        const double x = (l >= jj && l < piv) ? Cs[l][col] : 0.0;
+        const double xn = sqrt(wave_sum(x * x));
+        double alpha = Cs[piv][col];
+        if (fabs(alpha) <= PIVOT_NOISE * xn) alpha = 0.0;
+        if (xn == 0.0) continue;
+        const double beta = -copysign(hypot(alpha, xn), alpha);
+        const double tau = (beta - alpha) / beta;
+        const double scal = 1.0 / (alpha - beta);
+        __syncthreads();  // every lane has read the pivot
+        if (l >= jj && l < piv) Cs[l][col] = x * scal;
+        if (l == piv) Cs[piv][col] = beta;
+        __syncthreads();
+        if (l < col) {
+            double d = Cs[piv][l];
+            for (int i = jj; i < piv; ++i) d += Cs[i][col] * Cs[i][l];
+            const double f = tau * d;
+            for (int i = jj; i < piv; ++i) Cs[i][l] -= f * Cs[i][col];
+            Cs[piv][l] -= f;
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < N2; ++j)
+        if (l >= j && l < N2) Ls[N1 + l][N1 + j] = Cs[na - N2 + l][j];
+    __syncthreads();
+    // Li21 from Li22^T Li21 = B, one column per lane
+    if (l < N1) {
+        for (int i = N2 - 1; i >= 0; --i) {
+            double v = Ls[N1 + i][l];
+            for (int k = i + 1; k < N2; ++k) v -= Ls[N1 + k][N1 + i] * Ls[N1 + k][l];
+            Ls[N1 + i][l] = v / Ls[N1 + i][N1 + i];
+        }
+    }
+    __syncthreads();
+    // Li11 -= Li21^T Li21 (lower), row l
+    if (l < N1 && N2 > 0) {
+        for (int j = 0; j <= l; ++j) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int k = 0;
+            for (; k + 3 < N2; k += 4) {
+                s0 += Ls[N1 + k][l] * Ls[N1 + k][j];
+                s1 += Ls[N1 + k + 1][l] * Ls[N1 + k + 1][j];
+                s2 += Ls[N1 + k + 2][l] * Ls[N1 + k + 2][j];
+                s3 += Ls[N1 + k + 3][l] * Ls[N1 + k + 3][j];
+            }
+            for (; k < N2; ++k) s0 += Ls[N1 + k][l] * Ls[N1 + k][j];
+            Ls[l][j] -= (s0 + s1) + (s2 + s3);
+        }
+    }
+    __syncthreads();
+    // reversed Cholesky of the (1,1) block, last row first
+    for (int p = N1 - 1; p >= 0; --p) {
+        const double d = Ls[p][p];
+        if (!(d > 0.0)) {
+            if (l == 0) chp_record_failure(status, t.key, N1 - p);
+            return;
+        }
+        const double sd = sqrt(d), rs = 1.0 / sd;
+        __syncthreads();  // every lane has read the pivot
+        if (l < p) Ls[p][l] *= rs;
+        if (l == p) Ls[p][p] = sd;
+        __syncthreads();
+        if (l < p) {
+            const double ri = Ls[p][l];
+            for (int j = 0; j <= l; ++j) Ls[l][j] -= ri * Ls[p][j];
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < N; ++j)
+        if (l < N) Li[l + (i64)j * N] = Ls[l][j];
 }

@@ -145,7 +145,9 @@ struct chp_plan {
     struct Seq { bool built = false; std::vector<chpi::Launch> seq; };
     struct LevelSeq {
         bool built = false;
-        std::vector<std::vector<chpi::Launch>> shared, dir;  // per level, top-down
+        // per level, top-down: shared = the update matrices (or their factors),
+        // kept in their own arena d_supd; shared_t = inverses into scratch
+        std::vector<std::vector<chpi::Launch>> shared, shared_t, dir;
     };
     Seq comp[2], y2k[2], m2t[2];
     LevelSeq scale[8];
@@ -153,6 +155,7 @@ struct chp_plan {
     double* d_dscratch = nullptr;
     double* d_ddinv = nullptr;
     double* d_dsplitp = nullptr;
+    double* d_supd = nullptr;  // update arena of the Hessian's scaling sweeps (upd_len)
     i64 dscratch_len = 0, ddinv_len = 0;                       // required
     i64 dscratch_cap = 0, ddinv_cap = 0, dsplitp_cap = 0;      // allocated
     // HIP graphs of the sweeps, replayed on a plan-owned stream that is

@@ -592,7 +592,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTADD:
             CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, P->d_fi, blkval,
-                       P->d_upd, P->d_relidx, P->d_chidx, L.iarg);
+                       bt.p[1], P->d_relidx, P->d_chidx, L.iarg);
             break;
         case K_COPY_TOP:
             CHP_LAUNCH(k_copy_top, 0, (const int2*)list, P->d_fi, blkval,
@@ -617,7 +617,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, P->d_fi, blkval,
-                       P->d_upd, P->d_relidx);
+                       bt.p[1], P->d_relidx);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
@@ -641,8 +641,12 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_MATOP:
             CHP_LAUNCH(k_matop, 0, (const MatOp*)list, bt);
             break;
-        case K_UPDATE_FACTOR:
-            CHP_LAUNCH(k_update_factor, 0, (const UFTask*)list, bt, P->d_relidx, P->d_status);
+        case K_UPDATE_FACTOR:  // the 64-thread launches hold the small tasks
+            if (L.block == 64)
+                CHP_LAUNCH(k_update_factor_small, 0, (const UFTask*)list, bt, P->d_relidx,
+                           P->d_status);
+            else
+                CHP_LAUNCH(k_update_factor, 0, (const UFTask*)list, bt, P->d_relidx, P->d_status);
             break;
         default:
             chp_set_error("internal: unexpected launch kind");
@@ -934,7 +938,7 @@ void chp_plan_destroy(chp_plan* P) {
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_gptr, P->d_gent, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
-                    P->d_dscratch, P->d_ddinv, P->d_dsplitp};
+                    P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_supd};
     for (SolvePlan* S : P->solves) {
         for (auto& g : S->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (S->pg.d_lists) (void)hipFree(S->pg.d_lists);

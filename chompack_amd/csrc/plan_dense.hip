@@ -173,7 +173,7 @@ std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
 // Update matrices of the children of the fronts `fr`, read from the fronts
 // [blk (buffer fbuf); V or its factor (update arena)].
 void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
-                        bool factored, int fbuf, Bump& sc) {
+                        bool factored, int fbuf, Bump& sc, Bump& dv) {
     const std::vector<int> ch = children_of(P, fr);
     if (ch.empty()) return;
     if (!factored) {
@@ -181,7 +181,20 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
         push_tasks(P, seq, K_EXTRACT, xt, fbuf);
         return;
     }
-    std::vector<UFTask> ut;
+    std::vector<UFTask> ut, us;
+    // Large tasks leave the kernel early and finish on the whole device:
+    //   * N1 >= UF_SOLVE_N1 and N2 >= UF_SOLVE_N2: X = Li22^{-1} (blocked
+    //     triangular inverse), Z = X^T B (= Li21), Li11 = A - Z^T Z;
+    //   * those and all N1 >= UF_BLOCKED_N1: the reversed Cholesky of Li11 as
+    //     J A J = M M^T by the blocked potrf (J the reversal), Li11 = J M^T J,
+    //     through reversed-transpose copies.
+    // Smaller tasks are latency chains that stay inside the kernel.
+    constexpr int UF_BLOCKED_N1 = 192, UF_SOLVE_N1 = 32, UF_SOLVE_N2 = 32;
+    std::vector<MatOp> rt1, rt2;
+    std::vector<Mat> R, L22s, Xs;
+    std::vector<i64> Ws;
+    std::vector<int> rkeys, skeys;
+    std::vector<GemmTile> gz, gs;
     for (int c : ch) {
         const FrontInfo& g = P->fi[c];
         const FrontInfo& f = P->fi[g.parent];
@@ -194,15 +207,55 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
         t.relpos = g.relptr;
         t.nn = f.nn; t.na = f.na; t.N = g.na; t.key = g.parent;
         t.fbuf = (unsigned char)fbuf;
-        ut.push_back(t);
+        const int N = g.na, n2 = N - n1;
+        if (n1 >= UF_SOLVE_N1 && n2 >= UF_SOLVE_N2) {
+            t.defer_solve = 1;
+            const Mat L22{1, g.updptr + n1 + (i64)n1 * N, N, n2};
+            const Mat X{2, sc.take((i64)n2 * n2), n2, n2};
+            const i64 z = sc.take((i64)n2 * n1);
+            L22s.push_back(L22);
+            Xs.push_back(X);
+            Ws.push_back(sc.take((i64)n2 * n2));
+            skeys.push_back(g.parent);
+            // Z = X^T B (B = Li21 as gathered), Li11 -= Z^T Z (lower), Li21 = Z
+            add_gemm_tiles(P, gz, mk(2, X.off, n2, 1, 1, g.updptr + n1, N, 0, 2, z, n2, n2, n1, n2,
+                                     1.0, 0.0, 0, 2), false);
+            add_gemm_tiles(P, gs, mk(2, z, n2, 1, 2, z, n2, 0, 1, g.updptr, N, n1, n1, n2, -1.0,
+                                     1.0, 1), true);
+            add_matop(rt1, MO_COPY, 2, z, n2, 1, g.updptr + n1, N, n2, n1);
+        }
+        if (n1 >= UF_BLOCKED_N1 || t.defer_solve) {
+            t.defer_rchol = 1;
+            const i64 rs = sc.take((i64)n1 * n1);
+            add_matop(rt1, MO_RT, 1, g.updptr, g.na, 2, rs, n1, n1, n1);
+            add_matop(rt2, MO_RT, 2, rs, n1, 1, g.updptr, g.na, n1, n1);
+            R.push_back({2, rs, n1, n1});
+            rkeys.push_back(g.parent);
+        }
+        // tasks that fit one wave's LDS tiles run in 64-thread workgroups
+        const bool small = N <= 64 && f.na <= 64 && !t.defer_solve && !t.defer_rchol;
+        (small ? us : ut).push_back(t);
     }
+    push_tasks(P, seq, K_UPDATE_FACTOR, us);
+    if (!us.empty()) seq.back().block = 64;
     push_tasks(P, seq, K_UPDATE_FACTOR, ut);
+    if (!ut.empty()) seq.back().block = 1024;  // k_update_factor runs 16 waves
+    if (!L22s.empty()) {
+        emit_trtri(P, seq, L22s, Xs, Ws, skeys);
+        push_gemm(P, seq, gz);
+        push_gemm(P, seq, gs);
+    }
+    if (!R.empty()) {
+        push_tasks(P, seq, K_MATOP, rt1);
+        emit_potrf(P, seq, R, rkeys, dv);
+        push_tasks(P, seq, K_MATOP, rt2);
+    }
 }
 
 // One level of completion (completion.c:24-118), root to leaves.
 void build_completion_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
                             bool fu, Bump& sc, Bump& dv) {
-    emit_child_updates(P, seq, fr, fu, 0, sc);
+    emit_child_updates(P, seq, fr, fu, 0, sc, dv);
     std::vector<int> fa;
     for (int k : fr) if (P->fi[k].na > 0) fa.push_back(k);
     if (!fa.empty()) {
@@ -262,10 +315,10 @@ void build_completion_level(chp_plan* P, std::vector<Launch>& seq, const std::ve
 
 // One level of the scaling map (hessian.c:201-352): shared part (update
 // matrices, factors, inverses) and the per-direction part.
-void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>& dir,
-                       const std::vector<int>& fr, bool fu, bool inv, bool adj, Bump& sc,
-                       Bump& dv) {
-    emit_child_updates(P, sh, fr, fu, 6, sc);
+void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>& sht,
+                       std::vector<Launch>& dir, const std::vector<int>& fr, bool fu, bool inv,
+                       bool adj, Bump& sc, Bump& dv) {
+    emit_child_updates(P, sh, fr, fu, 6, sc, dv);
     std::vector<int> fa;
     for (int k : fr) if (P->fi[k].na > 0) fa.push_back(k);
     std::vector<Mat> V, Xv;
@@ -279,7 +332,7 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
         }
     }
     if (!fa.empty() && !fu) emit_potrf(P, sh, V, fa, dv);
-    if (!fa.empty() && inv) emit_trtri(P, sh, V, Xv, Wv, fa);
+    if (!fa.empty() && inv) emit_trtri(P, sht, V, Xv, Wv, fa);
     // M1 = L11^{-1} (trsm, !inv) or L11 itself (trmm, inv)
     std::vector<Mat> L11, X11;
     std::vector<i64> W11;
@@ -291,7 +344,7 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
             W11.push_back(sc.take((i64)f.nn * f.nn));
         }
     }
-    if (!inv) emit_trtri(P, sh, L11, X11, W11, std::vector<int>(fr.begin(), fr.end()));
+    if (!inv) emit_trtri(P, sht, L11, X11, W11, std::vector<int>(fr.begin(), fr.end()));
     // per direction: U11 symmetric, Tu = U op1(M1), U11 = op2(M1) Tu11 (upper
     // zeroed), U21 = op3(M3) Tu21
     std::vector<MatOp> sy;
@@ -368,7 +421,7 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
                      Bump& sc) {
     const double alpha = inv ? 1.0 : -1.0;
     if (inv) {
-        emit_child_updates(P, seq, fr, false, 0, sc);  // before this level's update
+        emit_child_updates(P, seq, fr, false, 0, sc, sc);  // before this level's update
     } else {
         // this level's V from the (finished) parents
         push_tasks(P, seq, K_EXTRACT, extract_tiles(P, fr), 0);
@@ -452,10 +505,12 @@ int build_scale(chp_plan* P, int idx, i64& sc_req, i64& dv_req) {
     const bool fu = idx & 4, inv = idx & 2, adj = idx & 1;
     auto& S = P->scale[idx];
     S.shared.assign(P->nlevels, {});
+    S.shared_t.assign(P->nlevels, {});
     S.dir.assign(P->nlevels, {});
     for (int L = P->nlevels - 1; L >= 0; --L) {
         Bump sc, dv;
-        build_scale_level(P, S.shared[L], S.dir[L], P->bylevel[L], fu, inv, adj, sc, dv);
+        build_scale_level(P, S.shared[L], S.shared_t[L], S.dir[L], P->bylevel[L], fu, inv, adj,
+                          sc, dv);
         b.level_done(sc, dv);
     }
     S.built = true;
@@ -537,17 +592,27 @@ int chp_hessian(chp_plan* P, const double* lblkval, const double* yblkval, doubl
         else if (!P->y2k[iv].built) build_y2k(P, iv);
     }
     if (prepare(P, sc, dv, s)) return -2;
+    if (!P->d_supd && alloc_doubles(&P->d_supd, P->upd_len)) return -2;
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    // The scaling sweeps keep their update matrices (factored or not; the
+    // same for adj and its adjoint) in d_supd, which the M2 sweeps between
+    // them leave alone: the second scaling of a call reuses them.
+    bool have_upd = false;
     auto run_scale = [&](int idx) -> int {
         const auto& S = P->scale[idx];
         for (int L = P->nlevels - 1; L >= 0; --L) {
-            int rc = run_seq(P, P->dense, S.shared[L], dense_tab(P, nullptr, lblkval, yblkval), s);
+            BufTab bt = dense_tab(P, nullptr, lblkval, yblkval);
+            bt.p[1] = P->d_supd;
+            int rc = have_upd ? 0 : run_seq(P, P->dense, S.shared[L], bt, s);
+            if (!rc) rc = run_seq(P, P->dense, S.shared_t[L], bt, s);
             if (rc) return rc;
             for (int u = 0; u < nu; ++u) {
-                rc = run_seq(P, P->dense, S.dir[L], dense_tab(P, ublkval[u], lblkval, yblkval), s);
+                bt.p[0] = ublkval[u];
+                rc = run_seq(P, P->dense, S.dir[L], bt, s);
                 if (rc) return rc;
             }
         }
+        have_upd = true;
         return 0;
     };
     for (int pass = 0; pass < npass; ++pass) {
