@@ -127,6 +127,14 @@ struct UFTask {
     unsigned char fbuf, defer_solve, defer_rchol, pad[5];
 };
 
+// One item of a chain of small fronts: the completion step of `front` (kind
+// 1), followed by the nch items of its children's update factors (kind 0;
+// kind 2 for the child that is the chain's next front).
+struct ChainItem {
+    UFTask uf;
+    int front, kind, nch, pad;
+};
+
 // Row tile of a front in the solve's scatter: rows r0 .. r0 + 63 of front k,
 // whose solved block sits at offset tofs of the solve's block buffer.
 struct STile {

@@ -26,6 +26,13 @@
 #define UF_ROWMAX 4096  // rows staged in LDS (longer rows are read from global memory)
 #define PIVOT_NOISE 1e-13
 
+// v of lane `lane` (uniform across the wave)
+__device__ inline double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -249,35 +256,63 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
 // The same update for small tasks (N <= 64 and na <= 64): one wave, with C
 // and Li held in LDS; lanes walk rows for the reductions and the copies and
 // own a column each in the reflections, the solve and the rank-1 updates.
-// The arithmetic of every entry follows k_update_factor's (the thread-per-
-// column reflection path, the serial column solve, the four partial sums).
-__global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __restrict__ tasks,
-                                                            BufTab bt,
-                                                            const int* __restrict__ relidx,
-                                                            unsigned long long* status) {
-    const UFTask t = tasks[blockIdx.x];
+// The reflections and the (1,1) update follow k_update_factor's arithmetic
+// (the thread-per-column path, the four partial sums); the solve runs down
+// the columns with the rows across the lanes.
+// Returns false after recording a failed reversed Cholesky.
+// FL / KL, when given, hold the parent's front F (FL[i][j] = F(i, j)) and the
+// factor Lk (KL[i][j] = Lk(i, j)) in LDS in place of global memory.
+__device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __restrict__ relidx,
+                                   unsigned long long* status, double (*Cs)[65],
+                                   double (*Ls)[65], int* s_r, const double (*FL)[65] = nullptr,
+                                   const double (*KL)[65] = nullptr) {
     const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
     const int l = threadIdx.x;
-    if (N > 64 || na > 64) {  // the planner sends only small tasks here
-        if (l == 0) chp_record_failure(status, t.key, 1);
-        return;
-    }
     const int* r = relidx + t.relpos;
     const double* F = bt.p[t.fbuf] + t.foff;
     const double* Lk = bt.p[1] + t.lkoff;
     double* Li = bt.p[1] + t.outoff;
-    __shared__ double Cs[64][65];  // C(i, j)
-    __shared__ double Ls[64][65];  // Li(i, j)
-    __shared__ int s_r[64];
     const int rl = l < N ? r[l] : 0;
     s_r[l] = rl;
     const int N1 = __popcll(__ballot(l < N && rl < nn)), N2 = N - N1;
     __syncthreads();
-    for (int j = 0; j < N; ++j)
-        Ls[l][j] = (l < N && j < N1 && l >= j) ? F[rl + (i64)s_r[j] * nj] : 0.0;
-    for (int j = 0; j < N2; ++j) {
-        const int cj = s_r[N1 + j] - nn;
-        Cs[l][j] = (l < na && l >= cj) ? Lk[l + (i64)cj * na] : 0.0;
+    // eight columns at a time, loads first (from a valid address also where
+    // the entry is zero), so that they are in flight together
+    if (FL) {
+        for (int j = 0; j < N; ++j)
+            Ls[l][j] = (j < N1 && l < N && l >= j) ? FL[rl][s_r[j]] : 0.0;
+        for (int j = 0; j < N2; ++j) {
+            const int cj = s_r[N1 + j] - nn;
+            Cs[l][j] = (l < na && l >= cj) ? KL[l][cj] : 0.0;
+        }
+    } else {
+    for (int j0 = 0; j0 < N; j0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            const bool ok = j < N1 && l < N && l >= j;
+            v[u] = F[ok ? rl + (i64)s_r[j] * nj : 0];
+            v[u] = ok ? v[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u < N) Ls[l][j0 + u] = v[u];
+    }
+    for (int j0 = 0; j0 < N2; j0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            const int cj = j < N2 ? s_r[N1 + j] - nn : 0;
+            const bool ok = j < N2 && l < na && l >= cj;
+            v[u] = Lk[ok ? l + (i64)cj * na : 0];
+            v[u] = ok ? v[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u < N2) Cs[l][j0 + u] = v[u];
+    }
     }
     __syncthreads();
     for (int js = 0; js < N2; ++js) {
@@ -309,13 +344,16 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
     for (int j = 0; j < N2; ++j)
         if (l >= j && l < N2) Ls[N1 + l][N1 + j] = Cs[na - N2 + l][j];
     __syncthreads();
-    // Li21 from Li22^T Li21 = B, one column per lane
-    if (l < N1) {
+    // Li21 from Li22^T Li21 = B, column by column, lane l holding row l: once
+    // x_i is known, rows l < i subtract Li22(i, l) x_i
+    for (int c = 0; c < N1; ++c) {
+        double z = l < N2 ? Ls[N1 + l][c] : 0.0;
         for (int i = N2 - 1; i >= 0; --i) {
-            double v = Ls[N1 + i][l];
-            for (int k = i + 1; k < N2; ++k) v -= Ls[N1 + k][N1 + i] * Ls[N1 + k][l];
-            Ls[N1 + i][l] = v / Ls[N1 + i][N1 + i];
+            const double zi = readlane_d(z, i) / Ls[N1 + i][N1 + i];
+            if (l == i) z = zi;
+            if (l < i) z -= Ls[N1 + i][N1 + l] * zi;
         }
+        if (l < N2) Ls[N1 + l][c] = z;
     }
     __syncthreads();
     // Li11 -= Li21^T Li21 (lower), row l
@@ -339,7 +377,7 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
         const double d = Ls[p][p];
         if (!(d > 0.0)) {
             if (l == 0) chp_record_failure(status, t.key, N1 - p);
-            return;
+            return false;
         }
         const double sd = sqrt(d), rs = 1.0 / sd;
         __syncthreads();  // every lane has read the pivot
@@ -354,4 +392,166 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
     }
     for (int j = 0; j < N; ++j)
         if (l < N) Li[l + (i64)j * N] = Ls[l][j];
+    __syncthreads();  // the LDS tiles are free again
+    return true;
+}
+
+__global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __restrict__ tasks,
+                                                            BufTab bt,
+                                                            const int* __restrict__ relidx,
+                                                            unsigned long long* status) {
+    __shared__ double Cs[64][65];  // C(i, j)
+    __shared__ double Ls[64][65];  // Li(i, j)
+    __shared__ int s_r[64];
+    const UFTask t = tasks[blockIdx.x];
+    if (t.N > 64 || t.na > 64) {  // the planner sends only small tasks here
+        if (threadIdx.x == 0) chp_record_failure(status, t.key, 1);
+        return;
+    }
+    update_factor_wave(t, bt, relidx, status, Cs, Ls, s_r);
+}
+
+// Completion (factored updates, completion.c:24-118) of a chain of small
+// fronts (nn + na <= 64, one front per level) in one 64-thread workgroup.
+// The items run in order: a front (kind 1, carrying the number nch of its
+// children) is followed by the update factors of its children (kind 0, and
+// kind 2 for the child that is the next front of the chain, which comes
+// last).  The front's columns B = [X11; X21] and the factor Lk of its update
+// matrix (V_k = Lk^T Lk) stay in LDS: the children's update factors read
+// them there, and the chain child's factor stays in LDS as the next front's
+// Lk (it is also stored to the update arena, as every child's is).  The step:
+//   X21 <- Lk^{-T} X21, X11 -= X21^T X21, X21 <- -Lk^{-1} X21 (substitution
+//   down each column, the rows across the lanes), X11 = Lr^T Lr (reversed
+//   Cholesky) and L11 = Lr^{-1} (forward substitution, a column per lane).
+// A zero pivot of Lk or a non-positive one of the reversed Cholesky records
+// a failure and ends the chain.
+__global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __restrict__ items,
+                                                         int nitems,
+                                                         const FrontInfo* __restrict__ fi,
+                                                         BufTab bt, const int* __restrict__ relidx,
+                                                         unsigned long long* status) {
+    __shared__ double T0[64][65], T1[64][65], T2[64][65], T3[64][65];
+    __shared__ int s_r[64];
+    double (*B)[65] = T0;   // the front's columns
+    double (*V)[65] = T1;   // Lk of the front
+    double (*Cs)[65] = T2;  // update-factor workspace
+    double (*Ls)[65] = T3;  // a child's update factor
+    bool have_v = false;
+    const int l = threadIdx.x;
+    for (int it = 0; it < nitems;) {
+        const ChainItem& ci = items[it];
+        const FrontInfo f = fi[ci.front];
+        const int nn = f.nn, na = f.na, nj = nn + na, nch = ci.nch;
+        double* blk = bt.p[0] + f.blkptr;
+        // loads first, eight columns at a time
+        for (int j0 = 0; j0 < nn; j0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                v[u] = blk[(l < nj && j0 + u < nn) ? l + (i64)(j0 + u) * nj : 0];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (l < nj && j0 + u < nn) B[l][j0 + u] = v[u];
+        }
+        if (!have_v) {
+            const double* Lk = bt.p[1] + f.updptr;
+            for (int j0 = 0; j0 < na; j0 += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = l < na && j0 + u < na && l >= j0 + u;
+                    v[u] = Lk[ok ? l + (i64)(j0 + u) * na : 0];
+                    v[u] = ok ? v[u] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (l < na && j0 + u < na) V[l][j0 + u] = v[u];
+            }
+        }
+        __syncthreads();
+        // the children's update factors, from B and V in LDS
+        bool keep = false;
+        for (int c = 1; c <= nch; ++c) {
+            const ChainItem& cu = items[it + c];
+            if (!update_factor_wave(cu.uf, bt, relidx, status, Cs, Ls, s_r, B, V)) return;
+            keep = cu.kind == 2;
+        }
+        it += 1 + nch;
+        if (na > 0) {
+            const int zero = __ffsll(__ballot(l < na && V[l][l] == 0.0));
+            if (zero) {
+                if (l == 0) chp_record_failure(status, ci.front, zero);
+                return;
+            }
+            // Lk^T Z = X21 and, below, Lk W = Z: column by column, lane l
+            // holding row l
+            for (int c = 0; c < nn; ++c) {
+                double z = l < na ? B[nn + l][c] : 0.0;
+                for (int i = na - 1; i >= 0; --i) {
+                    const double zi = readlane_d(z, i) / V[i][i];
+                    if (l == i) z = zi;
+                    if (l < i) z -= V[i][l] * zi;
+                }
+                if (l < na) B[nn + l][c] = z;
+            }
+            __syncthreads();
+            if (l < nn) {  // X11 -= Z^T Z, row l
+                for (int j = 0; j <= l; ++j) {
+                    double s0 = 0.0;
+                    for (int q = 0; q < na; ++q) s0 += B[nn + q][l] * B[nn + q][j];
+                    B[l][j] -= s0;
+                }
+            }
+            __syncthreads();
+            for (int c = 0; c < nn; ++c) {
+                double w = l < na ? B[nn + l][c] : 0.0;
+                for (int i = 0; i < na; ++i) {
+                    const double wi = readlane_d(w, i) / V[i][i];
+                    if (l == i) w = wi;
+                    if (l > i && l < na) w -= V[l][i] * wi;
+                }
+                if (l < na) B[nn + l][c] = -w;
+            }
+            __syncthreads();
+        }
+        // X11 = Lr^T Lr, last row first
+        for (int p = nn - 1; p >= 0; --p) {
+            const double d = B[p][p];
+            if (!(d > 0.0)) {
+                if (l == 0) chp_record_failure(status, ci.front, nn - p);
+                return;
+            }
+            const double sd = sqrt(d), rs = 1.0 / sd;
+            __syncthreads();  // every lane has read the pivot
+            if (l < p) B[p][l] *= rs;
+            if (l == p) B[p][p] = sd;
+            __syncthreads();
+            if (l < p) {
+                const double ri = B[p][l];
+                for (int j = 0; j <= l; ++j) B[l][j] -= ri * B[p][j];
+            }
+            __syncthreads();
+        }
+        // L11 = Lr^{-1} into Cs, column l
+        if (l < nn) {
+            Cs[l][l] = 1.0 / B[l][l];
+            for (int i = l + 1; i < nn; ++i) {
+                double v = 0.0;
+                for (int q = l; q < i; ++q) v -= B[i][q] * Cs[q][l];
+                Cs[i][l] = v / B[i][i];
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < nn; ++j) {
+            if (l < nn && l >= j) blk[l + (i64)j * nj] = Cs[l][j];
+            if (l >= nn && l < nj) blk[l + (i64)j * nj] = B[l][j];
+        }
+        __syncthreads();
+        if (keep) {  // the chain child's factor is the next front's Lk
+            double (*x)[65] = V;
+            V = Ls;
+            Ls = x;
+        }
+        have_v = keep;
+    }
 }

@@ -40,6 +40,8 @@ __global__ void k_diag_tasks(const DiagTask* tasks, BufTab bt, unsigned long lon
 __global__ void k_matop(const MatOp* ops, BufTab bt);
 __global__ void k_update_factor(const UFTask* tasks, BufTab bt, const int* relidx,
                                 unsigned long long* status);
+__global__ void k_completion_chain(const ChainItem* items, int nitems, const FrontInfo* fi,
+                                   BufTab bt, const int* relidx, unsigned long long* status);
 // N <= 64 and na <= 64; 64-thread workgroups
 __global__ void k_update_factor_small(const UFTask* tasks, BufTab bt, const int* relidx,
                                       unsigned long long* status);

@@ -641,6 +641,10 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_MATOP:
             CHP_LAUNCH(k_matop, 0, (const MatOp*)list, bt);
             break;
+        case K_COMP_CHAIN:
+            CHP_LAUNCH(k_completion_chain, 0, (const ChainItem*)list, L.iarg, P->d_fi, bt,
+                       P->d_relidx, P->d_status);
+            break;
         case K_UPDATE_FACTOR:  // the 64-thread launches hold the small tasks
             if (L.block == 64)
                 CHP_LAUNCH(k_update_factor_small, 0, (const UFTask*)list, bt, P->d_relidx,

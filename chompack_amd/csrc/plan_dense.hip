@@ -170,6 +170,20 @@ std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
     return ch;
 }
 
+// Update-factor task of child c from its parent's front (buffer fbuf) and the
+// factor of the parent's update matrix; the workspace offset is left at 0.
+UFTask uf_task(const chp_plan* P, int c, int fbuf) {
+    const FrontInfo& g = P->fi[c];
+    const FrontInfo& f = P->fi[g.parent];
+    UFTask t;
+    std::memset(&t, 0, sizeof(t));
+    t.foff = f.blkptr; t.lkoff = f.updptr; t.outoff = g.updptr;
+    t.relpos = g.relptr;
+    t.nn = f.nn; t.na = f.na; t.N = g.na; t.key = g.parent;
+    t.fbuf = (unsigned char)fbuf;
+    return t;
+}
+
 // Update matrices of the children of the fronts `fr`, read from the fronts
 // [blk (buffer fbuf); V or its factor (update arena)].
 void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
@@ -200,13 +214,8 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
         const FrontInfo& f = P->fi[g.parent];
         int n1 = 0;
         while (n1 < g.na && P->h_relidx[g.relptr + n1] < f.nn) ++n1;
-        UFTask t;
-        std::memset(&t, 0, sizeof(t));
-        t.foff = f.blkptr; t.lkoff = f.updptr; t.outoff = g.updptr;
+        UFTask t = uf_task(P, c, fbuf);
         t.woff = sc.take((i64)f.na * (g.na - n1));
-        t.relpos = g.relptr;
-        t.nn = f.nn; t.na = f.na; t.N = g.na; t.key = g.parent;
-        t.fbuf = (unsigned char)fbuf;
         const int N = g.na, n2 = N - n1;
         if (n1 >= UF_SOLVE_N1 && n2 >= UF_SOLVE_N2) {
             t.defer_solve = 1;
@@ -490,10 +499,48 @@ void note_needs(chp_plan* P, const DenseBuild& b, i64& sc_req, i64& dv_req) {
 int build_completion(chp_plan* P, int fu, i64& sc_req, i64& dv_req) {
     DenseBuild b(P);
     auto& seq = P->comp[fu].seq;
-    for (int L = P->nlevels - 1; L >= 0; --L) {
+    // With factored updates, runs of at least CHAIN_MIN levels that each hold
+    // one front with nn + na <= 64 go to one k_completion_chain launch.
+    constexpr int CHAIN_MIN = 3;
+    auto chain_front = [&](int L) {
+        const std::vector<int>& fr = P->bylevel[L];
+        return fr.size() == 1 && P->fi[fr[0]].nn + P->fi[fr[0]].na <= 64;
+    };
+    for (int L = P->nlevels - 1; L >= 0;) {
+        int L2 = L;
+        while (fu && L2 >= 0 && chain_front(L2)) --L2;
+        if (L - L2 >= CHAIN_MIN) {
+            std::vector<ChainItem> items;
+            for (int M = L; M > L2; --M) {
+                const int k = P->bylevel[M][0];
+                const int next = M - 1 > L2 ? P->bylevel[M - 1][0] : -1;
+                ChainItem ci;
+                std::memset(&ci, 0, sizeof(ci));
+                ci.front = k;
+                ci.kind = 1;
+                ci.nch = P->fi[k].chend - P->fi[k].chbeg;
+                items.push_back(ci);
+                for (int pass = 0; pass < 2; ++pass)  // the chain's next front last
+                    for (int l = P->fi[k].chbeg; l < P->fi[k].chend; ++l) {
+                        const int c = P->h_chidx[l];
+                        if ((c == next) != (pass == 1)) continue;
+                        std::memset(&ci, 0, sizeof(ci));
+                        ci.uf = uf_task(P, c, 0);
+                        ci.kind = c == next ? 2 : 0;
+                        items.push_back(ci);
+                    }
+            }
+            push_tasks(P, seq, K_COMP_CHAIN, items);
+            seq.back().grid = 1;
+            seq.back().block = 64;
+            seq.back().iarg = (int)items.size();
+            L = L2;
+            continue;
+        }
         Bump sc, dv;
         build_completion_level(P, seq, P->bylevel[L], fu != 0, sc, dv);
         b.level_done(sc, dv);
+        --L;
     }
     P->comp[fu].built = true;
     note_needs(P, b, sc_req, dv_req);

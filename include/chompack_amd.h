@@ -91,12 +91,13 @@ int chp_set_graphs(chp_plan* plan, int on);
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
  * (synchronizing after each launch, every launch on that stream) and clears
- * the counters; chp_profile_read fills ms/cnt/flops[23] per kernel kind
+ * the counters; chp_profile_read fills ms/cnt/flops[24] per kernel kind
  * (0 chol_small, 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm, 5 extract,
  * 6 projinv_small, 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd,
  * 11 solve_bwd, 12 copy_top, 13 splitk_reduce, 14 diag, 15 matop,
  * 16 update_factor, 17 sgather_fwd, 18 sgather_bwd, 19 sscatter,
- * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish) and returns 23.
+ * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish, 23 completion_chain) and
+ * returns 24.
  * chp_plan_flops: algorithmic GEMM flops of one cholesky (out[0]) and one
  * projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
