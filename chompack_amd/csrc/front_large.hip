@@ -943,21 +943,17 @@ __global__ __launch_bounds__(TPB) void k_neg_copy(
     double* blk = blkval + f.blkptr + nn;
     const double* S = scratch + sptr[k];
     const int nc = min(c0 + 16, nn) - c0;
-    const i64 total = (i64)nc * na;
-    // 16 columns x na rows over all threads, rows fastest, 4 in flight
-    for (i64 e0 = threadIdx.x; e0 < total; e0 += 4 * TPB) {
-        double v[4];
+    // 16 columns x na rows: rows over the threads, four columns in flight
+    for (int c = 0; c < nc; c += 4)
+        for (int i = threadIdx.x; i < na; i += TPB) {
+            double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const i64 e = e0 + (i64)u * TPB;
-            v[u] = (e < total) ? S[(e % na) + (i64)(c0 + e / na) * na] : 0.0;
-        }
+            for (int u = 0; u < 4; ++u)
+                v[u] = (c + u < nc) ? S[i + (i64)(c0 + c + u) * na] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const i64 e = e0 + (i64)u * TPB;
-            if (e < total) blk[(e % na) + (i64)(c0 + e / na) * nj] = -v[u];
+            for (int u = 0; u < 4; ++u)
+                if (c + u < nc) blk[i + (i64)(c0 + c + u) * nj] = -v[u];
         }
-    }
 }
 
 // scratch[sptr[k] + i + j*nj] = blk[i + j*nj] for the rows i < nn of the
