@@ -257,8 +257,9 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
 // and Li held in LDS; lanes walk rows for the reductions and the copies and
 // own a column each in the reflections, the solve and the rank-1 updates.
 // The reflections and the (1,1) update follow k_update_factor's arithmetic
-// (the thread-per-column path, the four partial sums); the solve runs down
-// the columns with the rows across the lanes.
+// (the thread-per-column path, the four partial sums); short bands sum
+// their norm in every lane, and the solve runs down the columns with the
+// rows across the lanes, multiplying by reciprocal pivots.
 // Returns false after recording a failed reversed Cholesky.
 // FL / KL, when given, hold the parent's front F (FL[i][j] = F(i, j)) and the
 // factor Lk (KL[i][j] = Lk(i, j)) in LDS in place of global memory.
@@ -279,8 +280,10 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     // eight columns at a time, loads first (from a valid address also where
     // the entry is zero), so that they are in flight together
     if (FL) {
+#pragma unroll 8
         for (int j = 0; j < N; ++j)
             Ls[l][j] = (j < N1 && l < N && l >= j) ? FL[rl][s_r[j]] : 0.0;
+#pragma unroll 8
         for (int j = 0; j < N2; ++j) {
             const int cj = s_r[N1 + j] - nn;
             Cs[l][j] = (l < na && l >= cj) ? KL[l][cj] : 0.0;
@@ -321,7 +324,13 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
         if (na - jj - (js + 1) == 0) continue;
         const int piv = na - 1 - js;
         const double x = (l >= jj && l < piv) ? Cs[l][col] : 0.0;
-        const double xn = sqrt(wave_sum(x * x));
+        double ss = 0.0;
+        if (piv - jj <= 8) {  // short band: every lane sums it (no reduction)
+            for (int i = jj; i < piv; ++i) ss += Cs[i][col] * Cs[i][col];
+        } else {
+            ss = wave_sum(x * x);
+        }
+        const double xn = sqrt(ss);
         double alpha = Cs[piv][col];
         if (fabs(alpha) <= PIVOT_NOISE * xn) alpha = 0.0;
         if (xn == 0.0) continue;
@@ -341,15 +350,18 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
         }
         __syncthreads();
     }
+#pragma unroll 8
     for (int j = 0; j < N2; ++j)
         if (l >= j && l < N2) Ls[N1 + l][N1 + j] = Cs[na - N2 + l][j];
     __syncthreads();
     // Li21 from Li22^T Li21 = B, column by column, lane l holding row l: once
     // x_i is known, rows l < i subtract Li22(i, l) x_i
+    // (multiplied by the reciprocal pivots, computed off the dependent chain)
+    const double rdg = l < N2 ? 1.0 / Ls[N1 + l][N1 + l] : 0.0;
     for (int c = 0; c < N1; ++c) {
         double z = l < N2 ? Ls[N1 + l][c] : 0.0;
         for (int i = N2 - 1; i >= 0; --i) {
-            const double zi = readlane_d(z, i) / Ls[N1 + i][N1 + i];
+            const double zi = readlane_d(z, i) * readlane_d(rdg, i);
             if (l == i) z = zi;
             if (l < i) z -= Ls[N1 + i][N1 + l] * zi;
         }
@@ -390,6 +402,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
         }
         __syncthreads();
     }
+#pragma unroll 8
     for (int j = 0; j < N; ++j)
         if (l < N) Li[l + (i64)j * N] = Ls[l][j];
     __syncthreads();  // the LDS tiles are free again
@@ -421,8 +434,9 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
 // them there, and the chain child's factor stays in LDS as the next front's
 // Lk (it is also stored to the update arena, as every child's is).  The step:
 //   X21 <- Lk^{-T} X21, X11 -= X21^T X21, X21 <- -Lk^{-1} X21 (substitution
-//   down each column, the rows across the lanes), X11 = Lr^T Lr (reversed
-//   Cholesky) and L11 = Lr^{-1} (forward substitution, a column per lane).
+//   down each column, the rows across the lanes, by reciprocal pivots),
+//   X11 = Lr^T Lr (reversed Cholesky) and L11 = Lr^{-1} (forward
+//   substitution, a column per lane).
 // A zero pivot of Lk or a non-positive one of the reversed Cholesky records
 // a failure and ends the chain.
 __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __restrict__ items,
@@ -485,10 +499,12 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
             }
             // Lk^T Z = X21 and, below, Lk W = Z: column by column, lane l
             // holding row l
+            // (multiplied by the reciprocal pivots)
+            const double rdg = l < na ? 1.0 / V[l][l] : 0.0;
             for (int c = 0; c < nn; ++c) {
                 double z = l < na ? B[nn + l][c] : 0.0;
                 for (int i = na - 1; i >= 0; --i) {
-                    const double zi = readlane_d(z, i) / V[i][i];
+                    const double zi = readlane_d(z, i) * readlane_d(rdg, i);
                     if (l == i) z = zi;
                     if (l < i) z -= V[i][l] * zi;
                 }
@@ -506,7 +522,7 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
             for (int c = 0; c < nn; ++c) {
                 double w = l < na ? B[nn + l][c] : 0.0;
                 for (int i = 0; i < na; ++i) {
-                    const double wi = readlane_d(w, i) / V[i][i];
+                    const double wi = readlane_d(w, i) * readlane_d(rdg, i);
                     if (l == i) w = wi;
                     if (l > i && l < na) w -= V[l][i] * wi;
                 }
