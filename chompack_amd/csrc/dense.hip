@@ -8,16 +8,18 @@
 //     beta = -sign(alpha) ||[alpha; x]||), giving Li22.  Pivots can vanish
 //     structurally; a pivot below PIVOT_NOISE ||x|| is taken as +0, the
 //     exact-arithmetic value, so the signs of Li22's rows do not follow
-//     rounding noise;
-//   * Li21 solves Li22^T Li21 = B (back substitution, one column per thread);
+//     rounding noise.  The reflections go in blocks of UF_NBH: the block's
+//     own columns take them one by one, the columns left of it all at once
+//     in compact WY form;
+//   * Li21 solves Li22^T Li21 = B;
 //   * Li11 is the reversed Cholesky factor of A - Li21^T Li21
 //     (A = Li11^T Li11, eliminated from the last row up).
 // All work is in global memory (workspace C in the scratch buffer); the
 // sequential steps are separated by workgroup barriers.  The steps are
 // latency-bound chains (a dependent FP64 operation costs tens of cycles on
-// one wave), so the workgroup has 16 waves: a reflection updates 16 columns
-// at once, the dot products keep two partial sums, and the triangular solve
-// runs row by row over all columns at once instead of one column per thread.
+// one wave), so the workgroup has 16 waves.  The planner hands the solve
+// and the (1,1) block of tasks with long separators (and the reversed
+// Cholesky of large (1,1) blocks) to device-wide launches (UFTask flags).
 #include "common.h"
 #include "kernels.h"
 
@@ -25,6 +27,8 @@
 #define UF_WAVES (UF_TPB / 64)
 #define UF_ROWMAX 4096  // rows staged in LDS (longer rows are read from global memory)
 #define PIVOT_NOISE 1e-13
+#define UF_NBH 8       // reflections per block
+#define UF_VMAX 1024   // rows of a block's V held in LDS
 
 // v of lane `lane` (uniform across the wave)
 __device__ inline double readlane_d(double v, int lane) {
@@ -54,6 +58,9 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
     __shared__ double red[UF_WAVES];
     __shared__ double s_tau, s_beta, s_scal, s_piv;
     __shared__ double rowbuf[UF_ROWMAX];
+    __shared__ double vbuf[UF_NBH][UF_VMAX];
+    __shared__ double gram[UF_NBH][UF_NBH], tmat[UF_NBH][UF_NBH], s_btau[UF_NBH];
+    __shared__ int s_bjj[UF_NBH];
     if (tid == 0) {
         int c = 0;
         while (c < N && r[c] < nn) ++c;
@@ -74,97 +81,179 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
     }
     __syncthreads();
     if (N2 > 0) {
-        for (int js = 0; js < N2; ++js) {
-            const int col = N2 - 1 - js;
-            const int jj = r[N - 1 - js] - nn;
-            if (na - jj - (js + 1) == 0) continue;
-            const int piv = na - 1 - js;
-            double* cc = C + (i64)col * na;
-            double s = 0.0;
-            for (int i = jj + tid; i < piv; i += UF_TPB) s += cc[i] * cc[i];
-            s = wave_sum(s);
-            if (lane == 0) red[wv] = s;
-            __syncthreads();
-            if (tid == 0) {
-                double ss = 0.0;
-                for (int w = 0; w < UF_WAVES; ++w) ss += red[w];
-                const double xn = sqrt(ss);
-                double alpha = cc[piv];
-                if (fabs(alpha) <= PIVOT_NOISE * xn) alpha = 0.0;
-                if (xn == 0.0) {
-                    s_tau = 0.0;
-                } else {
-                    const double beta = -copysign(hypot(alpha, xn), alpha);
-                    s_tau = (beta - alpha) / beta;
-                    s_beta = beta;
-                    s_scal = 1.0 / (alpha - beta);
+        // Blocks of UF_NBH consecutive reflections.  Within a block each
+        // reflection is formed and applied to the block's own columns in
+        // turn; the columns left of the block then take the whole block at
+        // once as H_{nb-1} ... H_0 = I - V T^T V^T (T upper triangular, the
+        // LAPACK dlarft forward recurrence), one wave per column.  A block
+        // whose bands span more than UF_VMAX rows applies its reflections one
+        // by one instead.
+        for (int js0 = 0; js0 < N2; js0 += UF_NBH) {
+            const int nb = min(UF_NBH, N2 - js0);
+            const int colmin = N2 - js0 - nb;  // leftmost column of the block
+            for (int b = 0; b < nb; ++b) {
+                const int js = js0 + b;
+                const int col = N2 - 1 - js;
+                const int jj = r[N - 1 - js] - nn;
+                if (tid == 0) {
+                    s_btau[b] = 0.0;
+                    s_bjj[b] = jj;
                 }
+                if (na - jj - (js + 1) == 0) continue;
+                const int piv = na - 1 - js;
+                double* cc = C + (i64)col * na;
+                double s = 0.0;
+                for (int i = jj + tid; i < piv; i += UF_TPB) s += cc[i] * cc[i];
+                s = wave_sum(s);
+                if (lane == 0) red[wv] = s;
+                __syncthreads();
+                if (tid == 0) {
+                    double ss = 0.0;
+                    for (int w = 0; w < UF_WAVES; ++w) ss += red[w];
+                    const double xn = sqrt(ss);
+                    double alpha = cc[piv];
+                    if (fabs(alpha) <= PIVOT_NOISE * xn) alpha = 0.0;
+                    if (xn == 0.0) {
+                        s_tau = 0.0;
+                    } else {
+                        const double beta = -copysign(hypot(alpha, xn), alpha);
+                        s_tau = (beta - alpha) / beta;
+                        s_beta = beta;
+                        s_scal = 1.0 / (alpha - beta);
+                    }
+                    s_btau[b] = s_tau;
+                }
+                __syncthreads();
+                const double tau = s_tau;
+                if (tau == 0.0) continue;
+                const double scal = s_scal;
+                for (int i = jj + tid; i < piv; i += UF_TPB) cc[i] *= scal;
+                if (tid == 0) cc[piv] = s_beta;
+                __syncthreads();
+                // H = I - tau v v^T, v = [cc[jj:piv]; 1], on rows jj..piv of
+                // the block's columns colmin..col-1.  A short band (most
+                // reflections of a nearly triangular C zero one or two
+                // entries) takes one thread per column, a long one one wave
+                // per column.
+                if (piv - jj < 32) {
+                    for (int c = colmin + tid; c < col; c += UF_TPB) {
+                        double* cd = C + (i64)c * na;
+                        double d = cd[piv];
+                        for (int i = jj; i < piv; ++i) d += cc[i] * cd[i];
+                        const double f = tau * d;
+                        for (int i = jj; i < piv; ++i) cd[i] -= f * cc[i];
+                        cd[piv] -= f;
+                    }
+                } else {
+                    for (int c = colmin + wv; c < col; c += UF_WAVES) {
+                        double* cd = C + (i64)c * na;
+                        double d0 = 0.0, d1 = 0.0;
+                        int i = jj + lane;
+                        for (; i + 64 < piv; i += 128) {
+                            d0 += cc[i] * cd[i];
+                            d1 += cc[i + 64] * cd[i + 64];
+                        }
+                        if (i < piv) d0 += cc[i] * cd[i];
+                        const double d = wave_sum(d0 + d1) + cd[piv];
+                        const double f = tau * d;
+                        for (int i = jj + lane; i < piv; i += 64) cd[i] -= f * cc[i];
+                        if (lane == 0) cd[piv] -= f;
+                    }
+                }
+                __syncthreads();
             }
             __syncthreads();
-            const double tau = s_tau;
-            if (tau == 0.0) continue;
-            const double scal = s_scal;
-            for (int i = jj + tid; i < piv; i += UF_TPB) cc[i] *= scal;
-            if (tid == 0) cc[piv] = s_beta;
-            __syncthreads();
-            // H = I - tau v v^T, v = [cc[jj:piv]; 1], on rows jj..piv of columns < col.
-            // A short band (most reflections of a nearly triangular C zero one
-            // or two entries) takes one thread per column, so the columns'
-            // memory round trips overlap; a long one one wave per column.
-            if (piv - jj < 32) {
-                for (int c = tid; c < col; c += UF_TPB) {
-                    double* cd = C + (i64)c * na;
-                    double d = cd[piv];
-                    for (int i = jj; i < piv; ++i) d += cc[i] * cd[i];
-                    const double f = tau * d;
-                    for (int i = jj; i < piv; ++i) cd[i] -= f * cc[i];
-                    cd[piv] -= f;
+            if (colmin == 0) continue;
+            const int hi = na - 1 - js0;  // the block's first pivot row
+            int lo = hi;
+            for (int b = 0; b < nb; ++b)
+                if (s_btau[b] != 0.0) lo = min(lo, s_bjj[b]);
+            if (lo == hi) continue;  // no reflection in the block
+            const int len = hi - lo + 1;
+            if (len <= UF_VMAX) {
+                // V over rows lo..hi: v_b zero outside its band, 1 at its pivot
+                for (int e = tid; e < UF_NBH * UF_VMAX; e += UF_TPB) {
+                    const int b = e / UF_VMAX, i = e % UF_VMAX, row = lo + i;
+                    double v = 0.0;
+                    if (b < nb && i < len && s_btau[b] != 0.0) {
+                        const int pb = na - 1 - (js0 + b);
+                        if (row == pb) v = 1.0;
+                        else if (row >= s_bjj[b] && row < pb) v = C[row + (i64)(N2 - 1 - js0 - b) * na];
+                    }
+                    vbuf[b][i] = v;
                 }
-            } else if (piv - jj <= UF_ROWMAX) {
-                // v staged in LDS; a wave updates two columns at once so that
-                // twice the loads are in flight
-                for (int i = tid; i < piv - jj; i += UF_TPB) rowbuf[i] = cc[jj + i];
                 __syncthreads();
-                const double* v = rowbuf - jj;
-                for (int c = wv; c < col; c += 2 * UF_WAVES) {
-                    const int c2 = c + UF_WAVES;
-                    const bool two = c2 < col;
-                    double* ca = C + (i64)c * na;
-                    double* cb = C + (i64)(two ? c2 : c) * na;
-                    double da = 0.0, db = 0.0;
-#pragma unroll 4
-                    for (int i = jj + lane; i < piv; i += 64) {
-                        const double vi = v[i];
-                        da += vi * ca[i];
-                        db += vi * cb[i];
+                for (int q = wv; q < UF_NBH * UF_NBH; q += UF_WAVES) {
+                    const int a = q / UF_NBH, b = q % UF_NBH;
+                    if (a >= b) continue;
+                    double g = 0.0;
+                    for (int i = lane; i < len; i += 64) g += vbuf[a][i] * vbuf[b][i];
+                    g = wave_sum(g);
+                    if (lane == 0) gram[a][b] = g;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    for (int b = 0; b < UF_NBH; ++b) {
+                        const double tb = b < nb ? s_btau[b] : 0.0;
+                        for (int a = 0; a < b; ++a) {
+                            double g = 0.0;
+                            for (int k = a; k < b; ++k) g += tmat[a][k] * gram[k][b];
+                            tmat[a][b] = -tb * g;
+                        }
+                        tmat[b][b] = tb;
+                        for (int a = b + 1; a < UF_NBH; ++a) tmat[a][b] = 0.0;
                     }
-                    const double fa = tau * (wave_sum(da) + ca[piv]);
-                    const double fb = tau * (wave_sum(db) + cb[piv]);
-#pragma unroll 4
-                    for (int i = jj + lane; i < piv; i += 64) {
-                        const double vi = v[i];
-                        ca[i] -= fa * vi;
-                        if (two) cb[i] -= fb * vi;
+                }
+                __syncthreads();
+                for (int c = wv; c < colmin; c += UF_WAVES) {
+                    // w = V^T c, y = T^T w, c -= V y; row lo + i in lane i % 64
+                    double* cd = C + (i64)c * na + lo;
+                    double w[UF_NBH];
+#pragma unroll
+                    for (int b = 0; b < UF_NBH; ++b) w[b] = 0.0;
+#pragma unroll 2
+                    for (int i = lane; i < len; i += 64) {
+                        const double x = cd[i];
+#pragma unroll
+                        for (int b = 0; b < UF_NBH; ++b) w[b] += vbuf[b][i] * x;
                     }
-                    if (lane == 0) {
-                        ca[piv] -= fa;
-                        if (two) cb[piv] -= fb;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                        for (int b = 0; b < UF_NBH; ++b) w[b] += __shfl_xor(w[b], o, 64);
+                    // y = T^T w: lane b < UF_NBH forms y_b from column b of T
+                    double yl = 0.0;
+#pragma unroll
+                    for (int a = 0; a < UF_NBH; ++a) yl += tmat[a][lane & (UF_NBH - 1)] * w[a];
+#pragma unroll
+                    for (int b = 0; b < UF_NBH; ++b) w[b] = readlane_d(yl, b);
+#pragma unroll 2
+                    for (int i = lane; i < len; i += 64) {
+                        double x = cd[i];
+#pragma unroll
+                        for (int b = 0; b < UF_NBH; ++b) x -= vbuf[b][i] * w[b];
+                        cd[i] = x;
                     }
                 }
             } else {
-                for (int c = wv; c < col; c += UF_WAVES) {
+                // one reflection after another; row i of a column always in
+                // lane i % 64, so no lane reads what another lane wrote
+                for (int c = wv; c < colmin; c += UF_WAVES) {
                     double* cd = C + (i64)c * na;
-                    double d0 = 0.0, d1 = 0.0;
-                    int i = jj + lane;
-                    for (; i + 64 < piv; i += 128) {
-                        d0 += cc[i] * cd[i];
-                        d1 += cc[i + 64] * cd[i + 64];
+                    for (int b = 0; b < nb; ++b) {
+                        const double tau = s_btau[b];
+                        if (tau == 0.0) continue;
+                        const int jb = s_bjj[b], pb = na - 1 - (js0 + b);
+                        const double* cc = C + (i64)(N2 - 1 - js0 - b) * na;
+                        double d0 = 0.0;
+                        for (int i = (jb & ~63) + lane; i < pb; i += 64)
+                            if (i >= jb) d0 += cc[i] * cd[i];
+                        if (lane == (pb & 63)) d0 += cd[pb];
+                        const double f = tau * wave_sum(d0);
+                        for (int i = (jb & ~63) + lane; i < pb; i += 64)
+                            if (i >= jb) cd[i] -= f * cc[i];
+                        if (lane == (pb & 63)) cd[pb] -= f;
                     }
-                    if (i < piv) d0 += cc[i] * cd[i];
-                    const double d = wave_sum(d0 + d1) + cd[piv];
-                    const double f = tau * d;
-                    for (int i = jj + lane; i < piv; i += 64) cd[i] -= f * cc[i];
-                    if (lane == 0) cd[piv] -= f;
                 }
             }
             __syncthreads();

@@ -203,7 +203,7 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
     //     J A J = M M^T by the blocked potrf (J the reversal), Li11 = J M^T J,
     //     through reversed-transpose copies.
     // Smaller tasks are latency chains that stay inside the kernel.
-    constexpr int UF_BLOCKED_N1 = 192, UF_SOLVE_N1 = 32, UF_SOLVE_N2 = 32;
+    constexpr int UF_BLOCKED_N1 = 192, UF_SOLVE_N1 = 1, UF_SOLVE_N2 = 64;
     std::vector<MatOp> rt1, rt2;
     std::vector<Mat> R, L22s, Xs;
     std::vector<i64> Ws;
