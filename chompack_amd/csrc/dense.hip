@@ -449,10 +449,14 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     const double rdg = l < N2 ? 1.0 / Ls[N1 + l][N1 + l] : 0.0;
     for (int c = 0; c < N1; ++c) {
         double z = l < N2 ? Ls[N1 + l][c] : 0.0;
+        const int lc = min(N1 + l, 64);  // Li22 column of lane l (clamped into the tile)
+        double sv = Ls[N1 + max(N2 - 1, 0)][lc];
         for (int i = N2 - 1; i >= 0; --i) {
+            const double sn = Ls[N1 + max(i - 1, 0)][lc];  // next row's, off the chain
             const double zi = readlane_d(z, i) * readlane_d(rdg, i);
             if (l == i) z = zi;
-            if (l < i) z -= Ls[N1 + i][N1 + l] * zi;
+            if (l < i) z -= sv * zi;
+            sv = sn;
         }
         if (l < N2) Ls[N1 + l][c] = z;
     }
@@ -592,10 +596,13 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
             const double rdg = l < na ? 1.0 / V[l][l] : 0.0;
             for (int c = 0; c < nn; ++c) {
                 double z = l < na ? B[nn + l][c] : 0.0;
+                double sv = V[na - 1][l];
                 for (int i = na - 1; i >= 0; --i) {
+                    const double sn = V[max(i - 1, 0)][l];  // next row's, off the chain
                     const double zi = readlane_d(z, i) * readlane_d(rdg, i);
                     if (l == i) z = zi;
-                    if (l < i) z -= V[i][l] * zi;
+                    if (l < i) z -= sv * zi;
+                    sv = sn;
                 }
                 if (l < na) B[nn + l][c] = z;
             }
@@ -610,10 +617,13 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
             __syncthreads();
             for (int c = 0; c < nn; ++c) {
                 double w = l < na ? B[nn + l][c] : 0.0;
+                double sv = V[l][0];
                 for (int i = 0; i < na; ++i) {
+                    const double sn = V[l][min(i + 1, 63)];  // next column's, off the chain
                     const double wi = readlane_d(w, i) * readlane_d(rdg, i);
                     if (l == i) w = wi;
-                    if (l > i && l < na) w -= V[l][i] * wi;
+                    if (l > i && l < na) w -= sv * wi;
+                    sv = sn;
                 }
                 if (l < na) B[nn + l][c] = -w;
             }
