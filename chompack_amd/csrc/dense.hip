@@ -205,34 +205,55 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
                     }
                 }
                 __syncthreads();
-                for (int c = wv; c < colmin; c += UF_WAVES) {
-                    // w = V^T c, y = T^T w, c -= V y; row lo + i in lane i % 64
-                    double* cd = C + (i64)c * na + lo;
-                    double w[UF_NBH];
+                for (int c = wv; c < colmin; c += 2 * UF_WAVES) {
+                    // w = V^T c, y = T^T w, c -= V y for columns c and c2 at
+                    // once (twice the loads in flight); row lo + i in lane i % 64
+                    const int c2 = c + UF_WAVES;
+                    const bool two = c2 < colmin;
+                    double* ca = C + (i64)c * na + lo;
+                    double* cb = C + (i64)(two ? c2 : c) * na + lo;
+                    double wa[UF_NBH], wb[UF_NBH];
 #pragma unroll
-                    for (int b = 0; b < UF_NBH; ++b) w[b] = 0.0;
-#pragma unroll 2
+                    for (int b = 0; b < UF_NBH; ++b) wa[b] = wb[b] = 0.0;
                     for (int i = lane; i < len; i += 64) {
-                        const double x = cd[i];
+                        const double xa = ca[i], xb = cb[i];
 #pragma unroll
-                        for (int b = 0; b < UF_NBH; ++b) w[b] += vbuf[b][i] * x;
+                        for (int b = 0; b < UF_NBH; ++b) {
+                            const double v = vbuf[b][i];
+                            wa[b] += v * xa;
+                            wb[b] += v * xb;
+                        }
                     }
 #pragma unroll
                     for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-                        for (int b = 0; b < UF_NBH; ++b) w[b] += __shfl_xor(w[b], o, 64);
+                        for (int b = 0; b < UF_NBH; ++b) {
+                            wa[b] += __shfl_xor(wa[b], o, 64);
+                            wb[b] += __shfl_xor(wb[b], o, 64);
+                        }
                     // y = T^T w: lane b < UF_NBH forms y_b from column b of T
-                    double yl = 0.0;
+                    double ya = 0.0, yb = 0.0;
 #pragma unroll
-                    for (int a = 0; a < UF_NBH; ++a) yl += tmat[a][lane & (UF_NBH - 1)] * w[a];
+                    for (int a = 0; a < UF_NBH; ++a) {
+                        const double ta = tmat[a][lane & (UF_NBH - 1)];
+                        ya += ta * wa[a];
+                        yb += ta * wb[a];
+                    }
 #pragma unroll
-                    for (int b = 0; b < UF_NBH; ++b) w[b] = readlane_d(yl, b);
-#pragma unroll 2
+                    for (int b = 0; b < UF_NBH; ++b) {
+                        wa[b] = readlane_d(ya, b);
+                        wb[b] = readlane_d(yb, b);
+                    }
                     for (int i = lane; i < len; i += 64) {
-                        double x = cd[i];
+                        double xa = ca[i], xb = cb[i];
 #pragma unroll
-                        for (int b = 0; b < UF_NBH; ++b) x -= vbuf[b][i] * w[b];
-                        cd[i] = x;
+                        for (int b = 0; b < UF_NBH; ++b) {
+                            const double v = vbuf[b][i];
+                            xa -= v * wa[b];
+                            xb -= v * wb[b];
+                        }
+                        ca[i] = xa;
+                        if (two) cb[i] = xb;
                     }
                 }
             } else {
