@@ -138,6 +138,33 @@ def two_cliques(n1=300, n2=100, n3=200, seed=6):
     return spd_from_edges(n1 + n2 + n3, I, J, rng)
 
 
+def caterpillar(spine=400, bandwidth=6, arrows=10, seed=7):
+    """A band of half-bandwidth `bandwidth` on `spine` vertices, each with a
+    pendant leaf, plus `arrows` vertices joined to every spine vertex.  Leaves
+    come first in the numbering (0..spine-1), then the spine, then the arrow
+    vertices, so the natural order is a perfect elimination order: every
+    spine supernode (one column, na = bandwidth + arrows) sits alone on its
+    elimination-tree level and has two children, its leaf and the previous
+    spine supernode."""
+    rng = np.random.default_rng(seed)
+    L, A = spine, arrows
+    s = np.arange(L)
+    I = [L + s]                    # spine vertex - its leaf
+    J = [s]
+    for d in range(1, bandwidth + 1):  # band along the spine
+        I.append(L + s[d:])
+        J.append(L + s[:-d])
+    for a in range(A):             # arrow vertex - every spine vertex
+        I.append(np.full(L, 2 * L + a))
+        J.append(L + s)
+    ia = np.arange(A)
+    Ia, Ja = np.meshgrid(ia, ia, indexing="ij")
+    m = Ia > Ja
+    I.append(2 * L + Ia[m])        # the arrow vertices form a clique
+    J.append(2 * L + Ja[m])
+    return spd_from_edges(2 * L + A, np.concatenate(I), np.concatenate(J), rng)
+
+
 def reference_test_matrix(n=31, density=0.15, seed=1):
     """The matrix of the reference's numeric test suite
     (tests/test_numeric.py:8-15 of the reference): ``random.sample`` of

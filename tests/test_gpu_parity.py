@@ -22,8 +22,11 @@ PROBLEMS = {
     "grid2d_64": lambda: problems.grid2d(64, seed=0),
     "grid3d_12": lambda: problems.grid3d(12, seed=1),
     "two_cliques": lambda: problems.two_cliques(),
+    # an elimination-tree chain of one-column fronts with two children each
+    # (the completion's chain kernel with non-chain children)
+    "caterpillar": lambda: problems.caterpillar(),
 }
-ORDER = {"two_cliques": None}
+ORDER = {"two_cliques": None, "caterpillar": None}
 
 
 def _setup(name, small_max):
