@@ -16,8 +16,9 @@
 // target entry is updated by exactly one workgroup, in the reference's order,
 // without atomics.  (llt.c:36-46 adds into an F22 already holding the
 // product, so zeroing is optional.)  Relative indices increase, so a child's
-// rows and columns landing in the tile are contiguous index ranges; each
-// thread owns one child row and keeps four of its columns in flight.
+// rows and columns landing in the tile are contiguous index ranges (their
+// bounds come with the tile list); each thread owns one child row and keeps
+// its up to 16 columns in flight.
 // Target (row, col) with col < nn lives in the blkval block (ld nj); with
 // col >= nn in k's update slot (ld na).
 __global__ __launch_bounds__(TPB) void k_extadd_tiles(
@@ -39,33 +40,21 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
                 U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
     }
     // The child index ranges landing in the tile (first child index with
-    // r >= c0, c1, r0, r1), for up to TPB / 4 children at a time: one thread
-    // per (child, bound), so the dependent binary-search loads of all
-    // children overlap instead of running one after another.
-    __shared__ int s_bnd[TPB / 4][4];
+    // r >= c0, c1, r0, r1), found on the host when the plan was built: one
+    // int4 per child in reverse child order, at tiles + t.w.
+    const int4* bnd = tiles + t.w;
     const int nch = f.chend - f.chbeg;
-    for (int b0 = 0; b0 < nch; b0 += TPB / 4) {
-        __syncthreads();
-        {
-            const int cb = tid >> 2, which = tid & 3;
-            if (b0 + cb < nch) {
-                const FrontInfo g = fi[chidx[f.chend - 1 - (b0 + cb)]];
-                const int* r = relidx + g.relptr;
-                const int v = which == 0 ? c0 : (which == 1 ? c1 : (which == 2 ? r0 : r1));
-                int lo = 0, hi = g.na;
-                while (lo < hi) { const int m = (lo + hi) >> 1; if (r[m] < v) lo = m + 1; else hi = m; }
-                s_bnd[cb][which] = lo;
-            }
-        }
-        __syncthreads();
-        for (int cb = 0; cb < min(TPB / 4, nch - b0); ++cb) {
-            const int l = f.chend - 1 - (b0 + cb);  // reverse child order
+    __syncthreads();  // the zeroed entries before any child adds into them
+    {
+        for (int cb = 0; cb < nch; ++cb) {
+            const int l = f.chend - 1 - cb;  // reverse child order
             const FrontInfo g = fi[chidx[l]];
             const int N = g.na;
             const int* r = relidx + g.relptr;
             const double* Uc = upd + g.updptr;
-            const int ja = s_bnd[cb][0], jb = s_bnd[cb][1], i = s_bnd[cb][2] + tid;
-            if (i < s_bnd[cb][3]) {
+            const int4 bc = bnd[cb];
+            const int ja = bc.x, jb = bc.y, i = bc.z + tid;
+            if (i < bc.w) {
                 // at most 16 child columns land in the 16 target columns
                 // (relative indices increase): all loads first, then the
                 // stores, so the row's memory round trips overlap

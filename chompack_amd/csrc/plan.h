@@ -224,7 +224,11 @@ int read_status(chp_plan* P, hipStream_t s, i64* info);
 int ensure_side_stream(chp_plan* P);
 // Extend-add work list of the listed fronts: (front, c0, r0) for the target
 // tiles of 16 columns x 256 rows that meet the lower triangle.
-std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts);
+// Extend-add tiles of the fronts (x = front, y = c0, z = r0, w = index in the
+// returned list of the first child's bounds), followed by the bounds: per
+// tile and child (reverse child order) the first child index with relative
+// index >= c0, c1, r0, r1.  *ntiles receives the number of tiles.
+std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts, int* ntiles);
 // Extract work list of the listed fronts: (front, j0, i0) for the 16-column x
 // 256-row blocks of V_k that meet the lower triangle.
 std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& fronts);

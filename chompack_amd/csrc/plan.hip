@@ -159,13 +159,30 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     if (P->pg->split_out.empty()) P->pg->splitp_cur = P->pg->side_cur = 0;
 }
 
-std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts) {
-    std::vector<int4> et;
+std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts, int* ntiles) {
+    std::vector<int4> et, bnd;
     for (int k : fronts) {
-        const int nj = P->fi[k].nn + P->fi[k].na;
+        const FrontInfo& f = P->fi[k];
+        const int nj = f.nn + f.na;
         for (int c0 = 0; c0 < nj; c0 += 16)
-            for (int r0 = c0 & ~255; r0 < nj; r0 += 256) et.push_back(make_int4(k, c0, r0, 0));
+            for (int r0 = c0 & ~255; r0 < nj; r0 += 256) {
+                // the children's index ranges landing in the tile (first child
+                // index with relative index >= c0, c1, r0, r1), children in
+                // reverse order
+                et.push_back(make_int4(k, c0, r0, (int)bnd.size()));
+                const int v[4] = {c0, std::min(c0 + 16, nj), r0, std::min(r0 + 256, nj)};
+                for (int l = f.chend - 1; l >= f.chbeg; --l) {
+                    const FrontInfo& g = P->fi[P->h_chidx[l]];
+                    const int* r = P->h_relidx.data() + g.relptr;
+                    int b[4];
+                    for (int q = 0; q < 4; ++q) b[q] = (int)(std::lower_bound(r, r + g.na, v[q]) - r);
+                    bnd.push_back(make_int4(b[0], b[1], b[2], b[3]));
+                }
+            }
     }
+    *ntiles = (int)et.size();
+    for (int4& t : et) t.w += *ntiles;  // the bounds follow the tiles in the list
+    et.insert(et.end(), bnd.begin(), bnd.end());
     return et;
 }
 
@@ -199,8 +216,9 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     }
     if (large.empty()) return;
     P->n_large_chol += (int)large.size();
-    const std::vector<int4> et = extadd_tiles(P, large);
-    P->chol.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1});
+    int net = 0;
+    const std::vector<int4> et = extadd_tiles(P, large, &net);
+    P->chol.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1});
     // Two-level blocking: 64-wide inner panels (diagonal factor, panel solve,
     // update confined to the current outer panel), then one trailing update
     // with K = the outer panel width for everything right of it.
@@ -512,8 +530,9 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
     P->llt.push_back({K_COPY_TOP, (int)ct.size(), 256, 0, P->pg->lists.add(ct), 0});
     push_gemm(P, P->llt, g1);
     push_gemm(P, P->llt, g2);
-    const std::vector<int4> et = extadd_tiles(P, large);
-    P->llt.push_back({K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0});
+    int net = 0;
+    const std::vector<int4> et = extadd_tiles(P, large, &net);
+    P->llt.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 0});
 }
 
 int alloc_doubles(double** d, i64 len) {
@@ -823,6 +842,11 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     P->upd_len = upd;
     P->dinv_len = dinv;
     P->solve_len = wun;
+    // host copies of the index maps (the sweep builders read them)
+    P->h_relidx.resize(relptr[nsn]);
+    P->h_chidx.resize(chptr[nsn]);
+    for (i64 i = 0; i < relptr[nsn]; ++i) P->h_relidx[i] = (int)relidx[i];
+    for (i64 i = 0; i < chptr[nsn]; ++i) P->h_chidx[i] = (int)chidx[i];
     for (int L = 0; L < H; ++L) build_chol_level(P, P->bylevel[L]);
     P->chol.push_back({K_JOIN, 0, 0, 0, 0, 2});
     // scratch of the projected inverse: X = L11^{-1} of every large front,
@@ -849,9 +873,6 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
     for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
     P->scratch_len = scratch;
     // device copies
-    std::vector<int> rel32(relptr[nsn]), ch32(chptr[nsn]);
-    for (i64 i = 0; i < relptr[nsn]; ++i) rel32[i] = (int)relidx[i];
-    for (i64 i = 0; i < chptr[nsn]; ++i) ch32[i] = (int)chidx[i];
     std::vector<i64> pv(perm, perm + n), sp(snptr, snptr + nsn + 1);
     std::vector<int> colfront(n), colidx(n);
     for (i64 k = 0; k < nsn; ++k)
@@ -861,10 +882,8 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         }
     int rc = 0;
     rc |= upload(&P->d_fi, P->fi);
-    rc |= upload(&P->d_relidx, rel32);
-    rc |= upload(&P->d_chidx, ch32);
-    P->h_relidx.swap(rel32);
-    P->h_chidx.swap(ch32);
+    rc |= upload(&P->d_relidx, P->h_relidx);
+    rc |= upload(&P->d_chidx, P->h_chidx);
     rc |= upload(&P->d_perm, pv);
     rc |= upload(&P->d_snptr, sp);
     rc |= upload(&P->d_dinvptr, P->h_dinvptr);

@@ -388,9 +388,10 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
 // One level of _Y2K (hessian.c:3-101), leaves to root, for one direction.
 void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv) {
     const double alpha = inv ? 1.0 : -1.0;
-    const std::vector<int4> et = extadd_tiles(P, fr);
+    int net = 0;
+    const std::vector<int4> et = extadd_tiles(P, fr, &net);
     if (!inv) {
-        Launch L{K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 1};
+        Launch L{K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1};
         seq.push_back(L);
     } else {
         std::vector<MatOp> z;
@@ -420,7 +421,7 @@ void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     push_gemm(P, seq, g2);
     push_gemm(P, seq, g3);
     if (inv) {
-        Launch L{K_EXTADD, (int)et.size(), 256, 0, P->pg->lists.add(et), 0};
+        Launch L{K_EXTADD, net, 256, 0, P->pg->lists.add(et), 0};
         seq.push_back(L);
     }
 }
