@@ -40,19 +40,18 @@ __global__ __launch_bounds__(TPB) void k_extadd_tiles(
                 U[(tr - nn) + (i64)(tc - nn) * na] = 0.0;
     }
     // The child index ranges landing in the tile (first child index with
-    // r >= c0, c1, r0, r1), found on the host when the plan was built: one
-    // int4 per child in reverse child order, at tiles + t.w.
+    // r >= c0, c1, r0, r1) and the child's na, relptr and updptr, found on
+    // the host when the plan was built: three int4 per child in reverse
+    // child order, at tiles + t.w.
     const int4* bnd = tiles + t.w;
     const int nch = f.chend - f.chbeg;
     __syncthreads();  // the zeroed entries before any child adds into them
     {
         for (int cb = 0; cb < nch; ++cb) {
-            const int l = f.chend - 1 - cb;  // reverse child order
-            const FrontInfo g = fi[chidx[l]];
-            const int N = g.na;
-            const int* r = relidx + g.relptr;
-            const double* Uc = upd + g.updptr;
-            const int4 bc = bnd[cb];
+            const int4 bc = bnd[3 * cb], g1 = bnd[3 * cb + 1], g2 = bnd[3 * cb + 2];
+            const int N = g1.x;
+            const int* r = relidx + (i64)(((unsigned long long)(unsigned)g1.w << 32) | (unsigned)g1.z);
+            const double* Uc = upd + (i64)(((unsigned long long)(unsigned)g2.y << 32) | (unsigned)g2.x);
             const int ja = bc.x, jb = bc.y, i = bc.z + tid;
             if (i < bc.w) {
                 // at most 16 child columns land in the 16 target columns

@@ -226,8 +226,10 @@ int ensure_side_stream(chp_plan* P);
 // tiles of 16 columns x 256 rows that meet the lower triangle.
 // Extend-add tiles of the fronts (x = front, y = c0, z = r0, w = index in the
 // returned list of the first child's bounds), followed by the bounds: per
-// tile and child (reverse child order) the first child index with relative
-// index >= c0, c1, r0, r1.  *ntiles receives the number of tiles.
+// tile and child (reverse child order) three int4, the first child index
+// with relative index >= c0, c1, r0, r1, then (na, 0, relptr lo, relptr hi)
+// and (updptr lo, updptr hi, 0, 0) of the child.  *ntiles receives the
+// number of tiles.
 std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts, int* ntiles);
 // Extract work list of the listed fronts: (front, j0, i0) for the 16-column x
 // 256-row blocks of V_k that meet the lower triangle.

@@ -177,6 +177,10 @@ std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts
                     int b[4];
                     for (int q = 0; q < 4; ++q) b[q] = (int)(std::lower_bound(r, r + g.na, v[q]) - r);
                     bnd.push_back(make_int4(b[0], b[1], b[2], b[3]));
+                    // and the child's own fields, so the kernel needs no
+                    // dependent loads of chidx and FrontInfo
+                    bnd.push_back(make_int4(g.na, 0, (int)(g.relptr & 0xffffffff), (int)(g.relptr >> 32)));
+                    bnd.push_back(make_int4((int)(g.updptr & 0xffffffff), (int)(g.updptr >> 32), 0, 0));
                 }
             }
     }
