@@ -375,7 +375,7 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
 // factor Lk (KL[i][j] = Lk(i, j)) in LDS in place of global memory.
 __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __restrict__ relidx,
                                    unsigned long long* status, double (*Cs)[65],
-                                   double (*Ls)[65], int* s_r, const double (*FL)[65] = nullptr,
+                                   double (*Ls)[65], const double (*FL)[65] = nullptr,
                                    const double (*KL)[65] = nullptr) {
     const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
     const int l = threadIdx.x;
@@ -383,8 +383,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     const double* F = bt.p[t.fbuf] + t.foff;
     const double* Lk = bt.p[1] + t.lkoff;
     double* Li = bt.p[1] + t.outoff;
-    const int rl = l < N ? r[l] : 0;
-    s_r[l] = rl;
+    const int rl = l < N ? r[l] : 0;  // lane j's r[j] is broadcast by readlane below
     const int N1 = __popcll(__ballot(l < N && rl < nn)), N2 = N - N1;
     __syncthreads();
     // eight columns at a time, loads first (from a valid address also where
@@ -392,10 +391,10 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     if (FL) {
 #pragma unroll 8
         for (int j = 0; j < N; ++j)
-            Ls[l][j] = (j < N1 && l < N && l >= j) ? FL[rl][s_r[j]] : 0.0;
+            Ls[l][j] = (j < N1 && l < N && l >= j) ? FL[rl][__builtin_amdgcn_readlane(rl, j)] : 0.0;
 #pragma unroll 8
         for (int j = 0; j < N2; ++j) {
-            const int cj = s_r[N1 + j] - nn;
+            const int cj = __builtin_amdgcn_readlane(rl, N1 + j) - nn;
             Cs[l][j] = (l < na && l >= cj) ? KL[l][cj] : 0.0;
         }
     } else {
@@ -405,7 +404,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
         for (int u = 0; u < 8; ++u) {
             const int j = j0 + u;
             const bool ok = j < N1 && l < N && l >= j;
-            v[u] = F[ok ? rl + (i64)s_r[j] * nj : 0];
+            v[u] = F[ok ? rl + (i64)__builtin_amdgcn_readlane(rl, j & 63) * nj : 0];
             v[u] = ok ? v[u] : 0.0;
         }
 #pragma unroll
@@ -417,7 +416,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int j = j0 + u;
-            const int cj = j < N2 ? s_r[N1 + j] - nn : 0;
+            const int cj = j < N2 ? __builtin_amdgcn_readlane(rl, (N1 + j) & 63) - nn : 0;
             const bool ok = j < N2 && l < na && l >= cj;
             v[u] = Lk[ok ? l + (i64)cj * na : 0];
             v[u] = ok ? v[u] : 0.0;
@@ -430,7 +429,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     __syncthreads();
     for (int js = 0; js < N2; ++js) {
         const int col = N2 - 1 - js;
-        const int jj = s_r[N - 1 - js] - nn;
+        const int jj = __builtin_amdgcn_readlane(rl, N - 1 - js) - nn;
         if (na - jj - (js + 1) == 0) continue;
         const int piv = na - 1 - js;
         const double x = (l >= jj && l < piv) ? Cs[l][col] : 0.0;
@@ -529,13 +528,12 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
                                                             unsigned long long* status) {
     __shared__ double Cs[64][65];  // C(i, j)
     __shared__ double Ls[64][65];  // Li(i, j)
-    __shared__ int s_r[64];
     const UFTask t = tasks[blockIdx.x];
     if (t.N > 64 || t.na > 64) {  // the planner sends only small tasks here
         if (threadIdx.x == 0) chp_record_failure(status, t.key, 1);
         return;
     }
-    update_factor_wave(t, bt, relidx, status, Cs, Ls, s_r);
+    update_factor_wave(t, bt, relidx, status, Cs, Ls);
 }
 
 // Completion (factored updates, completion.c:24-118) of a chain of small
@@ -559,7 +557,6 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
                                                          BufTab bt, const int* __restrict__ relidx,
                                                          unsigned long long* status) {
     __shared__ double T0[64][65], T1[64][65], T2[64][65], T3[64][65];
-    __shared__ int s_r[64];
     double (*B)[65] = T0;   // the front's columns
     double (*V)[65] = T1;   // Lk of the front
     double (*Cs)[65] = T2;  // update-factor workspace
@@ -601,7 +598,7 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
         bool keep = false;
         for (int c = 1; c <= nch; ++c) {
             const ChainItem& cu = items[it + c];
-            if (!update_factor_wave(cu.uf, bt, relidx, status, Cs, Ls, s_r, B, V)) return;
+            if (!update_factor_wave(cu.uf, bt, relidx, status, Cs, Ls, B, V)) return;
             keep = cu.kind == 2;
         }
         it += 1 + nch;
