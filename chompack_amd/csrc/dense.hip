@@ -427,7 +427,15 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
     }
     }
     __syncthreads();
-    for (int js = 0; js < N2; ++js) {
+    // Reflection js has nothing to reduce when its band is empty, i.e. when
+    // piv - jj = (na - 1 - js) - (r[N-1-js] - nn) is 0.  That difference
+    // never decreases with js (piv drops by one, r by at least one), so the
+    // empty reflections are a prefix: lane q tests reflection q, and the loop
+    // starts after the leading run.
+    const int rq = __shfl(rl, max(N - 1 - l, 0), 64);
+    const unsigned long long emp = __ballot(l < N2 && na - (rq - nn) - (l + 1) == 0);
+    const int js_first = ~emp ? __builtin_ctzll(~emp) : 64;
+    for (int js = js_first; js < N2; ++js) {
         const int col = N2 - 1 - js;
         const int jj = __builtin_amdgcn_readlane(rl, N - 1 - js) - nn;
         if (na - jj - (js + 1) == 0) continue;
