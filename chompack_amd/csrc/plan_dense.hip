@@ -431,7 +431,8 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
                      Bump& sc) {
     const double alpha = inv ? 1.0 : -1.0;
     if (inv) {
-        emit_child_updates(P, seq, fr, false, 0, sc, sc);  // before this level's update
+        Bump dv;  // unused: unfactored updates need no block inverses
+        emit_child_updates(P, seq, fr, false, 0, sc, dv);  // before this level's update
     } else {
         // this level's V from the (finished) parents
         push_tasks(P, seq, K_EXTRACT, extract_tiles(P, fr), 0);
