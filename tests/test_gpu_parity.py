@@ -132,6 +132,27 @@ def test_cholesky_failure_raises():
     assert X.is_factor is True
 
 
+@pytest.mark.parametrize("factored", [True, False])
+def test_completion_failure_raises(factored):
+    """A partial matrix with a negative diagonal entry in the middle of the
+    caterpillar's chain is not completable: the reversed Cholesky of that
+    front's (1,1) block meets a non-positive pivot.  The oracle fails as
+    well; the GPU sweep (the chain kernel with factored updates, the level
+    sweep without) reports it as ArithmeticError."""
+    import scipy.sparse as sp
+    A = problems.caterpillar()
+    d = np.zeros(A.shape[0])
+    d[400 + 200] = -A.diagonal()[400 + 200] - 10.0  # spine vertex 200
+    A = sp.csc_matrix(A + sp.diags(d))
+    symb = cp.symbolic(A, p=None)
+    ref = (cp.cspmatrix(symb, device="cpu") + A).blkval.numpy().copy()
+    assert on.completion(symb, ref, factored) != 0
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    with pytest.raises(ArithmeticError, match="completion failed"):
+        cp.completion(X, factored_updates=factored)
+    assert X.is_factor is True
+
+
 def test_graph_replays_match_eager():
     """The sweeps replay a captured HIP graph; every replay (with eager work
     and synchronizations in between) must equal the eager launch sequence."""
