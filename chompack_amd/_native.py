@@ -55,6 +55,8 @@ def _declare(L):
     L.chp_embed.restype = i64
     L.chp_relative_idx.argtypes = [i64, vp, vp, vp, vp, vp, vp]
     L.chp_relative_idx.restype = i64
+    L.chp_lmerge.argtypes = [vp, vp, i64, i64, vp]
+    L.chp_lmerge.restype = i64
     L.chp_amd_order.argtypes = [i64, vp, vp, vp]
     L.chp_amd_order.restype = i32
     # device plan and numeric sweeps (declared when present)

@@ -15,7 +15,10 @@ __all__ = ["order"]
 
 
 def order(A, uplo="L"):
-    assert sp.issparse(A) and A.shape[0] == A.shape[1], "A must be a square sparse matrix"
+    from .misc import as_sparse, is_sparse
+    assert is_sparse(A), "A must be a square sparse matrix"
+    A = as_sparse(A)
+    assert A.shape[0] == A.shape[1], "A must be a square sparse matrix"
     if uplo == "U":
         A = sp.csc_matrix(A).T
     S = sp.csc_matrix(symmetrize(A))

@@ -225,6 +225,22 @@ i64 chp_embed(i64 nsn, const i64* cpl, const i64* ril, const i64* snptr,
     return 0;
 }
 
+// Merge of two ascending lists into out[0..nl+nr) (cbase.c:64-101): the
+// smaller head is taken; equal heads are written once and both advance, so
+// a value repeated inside one list stays repeated.  Returns the merged
+// length.
+i64 chp_lmerge(const i64* left, const i64* right, i64 nl, i64 nr, i64* out) {
+    i64 a = 0, b = 0, k = 0;
+    while (a < nl && b < nr) {
+        if (left[a] < right[b]) out[k++] = left[a++];
+        else if (left[a] > right[b]) out[k++] = right[b++];
+        else { out[k++] = left[a++]; ++b; }
+    }
+    while (a < nl) out[k++] = left[a++];
+    while (b < nr) out[k++] = right[b++];
+    return k;
+}
+
 // Relative indices: for each non-root supernode k with parent p, the
 // positions of k's separator rows inside p's clique.  relptr[nsn+1] must be
 // prefilled with the prefix sums of the separator sizes.

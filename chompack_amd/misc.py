@@ -3,9 +3,10 @@
 Host-side mirror of chompack/misc.py (src/python/misc.py:4-204): ``tril``,
 ``triu``, ``symmetrize``, ``perm``, ``eye``, ``lmerge``, and the device
 versions of ``frontal_add_update`` / ``frontal_get_update`` (cbase.c:11-59).
-Sparse matrices are ``scipy.sparse`` matrices (cvxopt's ``spmatrix`` has no
-counterpart here); explicit zeros are kept as structural entries, as cvxopt
-keeps them.
+Sparse matrices are ``scipy.sparse`` matrices; a cvxopt-style ``spmatrix``
+(an object with ``size`` and ``CCS``, or ``size`` and ``I``/``J``/``V``) is
+accepted wherever a sparse matrix is and converted by ``as_sparse``.
+Explicit zeros are kept as structural entries, as cvxopt keeps them.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -13,14 +14,50 @@ import scipy.sparse as sp
 from . import _native
 
 __all__ = ["tril", "triu", "symmetrize", "perm", "eye", "lmerge",
-           "frontal_add_update", "frontal_get_update", "coo_triplets"]
+           "frontal_add_update", "frontal_get_update", "coo_triplets", "as_sparse",
+           "is_sparse"]
+
+
+def _vec(v):
+    """1-D numpy copy of a sequence or buffer (a cvxopt ``matrix`` column)."""
+    return np.asarray(v).reshape(-1, order="F")
+
+
+def is_sparse(A):
+    """True for a scipy sparse matrix or a cvxopt-style ``spmatrix``."""
+    return sp.issparse(A) or (hasattr(A, "size") and (
+        hasattr(A, "CCS") or (hasattr(A, "I") and hasattr(A, "J") and hasattr(A, "V"))))
+
+
+def as_sparse(A):
+    """scipy CSC matrix of a sparse argument.
+
+    scipy matrices pass through.  A cvxopt-style ``spmatrix`` is read through
+    the attributes cvxopt exposes: ``size`` (rows, cols) and ``CCS``
+    (colptr, rowind, values), or ``I``, ``J``, ``V`` triplets.  Values must
+    be real."""
+    if sp.issparse(A):
+        return A
+    if not is_sparse(A):
+        raise TypeError("expected a sparse matrix")
+    m, n = (int(x) for x in A.size)
+    if hasattr(A, "CCS"):
+        cp, ri, v = A.CCS
+        cp, ri, v = _vec(cp).astype(np.int64), _vec(ri).astype(np.int64), _vec(v)
+        if np.iscomplexobj(v):
+            raise TypeError("complex sparse matrices are not supported")
+        return sp.csc_matrix((v.astype(np.float64), ri, cp), shape=(m, n))
+    I, J, V = _vec(A.I).astype(np.int64), _vec(A.J).astype(np.int64), _vec(A.V)
+    if np.iscomplexobj(V):
+        raise TypeError("complex sparse matrices are not supported")
+    return sp.csc_matrix((V.astype(np.float64), (I, J)), shape=(m, n))
 
 
 def coo_triplets(A):
     """(I, J, V) of a sparse matrix with duplicates summed, column-major sorted.
 
     Explicit zeros are kept (they are part of the sparsity pattern)."""
-    A = sp.coo_matrix(A)
+    A = sp.coo_matrix(as_sparse(A))
     n0, n1 = A.shape
     I = np.asarray(A.row, dtype=np.int64)
     J = np.asarray(A.col, dtype=np.int64)
@@ -42,7 +79,8 @@ def _from_triplets(I, J, V, shape):
 
 def tril(A):
     """Lower triangular part of a sparse or dense matrix (misc.py:4-16)."""
-    if sp.issparse(A):
+    if is_sparse(A):
+        A = as_sparse(A)
         I, J, V = coo_triplets(A)
         m = I >= J
         return _from_triplets(I[m], J[m], V[m], A.shape)
@@ -51,7 +89,8 @@ def tril(A):
 
 def triu(A):
     """Upper triangular part of a sparse or dense matrix (misc.py:18-30)."""
-    if sp.issparse(A):
+    if is_sparse(A):
+        A = as_sparse(A)
         I, J, V = coo_triplets(A)
         m = I <= J
         return _from_triplets(I[m], J[m], V[m], A.shape)
@@ -61,7 +100,8 @@ def triu(A):
 def symmetrize(A):
     """Symmetric matrix from the lower triangle of a square sparse matrix
     (misc.py:32-40): tril(A) + strictly-lower(A)^T."""
-    assert sp.issparse(A), "argument must be a sparse matrix"
+    assert is_sparse(A), "argument must be a sparse matrix"
+    A = as_sparse(A)
     assert A.shape[0] == A.shape[1], "argument must me a square matrix"
     I, J, V = coo_triplets(A)
     lo = I >= J
@@ -74,7 +114,8 @@ def symmetrize(A):
 
 def perm(A, p):
     """Symmetric permutation A[p, p] of a sparse matrix (misc.py:42-53)."""
-    assert sp.issparse(A), "argument must be a sparse matrix"
+    assert is_sparse(A), "argument must be a sparse matrix"
+    A = as_sparse(A)
     assert A.shape[0] == A.shape[1], "A must be a square matrix"
     p = np.asarray(p, dtype=np.int64).ravel()
     assert A.shape[0] == len(p), "length of p must be equal to the order of A"
@@ -90,17 +131,36 @@ def eye(n):
 
 
 def lmerge(left, right, offsetl=0, offsetr=0, nl=None, nr=None):
-    """Sorted union of two sorted segments written back into ``left``
-    (cbase.c:64-101, misc.py:178-204).  Returns the merged length."""
+    """Merge the ascending segments left[offsetl:offsetl+nl] and
+    right[offsetr:offsetr+nr] and write the result to left[offsetl:...]
+    (cbase.c:64-101, misc.py:178-204).  Equal heads are taken once, so a value
+    repeated inside one segment stays repeated.  ``left`` is any mutable
+    integer sequence (list, numpy array, cvxopt ``matrix``); it must have room
+    for the merged segment.  Returns the merged length."""
     if nl is None:
         nl = len(left)
     if nr is None:
         nr = len(right)
-    a = np.asarray(left[offsetl:offsetl + nl])
-    b = np.asarray(right[offsetr:offsetr + nr])
-    m = np.union1d(a, b)
-    left[offsetl:offsetl + len(m)] = m
-    return int(len(m))
+    nl, nr, offsetl, offsetr = int(nl), int(nr), int(offsetl), int(offsetr)
+    if min(nl, nr, offsetl, offsetr) < 0:
+        raise ValueError("offsets and lengths must be non-negative")
+    a = np.ascontiguousarray(_vec(left)[offsetl:offsetl + nl], dtype=np.int64)
+    b = np.ascontiguousarray(_vec(right)[offsetr:offsetr + nr], dtype=np.int64)
+    if a.size != nl or b.size != nr:
+        raise ValueError("segment extends past the end of its list")
+    out = np.empty(nl + nr, dtype=np.int64)
+    k = int(_native.lib().chp_lmerge(_native.ptr(a), _native.ptr(b), nl, nr, _native.ptr(out)))
+    if offsetl + k > len(left):
+        raise ValueError("left has no room for the merged list (%d entries from %d)"
+                         % (k, offsetl))
+    if isinstance(left, list):
+        left[offsetl:offsetl + k] = [int(x) for x in out[:k]]
+    elif isinstance(left, np.ndarray):
+        left[offsetl:offsetl + k] = out[:k]
+    else:
+        for i in range(k):
+            left[offsetl + i] = int(out[i])
+    return k
 
 
 def _strides_of(T):
@@ -110,18 +170,38 @@ def _strides_of(T):
     return T.stride(0), T.stride(1)
 
 
+def _rel_rows(F, relidx, relptr, i):
+    """Host int64 copy of r = relidx[relptr[i]:relptr[i+1]], checked against
+    F: every index a row and column of F, no index twice (one GPU thread
+    updates each target entry, so a repeated index would race)."""
+    i = int(i)
+    rp = _vec(relptr)
+    if not 0 <= i < rp.size - 1:
+        raise IndexError("supernode index out of range")
+    r = _vec(relidx)[int(rp[i]):int(rp[i + 1])].astype(np.int64)
+    if r.size and (r.min() < 0 or r.max() >= min(F.shape[0], F.shape[1])):
+        raise IndexError("relative index outside the frontal matrix")
+    if np.unique(r).size != r.size:
+        raise ValueError("relative indices must be distinct")
+    return r
+
+
 def frontal_add_update(F, U, relidx, relptr, i, alpha=1.0):
     """F[r, r] += alpha * lower(U) with r = relidx[relptr[i]:relptr[i+1]]
-    (cbase.c:11-34).  F and U are float64 CUDA tensors, updated in place on
-    the current stream."""
+    (cbase.c:11-34): only U's lower triangle is added, as in the reference's
+    C version.  F and U are float64 CUDA tensors of any strides, F updated in
+    place on the current stream.  U must be N x N with N = len(r) (the
+    reference returns an error there too)."""
     import torch
     lib = _native.lib()
-    r = np.asarray(relidx[int(relptr[i]):int(relptr[i + 1])], dtype=np.int64)
-    N = len(r)
-    if U.shape[0] != N:
-        return None
     fr, fc = _strides_of(F)
     ur, uc = _strides_of(U)
+    r = _rel_rows(F, relidx, relptr, i)
+    N = len(r)
+    if U.shape[0] != N or U.shape[1] != N:
+        raise ValueError("U must be %d x %d" % (N, N))
+    if U.device != F.device:
+        raise ValueError("F and U must be on the same device")
     d_r = torch.as_tensor(r, device=F.device)
     stream = torch.cuda.current_stream(F.device).cuda_stream
     st = (ctypes_i64 * 4)(fr, fc, ur, uc)
@@ -133,13 +213,14 @@ def frontal_add_update(F, U, relidx, relptr, i, alpha=1.0):
 
 def frontal_get_update(F, relidx, relptr, i):
     """Return U = lower(F[r, r]) as a new N x N float64 CUDA tensor
-    (cbase.c:39-59); the strict upper part of U is zero."""
+    (cbase.c:39-59); the strict upper part of U is zero.  F may have any
+    strides."""
     import torch
     lib = _native.lib()
-    r = np.asarray(relidx[int(relptr[i]):int(relptr[i + 1])], dtype=np.int64)
+    fr, fc = _strides_of(F)
+    r = _rel_rows(F, relidx, relptr, i)
     N = len(r)
     U = torch.zeros((N, N), dtype=torch.float64, device=F.device)
-    fr, fc = _strides_of(F)
     d_r = torch.as_tensor(r, device=F.device)
     stream = torch.cuda.current_stream(F.device).cuda_stream
     st = (ctypes_i64 * 4)(fr, fc, U.stride(0), U.stride(1))

@@ -34,23 +34,34 @@ class SpMat(object):
         self._work = None  # row-major copy of X for several right-hand sides
 
     def residual(self, X, B, R):
-        """R = B - A X for column-major (n, k) float64 GPU tensors."""
+        """R = B - A X for float64 GPU vectors or (n, k) matrices (any
+        strides: operands that are not column-major go through column-major
+        copies, and R's is written back)."""
         import torch
+        ws = []
+        for name, T in (("X", X), ("B", B), ("R", R)):
+            if not (isinstance(T, torch.Tensor) and T.is_cuda and T.dtype == torch.float64):
+                raise TypeError("%s must be a float64 tensor in GPU memory" % name)
+            if T.dim() not in (1, 2) or T.shape[0] != self.n:
+                raise ValueError("%s must have %d rows" % (name, self.n))
+            ws.append(numeric.colmajor_work(T))
         k = 1 if X.dim() == 1 else X.shape[1]
-        ld = lambda T: T.shape[0] if T.dim() == 1 or k == 1 else T.stride(1)
-        for T in (X, B, R):
-            if T.dim() == 2 and k > 1 and T.stride(0) != 1:
-                raise TypeError("X, B and R must be column-major")
+        for T in (B, R):
+            if (1 if T.dim() == 1 else T.shape[1]) != k:
+                raise ValueError("X, B and R must have the same number of columns")
         work = None
         if k > 1:
             if self._work is None or self._work.numel() < self.n * k:
                 self._work = torch.empty(self.n * k, dtype=torch.float64, device=X.device)
             work = self._work.data_ptr()
+        (wx, ldx, _, _), (wb, ldb, _, _), (wr, ldr, _, rc_copy) = ws
         rc = _native.lib().chp_spmv_residual(self.n, self.rowptr.data_ptr(), self.colind.data_ptr(),
-                                             self.vals.data_ptr(), X.data_ptr(), ld(X),
-                                             B.data_ptr(), ld(B), R.data_ptr(), ld(R), k,
+                                             self.vals.data_ptr(), wx.data_ptr(), ldx,
+                                             wb.data_ptr(), ldb, wr.data_ptr(), ldr, k,
                                              work, numeric._stream(X))
         numeric._raise_rc(rc, "spmv")
+        if rc_copy:
+            R.copy_(wr)
 
 
 def cholesky(X):

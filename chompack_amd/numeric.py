@@ -196,8 +196,58 @@ def hessian(L, Y, U, adj=False, inv=False, factored_updates=True):
     del torch
 
 
+def colmajor_work(T):
+    """(work, ld, avail, copied) for a float64 GPU vector or matrix T.
+
+    ``work`` addresses T's elements column-major with leading dimension
+    ``ld`` from ``work.data_ptr()``: T itself when its strides allow that
+    (unit row stride, column stride >= rows), otherwise a column-major copy
+    (``copied``; the caller writes it back with ``T.copy_(work)``).
+    ``avail`` is the number of float64 elements of storage from
+    ``work.data_ptr()`` on, the bound for explicit ld / column counts."""
+    if T.dim() == 1:
+        rows, cols = T.shape[0], 1
+        ok = T.stride(0) == 1 or rows <= 1
+        ld = max(rows, 1)
+    elif T.dim() == 2:
+        rows, cols = T.shape
+        ok = (T.stride(0) == 1 or rows <= 1) and (cols <= 1 or T.stride(1) >= max(rows, 1))
+        ld = T.stride(1) if (ok and cols > 1) else max(rows, 1)
+    else:
+        raise TypeError("expected a vector or a matrix")
+    if ok:
+        avail = T.untyped_storage().nbytes() // 8 - T.storage_offset()
+        return T, ld, avail, False
+    import torch
+    work = torch.empty_strided(tuple(T.shape), (1,) if T.dim() == 1 else (1, max(rows, 1)),
+                               dtype=T.dtype, device=T.device)
+    work.copy_(T)
+    return work, max(rows, 1), max(rows, 1) * cols, True
+
+
+def host_dense(B):
+    """Writable float64 numpy view of a host dense matrix or vector (numpy
+    array, or an object exposing the buffer protocol / array interface such
+    as a cvxopt ``matrix``, whose storage is column-major), sharing its
+    memory, so an in-place routine can write its result back."""
+    a = np.asarray(B)
+    if a.dtype != np.float64:
+        raise TypeError("B must hold float64 values")
+    if a.ndim not in (1, 2):
+        raise TypeError("B must be a vector or a matrix")
+    if not a.flags.writeable:
+        raise ValueError("B must be writable (it is updated in place)")
+    return a
+
+
 def _triangular(fn, what, L, B, alpha, trans, nrhs, offsetB, ldB):
-    """Shared argument handling of trsm and trmm (cbase.c:965-1041)."""
+    """Shared argument handling of trsm and trmm (cbase.c:965-1041).
+
+    B is a float64 GPU tensor (any strides: storage that is not column-major
+    is solved through a column-major copy written back afterwards) or a
+    host dense matrix (numpy array or a buffer-protocol object such as a
+    cvxopt ``matrix``), which is staged through the GPU and overwritten in
+    place, as the reference overwrites its ``matrix`` argument."""
     torch = _torch()
     if not (isinstance(L, cspmatrix) and L.is_factor is True):
         raise ValueError("L must be a cspmatrix factor")
@@ -205,40 +255,60 @@ def _triangular(fn, what, L, B, alpha, trans, nrhs, offsetB, ldB):
         raise ValueError("trans must be 'N' or 'T'")
     plan, b = _plan(L)
     n = L.symb.n
-    if not (isinstance(B, torch.Tensor) and B.is_cuda and B.dtype == torch.float64):
-        raise TypeError("B must be a float64 tensor in GPU memory")
-    if B.dim() == 1:
-        work, cols, ld = B, 1, B.shape[0]
-    elif B.dim() == 2:
-        cols = B.shape[1]
-        if B.stride(0) == 1:          # column-major storage
-            work, ld = B, (B.stride(1) if cols > 1 else B.shape[0])
-        else:                         # row-major: work on a column-major copy
-            work, ld = B.t().contiguous(), B.shape[0]
+    host = None
+    if isinstance(B, torch.Tensor):
+        if not (B.is_cuda and B.dtype == torch.float64):
+            raise TypeError("B must be a float64 tensor in GPU memory or a host matrix")
+        if B.device != b.device:
+            raise ValueError("B must be on the device of L")
+        if B.dim() not in (1, 2):
+            raise TypeError("B must be a vector or a matrix")
+        work, ld, avail, copied = colmajor_work(B)
+        cols = 1 if B.dim() == 1 else B.shape[1]
     else:
-        raise TypeError("B must be a vector or a matrix")
+        host = host_dense(B)
+        # the column-major element sequence of the host matrix (its own
+        # storage for a Fortran-ordered buffer such as cvxopt's)
+        flat = host.reshape(-1, order="F")
+        work = torch.as_tensor(np.ascontiguousarray(flat)).to(b.device)
+        ld = max(host.shape[0], 1)
+        avail = work.numel()
+        copied = False
+        cols = 1 if host.ndim == 1 else host.shape[1]
     if ldB != -1 and ldB is not None:
         ld = int(ldB)
     k = cols if (nrhs == -1 or nrhs is None) else int(nrhs)
-    if ld < n:
+    if int(offsetB) < 0:
+        raise ValueError("offsetB must be non-negative")
+    if ld < max(n, 1):
         raise ValueError("ldB must be at least n")
-    if k > 0 and int(offsetB) + (k - 1) * ld + n > work.numel():
+    if k < 0:
+        raise ValueError("nrhs must be non-negative")
+    if k > 0 and int(offsetB) + (k - 1) * ld + n > avail:
         raise ValueError("B is too small for n, nrhs, offsetB and ldB")
     rc = fn(plan.handle, b.data_ptr(), work.data_ptr() + 8 * int(offsetB),
             int(ld), int(k), float(alpha), ord(trans), _stream(b))
     _raise_rc(rc, what)
-    if work is not B:
-        B.copy_(work.t())
+    if copied:
+        B.copy_(work)
+    if host is not None:
+        res = work.cpu().numpy()
+        if host.ndim == 1:
+            host[:] = res
+        else:
+            host[...] = res.reshape(host.shape, order="F")
 
 
 def trsm(L, B, alpha=1.0, trans='N', nrhs=-1, offsetB=0, ldB=-1):
     """B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B ('T') with the chordal
     factor L, B in the original ordering (cbase.c:965-1077).
 
-    B is a float64 CUDA tensor: a vector of length n, or an (n, k) matrix.
-    As in the reference, ``nrhs``, ``offsetB`` and ``ldB`` address B's
-    column-major storage; a row-major (n, k) tensor is solved through a
-    column-major copy that is written back."""
+    B is a float64 CUDA tensor (a vector of length n, or an (n, k) matrix)
+    or a host dense matrix (numpy, or a cvxopt-style ``matrix`` through the
+    buffer protocol), updated in place.  As in the reference, ``nrhs``,
+    ``offsetB`` and ``ldB`` address B's column-major storage; a tensor whose
+    strides are not column-major is solved through a column-major copy that
+    is written back."""
     _triangular(_native.lib().chp_trsm, "trsm", L, B, alpha, trans, nrhs, offsetB, ldB)
 
 

@@ -6,14 +6,15 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _native
-from .misc import symmetrize
+from .misc import symmetrize, as_sparse, is_sparse
 
 __all__ = ["maxcardsearch", "maxchord"]
 
 
 def _pattern(A, ve):
-    if not sp.issparse(A):
+    if not is_sparse(A):
         raise TypeError("A must be a sparse matrix")
+    A = as_sparse(A)
     n = A.shape[0]
     if A.shape[1] != n:
         raise ValueError("A must be a square matrix")

@@ -13,36 +13,54 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import numeric, _native
+from .misc import as_sparse, is_sparse
 from .symbolic import cspmatrix, symbolic
 
 __all__ = ["pfcholesky", "pfchol", "pftrsm", "pftrmm"]
 
 
 def _colmajor(T, name):
+    """(work, rows, cols, ld, copied) of a float64 GPU vector or matrix:
+    ``work`` is T, or a column-major copy of it when T's strides are not
+    column-major (written back by the caller)."""
     import torch
     if not (isinstance(T, torch.Tensor) and T.is_cuda and T.dtype == torch.float64):
         raise TypeError("%s must be a float64 tensor in GPU memory" % name)
-    if T.dim() == 1:
-        return T.shape[0], 1, T.shape[0]
-    if T.dim() != 2 or (T.shape[1] > 1 and T.stride(0) != 1):
-        raise TypeError("%s must be a vector or a column-major matrix" % name)
-    return T.shape[0], T.shape[1], (T.stride(1) if T.shape[1] > 1 else T.shape[0])
+    if T.dim() not in (1, 2):
+        raise TypeError("%s must be a vector or a matrix" % name)
+    work, ld, _, copied = numeric.colmajor_work(T)
+    return work, T.shape[0], (1 if T.dim() == 1 else T.shape[1]), ld, copied
+
+
+def _need(T, name, n, m):
+    if T[1] != n or T[2] < m:
+        raise ValueError("%s must have %d rows and at least %d columns" % (name, n, m))
+
+
+def _write_back(pairs):
+    for T, w in pairs:
+        if w[4]:
+            T.copy_(w[0])
 
 
 def pfchol(a, V, L, B):
     """Product-form factorization of I + V diag(a) V^T in place: on exit
     L and B hold the factors and V's columns 1.. are transformed as in
-    dpftrf (pfcholesky.c:52-87).  Raises ArithmeticError("factorization
-    failed") where the reference does."""
-    n, m, ldv = _colmajor(V, "V")
-    _, _, ldl = _colmajor(L, "L")
-    _, _, ldb = _colmajor(B, "B")
+    dpftrf (pfcholesky.c:52-87).  L and B need V's row count and at least
+    its column count.  Raises ArithmeticError("factorization failed") where
+    the reference does."""
+    wv, wl, wb = _colmajor(V, "V"), _colmajor(L, "L"), _colmajor(B, "B")
+    n, m = wv[1], wv[2]
+    _need(wl, "L", n, m)
+    _need(wb, "B", n, m)
     ah = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel()[:m])
     if ah.size < m:
         raise ValueError("a must have at least as many entries as V has columns")
-    rc = _native.lib().chp_pf_factor(n, m, _native.ptr(ah), V.data_ptr(), ldv, L.data_ptr(), ldl,
-                                     B.data_ptr(), ldb, numeric._stream(V))
+    rc = _native.lib().chp_pf_factor(n, m, _native.ptr(ah), wv[0].data_ptr(), wv[3],
+                                     wl[0].data_ptr(), wl[3], wb[0].data_ptr(), wb[3],
+                                     numeric._stream(V))
     numeric._raise_rc(rc, "pfchol")
+    _write_back([(V, wv), (L, wl), (B, wb)])
     if rc == 1:
         raise ArithmeticError("factorization failed")
 
@@ -50,15 +68,16 @@ def pfchol(a, V, L, B):
 def _pf_apply(fn, what, V, L, B, U, trans):
     if trans not in ('N', 'T'):
         raise ValueError("trans must be 'N' or 'T'")
-    n, m, ldv = _colmajor(V, "V")
-    _, _, ldl = _colmajor(L, "L")
-    _, _, ldb = _colmajor(B, "B")
-    nu_rows, nu, ldu = _colmajor(U, "U")
-    if nu_rows != n:
+    wv, wl, wb, wu = _colmajor(V, "V"), _colmajor(L, "L"), _colmajor(B, "B"), _colmajor(U, "U")
+    n, m = wv[1], wv[2]
+    _need(wl, "L", n, m)
+    _need(wb, "B", n, m)
+    if wu[1] != n:
         raise ValueError("U must have as many rows as V")
-    rc = fn(n, m, V.data_ptr(), ldv, L.data_ptr(), ldl, B.data_ptr(), ldb, U.data_ptr(), ldu, nu,
-            ord(trans), numeric._stream(U))
+    rc = fn(n, m, wv[0].data_ptr(), wv[3], wl[0].data_ptr(), wl[3], wb[0].data_ptr(), wb[3],
+            wu[0].data_ptr(), wu[3], wu[2], ord(trans), numeric._stream(U))
     numeric._raise_rc(rc, what)
+    _write_back([(U, wu)])
 
 
 def pftrsm(V, L, B, U, trans='N'):
@@ -91,12 +110,15 @@ class pfcholesky(object):
         if isinstance(X, cspmatrix):
             n = X.symb.n
             device = X.blkval.device
-        elif sp.issparse(X):
+        elif is_sparse(X):
+            X = as_sparse(X)
             n = X.shape[0]
             if X.shape[0] != X.shape[1]:
                 raise ValueError("X must be a square matrix")
         else:
             raise TypeError("X must be a cspmatrix or a sparse matrix")
+        if not isinstance(V, torch.Tensor):
+            V = np.asarray(V, dtype=np.float64)   # numpy, or a buffer such as a cvxopt matrix
         Vt = torch.as_tensor(V, dtype=torch.float64).to(device)
         if Vt.dim() == 1:
             Vt = Vt.reshape(-1, 1)

@@ -18,7 +18,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _native
-from .misc import coo_triplets, symmetrize, perm, tril
+from .misc import coo_triplets, symmetrize, perm, tril, as_sparse, is_sparse
 
 __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "post_order",
            "etree", "counts"]
@@ -26,7 +26,7 @@ __all__ = ["symbolic", "cspmatrix", "merge_size_fill", "peo", "post_order",
 
 def _csc_pattern(A):
     """(colptr, rowind) int64 arrays of a sparse matrix, rows sorted."""
-    A = sp.csc_matrix(A)
+    A = sp.csc_matrix(as_sparse(A))
     A.sort_indices()
     return _native.as_i64(A.indptr), _native.as_i64(A.indices)
 
@@ -130,8 +130,9 @@ def merge_size_fill(tsize=8, tfill=8):
 def peo(A, p):
     """True if p is a perfect elimination ordering of the lower triangle of A
     (symbolic.py:413-449)."""
+    assert is_sparse(A), "A must be a sparse matrix"
+    A = as_sparse(A)
     n = A.shape[0]
-    assert sp.issparse(A), "A must be a sparse matrix"
     assert A.shape[1] == n, "A must be a square matrix"
     p = np.asarray(p, dtype=np.int64).ravel()
     assert len(p) == n, "length of p must be equal to the order of A"
@@ -153,14 +154,17 @@ class symbolic(object):
     """Symbolic factorization object (symbolic.py:499-890).
 
     ``symbolic(A, p=None, merge_function=None, supernodal=True)`` where A is a
-    square ``scipy.sparse`` matrix (only its lower triangle is read), ``p`` a
+    square sparse matrix (``scipy.sparse``, or a cvxopt-style ``spmatrix``
+    exposing ``size`` and ``CCS`` or ``I``/``J``/``V``; only its lower
+    triangle is read), ``p`` a
     permutation vector or an ordering routine (e.g. ``chompack_amd.amd.order``),
     and ``merge_function`` an amalgamation heuristic such as
     ``merge_size_fill(tsize, tfill)``.
     """
 
     def __init__(self, A, p=None, merge_function=None, **kwargs):
-        assert sp.issparse(A), "A must be a sparse matrix"
+        assert is_sparse(A), "A must be a sparse matrix"
+        A = as_sparse(A)
         assert A.shape[0] == A.shape[1], "A must be a square matrix"
         supernodal = kwargs.get("supernodal", True)
         n = A.shape[0]
@@ -462,8 +466,8 @@ class cspmatrix(object):
             assert self.symb is B.symb, "Symbolic factorization mismatch"
             assert B.is_factor is False, "Addition of cspmatrix factors not supported"
             self.blkval.add_(B.blkval)
-        elif sp.issparse(B):
-            self._iadd_spmatrix(B)
+        elif is_sparse(B):
+            self._iadd_spmatrix(as_sparse(B))
         else:
             raise TypeError
         return self
@@ -474,8 +478,8 @@ class cspmatrix(object):
             assert self.symb is B.symb, "Symbolic factorization mismatch"
             assert B.is_factor is False, "Addition of cspmatrix factors not supported"
             self.blkval.sub_(B.blkval)
-        elif sp.issparse(B):
-            self._iadd_spmatrix(B, alpha=-1.0)
+        elif is_sparse(B):
+            self._iadd_spmatrix(as_sparse(B), alpha=-1.0)
         else:
             raise TypeError
         return self

@@ -45,6 +45,12 @@ int64_t chp_supernodes(int64_t n, const int64_t* parent, const int64_t* post,
 /* Clique row sets (symbolic.py:328-384); returns 0 or -1-k on a size mismatch. */
 int64_t chp_embed(int64_t nsn, const int64_t* cpl, const int64_t* ril, const int64_t* snptr,
                   const int64_t* snpar, const int64_t* sncolptr, int64_t* snrowidx);
+/* Sorted merge of left[0..nl) and right[0..nr) into out (room for nl + nr);
+ * equal heads are taken once.  Returns the merged length.  Replaces lmerge
+ * (cbase.c:64-101; Python twin misc.py:178-204), whose caller copies out
+ * back into left. */
+int64_t chp_lmerge(const int64_t* left, const int64_t* right, int64_t nl, int64_t nr,
+                   int64_t* out);
 /* Relative indices (symbolic.py:386-411); returns 0 or -1-k. */
 int64_t chp_relative_idx(int64_t nsn, const int64_t* snptr, const int64_t* snpar,
                          const int64_t* sncolptr, const int64_t* snrowidx,

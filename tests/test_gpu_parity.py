@@ -106,9 +106,11 @@ def test_cholesky_projinv_solve(name, small_max):
     assert abs(v - on.dot(symb, Y.blkval.cpu().numpy(), A0)) <= 1e-10 * symb.n
 
 
-@pytest.mark.parametrize("name", ["ref31", "grid2d_64"])
-def test_llt_roundtrip(name):
-    A, symb, X = _setup(name, 64)
+@pytest.mark.parametrize("small_max", [64, 1])
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_llt_roundtrip(name, small_max):
+    print("case llt", name, small_max, flush=True)
+    A, symb, X = _setup(name, small_max)
     A0 = X.blkval.cpu().numpy().copy()
     ref = A0.copy()
     on.cholesky(symb, ref)
