@@ -20,6 +20,10 @@ dbl = ctypes.c_double
 vp = ctypes.c_void_p
 P64 = ctypes.POINTER(ctypes.c_int64)
 
+# CHP_RESID_NOALLOC_MAX of include/chompack_amd.h: the fixed workspace (in
+# doubles) of chp_spmv_residual_noalloc, so n * nrhs must not exceed it
+RESID_NOALLOC_MAX = 8388608
+
 
 def lib():
     """Return the loaded library, loading it on first use (raises if absent)."""
@@ -85,6 +89,8 @@ def _declare(L):
     _opt(L, "chp_factor_export", [vp, vp, vp, vp], i32)
     _opt(L, "chp_cholesky_mixed", [vp, vp, vp, vp], i32)
     _opt(L, "chp_spmv_residual", [i64, vp, vp, vp, vp, i32, vp, i32, vp, i32, i32, vp, vp], i32)
+    _opt(L, "chp_spmv_residual_noalloc", [i64, vp, vp, vp, vp, i32, vp, i32, vp, i32, i32, vp],
+         i32)
     _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
     _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
     _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)

@@ -192,6 +192,19 @@ int chp_spmv_residual(int64_t n, const int64_t* rowptr, const int* colind, const
                       const double* X, int ldx, const double* B, int ldb, double* R, int ldr,
                       int nrhs, double* work, void* stream);
 
+/* The same residual without any allocation and without a work argument, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight refinement loops, hosts that forbid hipMalloc).  No counterpart in
+ * the reference.  The row-major copy of X lives in a fixed device array of
+ * CHP_RESID_NOALLOC_MAX doubles (64 MiB, held by each GPU the library runs
+ * on until the process ends); n * nrhs above that returns -1 with a message.
+ * Calls that overlap in time on one GPU share the array and race: serialize
+ * them.  Results are bit-identical to chp_spmv_residual. */
+#define CHP_RESID_NOALLOC_MAX 8388608
+int chp_spmv_residual_noalloc(int64_t n, const int64_t* rowptr, const int* colind,
+                              const double* vals, const double* X, int ldx, const double* B,
+                              int ldb, double* R, int ldr, int nrhs, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Product-form Cholesky factors of I + V diag(a) V^T (pfcholesky.c), V, L, B
  * device column-major n x m arrays, a a host array of m doubles.  Each factor
