@@ -56,8 +56,11 @@ def test_frontal_add_and_get_update(layout):
         G = cp.frontal_get_update(F, s.relidx, s.relptr, c)
         torch.cuda.synchronize()
         assert G.shape == (N, N)
-        Gh = on._extract(Fh, r)
-        assert np.array_equal(G.cpu().numpy(), Gh)   # a gather: bit-exact
+        # a gather: bit-exact against the oracle's extract of the GPU's own F
+        # (F itself differs from the oracle's by fused multiply-add rounding)
+        same = np.array_equal(G.cpu().numpy(), on._extract(F.cpu().numpy(), r))
+        assert same
+        assert _rel(G.cpu().numpy(), on._extract(Fh, r)) <= 1e-15
     e = _rel(F.cpu().numpy(), Fh)
     print("frontal_add_update rel err %.3e" % e, flush=True)
     assert e <= 1e-15  # one (possibly fused) multiply-add per entry and child
