@@ -531,6 +531,17 @@ def main():
             gbs = nbytes / (kms[i] * 1e-3) / 1e9
             hbm[kind] = {"bytes_per_step": nbytes, "ms": round(float(kms[i]), 4),
                          "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / PEAK["hbm_gbs"], 4)}
+    # whole-path roofline: each kernel class at its own bound, summed (the
+    # classes are separate launches): all algorithmic flops at the FP64
+    # matrix peak, the extend-add and extract bytes at the HBM peak
+    t_flops = (fl_chol + fl_pinv) / (PEAK["fp64_tflops"] * 1e12) * 1e3
+    t_bytes = (ext_b + xtr_b) / (PEAK["hbm_gbs"] * 1e9) * 1e3
+    whole = {"roofline_time_ms": round(t_flops + t_bytes, 4), "measured_ms": round(ms, 4),
+             "frac": round((t_flops + t_bytes) / ms, 4),
+             "flops_floor_ms": round(t_flops, 4), "bytes_floor_ms": round(t_bytes, 4),
+             "formula": "sum over kernel classes of max(flops / 78.6 TFLOP/s, bytes / 8 TB/s): "
+                        "cholesky + projected-inverse algorithmic flops, extend-add + extract "
+                        "algorithmic bytes"}
     out = {
         "metric": "Cholesky+projinv wall time, 40³ grid; mixed vs FP64 speedup at 1e-10 resid",
         "value": round(ms, 4),
@@ -549,6 +560,7 @@ def main():
                    "n": symb.n, "nnz_L": symb.nnz, "supernodes": symb.Nsn,
                    "clique_number": symb.clique_number},
         "roofline": roof,
+        "whole_path_roofline": whole,
         "effective_tflops": round((fl_chol + fl_pinv) / (ms * 1e-3) / 1e12, 3),
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,
@@ -601,13 +613,26 @@ def main():
                     times.append(dt)
             cpu_ms = statistics.median(times) * 1e3
             err = float(np.abs(gpu_result - b).max() / np.abs(b).max())
+            # the same workload on one host thread (one run)
+            ref.set_threads(1)
+            b1 = b0.copy()
+            t0 = time.perf_counter()
+            ref.cholesky(symb, b1)
+            ref.projected_inverse(symb, b1)
+            cpu1_ms = (time.perf_counter() - t0) * 1e3
+            ref.set_threads(threads)
             out["cpu_baseline"] = {"value": round(cpu_ms, 2), "unit": "ms", "cores": threads,
                                    "kind": "reference",
                                    "sample": "full %d^3 workload, cholesky + projected_inverse "
-                                             "with the reference C kernels (OpenBLAS, %d threads), "
-                                             "median of %d runs after 1 warm-up" %
-                                             (args.side, threads, args.cpu_runs)}
+                                             "with the reference C kernels (OpenBLAS, %d threads: "
+                                             "the host share of one GPU; %d CPUs visible), median "
+                                             "of %d runs after 1 warm-up; 1 thread: one run" %
+                                             (args.side, threads, os.cpu_count() or 0,
+                                              args.cpu_runs),
+                                   "value_1_thread": round(cpu1_ms, 2),
+                                   "cpus_visible": os.cpu_count()}
             out["speedup_vs_cpu"] = round(cpu_ms / ms, 2)
+            out["speedup_vs_cpu_1_thread"] = round(cpu1_ms / ms, 2)
             out["parity_rel_err_vs_reference"] = err
             if solve is None:
                 out["cpu_baseline"]["sample"] += "; solve section skipped"

@@ -609,6 +609,53 @@ def test_mixed_precision_refined_solve():
     assert res.max() <= 1e-10
 
 
+MIXED_CASES = {
+    # 40^3 grid (seed 1), 16 right-hand sides; RGG n = 200000 (seed 5), 64
+    "grid40_16rhs": (lambda: problems.grid3d(40, seed=1), 16),
+    "rgg200k_64rhs": (lambda: problems.rgg(200000, 12.0, seed=5), 64),
+}
+
+
+@pytest.mark.parametrize("case", list(MIXED_CASES))
+def test_mixed_refined_solve_full_size(case):
+    """The mixed-precision legs at full size: cholesky with FP32 Schur
+    updates, then FP64 iterative refinement of N(0,1) right-hand sides to
+    max_j ||b_j - A x_j||_2 / ||b_j||_2 <= 1e-10 (checked again on the host
+    with scipy).  The mixed factor's difference from the FP64 factor is
+    stated and bounded by 1e-4 relative (FP32 rounding of long Schur sums);
+    its logdet by 1e-7 relative."""
+    import torch
+    make, nrhs = MIXED_CASES[case]
+    print("case mixed", case, flush=True)
+    A = make()
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X0 = cp.cspmatrix(symb, device="cuda:0") + A
+    L64 = X0.copy()
+    cp.cholesky(L64)
+    L32 = X0.copy()
+    cp.mixed.cholesky(L32)
+    del X0
+    e = float((L32.blkval - L64.blkval).abs().max() / L64.blkval.abs().max())
+    l64, l32 = cp.logdet(L64), cp.logdet(L32)
+    print("mixed factor rel diff %.3e, logdet rel diff %.3e" % (e, abs(l32 - l64) / abs(l64)),
+          flush=True)
+    assert 0.0 < e <= 1e-4
+    assert abs(l32 - l64) <= 1e-7 * abs(l64)
+    del L64
+    Am = cp.mixed.SpMat(A)
+    rng = np.random.default_rng(0)
+    Bh = rng.standard_normal((symb.n, nrhs))
+    B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+    x, hist = cp.mixed.solve(Am, L32, B, tol=1e-10)
+    print("refinement history", ["%.2e" % h for h in hist], flush=True)
+    assert hist[-1] <= 1e-10
+    assert len(hist) <= 4
+    As = cp.symmetrize(A).tocsr()
+    res = np.linalg.norm(As @ x.cpu().numpy() - Bh, axis=0) / np.linalg.norm(Bh, axis=0)
+    print("host residual %.3e" % res.max(), flush=True)
+    assert res.max() <= 1e-10
+
+
 @pytest.mark.parametrize("nrhs", [1, 5, 70])
 def test_spmv_residual(nrhs):
     """R = B - A X on the GPU (device CSR, row-major copy of X for several
