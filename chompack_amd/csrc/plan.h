@@ -28,6 +28,8 @@ struct Launch {
     double flops = 0.0;  // algorithmic flops of a GEMM launch
     int bsel = 0;        // buffer (BufTab index) used as blkval by front kernels
     int side = 0;        // 0: the sweep's stream; 1, 2: side stream (after a K_FORK of it)
+    int fisel = 0;       // front table of the front kernels: 0 the compacted update
+                         // offsets (d_fi), 1 one slot per front (d_fi_full)
 };
 
 struct ListBuf {
@@ -109,6 +111,12 @@ struct chp_plan {
     int nlevels = 0;
     int small_max = 64;
     std::vector<FrontInfo> fi;
+    // the same fronts with one update slot per front (updptr = prefix sums of
+    // na^2): the layout of the Hessian scaling arena d_supd, which keeps every
+    // slot of a sweep for the second scaling of hessian(adj=None)
+    std::vector<FrontInfo> fi_full;
+    FrontInfo* d_fi_full = nullptr;
+    i64 upd_full_len = 0;
     std::vector<std::vector<int>> bylevel;
     std::vector<int> h_relidx, h_chidx;
     std::vector<i64> h_dinvptr, h_xptr, h_tptr, h_wunit, h_lltptr;
@@ -234,6 +242,8 @@ std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts
 // Extract work list of the listed fronts: (front, j0, i0) for the 16-column x
 // 256-row blocks of V_k that meet the lower triangle.
 std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& fronts);
+// Children of the listed fronts (each front's children in chidx order).
+std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr);
 BufTab main_tab(chp_plan* P, double* blkval);
 int alloc_doubles(double** d, i64 len);
 

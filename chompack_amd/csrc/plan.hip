@@ -16,6 +16,8 @@
 #include "../../include/chompack_amd.h"
 
 #include <algorithm>
+#include <map>
+#include <set>
 #include <cmath>
 #include <array>
 #include <cstring>
@@ -188,6 +190,13 @@ std::vector<int4> extadd_tiles(const chp_plan* P, const std::vector<int>& fronts
     for (int4& t : et) t.w += *ntiles;  // the bounds follow the tiles in the list
     et.insert(et.end(), bnd.begin(), bnd.end());
     return et;
+}
+
+std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
+    std::vector<int> ch;
+    for (int k : fr)
+        for (int l = P->fi[k].chbeg; l < P->fi[k].chend; ++l) ch.push_back(P->h_chidx[l]);
+    return ch;
 }
 
 std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& fronts) {
@@ -366,7 +375,10 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
 // from tbase on: each stream reuses its own region level after level.
 void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64 tbase,
                          i64& xcur, i64& scratch_need) {
-    const std::vector<int4> xt = extract_tiles(P, fr);
+    // V_c of the children of this level's fronts, extracted once the fronts
+    // are finished (end of the level): an update slot is touched only at its
+    // front's level and its parent's, which lets the arena reuse slots
+    const std::vector<int4> xt = extract_tiles(P, children_of(P, fr));
     std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
@@ -440,24 +452,21 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
         P->pg->split_side = false;
         for (size_t i = s0; i < P->projinv.size(); ++i) P->projinv[i].side = 2;
     }
-    // side stream 1: V_k for every front of the level (needs the parents'
-    // finished projected inverse, so it forks from the main stream here) and
-    // the small fronts
-    const bool fork = !large.empty() && (!xt.empty() || !small.empty());
+    // side stream 1: the small fronts (their V_k were extracted at the end of
+    // the parents' level), beside the large fronts
+    const bool fork = !large.empty() && !small.empty();
     if (fork) P->projinv.push_back({K_FORK, 0, 0, 0, 0, 1});
-    if (!xt.empty()) {
-        Launch Lx{K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0};
-        Lx.side = fork ? 1 : 0;
-        P->projinv.push_back(Lx);
-    }
     if (!small.empty()) {
         Launch Ls{K_PROJINV_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0};
         Ls.side = fork ? 1 : 0;
         P->projinv.push_back(Ls);
     }
-    if (large.empty()) return;
+    auto extract_children = [&]() {
+        if (!xt.empty())
+            P->projinv.push_back({K_EXTRACT, (int)xt.size(), 256, 0, P->pg->lists.add(xt), 0});
+    };
+    if (large.empty()) { extract_children(); return; }
     P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
-    if (fork) P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 1});
     // F11 = X^T X (lower, strict upper zeroed) and T = V B21
     std::vector<GemmTile> g3, g4;
     for (int k : large) {
@@ -482,6 +491,8 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
             for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
     if (!nt.empty())
         P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0});
+    if (fork) P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 1});
+    extract_children();
 }
 
 // llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
@@ -537,6 +548,72 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
     int net = 0;
     const std::vector<int4> et = extadd_tiles(P, large, &net);
     P->llt.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 0});
+}
+
+// Offsets of the update slots in the arena (fi[k].updptr), and the arena
+// length.  Every sweep writes and reads slot k (na_k^2 doubles) only at
+// the levels level(k) and level(parent(k)) (cholesky, llt, _Y2K: written
+// at k's level, read at the parent's; projected inverse, completion, the
+// scaling maps, _M2T: written at the parent's level, read at k's), and a
+// level finishes before the next begins.  So a slot is live over the closed
+// level interval [level(k), level(parent(k))], and slots whose intervals
+// meet get disjoint memory while the others share it: the arena holds the
+// largest set of slots live at one level instead of sum(na^2).  Levels are
+// visited bottom-up; slots whose interval ended below the level are
+// released (coalescing neighbours), then the level's new slots take the
+// best-fitting free block or the end of the arena.
+i64 place_update_slots(std::vector<FrontInfo>& fi, const std::vector<std::vector<int>>& bylevel) {
+    const int H = (int)bylevel.size();
+    std::map<i64, i64> by_off;              // free block offset -> length
+    std::set<std::pair<i64, i64>> by_len;   // (length, offset) of free blocks
+    std::vector<std::vector<int>> release(H + 1);
+    i64 top = 0;
+    auto slot_len = [&](int k) { return ((i64)fi[k].na * fi[k].na + 31) / 32 * 32; };
+    auto free_block = [&](i64 off, i64 len) {
+        auto nx = by_off.lower_bound(off);
+        if (nx != by_off.end() && off + len == nx->first) {  // merge with the next block
+            by_len.erase({nx->second, nx->first});
+            len += nx->second;
+            nx = by_off.erase(nx);
+        }
+        if (nx != by_off.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {  // merge into the previous block
+                by_len.erase({pv->second, pv->first});
+                off = pv->first;
+                len += pv->second;
+                by_off.erase(pv);
+            }
+        }
+        if (off + len == top) { top = off; return; }  // the arena's tail shrinks
+        by_off[off] = len;
+        by_len.insert({len, off});
+    };
+    for (int L = 0; L < H; ++L) {
+        for (int k : release[L]) free_block(fi[k].updptr, slot_len(k));
+        for (int k : bylevel[L]) {
+            FrontInfo& f = fi[k];
+            if (f.na == 0) { f.updptr = 0; continue; }
+            const i64 need = slot_len(k);
+            auto it = by_len.lower_bound({need, (i64)0});
+            if (it != by_len.end()) {
+                const i64 off = it->second, len = it->first;
+                by_len.erase(it);
+                by_off.erase(off);
+                f.updptr = off;
+                if (len > need) { by_off[off + need] = len - need; by_len.insert({len - need, off + need}); }
+            } else {
+                f.updptr = top;
+                top += need;
+            }
+            // released once the parent's level is over
+            const int end = fi[f.parent].level;
+            release[std::min(end + 1, H)].push_back(k);
+        }
+    }
+    i64 len = 0;
+    for (const FrontInfo& f : fi) len = std::max(len, f.updptr + (i64)f.na * f.na);
+    return len;
 }
 
 int alloc_doubles(double** d, i64 len) {
@@ -606,27 +683,28 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         }
         const hipStream_t s = (two && L.side) ? P->sstream[L.side] : s0;
         const void* list = pg.d_lists + L.list;
+        const FrontInfo* fi = L.fisel ? P->d_fi_full : P->d_fi;
         double* blkval = bt.p[L.bsel];
         dim3 g(L.grid), b(L.block);
         switch (L.kid) {
         case K_CHOL_SMALL:
-            CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
             break;
         case K_EXTADD:
-            CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, fi, blkval,
                        bt.p[1], P->d_relidx, P->d_chidx, L.iarg);
             break;
         case K_COPY_TOP:
-            CHP_LAUNCH(k_copy_top, 0, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_copy_top, 0, (const int2*)list, fi, blkval,
                        P->d_scratch, P->d_lltptr);
             break;
         case K_POTRF_DIAG:
-            CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, P->d_fi,
+            CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, fi,
                        blkval, P->d_dinv, P->d_dinvptr, P->d_status);
             break;
         case K_PANEL_FACTOR:
-            CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, P->d_fi, blkval, P->d_dinv,
+            CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, fi, blkval, P->d_dinv,
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
@@ -639,23 +717,23 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_EXTRACT:
-            CHP_LAUNCH(k_extract, 0, (const int4*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_extract, 0, (const int4*)list, fi, blkval,
                        bt.p[1], P->d_relidx);
             break;
         case K_PROJINV_SMALL:
-            CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, P->d_fi,
+            CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, fi,
                        blkval, P->d_upd, P->d_status);
             break;
         case K_TRTRI_DIAG:
-            CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, fi, blkval,
                        P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
             break;
         case K_NEG_COPY:
-            CHP_LAUNCH(k_neg_copy, 0, (const int2*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_neg_copy, 0, (const int2*)list, fi, blkval,
                        P->d_scratch, P->d_tptr);
             break;
         case K_LLT_SMALL:
-            CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, P->d_fi, blkval,
+            CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx);
             break;
         case K_DIAG:
@@ -665,7 +743,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             CHP_LAUNCH(k_matop, 0, (const MatOp*)list, bt);
             break;
         case K_COMP_CHAIN:
-            CHP_LAUNCH(k_completion_chain, 0, (const ChainItem*)list, L.iarg, P->d_fi, bt,
+            CHP_LAUNCH(k_completion_chain, 0, (const ChainItem*)list, L.iarg, fi, bt,
                        P->d_relidx, P->d_status);
             break;
         case K_UPDATE_FACTOR:  // the 64-thread launches hold the small tasks
@@ -787,7 +865,39 @@ const char* chp_last_error(void) { return g_err.c_str(); }
 
 int chp_version(void) { return 1; }
 
-// See include/chompack_amd.h.
+// See include/chompack_amd.h.  Host only: no device call.
+i64 chp_update_layout(i64 nsn, const i64* snpost, const i64* relptr, const i64* chptr,
+                      const i64* chidx, i64* updptr, i64* level_out) {
+    std::vector<FrontInfo> fi(nsn);
+    std::vector<int> level(nsn, 0);
+    for (i64 k = 0; k < nsn; ++k) {
+        std::memset(&fi[k], 0, sizeof(FrontInfo));
+        fi[k].parent = (int)k;
+        fi[k].na = (int)(relptr[k + 1] - relptr[k]);
+    }
+    for (i64 k = 0; k < nsn; ++k)
+        for (i64 l = chptr[k]; l < chptr[k + 1]; ++l) {
+            if (chidx[l] < 0 || chidx[l] >= nsn) { chp_set_error("invalid chidx"); return -1; }
+            fi[chidx[l]].parent = (int)k;
+        }
+    for (i64 t = 0; t < nsn; ++t) {
+        const i64 k = snpost[t];
+        if (k < 0 || k >= nsn) { chp_set_error("invalid snpost"); return -1; }
+        for (i64 l = chptr[k]; l < chptr[k + 1]; ++l)
+            level[k] = std::max(level[k], level[chidx[l]] + 1);
+    }
+    int H = 0;
+    for (i64 k = 0; k < nsn; ++k) { fi[k].level = level[k]; H = std::max(H, level[k] + 1); }
+    std::vector<std::vector<int>> bylevel(H);
+    for (i64 t = 0; t < nsn; ++t) bylevel[level[snpost[t]]].push_back((int)snpost[t]);
+    const i64 len = place_update_slots(fi, bylevel);
+    for (i64 k = 0; k < nsn; ++k) {
+        if (updptr) updptr[k] = fi[k].updptr;
+        if (level_out) level_out[k] = level[k];
+    }
+    return len;
+}
+
 chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
                           const i64* relptr, const i64* relidx, const i64* chptr,
                           const i64* chidx, const i64* blkptr, const i64* perm,
@@ -833,7 +943,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         f.parent = parent[k];
         f.level = level[k];
         f.blkptr = blkptr[k];
-        f.updptr = upd;
+        f.updptr = upd;  // one slot per front here; compacted below
         f.relptr = relptr[k];
         upd += (i64)f.na * f.na;
         if (!chol_is_small(P, f) || !projinv_is_small(P, f)) {
@@ -843,7 +953,9 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         P->h_wunit[k] = wun;
         wun += f.nn + f.na;
     }
-    P->upd_len = upd;
+    P->fi_full = P->fi;
+    P->upd_full_len = upd;
+    P->upd_len = place_update_slots(P->fi, P->bylevel);
     P->dinv_len = dinv;
     P->solve_len = wun;
     // host copies of the index maps (the sweep builders read them)
@@ -886,6 +998,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         }
     int rc = 0;
     rc |= upload(&P->d_fi, P->fi);
+    rc |= upload(&P->d_fi_full, P->fi_full);
     rc |= upload(&P->d_relidx, P->h_relidx);
     rc |= upload(&P->d_chidx, P->h_chidx);
     rc |= upload(&P->d_perm, pv);
@@ -961,7 +1074,7 @@ void chp_plan_destroy(chp_plan* P) {
         if (P->sev_join[i]) (void)hipEventDestroy(P->sev_join[i]);
         if (P->sstream[i]) (void)hipStreamDestroy(P->sstream[i]);
     }
-    void* ptrs[] = {P->d_fi, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
+    void* ptrs[] = {P->d_fi, P->d_fi_full, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_gptr, P->d_gent, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,

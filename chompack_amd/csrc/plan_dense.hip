@@ -163,13 +163,6 @@ template void push_tasks<MatOp>(chp_plan*, std::vector<Launch>&, Kid, const std:
 
 namespace {
 
-std::vector<int> children_of(const chp_plan* P, const std::vector<int>& fr) {
-    std::vector<int> ch;
-    for (int k : fr)
-        for (int l = P->fi[k].chbeg; l < P->fi[k].chend; ++l) ch.push_back(P->h_chidx[l]);
-    return ch;
-}
-
 // Update-factor task of child c from its parent's front (buffer fbuf) and the
 // factor of the parent's update matrix; the workspace offset is left at 0.
 UFTask uf_task(const chp_plan* P, int c, int fbuf) {
@@ -433,9 +426,6 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     if (inv) {
         Bump dv;  // unused: unfactored updates need no block inverses
         emit_child_updates(P, seq, fr, false, 0, sc, dv);  // before this level's update
-    } else {
-        // this level's V from the (finished) parents
-        push_tasks(P, seq, K_EXTRACT, extract_tiles(P, fr), 0);
     }
     std::vector<GemmTile> g1, g2, g3;
     for (int k : fr) {
@@ -456,6 +446,10 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     push_gemm(P, seq, g1);
     push_gemm(P, seq, g2);
     push_gemm(P, seq, g3);
+    // without inv: the children's V from the finished fronts of this level,
+    // here at the parents' level (an update slot is touched only at its
+    // front's level and its parent's)
+    if (!inv) push_tasks(P, seq, K_EXTRACT, extract_tiles(P, children_of(P, fr)), 0);
 }
 
 int ensure_dense(chp_plan* P, hipStream_t s) {
@@ -556,12 +550,18 @@ int build_scale(chp_plan* P, int idx, i64& sc_req, i64& dv_req) {
     S.shared.assign(P->nlevels, {});
     S.shared_t.assign(P->nlevels, {});
     S.dir.assign(P->nlevels, {});
+    // the scaling sweeps keep every update slot (d_supd): built on the
+    // one-slot-per-front table, and their front kernels read that table
+    std::swap(P->fi, P->fi_full);
     for (int L = P->nlevels - 1; L >= 0; --L) {
         Bump sc, dv;
         build_scale_level(P, S.shared[L], S.shared_t[L], S.dir[L], P->bylevel[L], fu, inv, adj,
                           sc, dv);
         b.level_done(sc, dv);
+        for (auto* v : {&S.shared[L], &S.shared_t[L], &S.dir[L]})
+            for (Launch& l : *v) l.fisel = 1;
     }
+    std::swap(P->fi, P->fi_full);
     S.built = true;
     note_needs(P, b, sc_req, dv_req);
     return 0;
@@ -641,7 +641,7 @@ int chp_hessian(chp_plan* P, const double* lblkval, const double* yblkval, doubl
         else if (!P->y2k[iv].built) build_y2k(P, iv);
     }
     if (prepare(P, sc, dv, s)) return -2;
-    if (!P->d_supd && alloc_doubles(&P->d_supd, P->upd_len)) return -2;
+    if (!P->d_supd && alloc_doubles(&P->d_supd, P->upd_full_len)) return -2;
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
     // The scaling sweeps keep their update matrices (factored or not; the
     // same for adj and its adjoint) in d_supd, which the M2 sweeps between

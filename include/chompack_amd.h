@@ -82,6 +82,17 @@ chp_plan* chp_plan_create(int64_t n, int64_t nsn, const int64_t* snpost,
                           const int64_t* chidx, const int64_t* blkptr,
                           const int64_t* p, int small_max);
 void chp_plan_destroy(chp_plan* plan);
+/* Update-arena layout of a plan (host only, no device call): updptr[k] =
+ * offset of supernode k's na_k x na_k update matrix, level_out[k] = its
+ * level (height in the supernodal elimination tree; either output may be
+ * NULL); returns the arena length in doubles.  Slot k is live over the
+ * levels [level(k), level(parent(k))]; slots live at a common level are
+ * disjoint, the others share memory, so the arena holds the largest live
+ * set instead of sum(na^2).  Replaces the reference's LIFO stack sizing
+ * stack_mem (symbolic.py:634-680), which assumes a sequential postorder. */
+int64_t chp_update_layout(int64_t nsn, const int64_t* snpost, const int64_t* relptr,
+                          const int64_t* chptr, const int64_t* chidx, int64_t* updptr,
+                          int64_t* level_out);
 /* out[8]: levels, large fronts (cholesky), large fronts (projected inverse),
  * GEMM tiles, update-arena doubles, diagonal-inverse doubles, scratch doubles,
  * cholesky launches. */
