@@ -24,7 +24,12 @@
 __global__ __launch_bounds__(TPB) void k_extadd_tiles(
     const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx, const int* __restrict__ chidx, int zero22) {
+    const int* __restrict__ relidx, const int* __restrict__ chidx, int zero22,
+    i64 bstride, i64 ustride) {
+    // grid y: the instance (a Hessian direction) whose blkval and update
+    // arena lie bstride and ustride doubles after the previous one's
+    blkval += (i64)blockIdx.y * bstride;
+    upd += (i64)blockIdx.y * ustride;
     const int4 t = tiles[blockIdx.x];
     const int k = t.x, c0 = t.y, r0 = t.z;
     const FrontInfo f = fi[k];

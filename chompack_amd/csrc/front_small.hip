@@ -109,8 +109,11 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
 __global__ __launch_bounds__(TPB) void k_extract(
     const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx) {
+    const int* __restrict__ relidx, i64 bstride, i64 ustride) {
     __shared__ double S[256][17];
+    // grid y: the instance (a Hessian direction), as in k_extadd_tiles
+    blkval += (i64)blockIdx.y * bstride;
+    upd += (i64)blockIdx.y * ustride;
     const int4 t = tiles[blockIdx.x];
     const int k = t.x, j0 = t.y, ib = t.z;
     const FrontInfo f = fi[k];

@@ -14,7 +14,7 @@ __global__ void k_chol_small(const int* fronts, const FrontInfo* fi, double* blk
                              double* upd, const int* relidx, const int* chidx,
                              unsigned long long* status);
 __global__ void k_extract(const int4* tiles, const FrontInfo* fi, const double* blkval,
-                          double* upd, const int* relidx);
+                          double* upd, const int* relidx, i64 bstride, i64 ustride);
 __global__ void k_projinv_small(const int* fronts, const FrontInfo* fi, double* blkval,
                                 const double* upd, unsigned long long* status);
 __global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkval,
@@ -22,7 +22,8 @@ __global__ void k_llt_small(const int* fronts, const FrontInfo* fi, double* blkv
 
 // large fronts
 __global__ void k_extadd_tiles(const int4* tiles, const FrontInfo* fi, double* blkval, double* upd,
-                               const int* relidx, const int* chidx, int zero22);
+                               const int* relidx, const int* chidx, int zero22, i64 bstride,
+                               i64 ustride);
 __global__ void k_copy_top(const int2* tiles, const FrontInfo* fi, const double* blkval,
                            double* scratch, const i64* sptr);
 __global__ void k_potrf_diag(const int* fronts, int c0, const FrontInfo* fi, double* blkval,

@@ -5,6 +5,7 @@
 #include "kernels.h"
 
 #include <cstring>
+#include <map>
 #include <vector>
 
 namespace chpi {
@@ -30,6 +31,9 @@ struct Launch {
     int side = 0;        // 0: the sweep's stream; 1, 2: side stream (after a K_FORK of it)
     int fisel = 0;       // front table of the front kernels: 0 the compacted update
                          // offsets (d_fi), 1 one slot per front (d_fi_full)
+    int gy = 1;          // extend-add / extract: instances along grid y, their
+    i64 bstride = 0;     // blkval and update arenas bstride / ustride doubles apart
+    i64 ustride = 0;
 };
 
 struct ListBuf {
@@ -158,8 +162,15 @@ struct chp_plan {
         // kept in their own arena d_supd; shared_t = inverses into scratch
         std::vector<std::vector<chpi::Launch>> shared, shared_t, dir;
     };
-    Seq comp[2], y2k[2], m2t[2];
-    LevelSeq scale[8];
+    // the Hessian maps work on a chunk of nb directions at once; their
+    // sequences are built per chunk size
+    Seq comp[2];
+    std::map<int, Seq> y2k[2], m2t[2];
+    std::map<int, LevelSeq> scale[8];
+    i64 blk_len = 0;            // blkptr[nsn]
+    double* d_ustack = nullptr; // a chunk of directions, blk stride apart
+    double* d_uupd = nullptr;   // their update arenas, upd stride apart
+    i64 ustack_cap = 0, uupd_cap = 0;
     std::vector<chpi::SolvePlan*> solves;  // cached per nrhs
     double* d_dscratch = nullptr;
     double* d_ddinv = nullptr;

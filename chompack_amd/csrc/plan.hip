@@ -685,7 +685,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         const void* list = pg.d_lists + L.list;
         const FrontInfo* fi = L.fisel ? P->d_fi_full : P->d_fi;
         double* blkval = bt.p[L.bsel];
-        dim3 g(L.grid), b(L.block);
+        dim3 g(L.grid, L.gy), b(L.block);
         switch (L.kid) {
         case K_CHOL_SMALL:
             CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, fi, blkval,
@@ -693,7 +693,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTADD:
             CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, fi, blkval,
-                       bt.p[1], P->d_relidx, P->d_chidx, L.iarg);
+                       bt.p[1], P->d_relidx, P->d_chidx, L.iarg, L.bstride, L.ustride);
             break;
         case K_COPY_TOP:
             CHP_LAUNCH(k_copy_top, 0, (const int2*)list, fi, blkval,
@@ -718,7 +718,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, fi, blkval,
-                       bt.p[1], P->d_relidx);
+                       bt.p[1], P->d_relidx, L.bstride, L.ustride);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, fi,
@@ -953,6 +953,7 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
         P->h_wunit[k] = wun;
         wun += f.nn + f.na;
     }
+    P->blk_len = blkptr[nsn];
     P->fi_full = P->fi;
     P->upd_full_len = upd;
     P->upd_len = place_update_slots(P->fi, P->bylevel);
@@ -1078,7 +1079,7 @@ void chp_plan_destroy(chp_plan* P) {
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_gptr, P->d_gent, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
                     P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
-                    P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_supd};
+                    P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_supd, P->d_ustack, P->d_uupd};
     for (SolvePlan* S : P->solves) {
         for (auto& g : S->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (S->pg.d_lists) (void)hipFree(S->pg.d_lists);

@@ -319,7 +319,7 @@ void build_completion_level(chp_plan* P, std::vector<Launch>& seq, const std::ve
 // matrices, factors, inverses) and the per-direction part.
 void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>& sht,
                        std::vector<Launch>& dir, const std::vector<int>& fr, bool fu, bool inv,
-                       bool adj, Bump& sc, Bump& dv) {
+                       bool adj, Bump& sc, Bump& dv, int nb, i64 bstride) {
     emit_child_updates(P, sh, fr, fu, 6, sc, dv);
     std::vector<int> fa;
     for (int k : fr) if (P->fi[k].na > 0) fa.push_back(k);
@@ -347,30 +347,36 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
         }
     }
     if (!inv) emit_trtri(P, sht, L11, X11, W11, std::vector<int>(fr.begin(), fr.end()));
-    // per direction: U11 symmetric, Tu = U op1(M1), U11 = op2(M1) Tu11 (upper
-    // zeroed), U21 = op3(M3) Tu21
+    // per direction (nb of them, blkval bstride doubles apart): U11
+    // symmetric, Tu = U op1(M1), U11 = op2(M1) Tu11 (upper zeroed),
+    // U21 = op3(M3) Tu21
     std::vector<MatOp> sy;
     std::vector<GemmTile> g1, g2;
     const bool t1 = !adj, t2 = adj;                       // op1 / op2 transposed
     const bool t3 = fu ? adj : !adj;                      // op3 transposed
-    size_t qa = 0;
-    for (size_t q = 0; q < fr.size(); ++q) {
-        const int k = fr[q];
-        const FrontInfo& f = P->fi[k];
-        const int nn = f.nn, na = f.na, nj = nn + na;
-        const Mat M1 = inv ? L11[q] : X11[q];
-        const i64 tu = sc.take((i64)nj * nn);
-        add_matop(sy, MO_SYMM, 0, f.blkptr, nj, 0, f.blkptr, nj, nn, nn);
-        add_gemm_tiles(P, g1, mk(0, f.blkptr, nj, 0, M1.buf, M1.off, M1.ld, t1 ? 1 : 0,
-                                 2, tu, nj, nj, nn, nn, 1.0, 0.0, 0, 0, t1 ? 2 : 1), false);
-        add_gemm_tiles(P, g2, mk(M1.buf, M1.off, M1.ld, t2 ? 1 : 0, 2, tu, nj, 0,
-                                 0, f.blkptr, nj, nn, nn, nn, 1.0, 0.0, 2, t2 ? 2 : 1), false);
-        if (na > 0) {
-            const Mat M3 = inv ? Xv[qa] : V[qa];
-            add_gemm_tiles(P, g2, mk(M3.buf, M3.off, M3.ld, t3 ? 1 : 0, 2, tu + nn, nj, 0,
-                                     0, f.blkptr + nn, nj, na, nn, na, 1.0, 0.0, 0, t3 ? 2 : 1),
+    for (int u = 0; u < nb; ++u) {
+        const i64 ub = (i64)u * bstride;
+        size_t qa = 0;
+        for (size_t q = 0; q < fr.size(); ++q) {
+            const int k = fr[q];
+            const FrontInfo& f = P->fi[k];
+            const int nn = f.nn, na = f.na, nj = nn + na;
+            const Mat M1 = inv ? L11[q] : X11[q];
+            const i64 tu = sc.take((i64)nj * nn);
+            add_matop(sy, MO_SYMM, 0, ub + f.blkptr, nj, 0, ub + f.blkptr, nj, nn, nn);
+            add_gemm_tiles(P, g1, mk(0, ub + f.blkptr, nj, 0, M1.buf, M1.off, M1.ld, t1 ? 1 : 0,
+                                     2, tu, nj, nj, nn, nn, 1.0, 0.0, 0, 0, t1 ? 2 : 1), false);
+            add_gemm_tiles(P, g2, mk(M1.buf, M1.off, M1.ld, t2 ? 1 : 0, 2, tu, nj, 0,
+                                     0, ub + f.blkptr, nj, nn, nn, nn, 1.0, 0.0, 2, t2 ? 2 : 1),
                            false);
-            ++qa;
+            if (na > 0) {
+                const Mat M3 = inv ? Xv[qa] : V[qa];
+                add_gemm_tiles(P, g2, mk(M3.buf, M3.off, M3.ld, t3 ? 1 : 0, 2, tu + nn, nj, 0,
+                                         0, ub + f.blkptr + nn, nj, na, nn, na, 1.0, 0.0, 0,
+                                         t3 ? 2 : 1),
+                               false);
+                ++qa;
+            }
         }
     }
     push_tasks(P, dir, K_MATOP, sy);
@@ -378,20 +384,50 @@ void build_scale_level(chp_plan* P, std::vector<Launch>& sh, std::vector<Launch>
     push_gemm(P, dir, g2);
 }
 
-// One level of _Y2K (hessian.c:3-101), leaves to root, for one direction.
-void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv) {
+// Copies of a GEMM task for nb directions: operands on a per-direction
+// buffer (str[buf] != 0) move str[buf] doubles per direction.
+void add_gemm_dirs(chp_plan* P, std::vector<GemmTile>& tiles, const GemmTask& t,
+                   bool skip_upper, int nb, const i64* str) {
+    for (int u = 0; u < nb; ++u) {
+        GemmTask c = t;
+        c.offA += u * str[c.bufA];
+        c.offB += u * str[c.bufB];
+        c.offC += u * str[c.bufC];
+        add_gemm_tiles(P, tiles, c, skip_upper);
+    }
+}
+
+// The extend-add / extract launches appended to seq since `from` cover nb
+// directions (grid y).
+void mark_dirs(std::vector<Launch>& seq, size_t from, int nb, const i64* str) {
+    for (size_t i = from; i < seq.size(); ++i)
+        if (seq[i].kid == K_EXTADD || seq[i].kid == K_EXTRACT) {
+            seq[i].gy = nb;
+            seq[i].bstride = str[0];
+            seq[i].ustride = str[1];
+        }
+}
+
+// One level of _Y2K (hessian.c:3-101), leaves to root, for nb directions:
+// str[0] / str[1] = the distance of consecutive directions' blkval / update
+// arena (buffers 0 and 1).
+void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv,
+                     int nb, const i64* str) {
     const double alpha = inv ? 1.0 : -1.0;
     int net = 0;
     const std::vector<int4> et = extadd_tiles(P, fr, &net);
+    const size_t from = seq.size();
     if (!inv) {
         Launch L{K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1};
         seq.push_back(L);
     } else {
         std::vector<MatOp> z;
-        for (int k : fr) {
-            const FrontInfo& f = P->fi[k];
-            if (f.na > 0) add_matop(z, MO_ZERO, 1, f.updptr, f.na, 1, f.updptr, f.na, f.na, f.na);
-        }
+        for (int u = 0; u < nb; ++u)
+            for (int k : fr) {
+                const FrontInfo& f = P->fi[k];
+                const i64 o = (i64)u * str[1] + f.updptr;
+                if (f.na > 0) add_matop(z, MO_ZERO, 1, o, f.na, 1, o, f.na, f.na, f.na);
+            }
         push_tasks(P, seq, K_MATOP, z);
     }
     std::vector<GemmTile> g1, g2, g3;
@@ -401,14 +437,14 @@ void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
         if (na == 0) continue;
         const i64 b21 = f.blkptr + nn;
         // F22 += alpha L21 F21^T
-        add_gemm_tiles(P, g1, mk(5, b21, nj, 0, 0, b21, nj, 1, 1, f.updptr, na,
-                                 na, na, nn, alpha, 1.0, 1), true);
+        add_gemm_dirs(P, g1, mk(5, b21, nj, 0, 0, b21, nj, 1, 1, f.updptr, na,
+                                na, na, nn, alpha, 1.0, 1), true, nb, str);
         // F21 += alpha L21 sym(F11)
-        add_gemm_tiles(P, g2, mk(5, b21, nj, 0, 0, f.blkptr, nj, 2, 0, b21, nj,
-                                 na, nn, nn, alpha, 1.0, 0), false);
+        add_gemm_dirs(P, g2, mk(5, b21, nj, 0, 0, f.blkptr, nj, 2, 0, b21, nj,
+                                na, nn, nn, alpha, 1.0, 0), false, nb, str);
         // F22 += alpha F21 L21^T
-        add_gemm_tiles(P, g3, mk(0, b21, nj, 0, 5, b21, nj, 1, 1, f.updptr, na,
-                                 na, na, nn, alpha, 1.0, 1), true);
+        add_gemm_dirs(P, g3, mk(0, b21, nj, 0, 5, b21, nj, 1, 1, f.updptr, na,
+                                na, na, nn, alpha, 1.0, 1), true, nb, str);
     }
     push_gemm(P, seq, g1);
     push_gemm(P, seq, g2);
@@ -417,12 +453,15 @@ void build_y2k_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
         Launch L{K_EXTADD, net, 256, 0, P->pg->lists.add(et), 0};
         seq.push_back(L);
     }
+    mark_dirs(seq, from, nb, str);
 }
 
-// One level of _M2T (hessian.c:103-198), root to leaves, for one direction.
+// One level of _M2T (hessian.c:103-198), root to leaves, for nb directions
+// (str as for _Y2K).
 void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr, bool inv,
-                     Bump& sc) {
+                     Bump& sc, int nb, const i64* str) {
     const double alpha = inv ? 1.0 : -1.0;
+    const size_t from = seq.size();
     if (inv) {
         Bump dv;  // unused: unfactored updates need no block inverses
         emit_child_updates(P, seq, fr, false, 0, sc, dv);  // before this level's update
@@ -434,14 +473,14 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
         if (na == 0) continue;
         const i64 b21 = f.blkptr + nn;
         // F11 += alpha F21^T L21 (lower)
-        add_gemm_tiles(P, g1, mk(0, b21, nj, 1, 5, b21, nj, 0, 0, f.blkptr, nj,
-                                 nn, nn, na, alpha, 1.0, 1), true);
+        add_gemm_dirs(P, g1, mk(0, b21, nj, 1, 5, b21, nj, 0, 0, f.blkptr, nj,
+                                nn, nn, na, alpha, 1.0, 1), true, nb, str);
         // F21 += alpha V_k L21 (V_k stored with both triangles)
-        add_gemm_tiles(P, g2, mk(1, f.updptr, na, 0, 5, b21, nj, 0, 0, b21, nj,
-                                 na, nn, na, alpha, 1.0, 0), false);
+        add_gemm_dirs(P, g2, mk(1, f.updptr, na, 0, 5, b21, nj, 0, 0, b21, nj,
+                                na, nn, na, alpha, 1.0, 0), false, nb, str);
         // F11 += alpha L21^T F21 (lower)
-        add_gemm_tiles(P, g3, mk(5, b21, nj, 1, 0, b21, nj, 0, 0, f.blkptr, nj,
-                                 nn, nn, na, alpha, 1.0, 1), true);
+        add_gemm_dirs(P, g3, mk(5, b21, nj, 1, 0, b21, nj, 0, 0, f.blkptr, nj,
+                                nn, nn, na, alpha, 1.0, 1), true, nb, str);
     }
     push_gemm(P, seq, g1);
     push_gemm(P, seq, g2);
@@ -450,6 +489,7 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     // here at the parents' level (an update slot is touched only at its
     // front's level and its parent's)
     if (!inv) push_tasks(P, seq, K_EXTRACT, extract_tiles(P, children_of(P, fr)), 0);
+    mark_dirs(seq, from, nb, str);
 }
 
 int ensure_dense(chp_plan* P, hipStream_t s) {
@@ -543,20 +583,31 @@ int build_completion(chp_plan* P, int fu, i64& sc_req, i64& dv_req) {
     return 0;
 }
 
-int build_scale(chp_plan* P, int idx, i64& sc_req, i64& dv_req) {
+// Distances of consecutive directions of a chunk: blkval (buffer 0) and
+// update arena (buffer 1), rounded to 256-byte multiples; 0 for the other
+// buffers.
+void dir_strides(const chp_plan* P, i64* str) {
+    for (int i = 0; i < 8; ++i) str[i] = 0;
+    str[0] = (P->blk_len + 31) / 32 * 32;
+    str[1] = (P->upd_len + 31) / 32 * 32;
+}
+
+int build_scale(chp_plan* P, int idx, int nb, i64& sc_req, i64& dv_req) {
     DenseBuild b(P);
     const bool fu = idx & 4, inv = idx & 2, adj = idx & 1;
-    auto& S = P->scale[idx];
+    auto& S = P->scale[idx][nb];
     S.shared.assign(P->nlevels, {});
     S.shared_t.assign(P->nlevels, {});
     S.dir.assign(P->nlevels, {});
+    i64 str[8];
+    dir_strides(P, str);
     // the scaling sweeps keep every update slot (d_supd): built on the
     // one-slot-per-front table, and their front kernels read that table
     std::swap(P->fi, P->fi_full);
     for (int L = P->nlevels - 1; L >= 0; --L) {
         Bump sc, dv;
         build_scale_level(P, S.shared[L], S.shared_t[L], S.dir[L], P->bylevel[L], fu, inv, adj,
-                          sc, dv);
+                          sc, dv, nb, str[0]);
         b.level_done(sc, dv);
         for (auto* v : {&S.shared[L], &S.shared_t[L], &S.dir[L]})
             for (Launch& l : *v) l.fisel = 1;
@@ -567,21 +618,28 @@ int build_scale(chp_plan* P, int idx, i64& sc_req, i64& dv_req) {
     return 0;
 }
 
-int build_y2k(chp_plan* P, int inv) {
+int build_y2k(chp_plan* P, int inv, int nb) {
     DenseBuild b(P);
-    for (int L = 0; L < P->nlevels; ++L) build_y2k_level(P, P->y2k[inv].seq, P->bylevel[L], inv != 0);
-    P->y2k[inv].built = true;
+    i64 str[8];
+    dir_strides(P, str);
+    auto& S = P->y2k[inv][nb];
+    for (int L = 0; L < P->nlevels; ++L)
+        build_y2k_level(P, S.seq, P->bylevel[L], inv != 0, nb, str);
+    S.built = true;
     return 0;
 }
 
-int build_m2t(chp_plan* P, int inv, i64& sc_req) {
+int build_m2t(chp_plan* P, int inv, int nb, i64& sc_req) {
     DenseBuild b(P);
+    i64 str[8];
+    dir_strides(P, str);
+    auto& S = P->m2t[inv][nb];
     for (int L = P->nlevels - 1; L >= 0; --L) {
         Bump sc, dv;
-        build_m2t_level(P, P->m2t[inv].seq, P->bylevel[L], inv != 0, sc);
+        build_m2t_level(P, S.seq, P->bylevel[L], inv != 0, sc, nb, str);
         b.level_done(sc, dv);
     }
-    P->m2t[inv].built = true;
+    S.built = true;
     sc_req = std::max(sc_req, b.sc_need);
     return 0;
 }
@@ -628,52 +686,88 @@ int chp_hessian(chp_plan* P, const double* lblkval, const double* yblkval, doubl
         chp_set_error("null argument");
         return -1;
     }
+    if (nu <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     const int fu = factored ? 1 : 0, iv = inv ? 1 : 0;
     const int a0 = adj < 0 ? iv : (adj ? 1 : 0);
     const int npass = adj < 0 ? 2 : 1;
+    i64 str[8];
+    dir_strides(P, str);
+    // Directions go in chunks of up to HESS_CHUNK, stacked in one device
+    // array (blkval bstride apart, each with its own update arena): every
+    // launch of a level covers the whole chunk.  The chunk's device memory is
+    // capped at HESS_BYTES.
+    constexpr int HESS_CHUNK = 32;
+    constexpr double HESS_BYTES = 16e9;
+    int nbmax = std::min(nu, HESS_CHUNK);
+    while (nbmax > 1 && 8.0 * nbmax * (double)(str[0] + str[1]) > HESS_BYTES) --nbmax;
     i64 sc = 0, dv = 0;
-    for (int pass = 0; pass < npass; ++pass) {
-        const int a = pass == 0 ? a0 : 1 - a0;
-        const int idx = 4 * fu + 2 * iv + a;
-        if (!P->scale[idx].built) build_scale(P, idx, sc, dv);
-        if (a) { if (!P->m2t[iv].built) build_m2t(P, iv, sc); }
-        else if (!P->y2k[iv].built) build_y2k(P, iv);
-    }
+    std::vector<int> sizes = {nbmax};
+    if (nu % nbmax) sizes.push_back(nu % nbmax);
+    for (int nb : sizes)
+        for (int pass = 0; pass < npass; ++pass) {
+            const int a = pass == 0 ? a0 : 1 - a0;
+            const int idx = 4 * fu + 2 * iv + a;
+            if (!P->scale[idx][nb].built) build_scale(P, idx, nb, sc, dv);
+            if (a) { if (!P->m2t[iv][nb].built) build_m2t(P, iv, nb, sc); }
+            else if (!P->y2k[iv][nb].built) build_y2k(P, iv, nb);
+        }
     if (prepare(P, sc, dv, s)) return -2;
     if (!P->d_supd && alloc_doubles(&P->d_supd, P->upd_full_len)) return -2;
+    const i64 need_u = (i64)nbmax * str[0], need_a = (i64)nbmax * str[1];
+    if (P->ustack_cap < need_u || P->uupd_cap < need_a) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (P->ustack_cap < need_u) {
+            if (P->d_ustack) HIP_CHECK(hipFree(P->d_ustack));
+            P->d_ustack = nullptr;
+            if (alloc_doubles(&P->d_ustack, need_u)) return -2;
+            P->ustack_cap = need_u;
+        }
+        if (P->uupd_cap < need_a) {
+            if (P->d_uupd) HIP_CHECK(hipFree(P->d_uupd));
+            P->d_uupd = nullptr;
+            if (alloc_doubles(&P->d_uupd, need_a)) return -2;
+            P->uupd_cap = need_a;
+        }
+    }
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
     // The scaling sweeps keep their update matrices (factored or not; the
     // same for adj and its adjoint) in d_supd, which the M2 sweeps between
-    // them leave alone: the second scaling of a call reuses them.
+    // them leave alone: after the first scaling sweep of the call the others
+    // (later chunks, the second pass of adj=None) reuse them.
     bool have_upd = false;
-    auto run_scale = [&](int idx) -> int {
-        const auto& S = P->scale[idx];
-        for (int L = P->nlevels - 1; L >= 0; --L) {
-            BufTab bt = dense_tab(P, nullptr, lblkval, yblkval);
-            bt.p[1] = P->d_supd;
-            int rc = have_upd ? 0 : run_seq(P, P->dense, S.shared[L], bt, s);
-            if (!rc) rc = run_seq(P, P->dense, S.shared_t[L], bt, s);
-            if (rc) return rc;
-            for (int u = 0; u < nu; ++u) {
-                bt.p[0] = ublkval[u];
-                rc = run_seq(P, P->dense, S.dir[L], bt, s);
+    for (int u0 = 0; u0 < nu; u0 += nbmax) {
+        const int nb = std::min(nbmax, nu - u0);
+        for (int u = 0; u < nb; ++u)
+            HIP_CHECK(hipMemcpyAsync(P->d_ustack + u * str[0], ublkval[u0 + u],
+                                     P->blk_len * sizeof(double), hipMemcpyDeviceToDevice, s));
+        auto run_scale = [&](int idx) -> int {
+            const auto& S = P->scale[idx][nb];
+            for (int L = P->nlevels - 1; L >= 0; --L) {
+                BufTab bt = dense_tab(P, P->d_ustack, lblkval, yblkval);
+                bt.p[1] = P->d_supd;
+                int rc = have_upd ? 0 : run_seq(P, P->dense, S.shared[L], bt, s);
+                if (!rc) rc = run_seq(P, P->dense, S.shared_t[L], bt, s);
+                if (!rc) rc = run_seq(P, P->dense, S.dir[L], bt, s);
                 if (rc) return rc;
             }
+            have_upd = true;
+            return 0;
+        };
+        for (int pass = 0; pass < npass; ++pass) {
+            const int a = pass == 0 ? a0 : 1 - a0;
+            const int idx = 4 * fu + 2 * iv + a;
+            int rc = 0;
+            if (a != iv && (rc = run_scale(idx))) return rc;
+            const auto& seq = a ? P->m2t[iv][nb].seq : P->y2k[iv][nb].seq;
+            BufTab bt = dense_tab(P, P->d_ustack, lblkval, yblkval);
+            bt.p[1] = P->d_uupd;
+            if ((rc = run_seq(P, P->dense, seq, bt, s))) return rc;
+            if (a == iv && (rc = run_scale(idx))) return rc;
         }
-        have_upd = true;
-        return 0;
-    };
-    for (int pass = 0; pass < npass; ++pass) {
-        const int a = pass == 0 ? a0 : 1 - a0;
-        const int idx = 4 * fu + 2 * iv + a;
-        int rc = 0;
-        if (a != iv && (rc = run_scale(idx))) return rc;
-        const auto& seq = a ? P->m2t[iv].seq : P->y2k[iv].seq;
-        for (int u = 0; u < nu; ++u)
-            if ((rc = run_seq(P, P->dense, seq, dense_tab(P, ublkval[u], lblkval, yblkval), s)))
-                return rc;
-        if (a == iv && (rc = run_scale(idx))) return rc;
+        for (int u = 0; u < nb; ++u)
+            HIP_CHECK(hipMemcpyAsync(ublkval[u0 + u], P->d_ustack + u * str[0],
+                                     P->blk_len * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     // a failed update factorization is reported in info but, as in the
     // reference (hessian.c:354-374 returns 0), is not an error

@@ -247,6 +247,29 @@ def test_hessian_modes(name, factored):
                 assert e <= TOL, (adj, inv)
 
 
+@pytest.mark.parametrize("factored", [True, False])
+def test_hessian_direction_chunks(factored):
+    """35 directions: the Hessian maps run on chunks of directions stacked on
+    the device (32, then 3); every direction against the oracle, for the
+    full map (adj=None) and its inverse, and G_X^adj."""
+    import torch
+    symb, L, Y = _factor_pair("grid3d_12")
+    Lc = cp.cspmatrix(symb, blkval=torch.as_tensor(L, device="cuda:0"), factor=True)
+    Yc = cp.cspmatrix(symb, blkval=torch.as_tensor(Y, device="cuda:0"))
+    rng = np.random.default_rng(12)
+    m = _lower_mask(symb)
+    for adj, inv in ((None, False), (None, True), (True, False)):
+        print("case hessian chunks", factored, adj, inv, flush=True)
+        us = [rng.standard_normal(len(L)) for _ in range(35)]
+        Us = [cp.cspmatrix(symb, blkval=torch.as_tensor(u.copy(), device="cuda:0")) for u in us]
+        cp.hessian(Lc, Yc, Us, adj=adj, inv=inv, factored_updates=factored)
+        refs = [u.copy() for u in us]
+        on.hessian(symb, L, Y, refs, adj=adj, inv=inv, factored_updates=factored)
+        e = max(_rel(U.blkval.cpu().numpy()[m], r[m]) for U, r in zip(Us, refs))
+        print("hessian chunks max rel err %.3e" % e, flush=True)
+        assert e <= TOL, (adj, inv)
+
+
 def test_logdet_and_gradient():
     """logdet from the GPU factor vs the dense slogdet; the autograd gradient
     (projected inverse) vs central finite differences, relative error <= 1e-6."""
