@@ -93,6 +93,7 @@ def _declare(L):
     _opt(L, "chp_spmv_residual_noalloc", [i64, vp, vp, vp, vp, i32, vp, i32, vp, i32, i32, vp],
          i32)
     _opt(L, "chp_completion", [vp, vp, i32, vp, vp], i32)
+    _opt(L, "chp_cholesky_adjoint", [vp, vp, vp, vp, vp, vp], i32)
     _opt(L, "chp_hessian", [vp, vp, vp, vp, i32, i32, i32, i32, vp, vp], i32)
     _opt(L, "chp_frontal_add_update", [vp, i64, vp, vp, vp, dbl, vp], i32)
     _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)

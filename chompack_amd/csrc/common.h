@@ -103,8 +103,12 @@ struct DiagTask {
 // the same for i >= j only; MO_COPY_LOWER_Z also writes 0 for i < j;
 // MO_RT dst(i, j) = src(n-1-j, n-1-i) for i >= j and 0 above (reversed
 // transpose of a lower triangle, square n); MO_SYMM dst(i, j) = dst(j, i)
-// for i < j; MO_ZERO_UPPER dst(i, j) = 0 for i < j; MO_ZERO dst = 0.
-enum MatOpKind { MO_COPY, MO_COPY_LOWER, MO_COPY_LOWER_Z, MO_RT, MO_SYMM, MO_ZERO_UPPER, MO_ZERO };
+// for i < j; MO_ZERO_UPPER dst(i, j) = 0 for i < j; MO_ZERO dst = 0;
+// MO_PHI dst(i, i) *= 1/2 and dst(i, j) = 0 for i < j (in place: the lower
+// triangle with the diagonal halved); MO_SYMAVG dst(i, j) = alpha (src(i, j)
+// + src(j, i)) for i >= j and 0 for i < j (square).
+enum MatOpKind { MO_COPY, MO_COPY_LOWER, MO_COPY_LOWER_Z, MO_RT, MO_SYMM, MO_ZERO_UPPER, MO_ZERO,
+                 MO_PHI, MO_SYMAVG };
 struct MatOp {
     i64 src, dst;
     int lds, ldd, m, n, i0, j0;

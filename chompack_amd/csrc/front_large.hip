@@ -556,6 +556,10 @@ __global__ __launch_bounds__(TPB) void k_matop(const MatOp* __restrict__ ops, Bu
         case MO_RT: *d = (i >= j) ? S[(o.n - 1 - j) + (i64)(o.n - 1 - i) * o.lds] : 0.0; break;
         case MO_SYMM: if (i < j) *d = D[j + (i64)i * o.ldd]; break;
         case MO_ZERO_UPPER: if (i < j) *d = 0.0; break;
+        case MO_PHI: if (i < j) *d = 0.0; else if (i == j) *d *= 0.5; break;
+        case MO_SYMAVG:
+            *d = (i >= j) ? o.alpha * (S[i + (i64)j * o.lds] + S[j + (i64)i * o.lds]) : 0.0;
+            break;
         default: *d = 0.0; break;
         }
     }

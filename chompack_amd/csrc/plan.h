@@ -165,6 +165,7 @@ struct chp_plan {
     // the Hessian maps work on a chunk of nb directions at once; their
     // sequences are built per chunk size
     Seq comp[2];
+    Seq chol_adj;  // reverse-mode sweep of the Cholesky factorization
     std::map<int, Seq> y2k[2], m2t[2];
     std::map<int, LevelSeq> scale[8];
     i64 blk_len = 0;            // blkptr[nsn]

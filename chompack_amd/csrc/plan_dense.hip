@@ -42,7 +42,7 @@ void add_matop(std::vector<MatOp>& ops, MatOpKind kind, int sbuf, i64 src, int l
             const bool above = i0 + 63 < j0;  // every element has i < j
             const bool below = i0 > j0 + 63;  // every element has i > j
             if (kind == MO_COPY_LOWER && above) continue;
-            if ((kind == MO_SYMM || kind == MO_ZERO_UPPER) && below) continue;
+            if ((kind == MO_SYMM || kind == MO_ZERO_UPPER || kind == MO_PHI) && below) continue;
             MatOp o;
             std::memset(&o, 0, sizeof(o));
             o.src = src; o.dst = dst; o.lds = lds; o.ldd = ldd; o.m = m; o.n = n;
@@ -492,6 +492,78 @@ void build_m2t_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<in
     mark_dirs(seq, from, nb, str);
 }
 
+// One level of the Cholesky adjoint sweep (the reverse of cholesky.c:26-82,
+// root to leaves).  Buffers: 0 the output Fbar (blkval-shaped), 5 the factor
+// [L11; B21], 6 the gradient [Gb11; Gb21] of the stored factor, 1 the update
+// arena, whose slot k holds Ub_k (both triangles) from the parent's level.
+// Per front, X11 = L11^{-1}, L21 = B21 L11, T = Gb21 X11^T:
+//   M = tril(Gb11) - B21^T T;  T -= 2 Ub L21;  Fbar21 = T X11 / 2;
+//   M -= 2 Fbar21^T L21;  P = Phi(L11^T tril(M));  S = X11^T (P X11);
+//   Fbar11 = lower((S + S^T) / 2);
+// then the children's Ub_c = Fbar[r_c, r_c] (k_extract on buffer 0 and the
+// arena).  The gradient with respect to X's lower-stored entries is Fbar
+// with the strictly-lower entries doubled.
+void build_chol_adjoint_level(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
+                              Bump& sc) {
+    std::vector<Mat> Ls, Xs;
+    std::vector<i64> Ws;
+    std::vector<int> keys;
+    std::vector<MatOp> m0, phi, sym;
+    std::vector<GemmTile> ga, gb, gc, gd, ge, gf, gg, gh;
+    for (int k : fr) {
+        const FrontInfo& f = P->fi[k];
+        const int nn = f.nn, na = f.na, nj = nn + na;
+        const i64 bp = f.blkptr;
+        const i64 nn2 = (i64)nn * nn;
+        const i64 x = sc.take(nn2), w = sc.take(nn2), m = sc.take(nn2), p = sc.take(nn2),
+                  w2 = sc.take(nn2), s = sc.take(nn2);
+        Ls.push_back({5, bp, nj, nn});
+        Xs.push_back({2, x, nn, nn});
+        Ws.push_back(w);
+        keys.push_back(k);
+        add_matop(m0, MO_COPY_LOWER_Z, 6, bp, nj, 2, m, nn, nn, nn);
+        if (na > 0) {
+            const i64 l21 = sc.take((i64)na * nn), t = sc.take((i64)na * nn);
+            // L21 = B21 L11 (L11 lower), T = Gb21 X11^T (X11^T upper)
+            add_gemm_tiles(P, ga, mk(5, bp + nn, nj, 0, 5, bp, nj, 0, 2, l21, na,
+                                     na, nn, nn, 1.0, 0.0, 0, 0, 1), false);
+            add_gemm_tiles(P, ga, mk(6, bp + nn, nj, 0, 2, x, nn, 1, 2, t, na,
+                                     na, nn, nn, 1.0, 0.0, 0, 0, 2), false);
+            // M -= B21^T T
+            add_gemm_tiles(P, gb, mk(5, bp + nn, nj, 1, 2, t, na, 0, 2, m, nn,
+                                     nn, nn, na, -1.0, 1.0, 0), false);
+            // T -= 2 Ub L21 (Ub symmetric, both triangles stored)
+            add_gemm_tiles(P, gc, mk(1, f.updptr, na, 0, 2, l21, na, 0, 2, t, na,
+                                     na, nn, na, -2.0, 1.0, 0), false);
+            // Fbar21 = T X11 / 2 (X11 lower)
+            add_gemm_tiles(P, gd, mk(2, t, na, 0, 2, x, nn, 0, 0, bp + nn, nj,
+                                     na, nn, nn, 0.5, 0.0, 0, 0, 1), false);
+            // M -= 2 Fbar21^T L21
+            add_gemm_tiles(P, ge, mk(0, bp + nn, nj, 1, 2, l21, na, 0, 2, m, nn,
+                                     nn, nn, na, -2.0, 1.0, 0), false);
+        }
+        // P = L11^T tril(M) (lower part), then Phi
+        add_gemm_tiles(P, gf, mk(5, bp, nj, 1, 2, m, nn, 0, 2, p, nn,
+                                 nn, nn, nn, 1.0, 0.0, 1, 2, 1), true);
+        add_matop(phi, MO_PHI, 2, p, nn, 2, p, nn, nn, nn);
+        // W2 = P X11 (both lower), S = X11^T W2
+        add_gemm_tiles(P, gg, mk(2, p, nn, 0, 2, x, nn, 0, 2, w2, nn,
+                                 nn, nn, nn, 1.0, 0.0, 0, 1, 1), false);
+        add_gemm_tiles(P, gh, mk(2, x, nn, 1, 2, w2, nn, 0, 2, s, nn,
+                                 nn, nn, nn, 1.0, 0.0, 0, 2, 0), false);
+        add_matop(sym, MO_SYMAVG, 2, s, nn, 0, bp, nj, nn, nn, 0.5);
+    }
+    emit_trtri(P, seq, Ls, Xs, Ws, keys);
+    push_tasks(P, seq, K_MATOP, m0);
+    for (auto* g : {&ga, &gb, &gc, &gd, &ge, &gf}) push_gemm(P, seq, *g);
+    push_tasks(P, seq, K_MATOP, phi);
+    push_gemm(P, seq, gg);
+    push_gemm(P, seq, gh);
+    push_tasks(P, seq, K_MATOP, sym);
+    // the children's Ub from this level's finished Fbar
+    push_tasks(P, seq, K_EXTRACT, extract_tiles(P, children_of(P, fr)), 0);
+}
+
 int ensure_dense(chp_plan* P, hipStream_t s) {
     if (sync_program(P->dense)) return -2;
     struct { double** d; i64* have; i64 need; } bufs[] = {
@@ -618,6 +690,18 @@ int build_scale(chp_plan* P, int idx, int nb, i64& sc_req, i64& dv_req) {
     return 0;
 }
 
+int build_chol_adjoint(chp_plan* P, i64& sc_req) {
+    DenseBuild b(P);
+    for (int L = P->nlevels - 1; L >= 0; --L) {
+        Bump sc, dv;
+        build_chol_adjoint_level(P, P->chol_adj.seq, P->bylevel[L], sc);
+        b.level_done(sc, dv);
+    }
+    P->chol_adj.built = true;
+    sc_req = std::max(sc_req, b.sc_need);
+    return 0;
+}
+
 int build_y2k(chp_plan* P, int inv, int nb) {
     DenseBuild b(P);
     i64 str[8];
@@ -676,6 +760,20 @@ int chp_completion(chp_plan* P, double* blkval, int factored, void* stream, i64*
     if (prepare(P, sc, dv, s)) return -2;
     hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
     int rc = run_seq(P, P->dense, P->comp[fu].seq, dense_tab(P, blkval, nullptr, nullptr), s);
+    if (rc) return rc;
+    return read_status(P, s, info);
+}
+
+int chp_cholesky_adjoint(chp_plan* P, const double* lblkval, const double* gblkval, double* out,
+                         void* stream, i64* info) {
+    if (!P || !lblkval || !gblkval || !out) { chp_set_error("null argument"); return -1; }
+    hipStream_t s = (hipStream_t)stream;
+    i64 sc = 0, dv = 0;
+    if (!P->chol_adj.built) build_chol_adjoint(P, sc);
+    if (prepare(P, sc, dv, s)) return -2;
+    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
+    BufTab bt = dense_tab(P, out, lblkval, gblkval);
+    int rc = run_seq(P, P->dense, P->chol_adj.seq, bt, s);
     if (rc) return rc;
     return read_status(P, s, info);
 }

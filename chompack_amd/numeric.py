@@ -14,7 +14,7 @@ import numpy as np
 from . import _native
 from .symbolic import cspmatrix
 
-__all__ = ["Plan", "cholesky", "llt", "projected_inverse", "completion", "hessian", "trmm",
+__all__ = ["Plan", "cholesky", "cholesky_adjoint", "llt", "projected_inverse", "completion", "hessian", "trmm",
            "trsm", "dot", "logdet", "logdiag", "copy_blkval"]
 
 SMALL_MAX = 64  # largest clique order handled by the LDS-resident front kernels
@@ -127,6 +127,34 @@ def projected_inverse(L):
     _raise_rc(rc, "projected_inverse")
     if rc == 1:
         raise ArithmeticError("partial inverse failed")
+
+
+def cholesky_adjoint(L, G):
+    """Reverse-mode sweep of ``cholesky`` (cholesky.c:26-82 run backwards):
+    L is a factor (stored convention), G a cspmatrix on the same symbolic
+    object holding the gradient of a scalar loss with respect to L's stored
+    entries (lower trapezoid of each block).  Returns a new cspmatrix Fbar:
+    per supernode the lower trapezoid of the symmetric gradient of its
+    frontal matrix; the gradient with respect to the lower-stored entries of
+    the factored matrix is Fbar with its strictly-lower entries doubled.
+    Raises ArithmeticError on a zero pivot of L."""
+    torch = _torch()
+    if not (isinstance(L, cspmatrix) and L.is_factor is True):
+        raise ValueError("L must be a cspmatrix factor")
+    if not isinstance(G, cspmatrix) or G.symb is not L.symb:
+        raise ValueError("symbolic factorizations must be the same")
+    plan, lb = _plan(L)
+    gb = _check_blkval(G)
+    if gb.device != lb.device:
+        raise ValueError("all blkval arrays must be on the same device")
+    out = torch.empty_like(lb)
+    info = np.zeros(2, dtype=np.int64)
+    rc = _native.lib().chp_cholesky_adjoint(plan.handle, lb.data_ptr(), gb.data_ptr(),
+                                            out.data_ptr(), _stream(lb), _native.ptr(info))
+    _raise_rc(rc, "cholesky_adjoint")
+    if rc == 1:
+        raise ArithmeticError("factor has a zero pivot")
+    return cspmatrix(L.symb, blkval=out)
 
 
 def completion(X, factored_updates=True):

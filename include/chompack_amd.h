@@ -146,6 +146,18 @@ int chp_llt(chp_plan* plan, double* blkval, void* stream);
  * refinement (chp_spmv_residual + chp_trsm).  No reference counterpart. */
 int chp_cholesky_mixed(chp_plan* plan, double* blkval, void* stream, int64_t* info);
 
+/* Reverse-mode sweep of chp_cholesky (cholesky.c:26-82 backwards, root to
+ * leaves; no reference counterpart: the backward of a differentiable
+ * cholesky).  lblkval: the factor in the stored convention; gblkval: the
+ * gradient of a scalar loss with respect to that stored factor (lower
+ * trapezoid of each block read); out (blkval length, distinct from both):
+ * per supernode the lower trapezoid of the symmetric gradient of its
+ * frontal matrix.  The gradient with respect to the lower-stored entries of
+ * X is out with the strictly-lower entries doubled.  Returns 1 on a zero
+ * pivot of L11 (info as for chp_cholesky). */
+int chp_cholesky_adjoint(chp_plan* plan, const double* lblkval, const double* gblkval,
+                         double* out, void* stream, int64_t* info);
+
 /* Projected inverse Y = P(X^{-1}) from the factor; replaces
  * projected_inverse() (projected_inverse.c:3-75, cbase.c:470-474). */
 int chp_projected_inverse(chp_plan* plan, double* blkval, void* stream, int64_t* info);

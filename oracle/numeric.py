@@ -20,7 +20,7 @@ blkptr[k+1]], a column-major nj x nn array; ``_blk`` returns a writable
 import numpy as np
 import scipy.linalg as sla
 
-__all__ = ["cholesky", "llt", "projected_inverse", "completion", "update_factor", "hessian",
+__all__ = ["cholesky", "cholesky_adjoint", "blk_weights", "llt", "projected_inverse", "completion", "update_factor", "hessian",
            "trsm", "trmm", "pf_factor", "pf_solve", "pf_product", "dot", "logdet"]
 
 
@@ -157,6 +157,68 @@ def projected_inverse(symb, blkval):
             stack.append(_extract(F, S.rel(c)))
         _write_lower(B, F[:, :nn], nn)
     return 0
+
+
+def cholesky_adjoint(symb, lblkval, gblkval):
+    """Reverse-mode sweep of the multifrontal Cholesky (cholesky.c:26-82 run
+    backwards, root to leaves).  lblkval: the factor in the stored convention
+    ([L11; B21] per supernode, B21 = L21 L11^{-1}); gblkval: the gradient of a
+    scalar loss with respect to that stored factor (lower trapezoid read).
+    Returns fbar (blkval-shaped): per supernode the lower trapezoid of the
+    symmetric gradient of its assembled frontal matrix, so that the gradient
+    with respect to the lower-stored entries of X is fbar * w with w = 2 on
+    strictly-lower entries and 1 on the diagonal.  Per front, with X11 =
+    L11^{-1}, L21 = B21 L11 and Ub the symmetric gradient of its update
+    matrix (from the parent; 0 at a root):
+        T = Gb21 X11^T,  Lb21 = T - 2 Ub L21,  G21 = Lb21 X11,
+        Lb = tril(Gb11 - B21^T T - G21^T L21),  P = Phi(L11^T Lb),
+        S11 = sym(X11^T P X11),  Fbar = [S11, G21^T / 2; G21 / 2, Ub],
+    Phi = lower triangle with the diagonal halved; each child c receives
+    Ub_c = Fbar[r_c, r_c]."""
+    S = _S(symb)
+    out = np.zeros_like(lblkval)
+    ubar = {}
+    for k in reversed(S.snpost):
+        nn, na, nj = S.dims(k)
+        B = _blk(lblkval, S, k)
+        G = _blk(gblkval, S, k)
+        L11 = np.tril(B[:nn, :])
+        X11 = sla.solve_triangular(L11, np.eye(nn), lower=True)
+        B21 = B[nn:, :]
+        L21 = B21 @ L11
+        Ub = ubar.pop(k, np.zeros((na, na)))
+        T = G[nn:, :] @ X11.T
+        Lb21 = T - 2.0 * Ub @ L21
+        G21 = Lb21 @ X11
+        Lb = np.tril(np.tril(G[:nn, :]) - B21.T @ T - G21.T @ L21)
+        P = np.tril(L11.T @ Lb)
+        P[np.diag_indices(nn)] *= 0.5
+        Sm = X11.T @ P @ X11
+        F = np.zeros((nj, nj))
+        F[:nn, :nn] = 0.5 * (Sm + Sm.T)
+        F[nn:, :nn] = 0.5 * G21
+        F[:nn, nn:] = 0.5 * G21.T
+        F[nn:, nn:] = Ub
+        for c in S.children(k):
+            r = S.rel(c)
+            ubar[c] = F[np.ix_(r, r)]
+        _write_lower(_blk(out, S, k), F[:, :nn], nn)
+    return out
+
+
+def blk_weights(symb):
+    """w of cholesky_adjoint: 2 on strictly-lower entries of every block,
+    1 on the diagonal, 0 above it (the gradient with respect to a lower-stored
+    entry x_ij counts both x_ij and x_ji)."""
+    S = _S(symb)
+    w = np.zeros(int(S.blkptr[-1]))
+    for k in range(len(S.snptr) - 1):
+        nn, na, nj = S.dims(k)
+        Wk = _blk(w, S, k)
+        ii, jj = np.tril_indices(nj, -1, nn)
+        Wk[ii, jj] = 2.0
+        Wk[np.arange(nn), np.arange(nn)] = 1.0
+    return w
 
 
 def trsm(symb, lblkval, B, alpha=1.0, trans="N"):
