@@ -258,6 +258,14 @@ def differentiation_section(dev, with_cpu):
         cp.cholesky(Lt)
         return cp.numeric.logdiag(Lt)
     fd = -2.0 * float((logdiag_at(h) - logdiag_at(-h)).sum()) / (2 * h)
+    # loss = <C, stored factor> through autograd.cholesky: forward + the
+    # adjoint sweep
+    Cw = torch.as_tensor(rng.standard_normal(X.blkval.numel()), device=dev) * w
+
+    def chol_grad():
+        Xg = cp.cspmatrix(symb, blkval=X.blkval.clone().requires_grad_(True))
+        (cp.autograd.cholesky(Xg) * Cw).sum().backward()
+    t_cgrad = _timed(chol_grad, 3)
     L = X.copy()
     cp.cholesky(L)
     Y = L.copy()
@@ -280,7 +288,19 @@ def differentiation_section(dev, with_cpu):
         Xb.is_factor = False
         cp.completion(Xb)
     t_comp = _timed(compl, 1)
+    # gradient of the max log det over completions (llt backward) at the
+    # projected inverse of the band-plus-arrow matrix
+    Yb = Xb0.copy()
+    cp.cholesky(Yb)
+    cp.projected_inverse(Yb)
+
+    def maxdet_grad():
+        Yg = cp.cspmatrix(bsymb, blkval=Yb.blkval.clone().requires_grad_(True))
+        cp.autograd.maxdet(Yg).backward()
+    t_md = _timed(maxdet_grad, 1)
     out = {"grid32_n": int(symb.n), "gradient_ms": round(t_grad, 3),
+           "cholesky_forward_backward_ms": round(t_cgrad, 3),
+           "band_arrow_maxdet_gradient_ms": round(t_md, 1),
            "gradient_fd_rel_err": abs(dd - fd) / abs(dd), "hessian_32_directions_ms": round(t_hess, 2),
            "band_arrow_n": int(bsymb.n), "band_arrow_supernodes": int(bsymb.Nsn),
            "band_arrow_completion_ms": round(t_comp, 1),
