@@ -117,10 +117,15 @@ def batched_section(dev):
     ld64 = res["ld"]
     t32 = _timed(run(cp.mixed.cholesky), 10)
     err = float(np.max(np.abs(res["ld"] - ld64) / np.abs(ld64)))
+    tbf = _timed(run(lambda Z: cp.mixed.cholesky(Z, schur="bf16x3")), 10)
+    errbf = float(np.max(np.abs(res["ld"] - ld64) / np.abs(ld64)))
     return {"instances": 512, "n_total": int(bs.symb.n), "fp64_ms": round(t64, 3),
             "mixed_ms": round(t32, 3), "mixed_vs_fp64_speedup": round(t64 / t32, 3),
             "mixed_logdet_max_rel_err": err,
-            "what": "cholesky + per-instance logdet of the whole batch"}
+            "bf16x3_ms": round(tbf, 3), "bf16x3_vs_fp64_speedup": round(t64 / tbf, 3),
+            "bf16x3_logdet_max_rel_err": errbf,
+            "what": "cholesky + per-instance logdet of the whole batch (mixed: FP32 Schur "
+                    "updates; bf16x3: BF16x3 split-precision Schur updates)"}
 
 
 def rgg_section(dev):
@@ -150,22 +155,44 @@ def rgg_section(dev):
         cp.trsm(X, Bs)
         cp.trsm(X, Bs, trans="T")
 
-    def mixed():
-        cp.numeric.copy_blkval(X.blkval, X0.blkval)
-        X.is_factor = False
-        cp.mixed.cholesky(X)
-        _, h = cp.mixed.solve(Am, X, B0, tol=1e-10)
-        hist[:] = h
+    def mixed(schur):
+        def f():
+            cp.numeric.copy_blkval(X.blkval, X0.blkval)
+            X.is_factor = False
+            cp.mixed.cholesky(X, schur=schur)
+            _, h = cp.mixed.solve(Am, X, B0, tol=1e-10)
+            hist[:] = h
+        return f
+
+    def factor_only(schur):
+        def f():
+            cp.numeric.copy_blkval(X.blkval, X0.blkval)
+            X.is_factor = False
+            if schur is None:
+                cp.cholesky(X)
+            else:
+                cp.mixed.cholesky(X, schur=schur)
+        return f
     t64 = _timed(fp64, 3)
     xs = Bs.cpu().numpy()
     As = cp.symmetrize(A).tocsr()
     r64 = float(np.max(np.linalg.norm(As @ xs - Bh, axis=0) / np.linalg.norm(Bh, axis=0)))
-    t32 = _timed(mixed, 3)
+    t32 = _timed(mixed("fp32"), 3)
+    h32 = list(hist)
+    tbf = _timed(mixed("bf16x3"), 3)
+    hbf = list(hist)
+    f64, f32, fbf = (_timed(factor_only(m), 3) for m in (None, "fp32", "bf16x3"))
     return {"n": int(symb.n), "nnz_L": int(symb.nnz), "nrhs": 64,
             "fp64_ms": round(t64, 3), "fp64_residual": r64,
             "mixed_ms": round(t32, 3), "mixed_vs_fp64_speedup": round(t64 / t32, 3),
-            "mixed_residual_history": hist,
-            "what": "cholesky + solve of 64 N(0,1) right-hand sides, residual <= 1e-10"}
+            "mixed_residual_history": h32,
+            "bf16x3_ms": round(tbf, 3), "bf16x3_vs_fp64_speedup": round(t64 / tbf, 3),
+            "bf16x3_residual_history": hbf,
+            "factorization_only_ms": {"fp64": round(f64, 3), "fp32_schur": round(f32, 3),
+                                      "bf16x3_schur": round(fbf, 3)},
+            "what": "cholesky + solve of 64 N(0,1) right-hand sides, residual <= 1e-10 (mixed: "
+                    "FP32 Schur updates + FP64 refinement; bf16x3: BF16x3 split-precision Schur "
+                    "updates + FP64 refinement)"}
 
 
 def dense_completions_section(dev, with_cpu):
@@ -476,17 +503,31 @@ def main():
             cp.trsm(Lm, Bs)
             cp.trsm(Lm, Bs, trans="T")
 
-        hist = []
+        hists = {}
 
-        def mixed_path():
-            cp.numeric.copy_blkval(Lm.blkval, pristine)
-            Lm.is_factor = False
-            cp.mixed.cholesky(Lm)
-            x, h = cp.mixed.solve(Am, Lm, B0, tol=1e-10)
-            hist[:] = h
+        def mixed_path(schur):
+            def f():
+                cp.numeric.copy_blkval(Lm.blkval, pristine)
+                Lm.is_factor = False
+                cp.mixed.cholesky(Lm, schur=schur)
+                x, h = cp.mixed.solve(Am, Lm, B0, tol=1e-10)
+                hists[schur] = h
+            return f
+
+        def factor_path(schur):
+            def f():
+                cp.numeric.copy_blkval(Lm.blkval, pristine)
+                Lm.is_factor = False
+                if schur is None:
+                    cp.cholesky(Lm)
+                else:
+                    cp.mixed.cholesky(Lm, schur=schur)
+            return f
 
         res_ms = {}
-        for name, fn in (("fp64", fp64_path), ("mixed", mixed_path)):
+        for name, fn in (("fp64", fp64_path), ("mixed", mixed_path("fp32")),
+                         ("bf16x3", mixed_path("bf16x3")), ("f64", factor_path(None)),
+                         ("f32", factor_path("fp32")), ("fbf", factor_path("bf16x3"))):
             for _ in range(2):
                 fn()
             torch.cuda.synchronize()
@@ -498,8 +539,15 @@ def main():
         mixed = {"what": "cholesky + solve of 16 N(0,1) right-hand sides to max_j ||b_j - A x_j|| / ||b_j|| <= 1e-10",
                  "fp64_ms": round(res_ms["fp64"], 3), "mixed_ms": round(res_ms["mixed"], 3),
                  "mixed_vs_fp64_speedup": round(res_ms["fp64"] / res_ms["mixed"], 3),
-                 "mixed_residual_history": hist,
-                 "mixed": "FP32 MFMA Schur updates (k_gemm_tiles32); diagonal blocks, panels, solves and residuals FP64"}
+                 "mixed_residual_history": hists.get("fp32"),
+                 "mixed": "FP32 MFMA Schur updates (k_gemm_tiles32); diagonal blocks, panels, solves and residuals FP64",
+                 "bf16x3_ms": round(res_ms["bf16x3"], 3),
+                 "bf16x3_vs_fp64_speedup": round(res_ms["fp64"] / res_ms["bf16x3"], 3),
+                 "bf16x3_residual_history": hists.get("bf16x3"),
+                 "bf16x3": "BF16x3 split-precision Schur updates (k_gemm_tiles_bf3, FP32 accumulation)",
+                 "factorization_only_ms": {"fp64": round(res_ms["f64"], 3),
+                                           "fp32_schur": round(res_ms["f32"], 3),
+                                           "bf16x3_schur": round(res_ms["fbf"], 3)}}
 
     # instrumented step: HIP events around every launch of both sweeps
     plan = symb._plan(dev)

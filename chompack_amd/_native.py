@@ -89,6 +89,7 @@ def _declare(L):
     _opt(L, "chp_copy", [vp, vp, i64, vp], i32)
     _opt(L, "chp_factor_export", [vp, vp, vp, vp], i32)
     _opt(L, "chp_cholesky_mixed", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_cholesky_lowprec", [vp, vp, i32, vp, vp], i32)
     _opt(L, "chp_spmv_residual", [i64, vp, vp, vp, vp, i32, vp, i32, vp, i32, i32, vp, vp], i32)
     _opt(L, "chp_spmv_residual_noalloc", [i64, vp, vp, vp, vp, i32, vp, i32, vp, i32, i32, vp],
          i32)

@@ -926,6 +926,187 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles32(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same tiles in BF16x3 split precision (mixed-precision Schur updates):
+// each FP64 operand is rounded to FP32, f, and split as it is staged in LDS
+// into hi = bf16(f) and lo = bf16(f - hi) (round to nearest even; hi + lo
+// carries about 16 bits of f).  Each 16 x 16 x 16 block product takes three
+// v_mfma_f32_16x16x16bf16_1k (lo*hi, hi*lo, hi*hi; lo*lo is dropped) into
+// one FP32 accumulator; C = beta C + alpha acc is formed in FP64.  Relative
+// error about 2^-16 per product, removed by FP64 iterative refinement.
+// Operand layout of the MFMA: lane l holds row (A) / column (B) l % 16 at
+// k = 4 (l / 16) .. +3, so LDS keeps each row's k values contiguous and a
+// lane reads its four bf16 values in one 8-byte load.
+typedef short bf4 __attribute__((ext_vector_type(4)));
+#define BKP (BK + 4)   // LDS row length (bf16 elements) of the split tiles
+
+__device__ inline void split_bf16(double x, unsigned short& hi, unsigned short& lo) {
+    const float f = (float)x;
+    const unsigned u = __float_as_uint(f);
+    const unsigned h = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    const float d = f - __uint_as_float(h << 16);  // exact in FP32
+    const unsigned v = __float_as_uint(d);
+    hi = (unsigned short)h;
+    lo = (unsigned short)((v + 0x7fffu + ((v >> 16) & 1u)) >> 16);
+}
+
+template <int OPA, int OPB, bool CHK>
+__device__ inline void gemm_tile_bf3(const GemmTask& T, const double* __restrict__ A,
+                                     const double* __restrict__ B, double* __restrict__ C,
+                                     int row0, int col0, int kbase, int kend, double* Pp,
+                                     unsigned short (*Ah)[BKP], unsigned short (*Al)[BKP],
+                                     unsigned short (*Bh)[BKP], unsigned short (*Bl)[BKP]) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    constexpr bool AK = (OPA == 1);
+    constexpr bool BK_ = (OPB != 1);
+    const int ai = AK ? (tid >> 5) : (tid & 63);
+    const int ak = AK ? (tid & 31) : (tid >> 6);
+    const int bj = BK_ ? (tid >> 5) : (tid & 63);
+    const int bk = BK_ ? (tid & 31) : (tid >> 6);
+    double ra[8], rb[8];
+    const bool tile_in = row0 + TILE <= T.M && col0 + TILE <= T.N;
+    auto chunk_plain = [&](int kb) -> bool {
+        if (!CHK) return true;
+        if (!tile_in || kb + BK > T.K) return false;
+        if (T.triA == 1 && !(kb + BK - 1 <= row0)) return false;
+        if (T.triA == 2 && !(kb >= row0 + TILE - 1)) return false;
+        if (T.pad == 1 && !(kb >= col0 + TILE - 1)) return false;
+        if (T.pad == 2 && !(kb + BK - 1 <= col0)) return false;
+        return OPA != 2 && (OPB != 2 || kb >= col0 + TILE - 1);
+    };
+    auto fetch = [&](int kb) {
+        const bool plain = chunk_plain(kb);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+            const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+            if (plain) {
+                ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+                rb[q] = (OPB == 1) ? B[gj + (i64)gk2 * T.ldb] : B[gk2 + (i64)gj * T.ldb];
+            } else {
+                ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+                rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+            }
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = ai + (AK ? 8 * q : 0), k = ak + (AK ? 0 : 4 * q);
+            split_bf16(ra[q], Ah[i][k], Al[i][k]);
+            const int j = bj + (BK_ ? 8 * q : 0), k2 = bk + (BK_ ? 0 : 4 * q);
+            split_bf16(rb[q], Bh[j][k2], Bl[j][k2]);
+        }
+    };
+    flt4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (flt4){0.f, 0.f, 0.f, 0.f};
+    if (kbase < kend) fetch(kbase);
+    for (int kb = kbase; kb < kend; kb += BK) {
+        stash();
+        __syncthreads();
+        if (kb + BK < kend) fetch(kb + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK / 16; ++ks) {
+            const int kq = ks * 16 + 4 * (lane >> 4);
+            bf4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int r = wr * 32 + a * 16 + (lane & 15);
+                ah[a] = *reinterpret_cast<const bf4*>(&Ah[r][kq]);
+                al[a] = *reinterpret_cast<const bf4*>(&Al[r][kq]);
+                const int c = wc * 32 + a * 16 + (lane & 15);
+                bh[a] = *reinterpret_cast<const bf4*>(&Bh[c][kq]);
+                bl[a] = *reinterpret_cast<const bf4*>(&Bl[c][kq]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[a], bh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row0 + wr * 32 + a * 16 + 4 * (lane >> 4) + v;
+                const int j = col0 + wc * 32 + b * 16 + (lane & 15);
+                if (CHK && (i >= T.M || j >= T.N)) continue;
+                const double r = (double)acc[a][b][v];
+                if (Pp) { Pp[i + (i64)j * T.M] = alpha * r; continue; }
+                double* c = C + i + (i64)j * T.ldc;
+                if (T.tri == 0 || i >= j) *c = alpha * r + (beta != 0.0 ? beta * (*c) : 0.0);
+                else if (T.tri == 2) *c = 0.0;
+            }
+}
+
+__global__ __launch_bounds__(TPB) void k_gemm_tiles_bf3(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
+    __shared__ __align__(16) unsigned short Ah[TILE][BKP];
+    __shared__ __align__(16) unsigned short Al[TILE][BKP];
+    __shared__ __align__(16) unsigned short Bh[TILE][BKP];
+    __shared__ __align__(16) unsigned short Bl[TILE][BKP];
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    const double* A = bt.p[T.bufA] + T.offA;
+    const double* B = bt.p[T.bufB] + T.offB;
+    double* C = bt.p[T.bufC] + T.offC;
+    const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
+    int kbase = 0, kend = T.K;
+    double* Pp = nullptr;
+    if (T.ksplit > 1) {
+        kbase = tl.ks * T.kchunk;
+        kend = min(T.K, kbase + T.kchunk);
+        Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
+        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // the reduction handles it
+    }
+    if (T.triA == 1) kend = min(kend, row0 + TILE);
+    if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
+    if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
+    if (T.pad == 2) kend = min(kend, col0 + TILE);
+    if (T.tri == 2 && row0 + TILE - 1 < col0) {
+        for (int e = threadIdx.x; e < TILE * TILE; e += TPB) {
+            const int i = row0 + e % TILE, j = col0 + e / TILE;
+            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
+        }
+        return;
+    }
+    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && ((kend - kbase) % BK) == 0 &&
+                      T.triA == 0 && T.pad == 0 && T.opA < 2 && T.opB < 2;
+    const int key = T.opA * 3 + T.opB;
+    if (full) {
+        switch (key) {
+        case 0: gemm_tile_bf3<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 1: gemm_tile_bf3<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 3: gemm_tile_bf3<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        default: gemm_tile_bf3<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        }
+    } else {
+        switch (key) {
+        case 0: gemm_tile_bf3<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 1: gemm_tile_bf3<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 2: gemm_tile_bf3<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 3: gemm_tile_bf3<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 4: gemm_tile_bf3<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 5: gemm_tile_bf3<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 6: gemm_tile_bf3<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        case 7: gemm_tile_bf3<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        default: gemm_tile_bf3<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, Ah, Al, Bh, Bl); break;
+        }
+    }
+}
+
 // blk[nn + i, j] = -S[i + j*na] for the listed (front, 16-column) tiles:
 // F21 = -V_k B21 written back after F11 has consumed B21
 // (projected_inverse.c:51-56).

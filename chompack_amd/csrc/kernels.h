@@ -36,6 +36,7 @@ __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const doub
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+__global__ void k_gemm_tiles_bf3(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 // descriptor-addressed dense operations (completion, Hessian)
 __global__ void k_diag_tasks(const DiagTask* tasks, BufTab bt, unsigned long long* status);
 __global__ void k_matop(const MatOp* ops, BufTab bt);

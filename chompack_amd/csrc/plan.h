@@ -204,8 +204,10 @@ struct chp_plan {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int pending = 0;          // mode of a timed call whose end event is unread
     };
-    Cached g_chol, g_projinv, g_llt, g_chol32;
-    bool schur32 = false;  // the cholesky sweep in progress runs Schur updates in FP32
+    Cached g_chol, g_projinv, g_llt, g_chol32, g_cholbf;
+    // arithmetic of the Schur updates of the cholesky sweep in progress:
+    // 0 FP64, 1 FP32, 2 BF16x3 split precision
+    int schur_mode = 0;
     // profiling: per-kernel-kind time from HIP events around every launch
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -708,8 +708,10 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            if (L.iarg == 1 && P->schur32)  // Schur update in FP32 arithmetic
+            if (L.iarg == 1 && P->schur_mode == 1)  // Schur update in FP32 arithmetic
                 CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            else if (L.iarg == 1 && P->schur_mode == 2)  // ... in BF16x3 split precision
+                CHP_LAUNCH(k_gemm_tiles_bf3, 0, (const GemmTile*)list, pg.d_tasks, bt);
             else
                 CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
@@ -1062,7 +1064,7 @@ void chp_plan_destroy(chp_plan* P) {
     (void)hipDeviceSynchronize();  // no launch of this plan may still be in flight
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
-    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32}) {
+    for (chp_plan::Cached* G : {&P->g_chol, &P->g_projinv, &P->g_llt, &P->g_chol32, &P->g_cholbf}) {
         if (G->exec) (void)hipGraphExecDestroy(G->exec);
         if (G->e0) (void)hipEventDestroy(G->e0);
         if (G->e1) (void)hipEventDestroy(G->e1);
@@ -1189,14 +1191,19 @@ int chp_cholesky(chp_plan* P, double* blkval, void* stream, i64* info) {
     return read_status(P, s, info);
 }
 
-int chp_cholesky_mixed(chp_plan* P, double* blkval, void* stream, i64* info) {
+int chp_cholesky_lowprec(chp_plan* P, double* blkval, int schur, void* stream, i64* info) {
     if (!P || !blkval) { chp_set_error("null argument"); return -1; }
+    if (schur != 1 && schur != 2) { chp_set_error("schur must be 1 (FP32) or 2 (BF16x3)"); return -1; }
     hipStream_t s = (hipStream_t)stream;
-    P->schur32 = true;
-    int rc = run_sweep(P, P->chol, P->g_chol32, blkval, s, true);
-    P->schur32 = false;
+    P->schur_mode = schur;
+    int rc = run_sweep(P, P->chol, schur == 1 ? P->g_chol32 : P->g_cholbf, blkval, s, true);
+    P->schur_mode = 0;
     if (rc) return rc;
     return read_status(P, s, info);
+}
+
+int chp_cholesky_mixed(chp_plan* P, double* blkval, void* stream, i64* info) {
+    return chp_cholesky_lowprec(P, blkval, 1, stream, info);
 }
 
 int chp_projected_inverse(chp_plan* P, double* blkval, void* stream, i64* info) {

@@ -1,9 +1,12 @@
 """Mixed-precision factorization with FP64 iterative refinement.
 
-``cholesky(X)`` factors X with the Schur-complement updates of the large
-fronts (the bulk of the flops) in FP32 arithmetic on the matrix cores, while
-the diagonal blocks, the panels, the stored-convention solves and the small
-LDS-resident fronts stay FP64; the factor is accurate to about FP32 precision.
+``cholesky(X, schur)`` factors X with the Schur-complement updates of the
+large fronts (the bulk of the flops) in low precision on the matrix cores --
+FP32 (``schur="fp32"``) or BF16x3 split precision with FP32 accumulation
+(``"bf16x3"``) -- while the diagonal blocks, the panels, the
+stored-convention solves and the small LDS-resident fronts stay FP64; the
+factor is accurate to about FP32 precision (FP32) or about 1e-5 relative
+(BF16x3).
 ``solve(A, L, B, tol)`` then solves A x = b to a relative residual
 max_j ||b_j - A x_j||_2 / ||b_j||_2 <= tol by iterative refinement: the
 residual is formed in FP64 with the original A (``SpMat``, a device CSR copy
@@ -64,15 +67,22 @@ class SpMat(object):
             R.copy_(wr)
 
 
-def cholesky(X):
-    """Cholesky factorization in place with FP32 Schur-complement updates
-    (same conventions, state machine and errors as numeric.cholesky)."""
+SCHUR = {"fp32": 1, "bf16x3": 2}
+
+
+def cholesky(X, schur="fp32"):
+    """Cholesky factorization in place with low-precision Schur-complement
+    updates: ``schur="fp32"`` (FP32 MFMA) or ``"bf16x3"`` (BF16x3 split
+    precision with FP32 accumulation, about 2^-16 relative per product).
+    Same conventions, state machine and errors as numeric.cholesky."""
     if not (isinstance(X, cspmatrix) and X.is_factor is False):
         raise ValueError("X must be a cspmatrix")
+    if schur not in SCHUR:
+        raise ValueError("schur must be 'fp32' or 'bf16x3'")
     plan, b = numeric._plan(X)
     info = np.zeros(2, dtype=np.int64)
-    rc = _native.lib().chp_cholesky_mixed(plan.handle, b.data_ptr(), numeric._stream(b),
-                                          _native.ptr(info))
+    rc = _native.lib().chp_cholesky_lowprec(plan.handle, b.data_ptr(), SCHUR[schur],
+                                            numeric._stream(b), _native.ptr(info))
     X.is_factor = True
     numeric._raise_rc(rc, "cholesky")
     if rc == 1:

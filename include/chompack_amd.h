@@ -145,6 +145,13 @@ int chp_llt(chp_plan* plan, double* blkval, void* stream);
  * factor is accurate to about FP32 precision; pair it with iterative
  * refinement (chp_spmv_residual + chp_trsm).  No reference counterpart. */
 int chp_cholesky_mixed(chp_plan* plan, double* blkval, void* stream, int64_t* info);
+/* The same with a choice of Schur-update arithmetic: schur = 1 FP32 (as
+ * chp_cholesky_mixed), 2 BF16x3 split precision (each operand rounded to
+ * FP32 and split into bf16 hi + lo; lo*hi + hi*lo + hi*hi on
+ * v_mfma_f32_16x16x16bf16_1k with FP32 accumulation, about 2^-16 relative
+ * per product).  No reference counterpart. */
+int chp_cholesky_lowprec(chp_plan* plan, double* blkval, int schur, void* stream,
+                         int64_t* info);
 
 /* Reverse-mode sweep of chp_cholesky (cholesky.c:26-82 backwards, root to
  * leaves; no reference counterpart: the backward of a differentiable
