@@ -113,6 +113,8 @@ def batched_section(dev):
             chol(X)
             res["ld"] = bs.logdet(X)
         return f
+    t_default = _timed(run(cp.cholesky), 10)
+    tuned = cp.autotune.tune(X0, sweeps=("cholesky",))
     t64 = _timed(run(cp.cholesky), 10)
     ld64 = res["ld"]
     t32 = _timed(run(cp.mixed.cholesky), 10)
@@ -120,6 +122,7 @@ def batched_section(dev):
     tbf = _timed(run(lambda Z: cp.mixed.cholesky(Z, schur="bf16x3")), 10)
     errbf = float(np.max(np.abs(res["ld"] - ld64) / np.abs(ld64)))
     return {"instances": 512, "n_total": int(bs.symb.n), "fp64_ms": round(t64, 3),
+            "fp64_default_plan_ms": round(t_default, 3), "autotune": tuned,
             "mixed_ms": round(t32, 3), "mixed_vs_fp64_speedup": round(t64 / t32, 3),
             "mixed_logdet_max_rel_err": err,
             "bf16x3_ms": round(tbf, 3), "bf16x3_vs_fp64_speedup": round(t64 / tbf, 3),
@@ -370,6 +373,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)  # covers the sweeps' launch-mode tuning (4 calls)
     ap.add_argument("--side", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default plan parameters instead of autotuning them")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true", help="skip the 16-RHS solve section")
     ap.add_argument("--no-configs", action="store_true",
@@ -395,6 +400,15 @@ def main():
     X = cp.cspmatrix(symb, blkval=torch.empty_like(pristine))
     print("# n=%d Nsn=%d nnz(L)=%d clique=%d symbolic %.2fs" %
           (symb.n, symb.Nsn, symb.nnz, symb.clique_number, t_symb), file=sys.stderr, flush=True)
+    # plan parameters chosen by measurement on this matrix (setup, like the
+    # symbolic analysis: not part of a step)
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = cp.autotune.tune(cp.cspmatrix(symb, blkval=pristine))
+        tuned["tune_s"] = round(time.perf_counter() - t0, 2)
+        print("# autotune: %s %.3f ms (default %.3f ms)" % (tuned["best"], tuned["best_ms"],
+              tuned["default_ms"]), file=sys.stderr, flush=True)
 
     def step():
         cp.numeric.copy_blkval(X.blkval, pristine)
@@ -629,6 +643,7 @@ def main():
                    "clique_number": symb.clique_number},
         "roofline": roof,
         "whole_path_roofline": whole,
+        "autotune": tuned,
         "effective_tflops": round((fl_chol + fl_pinv) / (ms * 1e-3) / 1e12, 3),
         "algorithmic_flops": {"cholesky": fl_chol, "projected_inverse": fl_pinv},
         "components_instrumented_ms": comps,

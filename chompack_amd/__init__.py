@@ -15,7 +15,7 @@ from . import amd
 
 from .numeric import cholesky, llt, projected_inverse, completion, hessian, trsm, trmm, dot, \
     logdet
-from . import autograd, batch, mixed
+from . import autograd, autotune, batch, mixed
 from .pfcholesky import pfcholesky
 from .orderings import maxcardsearch, maxchord
 from .completions import psdcompletion, mrcompletion, edmcompletion

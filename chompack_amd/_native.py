@@ -67,6 +67,8 @@ def _declare(L):
     _opt(L, "chp_version", [], i32)
     _opt(L, "chp_plan_create", [i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, i32], vp)
     _opt(L, "chp_plan_destroy", [vp], None)
+    _opt(L, "chp_plan_create_params", [i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32], vp)
+    _opt(L, "chp_plan_params", [vp, vp], i32)
     _opt(L, "chp_update_layout", [i64, vp, vp, vp, vp, vp, vp], i64)
     _opt(L, "chp_plan_info", [vp, vp], i32)
     _opt(L, "chp_cholesky", [vp, vp, vp, vp], i32)

@@ -38,13 +38,12 @@ __device__ inline void lds_extend_add(double* F, int nj, const FrontInfo& f,
 // Supernodal Cholesky of one small front (cholesky.c:30-81):
 // assemble, factor F11, L21 = F21 L11^-T, F22 -= L21 L21^T,
 // B21 = L21 L11^-1 (stored convention), write block and update matrix.
-__global__ __launch_bounds__(TPB) void k_chol_small(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+__device__ inline void chol_small_front(
+    const int k, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ upd,
     const int* __restrict__ relidx, const int* __restrict__ chidx,
     unsigned long long* status) {
     extern __shared__ double F[];
-    const int k = fronts[blockIdx.x];
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
     const int tid = threadIdx.x;
@@ -99,6 +98,20 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
         }
     }
 }
+
+// A persistent launch over the level's small fronts (largest first): each
+// workgroup takes fronts blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ __launch_bounds__(TPB) void k_chol_small(
+    const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx, const int* __restrict__ chidx,
+    unsigned long long* status) {
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        chol_small_front(fronts[q], fi, blkval, upd, relidx, chidx, status);
+        __syncthreads();  // the front's LDS is free before the next one loads
+    }
+}
+
 
 // Gather V_k = F_p[r, r] from the parent's finished front into k's update
 // slot, both triangles (projected_inverse.c:60-70 / cbase.c:39-59, done by
@@ -163,12 +176,11 @@ __global__ __launch_bounds__(TPB) void k_extract(
 // Projected inverse of one small front (projected_inverse.c:29-72), with
 // V_k already gathered into k's update slot.
 // LDS: Li (nn x nn), C (nn x nn), B (na x nn), T (na x nn), tmp (nn).
-__global__ __launch_bounds__(TPB) void k_projinv_small(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+__device__ inline void projinv_small_front(
+    const int k, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, const double* __restrict__ upd,
     unsigned long long* status) {
     extern __shared__ double S[];
-    const int k = fronts[blockIdx.x];
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
     const int tid = threadIdx.x;
@@ -249,14 +261,25 @@ __global__ __launch_bounds__(TPB) void k_projinv_small(
     }
 }
 
+// Persistent over the level's small fronts, as k_chol_small.
+__global__ __launch_bounds__(TPB) void k_projinv_small(
+    const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, const double* __restrict__ upd,
+    unsigned long long* status) {
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        projinv_small_front(fronts[q], fi, blkval, upd, status);
+        __syncthreads();
+    }
+}
+
+
 // Chordal product X = L L^T for one small front (llt.c:30-57).
 // LDS: A (nj x nn, the block as stored) and F (nj x nj).
-__global__ __launch_bounds__(TPB) void k_llt_small(
-    const int* __restrict__ fronts, const FrontInfo* __restrict__ fi,
+__device__ inline void llt_small_front(
+    const int k, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ upd,
     const int* __restrict__ relidx, const int* __restrict__ chidx) {
     extern __shared__ double S[];
-    const int k = fronts[blockIdx.x];
     const FrontInfo f = fi[k];
     const int nn = f.nn, na = f.na, nj = nn + na;
     const int tid = threadIdx.x;
@@ -296,3 +319,15 @@ __global__ __launch_bounds__(TPB) void k_llt_small(
         }
     }
 }
+
+// Persistent over the level's small fronts, as k_chol_small.
+__global__ __launch_bounds__(TPB) void k_llt_small(
+    const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx, const int* __restrict__ chidx) {
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        llt_small_front(fronts[q], fi, blkval, upd, relidx, chidx);
+        __syncthreads();
+    }
+}
+

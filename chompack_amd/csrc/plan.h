@@ -113,7 +113,16 @@ struct chp_plan {
     int device = 0;
     i64 n = 0, nsn = 0;
     int nlevels = 0;
+    // tunable parameters (chp_plan_create_params): largest clique order of the
+    // LDS-resident small-front kernels, the LDS a small-front workgroup may
+    // use, the outer panel width of large fronts, the split-K cap
     int small_max = 64;
+    size_t lds_cap = 64 * 1024;
+    int nb2 = NB2;
+    int splitk_max = 16;
+    // device: compute units, LDS per compute unit
+    int ncu = 256;
+    size_t lds_per_cu = 64 * 1024;
     std::vector<FrontInfo> fi;
     // the same fronts with one update slot per front (updptr = prefix sums of
     // na^2): the layout of the Hessian scaling arena d_supd, which keeps every

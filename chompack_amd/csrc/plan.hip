@@ -28,8 +28,6 @@ void chp_set_error(const std::string& msg) { g_err = msg; }
 
 namespace chpi {
 
-const size_t LDS_SMALL_CAP = 64 * 1024;
-
 size_t chol_small_lds(const FrontInfo& f) {
     size_t nj = f.nn + f.na;
     return nj * nj * sizeof(double);
@@ -43,10 +41,10 @@ size_t llt_small_lds(const FrontInfo& f) {
     return (nj * nn + nj * nj) * sizeof(double);
 }
 bool chol_is_small(const chp_plan* P, const FrontInfo& f) {
-    return f.nn + f.na <= P->small_max && chol_small_lds(f) <= LDS_SMALL_CAP;
+    return f.nn + f.na <= P->small_max && chol_small_lds(f) <= P->lds_cap;
 }
 bool projinv_is_small(const chp_plan* P, const FrontInfo& f) {
-    return f.nn + f.na <= P->small_max && projinv_small_lds(f) <= LDS_SMALL_CAP;
+    return f.nn + f.na <= P->small_max && projinv_small_lds(f) <= P->lds_cap;
 }
 
 GemmTask mk(int bufA, i64 offA, int lda, int opA, int bufB, i64 offB, int ldb, int opB,
@@ -99,15 +97,17 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
         for (int tm = 0; tm < mt; ++tm)
             if (!(skip_upper && tm < tn)) ++ntiles;
     int ks = 1;
-    if (ntiles < 128 && t.K >= 512) {
+    if (P->splitk_max <= 1) {
+        ks = 1;
+    } else if (ntiles < 128 && t.K >= 512) {
         ks = std::max(1, std::min(t.K / 256, (512 + ntiles - 1) / ntiles));
-        ks = std::min(ks, 16);
+        ks = std::min(ks, std::min(16, P->splitk_max));
     } else if (ntiles < 1024 && t.K >= 768) {
         // a few tiles per CU: split up to 4 ways so that the launch holds
         // about 2048 units of K >= 384 (the tail of a 2-3 tile-per-CU launch
         // otherwise leaves CUs idle)
         ks = std::max(1, std::min(t.K / 384, (2048 + ntiles - 1) / ntiles));
-        ks = std::min(ks, 4);
+        ks = std::min(ks, std::min(4, P->splitk_max));
     }
     if (ks > 1) {
         int chunk = (t.K + ks - 1) / ks;
@@ -209,6 +209,23 @@ std::vector<int4> extract_tiles(const chp_plan* P, const std::vector<int>& front
     return xt;
 }
 
+// A persistent launch of a small-front kernel over `fronts`: the list in
+// decreasing order of work (nj^2 (nn + 1), largest first, so the longest
+// fronts start first and the tail is short), and at most as many
+// workgroups as the device holds at once with `lds` bytes each; iarg
+// carries the front count.
+Launch small_launch(chp_plan* P, Kid kid, std::vector<int> fronts, size_t lds) {
+    std::stable_sort(fronts.begin(), fronts.end(), [&](int a, int b) {
+        const FrontInfo &fa = P->fi[a], &fb = P->fi[b];
+        const double ja = fa.nn + fa.na, jb = fb.nn + fb.na;
+        return ja * ja * (fa.nn + 1) > jb * jb * (fb.nn + 1);
+    });
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, P->lds_per_cu / std::max<size_t>(lds, 1)));
+    const int count = (int)fronts.size();
+    const int grid = std::min(count, P->ncu * per_cu);
+    return Launch{kid, grid, 256, lds, P->pg->lists.add(fronts), count};
+}
+
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
 void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     std::vector<int> small, large;
@@ -223,7 +240,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     const bool fork = !small.empty() && !large.empty();
     if (fork) P->chol.push_back({K_FORK, 0, 0, 0, 0, 1});
     if (!small.empty()) {
-        Launch Ls{K_CHOL_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0};
+        Launch Ls = small_launch(P, K_CHOL_SMALL, small, lds);
         Ls.side = fork ? 1 : 0;
         P->chol.push_back(Ls);
     }
@@ -237,8 +254,9 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // with K = the outer panel width for everything right of it.
     int maxnn = 0;
     for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
-    for (int p0 = 0; p0 < maxnn; p0 += NB2) {
-        for (int c0 = p0; c0 < std::min(p0 + NB2, maxnn); c0 += NB) {
+    const int nb2 = P->nb2;
+    for (int p0 = 0; p0 < maxnn; p0 += nb2) {
+        for (int c0 = p0; c0 < std::min(p0 + nb2, maxnn); c0 += NB) {
             // diagonal block and the panel below it, one launch: a workgroup
             // per 64 panel rows (plus one writing the diagonal block)
             std::vector<int4> pf;
@@ -247,7 +265,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                 const FrontInfo& f = P->fi[k];
                 int nn = f.nn, nj = nn + f.na;
                 if (nn <= c0) continue;
-                int c1 = std::min(c0 + NB, nn), p1 = std::min(p0 + NB2, nn);
+                int c1 = std::min(c0 + NB, nn), p1 = std::min(p0 + nb2, nn);
                 pf.push_back(make_int4(k, c0, -1, 0));
                 for (int r0 = c1; r0 < nj; r0 += NB) pf.push_back(make_int4(k, c0, r0, 0));
                 // update inside the outer panel: F[c1:, c1:p1] -= L[c1:, c0:c1] L[c1:p1, c0:c1]^T
@@ -266,7 +284,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
             const FrontInfo& f = P->fi[k];
             int nn = f.nn, na = f.na, nj = nn + na;
             if (nn <= p0) continue;
-            int p1 = std::min(p0 + NB2, nn);
+            int p1 = std::min(p0 + nb2, nn);
             if (p1 < nn)
                 add_gemm_tiles(P, gt, mk(0, f.blkptr + p1 + (i64)p0 * nj, nj, 0,
                                          0, f.blkptr + p1 + (i64)p0 * nj, nj, 1,
@@ -311,7 +329,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     }
     if (!inv.empty())
         P->chol.push_back({K_DIAG, (int)inv.size(), 256, 0, P->pg->lists.add(inv), 0});
-    const int GB = NB2 / NB;
+    const int GB = nb2 / NB;
     int maxgrp = 0;
     for (int k : large)
         if (P->fi[k].na > 0) maxgrp = std::max(maxgrp, ((P->fi[k].nn + NB - 1) / NB + GB - 1) / GB);
@@ -457,7 +475,7 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     const bool fork = !large.empty() && !small.empty();
     if (fork) P->projinv.push_back({K_FORK, 0, 0, 0, 0, 1});
     if (!small.empty()) {
-        Launch Ls{K_PROJINV_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0};
+        Launch Ls = small_launch(P, K_PROJINV_SMALL, small, lds);
         Ls.side = fork ? 1 : 0;
         P->projinv.push_back(Ls);
     }
@@ -503,7 +521,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
     size_t lds = 0;
     for (int k : fr) {
         const FrontInfo& f = P->fi[k];
-        if (f.nn + f.na <= P->small_max && llt_small_lds(f) <= LDS_SMALL_CAP) {
+        if (f.nn + f.na <= P->small_max && llt_small_lds(f) <= P->lds_cap) {
             small.push_back(k);
             lds = std::max(lds, llt_small_lds(f));
         } else {
@@ -511,7 +529,7 @@ void build_llt_level(chp_plan* P, const std::vector<int>& fr, i64& scratch_need)
         }
     }
     if (!small.empty())
-        P->llt.push_back({K_LLT_SMALL, (int)small.size(), 256, lds, P->pg->lists.add(small), 0});
+        P->llt.push_back(small_launch(P, K_LLT_SMALL, small, lds));
     if (large.empty()) return;
     // scratch A (nj x nn, ld nj) per large front; offsets in h_xptr are per
     // sweep and the levels run one after the other, so they are reused
@@ -688,7 +706,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         dim3 g(L.grid, L.gy), b(L.block);
         switch (L.kid) {
         case K_CHOL_SMALL:
-            CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, fi, blkval,
+            CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, L.iarg, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
             break;
         case K_EXTADD:
@@ -723,7 +741,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        bt.p[1], P->d_relidx, L.bstride, L.ustride);
             break;
         case K_PROJINV_SMALL:
-            CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, fi,
+            CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, L.iarg, fi,
                        blkval, P->d_upd, P->d_status);
             break;
         case K_TRTRI_DIAG:
@@ -735,7 +753,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_scratch, P->d_tptr);
             break;
         case K_LLT_SMALL:
-            CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, fi, blkval,
+            CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, L.iarg, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx);
             break;
         case K_DIAG:
@@ -904,11 +922,45 @@ chp_plan* chp_plan_create(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
                           const i64* relptr, const i64* relidx, const i64* chptr,
                           const i64* chidx, const i64* blkptr, const i64* perm,
                           int small_max) {
+    const int params[1] = {small_max};
+    return chp_plan_create_params(n, nsn, snpost, snptr, relptr, relidx, chptr, chidx, blkptr,
+                                  perm, params, 1);
+}
+
+chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* snptr,
+                                 const i64* relptr, const i64* relidx, const i64* chptr,
+                                 const i64* chidx, const i64* blkptr, const i64* perm,
+                                 const int* params, int nparams) {
     chp_plan* P = new chp_plan();
     auto fail = [&](const std::string& m) -> chp_plan* { chp_set_error(m); delete P; return nullptr; };
     if (hipGetDevice(&P->device) != hipSuccess) return fail("no HIP device");
     P->n = n; P->nsn = nsn;
-    P->small_max = small_max > 0 ? small_max : 64;
+    auto param = [&](int i) { return (params && i < nparams) ? params[i] : 0; };
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, P->device) == hipSuccess && v > 0)
+        P->ncu = v;
+    size_t lds_block = 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device) == hipSuccess && v > 0)
+        lds_block = (size_t)v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, P->device) == hipSuccess && v > 0)
+        P->lds_per_cu = (size_t)v;
+    else
+        P->lds_per_cu = lds_block;
+    P->small_max = param(0) > 0 ? param(0) : 64;
+    if (param(1) > 0) {
+        if (param(1) % NB) return fail("outer panel width must be a multiple of 64");
+        P->nb2 = param(1);
+    }
+    if (param(2) > 0) P->splitk_max = param(2);
+    P->lds_cap = std::min<size_t>(param(3) > 0 ? (size_t)param(3) : 64 * 1024, lds_block);
+    if (P->lds_cap > 64 * 1024) {  // small-front kernels may take more than 64 KB of LDS
+        for (const void* k : {(const void*)k_chol_small, (const void*)k_projinv_small,
+                              (const void*)k_llt_small})
+            if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)P->lds_cap) != hipSuccess)
+                return fail("the device refuses " + std::to_string(P->lds_cap) +
+                            " bytes of LDS for a small-front kernel");
+    }
     std::vector<int> parent(nsn), level(nsn, 0);
     for (i64 k = 0; k < nsn; ++k) parent[k] = (int)k;
     for (i64 k = 0; k < nsn; ++k)
@@ -1095,6 +1147,13 @@ void chp_plan_destroy(chp_plan* P) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete P;
+}
+
+// params[0..3] of the plan in use: small_max, nb2, splitk_max, lds_cap.
+int chp_plan_params(chp_plan* P, int* out) {
+    if (!P || !out) { chp_set_error("null argument"); return -1; }
+    out[0] = P->small_max; out[1] = P->nb2; out[2] = P->splitk_max; out[3] = (int)P->lds_cap;
+    return 0;
 }
 
 // out[0..7] = nlevels, n_large_chol, n_large_projinv, n_gemm_tiles,

@@ -52,7 +52,12 @@ class Plan(object):
     and work arrays resident in HBM).  Built on first use, cached on the
     symbolic object, freed with it."""
 
-    def __init__(self, symb, device, small_max=None):
+    PARAMS = ("small_max", "panel_width", "splitk_max", "lds_cap")
+
+    def __init__(self, symb, device, small_max=None, **params):
+        """``params``: panel_width (outer panel of large fronts, a multiple
+        of 64), splitk_max (1 = no split-K), lds_cap (bytes of LDS a
+        small-front workgroup may use); omitted ones keep their defaults."""
         torch = _torch()
         lib = _native.lib()
         self.symb_arrays = [_native.as_i64(a) for a in (
@@ -60,13 +65,24 @@ class Plan(object):
             symb.blkptr, symb.p)]
         a = self.symb_arrays
         self.device = torch.device(device)
+        unknown = set(params) - set(self.PARAMS[1:])
+        if unknown:
+            raise TypeError("unknown plan parameters: %s" % ", ".join(sorted(unknown)))
+        pv = np.array([int(small_max or SMALL_MAX)] +
+                      [int(params.get(k) or 0) for k in self.PARAMS[1:]], dtype=np.int32)
         with torch.cuda.device(self.device):
-            h = lib.chp_plan_create(symb.n, symb.Nsn, *[_native.ptr(x) for x in a],
-                                    int(small_max or SMALL_MAX))
+            h = lib.chp_plan_create_params(symb.n, symb.Nsn, *[_native.ptr(x) for x in a],
+                                           pv.ctypes.data, len(pv))
         if not h:
             raise RuntimeError("plan creation failed: %s" % lib.chp_last_error().decode())
         self.handle = h
         self._lib = lib
+
+    def params(self):
+        """The plan's parameters: small_max, panel_width, splitk_max, lds_cap."""
+        out = np.zeros(4, dtype=np.int32)
+        self._lib.chp_plan_params(self.handle, out.ctypes.data)
+        return dict(zip(self.PARAMS, [int(v) for v in out]))
 
     def info(self):
         out = np.zeros(8, dtype=np.int64)

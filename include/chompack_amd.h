@@ -81,6 +81,20 @@ chp_plan* chp_plan_create(int64_t n, int64_t nsn, const int64_t* snpost,
                           const int64_t* relidx, const int64_t* chptr,
                           const int64_t* chidx, const int64_t* blkptr,
                           const int64_t* p, int small_max);
+/* The same with tunable parameters (no reference counterpart; chosen per
+ * plan by chompack_amd.autotune): params[0] small_max (as above), [1] outer
+ * panel width of the large-front factorization (a multiple of 64, default
+ * 256), [2] cap of the split-K factor of GEMM tasks (1 = no split-K,
+ * default 16), [3] LDS bytes a small-front workgroup may use (default
+ * 65536, capped by the device); entries <= 0 or beyond nparams keep their
+ * defaults. */
+chp_plan* chp_plan_create_params(int64_t n, int64_t nsn, const int64_t* snpost,
+                                 const int64_t* snptr, const int64_t* relptr,
+                                 const int64_t* relidx, const int64_t* chptr,
+                                 const int64_t* chidx, const int64_t* blkptr,
+                                 const int64_t* p, const int* params, int nparams);
+/* out[4]: the parameters of a plan (small_max, panel width, split-K cap, LDS cap). */
+int chp_plan_params(chp_plan* plan, int* out);
 void chp_plan_destroy(chp_plan* plan);
 /* Update-arena layout of a plan (host only, no device call): updptr[k] =
  * offset of supernode k's na_k x na_k update matrix, level_out[k] = its
