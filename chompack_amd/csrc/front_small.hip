@@ -26,10 +26,12 @@ __device__ inline void lds_extend_add(double* F, int nj, const FrontInfo& f,
         const int N = g.na;
         const int* r = relidx + g.relptr;
         const double* U = upd + g.updptr;
-        for (int j = 0; j < N; ++j) {
-            const int cj = r[j] * nj;
-            for (int i = j + (int)threadIdx.x; i < N; i += TPB)
-                F[r[i] + cj] += U[i + (i64)j * N];
+        // one pass over the child's N x N array (lower part used): a thread's
+        // loads are independent of each other, so they are all in flight at
+        // once instead of one global round trip per column
+        for (int e = threadIdx.x; e < N * N; e += TPB) {
+            const int i = e % N, j = e / N;
+            if (i >= j) F[r[i] + r[j] * nj] += U[e];
         }
         __syncthreads();
     }
@@ -175,7 +177,8 @@ __global__ __launch_bounds__(TPB) void k_extract(
 
 // Projected inverse of one small front (projected_inverse.c:29-72), with
 // V_k already gathered into k's update slot.
-// LDS: Li (nn x nn), C (nn x nn), B (na x nn), T (na x nn), tmp (nn).
+// LDS: Li (nn x nn), C (nn x nn), B (na x nn), T (na x nn), tmp (nn),
+// V (na x na, both triangles: the extraction stores both).
 __device__ inline void projinv_small_front(
     const int k, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, const double* __restrict__ upd,
@@ -190,10 +193,15 @@ __device__ inline void projinv_small_front(
     double* B = C + nn * nn;
     double* T = B + na * nn;
     double* tmp = T + na * nn;
+    double* Vs = tmp + nn;
 
     for (int e = tid; e < nn * nn; e += TPB) {
         int i = e % nn, j = e / nn;
         Li[e] = (i >= j) ? blk[i + j * nj] : 0.0;
+    }
+    if (na > 0) {  // one flat copy: every thread's loads in flight at once
+        const double* V = upd + f.updptr;
+        for (int e = tid; e < na * na; e += TPB) Vs[e] = V[e];
     }
     for (int e = tid; e < na * nn; e += TPB) {
         int i = e % na, j = e / na;
@@ -229,16 +237,17 @@ __device__ inline void projinv_small_front(
         C[e] = s;
     }
     if (na > 0) {
-        const double* V = upd + f.updptr;
-        // T = V B21 with V symmetric (lower stored, ld na)
+        // T = V B21 (V in LDS, symmetric with both triangles stored)
         for (int e = tid; e < na * nn; e += TPB) {
             int i = e % na, j = e / na;
-            double s = 0.0;
-            for (int m = 0; m < na; ++m) {
-                double v = (i >= m) ? V[i + (i64)m * na] : V[m + (i64)i * na];
-                s += v * B[m + j * na];
+            double s0 = 0.0, s1 = 0.0;
+            int m = 0;
+            for (; m + 1 < na; m += 2) {
+                s0 += Vs[i + m * na] * B[m + j * na];
+                s1 += Vs[i + (m + 1) * na] * B[m + 1 + j * na];
             }
-            T[e] = s;
+            if (m < na) s0 += Vs[i + m * na] * B[m + j * na];
+            T[e] = s0 + s1;
         }
         __syncthreads();
         // C += T^T B21
@@ -258,6 +267,42 @@ __device__ inline void projinv_small_front(
     for (int e = tid; e < na * nn; e += TPB) {
         int i = e % na, j = e / na;
         blk[nn + i + j * nj] = -T[e];
+    }
+}
+
+// V_c = F_p[r, r] (both triangles) for child c of the finished front f,
+// by the whole workgroup: the element-by-element form of k_extract.
+__device__ inline void extract_child(int c, const FrontInfo& f, const FrontInfo* __restrict__ fi,
+                                     const double* __restrict__ blkval, double* __restrict__ upd,
+                                     const int* __restrict__ relidx) {
+    const FrontInfo g = fi[c];
+    const int N = g.na;
+    const int* r = relidx + g.relptr;
+    const int nnp = f.nn, njp = f.nn + f.na, nap = f.na;
+    const double* pblk = blkval + f.blkptr;
+    const double* pV = upd + f.updptr;
+    double* V = upd + g.updptr;
+    for (int e = threadIdx.x; e < N * N; e += TPB) {
+        const int i = e % N, j = e / N;
+        const int ra = r[max(i, j)], rb = r[min(i, j)];  // ra >= rb
+        V[e] = (rb < nnp) ? pblk[ra + (i64)rb * njp] : pV[(ra - nnp) + (i64)(rb - nnp) * nap];
+    }
+}
+
+// Projected inverse along a chain of small fronts (one per level, parent
+// first) in one workgroup: each front, then its children's V from it, so
+// the next front of the chain finds its V in its update slot.
+__global__ __launch_bounds__(TPB) void k_projinv_chain(
+    const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,
+    double* __restrict__ blkval, double* __restrict__ upd, const int* __restrict__ relidx,
+    const int* __restrict__ chidx, unsigned long long* status) {
+    for (int q = 0; q < count; ++q) {
+        const int k = fronts[q];
+        projinv_small_front(k, fi, blkval, upd, status);
+        __syncthreads();
+        const FrontInfo f = fi[k];
+        for (int l = f.chbeg; l < f.chend; ++l) extract_child(chidx[l], f, fi, blkval, upd, relidx);
+        __syncthreads();
     }
 }
 

@@ -34,7 +34,7 @@ size_t chol_small_lds(const FrontInfo& f) {
 }
 size_t projinv_small_lds(const FrontInfo& f) {
     size_t nn = f.nn, na = f.na;
-    return (2 * nn * nn + 2 * na * nn + nn) * sizeof(double);
+    return (2 * nn * nn + 2 * na * nn + nn + na * na) * sizeof(double);
 }
 size_t llt_small_lds(const FrontInfo& f) {
     size_t nn = f.nn, nj = f.nn + f.na;
@@ -224,6 +224,18 @@ Launch small_launch(chp_plan* P, Kid kid, std::vector<int> fronts, size_t lds) {
     const int count = (int)fronts.size();
     const int grid = std::min(count, P->ncu * per_cu);
     return Launch{kid, grid, 256, lds, P->pg->lists.add(fronts), count};
+}
+
+// Chains: runs of at least CHAIN_MIN consecutive levels that hold one small
+// front each (an elimination-tree path, as in banded and arrow patterns) go
+// to one launch of a single workgroup that walks the fronts in order: child
+// before parent for cholesky and llt (a front reads its chain child's update
+// matrix right after the child wrote it), parent before child for the
+// projected inverse.  A level schedule would spend launches per front there.
+constexpr int CHAIN_MIN = 3;
+
+Launch chain_launch(chp_plan* P, Kid kid, const std::vector<int>& fronts, size_t lds) {
+    return Launch{kid, 1, 256, lds, P->pg->lists.add(fronts), (int)fronts.size()};
 }
 
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
@@ -744,6 +756,10 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, L.iarg, fi,
                        blkval, P->d_upd, P->d_status);
             break;
+        case K_PROJINV_CHAIN:
+            CHP_LAUNCH(k_projinv_chain, L.lds, (const int*)list, L.iarg, fi, blkval, P->d_upd,
+                       P->d_relidx, P->d_chidx, P->d_status);
+            break;
         case K_TRTRI_DIAG:
             CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, fi, blkval,
                        P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
@@ -1018,7 +1034,27 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     P->h_chidx.resize(chptr[nsn]);
     for (i64 i = 0; i < relptr[nsn]; ++i) P->h_relidx[i] = (int)relidx[i];
     for (i64 i = 0; i < chptr[nsn]; ++i) P->h_chidx[i] = (int)chidx[i];
-    for (int L = 0; L < H; ++L) build_chol_level(P, P->bylevel[L]);
+    auto one_small = [&](int L, size_t (*need)(const FrontInfo&), size_t& lds) {
+        if (P->bylevel[L].size() != 1) return false;
+        const FrontInfo& f = P->fi[P->bylevel[L][0]];
+        if (f.nn + f.na > P->small_max || need(f) > P->lds_cap) return false;
+        lds = std::max(lds, need(f));
+        return true;
+    };
+    for (int L = 0; L < H;) {
+        int L2 = L;
+        size_t lds = 0;
+        while (L2 < H && one_small(L2, chol_small_lds, lds)) ++L2;
+        if (L2 - L >= CHAIN_MIN) {
+            std::vector<int> run;
+            for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
+            P->chol.push_back(chain_launch(P, K_CHOL_SMALL, run, lds));
+            L = L2;
+            continue;
+        }
+        build_chol_level(P, P->bylevel[L]);
+        ++L;
+    }
     P->chol.push_back({K_JOIN, 0, 0, 0, 0, 2});
     // scratch of the projected inverse: X = L11^{-1} of every large front,
     // then the largest level's W, then the largest level's T
@@ -1037,11 +1073,36 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     i64 scratch = wbase + wmax, xcur = 0;
     // side stream 2 (the inversions) starts from the sweep's stream
     P->projinv.push_back({K_FORK, 0, 0, 0, 0, 2});
-    for (int L = H - 1; L >= 0; --L)
+    for (int L = H - 1; L >= 0;) {
+        int L2 = L;
+        size_t lds = 0;
+        while (L2 >= 0 && one_small(L2, projinv_small_lds, lds)) --L2;
+        if (L - L2 >= CHAIN_MIN) {
+            std::vector<int> run;
+            for (int M = L; M > L2; --M) run.push_back(P->bylevel[M][0]);
+            P->projinv.push_back(chain_launch(P, K_PROJINV_CHAIN, run, lds));
+            L = L2;
+            continue;
+        }
         build_projinv_level(P, P->bylevel[L], wbase, wbase + wmax, xcur, scratch);
+        --L;
+    }
     P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 2});
     if (xcur != wbase) return fail("projected-inverse scratch layout");
-    for (int L = 0; L < H; ++L) build_llt_level(P, P->bylevel[L], scratch);
+    for (int L = 0; L < H;) {
+        int L2 = L;
+        size_t lds = 0;
+        while (L2 < H && one_small(L2, llt_small_lds, lds)) ++L2;
+        if (L2 - L >= CHAIN_MIN) {
+            std::vector<int> run;
+            for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
+            P->llt.push_back(chain_launch(P, K_LLT_SMALL, run, lds));
+            L = L2;
+            continue;
+        }
+        build_llt_level(P, P->bylevel[L], scratch);
+        ++L;
+    }
     P->scratch_len = scratch;
     // device copies
     std::vector<i64> pv(perm, perm + n), sp(snptr, snptr + nsn + 1);

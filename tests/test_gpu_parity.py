@@ -25,8 +25,11 @@ PROBLEMS = {
     # an elimination-tree chain of one-column fronts with two children each
     # (the completion's chain kernel with non-chain children)
     "caterpillar": lambda: problems.caterpillar(),
+    # an elimination-tree path of one-column fronts (natural order): the
+    # chain launches of cholesky, llt and the projected inverse
+    "band_arrow": lambda: problems.band_arrow(1500, 8, 12, seed=4),
 }
-ORDER = {"two_cliques": None, "caterpillar": None}
+ORDER = {"two_cliques": None, "caterpillar": None, "band_arrow": None}
 
 
 def _setup(name, small_max):
