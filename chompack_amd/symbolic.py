@@ -257,36 +257,28 @@ class symbolic(object):
         blkptr = np.zeros(Nsn + 1, dtype=np.int64)
         np.cumsum(nnk * (nnk + nak), out=blkptr[1:])
 
-        # memory plan of the sequential LIFO sweep (symbolic.py:606-680)
-        na_l = (relptr[1:] - relptr[:-1]).tolist()
-        nn_l = nnk.tolist()
-        nch = np.diff(chptr).tolist()
-        stack = []
-        stack_size = stack_mem2 = stack_max = frontal_max = 0
-        stack_depth = stack_mem = stack_tmp = cln = stack_solve = stack_stmp = 0
-        for k in range(Nsn):
-            na = na_l[k]
-            nj = na + nn_l[k]
-            frontal_max = max(frontal_max, nj * nj)
-            cln = max(cln, nj)
-            for _ in range(nch[k]):
-                v = stack.pop()
-                stack_mem2 -= v * v
-                stack_tmp -= v * v
-                stack_stmp -= v
-            if na > 0:
-                stack.append(na)
-                stack_mem2 += na * na
-                stack_max = max(stack_max, stack_mem2)
-                stack_size = max(stack_size, len(stack))
-                stack_tmp += na * na
-                stack_mem = max(stack_tmp, stack_mem)
-                stack_stmp += na
-                stack_solve = max(stack_stmp, stack_solve)
-            stack_depth = max(stack_depth, len(stack))
-        self.frontal_len = frontal_max
-        self.stack_len = stack_max
-        self.stack_size = stack_size
+        # memory plan of the reference's sequential sweep (symbolic.py:606-680):
+        # its LIFO stack holds supernode j's update matrix from step j (after
+        # its push) until the step of its parent (which pops it before its
+        # own push), so the stack after step k holds exactly the j <= k whose
+        # parent is > k.  A difference array over those intervals gives the
+        # stack's size, depth and element count after every step at once.
+        na_a = (relptr[1:] - relptr[:-1]).astype(np.int64)
+        nj_a = nnk.astype(np.int64) + na_a
+        live = np.nonzero(na_a > 0)[0]           # the pushed (non-root) supernodes
+        stats = {}
+        for key, wgt in (("stack_mem", na_a * na_a), ("stack_depth", np.ones(Nsn, np.int64)),
+                         ("stack_solve", na_a)):
+            d = np.zeros(Nsn + 1, dtype=np.int64)
+            np.add.at(d, live, wgt[live])
+            np.add.at(d, snpar[live], -wgt[live])
+            stats[key] = int(np.cumsum(d[:Nsn]).max()) if Nsn else 0
+        cln = int(nj_a.max()) if Nsn else 0
+        stack_depth, stack_mem, stack_solve = (stats["stack_depth"], stats["stack_mem"],
+                                               stats["stack_solve"])
+        self.frontal_len = cln * cln
+        self.stack_len = stack_mem
+        self.stack_size = stack_depth
 
         self._clique_number = int(cln)
         self._n = int(n)

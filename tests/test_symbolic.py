@@ -173,3 +173,35 @@ def test_amd_fill_quality():
     s_mmd = cp.symbolic(A, p=lu.perm_c)
     assert s_amd.nnz < 0.6 * s_nat.nnz
     assert s_amd.nnz < 1.25 * s_mmd.nnz
+
+
+def _lifo_memory(s):
+    """The reference's memory plan by simulating its sequential sweep
+    (symbolic.py:606-680): supernodes in order, each popping its children's
+    update matrices and pushing its own."""
+    nnk = np.diff(s.snptr)
+    na = np.diff(s.relptr)
+    nch = np.diff(s.chptr)
+    stack, mem, solve, depth, cur, cur_s = [], 0, 0, 0, 0, 0
+    for k in range(s.Nsn):
+        for _ in range(int(nch[k])):
+            v = stack.pop()
+            cur -= v * v
+            cur_s -= v
+        if na[k] > 0:
+            stack.append(int(na[k]))
+            cur += int(na[k]) ** 2
+            cur_s += int(na[k])
+            mem, solve = max(mem, cur), max(solve, cur_s)
+        depth = max(depth, len(stack))
+    cln = int((nnk + na).max())
+    return {"stack_depth": depth, "stack_mem": mem, "frontal_mem": cln * cln, "stack_solve": solve}
+
+
+@pytest.mark.parametrize("make,order", [
+    (lambda: problems.grid3d(14, seed=1), "amd"), (lambda: problems.grid2d(40, seed=0), "amd"),
+    (lambda: problems.caterpillar(), None), (lambda: problems.two_cliques(), None),
+    (lambda: A23(), "amd"), (lambda: A17(), None)])
+def test_memory_plan_matches_lifo_simulation(make, order):
+    s = cp.symbolic(make(), p=cp.amd.order if order else None)
+    assert s.memory == _lifo_memory(s)
