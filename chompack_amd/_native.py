@@ -102,7 +102,10 @@ def _declare(L):
     _opt(L, "chp_frontal_get_update", [vp, i64, vp, vp, vp, vp], i32)
     _opt(L, "chp_profile", [vp, i32], i32)
     _opt(L, "chp_set_graphs", [vp, i32], i32)
-    _opt(L, "chp_profile_read", [vp, vp, vp, vp], i32)
+    _opt(L, "chp_profile_read", [vp, vp, vp, vp, i32], i32)
+    _opt(L, "chp_dag_trace", [vp, i32], i64)
+    _opt(L, "chp_dag_trace_read", [vp, vp, i64], i64)
+    _opt(L, "chp_set_dag_wait_limit", [vp, i32], i32)
     _opt(L, "chp_plan_flops", [vp, vp], i32)
     _opt(L, "chp_plan_gemm_bytes", [vp, vp], i32)
     _opt(L, "chp_profile_launches", [vp, vp, i64], i64)

@@ -1,7 +1,10 @@
 """Per-plan autotuning by measurement.
 
-The level schedule has four parameters that trade launch count against
-per-launch efficiency, and their best values depend on the pattern: the
+The level schedule has five parameters that trade launch count against
+per-launch efficiency, and their best values depend on the pattern: how
+the large fronts of the cholesky sweep are factored (``chol_dag``: 1 one
+persistent tile-DAG launch per level, 2 panel and GEMM launches, the
+default), the
 largest clique order handled by the LDS-resident small-front kernels
 (``small_max``; fronts above it use the tiled large-front kernels) and the
 LDS such a workgroup may take (``lds_cap``, up to the device's per-block
@@ -10,8 +13,8 @@ factorization (``panel_width``: the K of its trailing Schur updates), and
 the split-K cap of the GEMM tasks (``splitk_max``).
 
 ``tune(X)`` times the requested sweeps on the caller's own matrix for each
-candidate (coordinate descent: small_max with its LDS cap, then the panel
-width, then split-K), keeps the fastest plan on the symbolic object, so
+candidate (coordinate descent: the large-front cholesky, then small_max
+with its LDS cap, then the panel width, then split-K), keeps the fastest plan on the symbolic object, so
 every later call on that pattern uses it, and returns the measurements.
 Every candidate produces the same factor up to rounding: the parameters
 change only how the work is cut into launches.
@@ -28,6 +31,7 @@ __all__ = ["tune"]
 SMALL = [(32, 65536), (64, 65536), (96, 163840), (128, 163840)]
 PANEL = [128, 256, 512]
 SPLITK = [1, 16]
+DAG = [1, 2]
 
 
 def _time_sweeps(symb, plan, X0, sweeps, warmup, reps):
@@ -79,17 +83,22 @@ def tune(X, sweeps=("cholesky", "projected_inverse"), warmup=5, reps=5):
         k = tuple(sorted(p.items()))
         if k not in cache:
             plan = numeric.Plan(symb, dev, small_max=p["small_max"], panel_width=p["panel_width"],
-                                splitk_max=p["splitk_max"], lds_cap=p["lds_cap"])
+                                splitk_max=p["splitk_max"], lds_cap=p["lds_cap"],
+                                chol_dag=p["chol_dag"])
             p = plan.params()     # the values in force (LDS capped by the device)
             ms = _time_sweeps(symb, plan, X, sweeps, warmup, reps)
             cache[k] = (ms, plan, p)
             trials.append((p, round(ms, 4)))
         return cache[k]
 
-    best = {"small_max": 64, "lds_cap": 65536, "panel_width": 256, "splitk_max": 16}
+    best = {"small_max": 64, "lds_cap": 65536, "panel_width": 256, "splitk_max": 16,
+            "chol_dag": 2}
     default_ms = run(best)[0]
-    for axis in ("small", "panel", "splitk"):
-        if axis == "small":
+    axes = ("dag", "small", "panel", "splitk") if "cholesky" in sweeps else ("small", "panel", "splitk")
+    for axis in axes:
+        if axis == "dag":
+            cands = [dict(best, chol_dag=d) for d in DAG]
+        elif axis == "small":
             cands = [dict(best, small_max=s, lds_cap=l) for s, l in SMALL]
         elif axis == "panel":
             cands = [dict(best, panel_width=w) for w in PANEL]

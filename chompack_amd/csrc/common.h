@@ -22,7 +22,9 @@ struct FrontInfo {
     i64 blkptr, updptr, relptr;
 };
 
-// Failure status word: min over failing fronts of (k << 32) | info.
+// Failure status word: min over failing fronts of (k << 32) | info.  The
+// word after it is an error word: nonzero when a dependency wait of the
+// tile-DAG factorization ran out of its bound (an internal error).
 #define CHP_STATUS_OK 0xffffffffffffffffULL
 
 __device__ inline void chp_record_failure(unsigned long long* status, int k, int info) {
@@ -130,6 +132,26 @@ struct UFTask {
     // afterwards as a blocked multi-workgroup potrf)
     unsigned char fbuf, defer_solve, defer_rchol, pad[5];
 };
+
+// Tile-DAG factorization of a level's large fronts.  A front is cut into
+// 64 x 64 tiles: row (and column) tiles 0 .. P-1 cover its nn supernode
+// columns from 0, tiles P .. R-1 its na separator rows from nn.  Tile (i, j),
+// i >= j, lives in blkval (ld nj) when j < P, otherwise in the front's
+// update slot (ld na).  cbase: offset of the front's tile counters
+// (R (R + 1) / 2, row-major lower triangle); dinvoff: its diagonal-block
+// inverses (64 x 64 column-major per block); key: the supernode.
+struct DagFront {
+    i64 blkptr, updptr, dinvoff;
+    int nn, na, P, R, cbase, key;
+};
+// One task: tile (i, j) and step k; kt = k | type << 16 with type 0 the
+// factorization of diagonal tile (k, k) and the inverse of its factor, 1
+// the solve of tile (i, k) with that factor, 2 the update of tile (i, j)
+// with tiles (i, k) and (j, k).
+struct DagTask {
+    int f, i, j, kt;
+};
+enum { DAG_POTRF = 0, DAG_TRSM = 1, DAG_UPDATE = 2 };
 
 // One item of a chain of small fronts: the completion step of `front` (kind
 // 1), followed by the nch items of its children's update factors (kind 0;

@@ -577,27 +577,46 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 #define BK 32
 
 
+// Global loads and stores of the GEMM tiles.  COH = true: agent-scope
+// atomic (cache-bypassing, coherent across the XCDs' L2 caches) accesses,
+// for data that other workgroups of the same launch write (k_chol_dag).
+template <bool COH>
+__device__ inline double gld(const double* p) {
+    if (COH)
+        return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    return *p;
+}
+template <bool COH>
+__device__ inline void gst(double* p, double v) {
+    if (COH)
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
 // Element (gi, gk) of opA and (gk, gj) of opB with the task's bounds and
 // triangular masks; 0 outside.
-template <int OPA>
+template <int OPA, bool COH = false>
 __device__ inline double ldA_chk(const GemmTask& T, const double* A, int gi, int gk) {
     bool ok = gi < T.M && gk < T.K;
     if (T.triA == 1) ok = ok && gk <= gi;
     else if (T.triA == 2) ok = ok && gk >= gi;
     if (!ok) return 0.0;
-    if (OPA == 0) return A[gi + (i64)gk * T.lda];
-    if (OPA == 1) return A[gk + (i64)gi * T.lda];
-    return (gi >= gk) ? A[gi + (i64)gk * T.lda] : A[gk + (i64)gi * T.lda];
+    if (OPA == 0) return gld<COH>(A + gi + (i64)gk * T.lda);
+    if (OPA == 1) return gld<COH>(A + gk + (i64)gi * T.lda);
+    return (gi >= gk) ? gld<COH>(A + gi + (i64)gk * T.lda) : gld<COH>(A + gk + (i64)gi * T.lda);
 }
-template <int OPB>
+template <int OPB, bool COH = false>
 __device__ inline double ldB_chk(const GemmTask& T, const double* B, int gk, int gj) {
     bool ok = gj < T.N && gk < T.K;
     if (T.pad == 1) ok = ok && gk >= gj;
     else if (T.pad == 2) ok = ok && gk <= gj;
     if (!ok) return 0.0;
-    if (OPB == 0) return B[gk + (i64)gj * T.ldb];
-    if (OPB == 1) return B[gj + (i64)gk * T.ldb];
-    return (gk >= gj) ? B[gk + (i64)gj * T.ldb] : B[gj + (i64)gk * T.ldb];
+    if (OPB == 0) return gld<COH>(B + gk + (i64)gj * T.ldb);
+    if (OPB == 1) return gld<COH>(B + gj + (i64)gk * T.ldb);
+    return (gk >= gj) ? gld<COH>(B + gk + (i64)gj * T.ldb) : gld<COH>(B + gj + (i64)gk * T.ldb);
 }
 
 // One 64 x 64 tile.  OPA: 0 A(i,k) = A[i + k lda], 1 A[k + i lda],
@@ -606,7 +625,7 @@ __device__ inline double ldB_chk(const GemmTask& T, const double* B, int gk, int
 // CHK: per-element bounds and masks (edge tiles, triangular operands);
 // otherwise a check-free interior path.  Threads are mapped so consecutive
 // lanes read consecutive addresses of the stored operand.
-template <int OPA, int OPB, bool CHK>
+template <int OPA, int OPB, bool CHK, bool COH = false>
 __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A,
                                  const double* __restrict__ B, double* __restrict__ C,
                                  int row0, int col0, int kbase, int kend, double* Pp,
@@ -641,17 +660,17 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
-                ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+                ra[q] = (OPA == 1) ? gld<COH>(A + gk + (i64)gi * T.lda) : gld<COH>(A + gi + (i64)gk * T.lda);
                 const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
-                rb[q] = (OPB == 1) ? B[gj + (i64)gk2 * T.ldb] : B[gk2 + (i64)gj * T.ldb];
+                rb[q] = (OPB == 1) ? gld<COH>(B + gj + (i64)gk2 * T.ldb) : gld<COH>(B + gk2 + (i64)gj * T.ldb);
             }
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
-                ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+                ra[q] = ldA_chk<OPA, COH>(T, A, gi, gk);
                 const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
-                rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+                rb[q] = ldB_chk<OPB, COH>(T, B, gk2, gj);
             }
         }
     };
@@ -700,8 +719,10 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
                 if (CHK && (i >= T.M || j >= T.N)) continue;
                 if (Pp) { Pp[i + (i64)j * T.M] = alpha * acc[a][b][v]; continue; }
                 double* c = C + i + (i64)j * T.ldc;
-                if (T.tri == 0 || i >= j) *c = alpha * acc[a][b][v] + (beta != 0.0 ? beta * *c : 0.0);
-                else if (T.tri == 2) *c = 0.0;
+                if (T.tri == 0 || i >= j)
+                    gst<COH>(c, alpha * acc[a][b][v] + (beta != 0.0 ? beta * gld<COH>(c) : 0.0));
+                else if (T.tri == 2)
+                    gst<COH>(c, 0.0);
             }
 }
 
@@ -1176,4 +1197,299 @@ __global__ __launch_bounds__(TPB) void k_splitk_reduce(
         for (int ks = 0; ks < T.ksplit; ++ks) s += Pp[ks * mn + i + (i64)j * T.M];
         *c = (T.beta != 0.0 ? T.beta * (*c) : 0.0) + s;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Tile-DAG factorization of one level's large fronts (the dense part of
+// cholesky.c:55-72 for every large front of the level, in one persistent
+// launch).  Each front is cut into 64 x 64 tiles (DagFront); the tasks are
+//   POTRF(k):      factor diagonal tile (k, k) in place and write the inverse
+//                  of its factor (column-major, 64 x 64) to the front's
+//                  inverse area, block k;
+//   TRSM(i, k):    L(i, k) = A(i, k) L(k, k)^{-T}, a product with that inverse;
+//   UPDATE(i,j,k): A(i, j) -= L(i, k) L(j, k)^T (lower part of diagonal tiles;
+//                  tiles right of the supernode columns are the update matrix).
+// A tile's counter counts the operations finished on it: UPDATE k = 0, 1, ...
+// in order, then its POTRF or TRSM.  A task waits until its tile's counter
+// equals its k (the updates before it are done; the accumulation order is
+// fixed, so results do not depend on timing) and its operand tiles' counters
+// equal k + 1.  Workgroups take tasks from a queue in a topological order
+// (the host lists them in the start order of a simulated list schedule that
+// favours the critical path), so every task waited for has been taken by a
+// running workgroup: the waits end without any assumption on how many
+// workgroups are resident.  Each wait is also bounded (spin_limit polls); a
+// wait that runs out records an internal error in status[1] and goes on.
+// Tiles cross workgroups (and the eight XCDs' L2 caches, which are not
+// coherent with each other) as agent-scope atomic loads and stores of their
+// elements, which bypass those caches; the counter updates follow the
+// stores, and the tile loads the polls, by waits on the memory counters.
+
+// X = L^{-1} (64 x 64) from L in LDS (lower, zero above) and rdiag = 1/diag:
+// recursive doubling as in block_trtri_rd, with the temporary W = L21 X11 of
+// each level kept in the X21 block it is about to be replaced by, so only
+// two 64 x 65 arrays are needed.
+__device__ inline void trtri_lean(const double (*Ls)[NB + 1], const double* rdiag,
+                                  double (*X)[NB + 1]) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += TPB) X[e / NB][e % NB] = 0.0;
+    __syncthreads();
+    if (tid < NB / 4) {
+        const int b = 4 * tid;
+        const double y0 = rdiag[b], y1 = rdiag[b + 1], y2 = rdiag[b + 2], y3 = rdiag[b + 3];
+        const double l10 = Ls[b + 1][b], l20 = Ls[b + 2][b], l30 = Ls[b + 3][b];
+        const double l21 = Ls[b + 2][b + 1], l31 = Ls[b + 3][b + 1], l32 = Ls[b + 3][b + 2];
+        const double x10 = -y1 * l10 * y0;
+        const double x21 = -y2 * l21 * y1;
+        const double x20 = -y2 * (l20 * y0 + l21 * x10);
+        const double x32 = -y3 * l32 * y2;
+        const double x31 = -y3 * (l31 * y1 + l32 * x21);
+        const double x30 = -y3 * (l30 * y0 + l31 * x10 + l32 * x20);
+        X[b][b] = y0; X[b + 1][b + 1] = y1; X[b + 2][b + 2] = y2; X[b + 3][b + 3] = y3;
+        X[b + 1][b] = x10; X[b + 2][b] = x20; X[b + 3][b] = x30;
+        X[b + 2][b + 1] = x21; X[b + 3][b + 1] = x31; X[b + 3][b + 2] = x32;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int bs = 4; bs < NB; bs *= 2) {
+        const int nout = (NB / (2 * bs)) * bs * bs;  // <= 4 * TPB
+        // W = L21 X11 into the X21 block (read by no thread of this phase)
+        for (int e = tid; e < nout; e += TPB) {
+            const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
+            const int a0 = pr * 2 * bs;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int k = j;
+            for (; k + 3 < bs; k += 4) {
+                s0 += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
+                s1 += Ls[a0 + bs + i][a0 + k + 1] * X[a0 + k + 1][a0 + j];
+                s2 += Ls[a0 + bs + i][a0 + k + 2] * X[a0 + k + 2][a0 + j];
+                s3 += Ls[a0 + bs + i][a0 + k + 3] * X[a0 + k + 3][a0 + j];
+            }
+            for (; k < bs; ++k) s0 += Ls[a0 + bs + i][a0 + k] * X[a0 + k][a0 + j];
+            X[a0 + bs + i][a0 + j] = (s0 + s1) + (s2 + s3);
+        }
+        __syncthreads();
+        // X21 = -X22 W, formed in registers and stored once every thread has read W
+        double r[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int e = tid + m * TPB;
+            r[m] = 0.0;
+            if (e >= nout) continue;
+            const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
+            const int a0 = pr * 2 * bs;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int k = 0;
+            for (; k + 3 <= i; k += 4) {
+                s0 += X[a0 + bs + i][a0 + bs + k] * X[a0 + bs + k][a0 + j];
+                s1 += X[a0 + bs + i][a0 + bs + k + 1] * X[a0 + bs + k + 1][a0 + j];
+                s2 += X[a0 + bs + i][a0 + bs + k + 2] * X[a0 + bs + k + 2][a0 + j];
+                s3 += X[a0 + bs + i][a0 + bs + k + 3] * X[a0 + bs + k + 3][a0 + j];
+            }
+            for (; k <= i; ++k) s0 += X[a0 + bs + i][a0 + bs + k] * X[a0 + bs + k][a0 + j];
+            r[m] = -((s0 + s1) + (s2 + s3));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int e = tid + m * TPB;
+            if (e >= nout) continue;
+            const int pr = e / (bs * bs), i = (e / bs) % bs, j = e % bs;
+            const int a0 = pr * 2 * bs;
+            X[a0 + bs + i][a0 + j] = r[m];
+        }
+        __syncthreads();
+    }
+}
+
+__device__ inline int dag_row0(const DagFront& F, int i) {
+    return i < F.P ? NB * i : F.nn + NB * (i - F.P);
+}
+__device__ inline int dag_rows(const DagFront& F, int i) {
+    const int r0 = dag_row0(F, i);
+    return min(NB, (i < F.P ? F.nn : F.nn + F.na) - r0);
+}
+
+// Ordering without cache maintenance (the tiles themselves are read and
+// written with coherent accesses, gld<true> / gst<true>): wait until this
+// thread's memory operations have completed, so that a counter update after
+// the next barrier follows the stores, and the tile loads after a poll
+// follow it.  Agent-scope fences would write back and invalidate the L2
+// caches on every task.
+__device__ inline void dag_order_after_stores() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+}
+__device__ inline void dag_order_after_loads() {
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Polls counter c until it reaches target, at most `limit` times; gives up at
+// once when another wait has already run out (err != 0), so that a launch
+// with a failed wait drains quickly.
+__device__ inline bool dag_wait(int* c, int target, int limit, unsigned long long* err) {
+    int polls = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++polls > limit) return false;
+        if ((polls & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// POTRF task of k_chol_dag (kept out of line: inlined into the task loop its
+// unrolled pivot steps would take the registers of a second workgroup per CU)
+// The LDS of k_chol_dag.  POTRF: L and its inverse (64 x 65 each), the pivot
+// exchange arrays and 1/diag; TRSM and UPDATE: the GEMM's two 32 x 66
+// staging arrays at the start.  (At namespace scope, so that the out-of-line
+// POTRF addresses it as LDS.)
+__shared__ __attribute__((aligned(16))) double dag_smem[DAG_LDS_DOUBLES];
+
+__device__ __noinline__ void dag_potrf(const DagFront F, int k, double* blkval, double* dinv,
+                                       unsigned long long* status) {
+    double (*Ls)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(dag_smem);
+    double (*X)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(dag_smem + NB * (NB + 1));
+    double (*Pp)[5] = reinterpret_cast<double (*)[5]>(dag_smem + 2 * NB * (NB + 1));
+    double (*Qp)[5] = reinterpret_cast<double (*)[5]>(dag_smem + 2 * NB * (NB + 1) + NB * 5);
+    double* rdiag = dag_smem + 2 * NB * (NB + 1) + 2 * NB * 5;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int nj = F.nn + F.na;
+    double* blk = blkval + F.blkptr;
+    const int c0 = NB * k, kw = min(NB, F.nn - c0);
+    double* dg = blk + c0 + (i64)c0 * nj;
+    double a[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int jj = w + 4 * q;
+        a[q] = (l < kw && jj < kw && l >= jj) ? gld<true>(dg + l + (i64)jj * nj) : (l == jj ? 1.0 : 0.0);
+    }
+    int bad = 0;
+    if (!block_potrf4(a, Pp, Qp, rdiag, w, l, &bad)) {
+        if (tid == 0) chp_record_failure(status, F.key, c0 + bad + 1);
+    } else {
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int jj = w + 4 * q;
+            Ls[l][jj] = (l >= jj) ? a[q] : 0.0;
+            if (l < kw && jj < kw && l >= jj) gst<true>(dg + l + (i64)jj * nj, a[q]);
+        }
+        __syncthreads();
+        trtri_lean(Ls, rdiag, X);
+        double* inv = dinv + F.dinvoff + (i64)k * NB * NB;
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int c = w + 4 * q;  // column c, row l of the inverse
+            gst<true>(inv + l + c * NB, (l < kw && c < kw) ? X[l][c] : 0.0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_chol_dag(const int4* __restrict__ list,
+                                                  double* __restrict__ blkval,
+                                                  double* __restrict__ upd,
+                                                  double* __restrict__ dinv, int* cnt,
+                                                  unsigned long long* status, int spin_limit,
+                                                  i64* trace) {
+    __shared__ int s_task;
+    double (*As)[TILE + 2] = reinterpret_cast<double (*)[TILE + 2]>(dag_smem);
+    double (*Bs)[TILE + 2] = reinterpret_cast<double (*)[TILE + 2]>(dag_smem + BK * (TILE + 2));
+
+    const int4 h = list[0];
+    const DagFront* fronts = reinterpret_cast<const DagFront*>(list + 1);
+    const DagTask* tasks = reinterpret_cast<const DagTask*>(fronts + h.y);
+    int* qhead = cnt + h.z;
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    // Thread 0 takes the first task here and the next one together with the
+    // counter update at the end of each iteration, one thread-0 block per
+    // iteration between two barriers: two thread-0 blocks on either side of
+    // the back edge would be merged by the compiler into a path on which lane
+    // 0 leaves the loop while the other lanes of its wave run the loop body
+    // again with a stale task index.  The index itself is read as a
+    // wave-uniform value, so the loop exit is uniform.
+    if (tid == 0) s_task = atomicAdd(qhead, 1);
+    __syncthreads();
+    i64 t_grab = 0, t_ready = 0;
+    for (;;) {
+        const int t = __builtin_amdgcn_readfirstlane(s_task);
+        if (t >= h.x) break;
+        if (trace && tid == 0) t_grab = wall_clock64();
+        const DagTask T = tasks[t];
+        const DagFront F = fronts[T.f];
+        const int k = T.kt & 0xffff, type = T.kt >> 16, i = T.i, j = T.j;
+        int* ctile = cnt + F.cbase + i * (i + 1) / 2 + j;
+        if (tid == 0) {
+            unsigned long long* err = status + 1;
+            bool ok = dag_wait(ctile, k, spin_limit, err);
+            if (type == DAG_TRSM) {
+                ok = ok && dag_wait(cnt + F.cbase + k * (k + 1) / 2 + k, k + 1, spin_limit, err);
+            } else if (type == DAG_UPDATE) {
+                ok = ok && dag_wait(cnt + F.cbase + i * (i + 1) / 2 + k, k + 1, spin_limit, err);
+                if (j != i)
+                    ok = ok && dag_wait(cnt + F.cbase + j * (j + 1) / 2 + k, k + 1, spin_limit, err);
+            }
+            if (!ok) atomicOr(status + 1, 1ULL);
+            dag_order_after_loads();
+            if (trace) t_ready = wall_clock64();
+        }
+        __syncthreads();
+        const int nj = F.nn + F.na;
+        double* blk = blkval + F.blkptr;
+        const int c0 = NB * k, kw = min(NB, F.nn - c0);
+        if (type == DAG_POTRF) {
+            dag_potrf(F, k, blkval, dinv, status);
+        } else {
+            const int ri = dag_row0(F, i);
+            GemmTask G;
+            G.offA = G.offB = G.offC = G.offP = 0;
+            G.ksplit = 1; G.kchunk = 0;
+            G.bufA = G.bufB = G.bufC = 0;
+            G.opA = 0; G.opB = 1; G.triA = 0; G.pad = 0;
+            G.lda = nj;
+            const double* Ap = blk + ri + (i64)c0 * nj;
+            const double* Bp;
+            double* Cp;
+            if (type == DAG_TRSM) {  // in place: the tile is read in full before it is written
+                Bp = dinv + F.dinvoff + (i64)k * NB * NB;
+                G.ldb = NB;
+                Cp = blk + ri + (i64)c0 * nj;
+                G.ldc = nj;
+                G.M = dag_rows(F, i); G.N = kw; G.K = kw;
+                G.alpha = 1.0; G.beta = 0.0; G.tri = 0;
+            } else {
+                const int rj = dag_row0(F, j);
+                Bp = blk + rj + (i64)c0 * nj;
+                G.ldb = nj;
+                if (j < F.P) {
+                    Cp = blk + ri + (i64)rj * nj;
+                    G.ldc = nj;
+                } else {
+                    Cp = upd + F.updptr + (ri - F.nn) + (i64)(rj - F.nn) * F.na;
+                    G.ldc = F.na;
+                }
+                G.M = dag_rows(F, i); G.N = dag_rows(F, j); G.K = kw;
+                G.alpha = -1.0; G.beta = 1.0; G.tri = (i == j) ? 1 : 0;
+            }
+            if (G.M == TILE && G.N == TILE && (G.K % BK) == 0)
+                gemm_tile<0, 1, false, true>(G, Ap, Bp, Cp, 0, 0, 0, G.K, nullptr, As, Bs);
+            else
+                gemm_tile<0, 1, true, true>(G, Ap, Bp, Cp, 0, 0, 0, G.K, nullptr, As, Bs);
+        }
+        dag_order_after_stores();
+        __syncthreads();
+        if (tid == 0) {
+            __hip_atomic_fetch_add(ctile, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (trace) {  // per task: kind, grab, dependencies met, done (100 MHz clock)
+                i64* r = trace + 4 * ((i64)h.w + t);
+                r[0] = (i64)type | ((i64)k << 8) | ((i64)i << 24) | ((i64)j << 40) | ((i64)blockIdx.x << 52);
+                r[1] = t_grab; r[2] = t_ready; r[3] = wall_clock64();
+            }
+            s_task = atomicAdd(qhead, 1);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_fill_int(int* p, i64 n, int v) {
+    for (i64 e = (i64)blockIdx.x * TPB + threadIdx.x; e < n; e += (i64)gridDim.x * TPB) p[e] = v;
 }

@@ -10,7 +10,10 @@
 #define TPB 256
 
 __global__ void k_status_init(unsigned long long* status) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *status = CHP_STATUS_OK;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        status[0] = CHP_STATUS_OK;
+        status[1] = 0;
+    }
 }
 
 // Extend-add of all children of front k (reverse child order, one child at a

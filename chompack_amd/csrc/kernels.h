@@ -6,6 +6,10 @@
 #define NB 64          // inner panel width / diagonal block order of the large-front algorithms
 #define NB2 256        // outer panel width (K of the big trailing updates)
 #define TILE 64        // GEMM output tile (TILE x TILE)
+// LDS of a tile-DAG workgroup: L and its inverse (64 x 65 each), the pivot
+// exchange arrays (2 x 64 x 5), 1/diag (64), and the task index
+#define DAG_LDS_DOUBLES (2 * NB * (NB + 1) + 2 * NB * 5 + NB)
+#define DAG_LDS_BYTES (DAG_LDS_DOUBLES * 8 + 16)
 #define SOLVE_RHS_CHUNK 64  // right-hand sides per workgroup of the slab gather/scatter (grid y)
 
 __global__ void k_status_init(unsigned long long* status);
@@ -37,6 +41,13 @@ __global__ void k_panel_factor(const int4* tasks, const FrontInfo* fi, double* b
 __global__ void k_trtri_diag(const int2* blocks, const FrontInfo* fi, const double* blkval,
                              double* dinv, const i64* dinvptr, double* X, const i64* xptr,
                              unsigned long long* status);
+// the tile-DAG factorization of one level's large fronts (persistent);
+// list: int4 (ntasks, nfronts, queue counter index, 0), DagFront[nfronts],
+// DagTask[ntasks]
+// trace (optional, else null): 4 int64 per task, see chp_dag_trace
+__global__ void k_chol_dag(const int4* list, double* blkval, double* upd, double* dinv, int* cnt,
+                           unsigned long long* status, int spin_limit, i64* trace);
+__global__ void k_fill_int(int* p, i64 n, int v);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);

@@ -15,10 +15,10 @@ enum Kid {
     K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
     K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_TRMM_FWD, K_TRMM_BWD, K_TRMM_FIN, K_COMP_CHAIN,
-    K_PROJINV_CHAIN,
+    K_PROJINV_CHAIN, K_CHOL_DAG, K_FILL_INT,
     K_FORK, K_JOIN
 };
-const int NKIDS = 25;  // kernel kinds (K_FORK / K_JOIN are stream markers)
+const int NKIDS = 27;  // kernel kinds (K_FORK / K_JOIN are stream markers)
 
 struct Launch {
     Kid kid;
@@ -158,7 +158,19 @@ struct chp_plan {
     i64 work_len = 0;
     unsigned long long* d_status = nullptr;
     i64 upd_len = 0, dinv_len = 0, scratch_len = 0, solve_len = 0;
-    std::vector<chpi::Launch> chol, projinv, llt;
+    // chol: large fronts by the tile-DAG launch (FP64); chol_lp: the same
+    // sweep with panel launches and GEMM launches for the Schur updates,
+    // used by the low-precision factorizations
+    std::vector<chpi::Launch> chol, chol_lp, projinv, llt;
+    // tile counters and queue heads of the DAG launches (zeroed at the start
+    // of every cholesky sweep); polls a dependency wait may take
+    int* d_dagcnt = nullptr;
+    i64 dagcnt_len = 0;
+    int dag_wait_limit = 1 << 18;
+    bool chol_dag = false;  // large fronts of the FP64 cholesky by the tile-DAG launch
+    i64 dag_tasks = 0;           // tasks of all DAG launches of the sweep
+    i64* d_dagtrace = nullptr;   // per-task trace (chp_dag_trace), when enabled
+    bool dag_bad = false;  // a DAG queue order failed its host-side check
     chpi::Program main, dense;
     chpi::Program* pg = &main;  // program the builders append to
     int n_large_chol = 0, n_large_projinv = 0;

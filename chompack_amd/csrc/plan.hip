@@ -22,6 +22,7 @@
 #include <array>
 #include <cstring>
 #include <functional>
+#include <queue>
 
 static thread_local std::string g_err;
 void chp_set_error(const std::string& msg) { g_err = msg; }
@@ -238,8 +239,153 @@ Launch chain_launch(chp_plan* P, Kid kid, const std::vector<int>& fronts, size_t
     return Launch{kid, 1, 256, lds, P->pg->lists.add(fronts), (int)fronts.size()};
 }
 
+// Tile-DAG launch for the large fronts of one level (k_chol_dag).  The tasks
+// of each front are generated step by step (POTRF(k), TRSM(., k), UPDATE(.,
+// ., k)), each with its dependencies: the previous operation on its tile and
+// the operations that produce its operand tiles.  The queue order is the
+// start order of a list schedule on as many workers as the launch has
+// workgroups, in which a free worker takes the ready task with the longest
+// path to the end of the DAG (estimated durations): a topological order that
+// starts the pivot chain's tasks as soon as they are ready.
+namespace {
+double dag_cost(int type, int m, int n, int k) {
+    // microseconds, from the task shapes (fixed costs of the waits and fences
+    // included); only the relative sizes matter for the order
+    const double f = (double)m * n * k / (64.0 * 64.0 * 64.0);
+    if (type == DAG_POTRF) return 10.0;
+    return 1.5 + 1.5 * f;
+}
+}  // namespace
+
+void build_chol_dag(chp_plan* P, const std::vector<int>& large, std::vector<Launch>& seq) {
+    std::vector<DagFront> fr;
+    std::vector<DagTask> tk;
+    std::vector<double> dur, fl;
+    std::vector<int> dptr{0}, dep;  // dependencies of task t: dep[dptr[t] .. dptr[t + 1])
+    for (int key : large) {
+        const FrontInfo& f = P->fi[key];
+        DagFront F;
+        std::memset(&F, 0, sizeof(F));
+        F.blkptr = f.blkptr; F.updptr = f.updptr; F.dinvoff = P->h_dinvptr[key];
+        F.nn = f.nn; F.na = f.na; F.key = key;
+        F.P = (f.nn + NB - 1) / NB;
+        F.R = F.P + (f.na + NB - 1) / NB;
+        F.cbase = (int)P->dagcnt_len;
+        P->dagcnt_len += (i64)F.R * (F.R + 1) / 2;
+        const int fidx = (int)fr.size();
+        fr.push_back(F);
+        const int R = F.R;
+        auto rows = [&](int i) {
+            return i < F.P ? std::min(NB, f.nn - NB * i) : std::min(NB, f.na - NB * (i - F.P));
+        };
+        std::vector<int> last((size_t)R * (R + 1) / 2, -1);
+        auto at = [&](int i, int j) -> int& { return last[(size_t)i * (i + 1) / 2 + j]; };
+        auto add = [&](int type, int i, int j, int k, std::initializer_list<int> deps) {
+            const int t = (int)tk.size();
+            tk.push_back({fidx, i, j, k | (type << 16)});
+            const int kw = std::min(NB, f.nn - NB * k);
+            const int m = rows(i), n = type == DAG_POTRF ? kw : (type == DAG_TRSM ? kw : rows(j));
+            dur.push_back(dag_cost(type, m, n, kw));
+            double flops = 0.0;
+            if (type == DAG_POTRF) flops = (double)kw * kw * kw / 3.0;
+            else if (type == DAG_TRSM) flops = (double)m * kw * kw;
+            else flops = (i == j ? (double)m * (m + 1) : 2.0 * m * n) * kw;
+            fl.push_back(flops);
+            for (int d : deps)
+                if (d >= 0 && std::find(dep.begin() + dptr.back(), dep.end(), d) == dep.end())
+                    dep.push_back(d);
+            dptr.push_back((int)dep.size());
+            return t;
+        };
+        for (int k = 0; k < F.P; ++k) {
+            at(k, k) = add(DAG_POTRF, k, k, k, {at(k, k)});
+            for (int i = k + 1; i < R; ++i) at(i, k) = add(DAG_TRSM, i, k, k, {at(i, k), at(k, k)});
+            for (int j = k + 1; j < R; ++j)
+                for (int i = j; i < R; ++i)
+                    at(i, j) = add(DAG_UPDATE, i, j, k, {at(i, j), at(i, k), at(j, k)});
+        }
+    }
+    const int T = (int)tk.size();
+    if (T == 0) return;
+    // longest path to the end (tasks are created in a topological order)
+    std::vector<int> sptr(T + 1, 0), succ(dep.size());
+    for (int d : dep) ++sptr[d + 1];
+    for (int t = 0; t < T; ++t) sptr[t + 1] += sptr[t];
+    {
+        std::vector<int> fill(sptr.begin(), sptr.end() - 1);
+        for (int t = 0; t < T; ++t)
+            for (int e = dptr[t]; e < dptr[t + 1]; ++e) succ[fill[dep[e]]++] = t;
+    }
+    std::vector<double> bl(T);
+    for (int t = T - 1; t >= 0; --t) {
+        double m = 0.0;
+        for (int e = sptr[t]; e < sptr[t + 1]; ++e) m = std::max(m, bl[succ[e]]);
+        bl[t] = dur[t] + m;
+    }
+    // list schedule on W workers
+    const size_t lds = DAG_LDS_BYTES;
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, P->lds_per_cu / lds));
+    const int W = std::min(T, P->ncu * per_cu);
+    std::vector<int> indeg(T);
+    for (int t = 0; t < T; ++t) indeg[t] = dptr[t + 1] - dptr[t];
+    auto cmp = [&](int a, int b) { return bl[a] < bl[b] || (bl[a] == bl[b] && a > b); };
+    std::priority_queue<int, std::vector<int>, decltype(cmp)> ready(cmp);
+    typedef std::pair<double, int> Ev;
+    std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> running;
+    for (int t = 0; t < T; ++t)
+        if (!indeg[t]) ready.push(t);
+    std::vector<DagTask> order;
+    order.reserve(T);
+    double now = 0.0, flops = 0.0;
+    int idle = W;
+    while ((int)order.size() < T) {
+        while (idle > 0 && !ready.empty()) {
+            const int t = ready.top();
+            ready.pop();
+            order.push_back(tk[t]);
+            flops += fl[t];
+            running.push({now + dur[t], t});
+            --idle;
+        }
+        if (running.empty()) break;  // (cannot happen: the DAG is acyclic)
+        const Ev e = running.top();
+        running.pop();
+        now = e.first;
+        ++idle;
+        for (int q = sptr[e.second]; q < sptr[e.second + 1]; ++q)
+            if (--indeg[succ[q]] == 0) ready.push(succ[q]);
+    }
+    // the queue order runs sequentially: every wait is met by a task listed
+    // before it (what makes the launch's waits end)
+    {
+        std::vector<int> c(P->dagcnt_len, 0);
+        for (const DagTask& t : order) {
+            const DagFront& F = fr[t.f];
+            const int k = t.kt & 0xffff, type = t.kt >> 16;
+            auto cn = [&](int i, int j) -> int& { return c[F.cbase + (size_t)i * (i + 1) / 2 + j]; };
+            bool ok = cn(t.i, t.j) == k;
+            if (type == DAG_TRSM) ok = ok && cn(k, k) == k + 1;
+            if (type == DAG_UPDATE) ok = ok && cn(t.i, k) == k + 1 && cn(t.j, k) == k + 1;
+            if (!ok) { P->dag_bad = true; break; }
+            ++cn(t.i, t.j);
+        }
+    }
+    const int qidx = (int)P->dagcnt_len++;
+    ListBuf& lb = P->pg->lists;
+    std::vector<int4> hdr{make_int4(T, (int)fr.size(), qidx, (int)P->dag_tasks)};
+    P->dag_tasks += T;
+    const i64 off = lb.add(hdr);  // fronts and tasks follow the header contiguously
+    const size_t fb = fr.size() * sizeof(DagFront), tb = order.size() * sizeof(DagTask);
+    lb.bytes.resize(off + sizeof(int4) + fb + tb);
+    std::memcpy(lb.bytes.data() + off + sizeof(int4), fr.data(), fb);
+    std::memcpy(lb.bytes.data() + off + sizeof(int4) + fb, order.data(), tb);
+    Launch L{K_CHOL_DAG, W, 256, 0, off, T};
+    L.flops = flops;
+    seq.push_back(L);
+}
+
 // Cholesky launches for one level (cholesky.c:26-82, all fronts of the level).
-void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
+void build_chol_level(chp_plan* P, const std::vector<int>& fr, std::vector<Launch>& seq, bool dag) {
     std::vector<int> small, large;
     size_t lds = 0;
     for (int k : fr) {
@@ -250,23 +396,27 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // small fronts (LDS kernel) and large fronts of one level write disjoint
     // front and update storage: the small-front launch runs on the side stream
     const bool fork = !small.empty() && !large.empty();
-    if (fork) P->chol.push_back({K_FORK, 0, 0, 0, 0, 1});
+    if (fork) seq.push_back({K_FORK, 0, 0, 0, 0, 1});
     if (!small.empty()) {
         Launch Ls = small_launch(P, K_CHOL_SMALL, small, lds);
         Ls.side = fork ? 1 : 0;
-        P->chol.push_back(Ls);
+        seq.push_back(Ls);
     }
     if (large.empty()) return;
-    P->n_large_chol += (int)large.size();
+    if (dag) P->n_large_chol += (int)large.size();
     int net = 0;
     const std::vector<int4> et = extadd_tiles(P, large, &net);
-    P->chol.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1});
+    seq.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1});
     // Two-level blocking: 64-wide inner panels (diagonal factor, panel solve,
     // update confined to the current outer panel), then one trailing update
     // with K = the outer panel width for everything right of it.
     int maxnn = 0;
     for (int k : large) maxnn = std::max(maxnn, P->fi[k].nn);
     const int nb2 = P->nb2;
+    if (dag)  // one persistent launch for the whole level (diagonal blocks and
+              // their inverses included)
+        build_chol_dag(P, large, seq);
+    else
     for (int p0 = 0; p0 < maxnn; p0 += nb2) {
         for (int c0 = p0; c0 < std::min(p0 + nb2, maxnn); c0 += NB) {
             // diagonal block and the panel below it, one launch: a workgroup
@@ -287,8 +437,8 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                              0, f.blkptr + c1 + (i64)c1 * nj, nj,
                                              nj - c1, p1 - c1, c1 - c0, -1.0, 1.0, 1), true);
             }
-            P->chol.push_back({K_PANEL_FACTOR, (int)pf.size(), 256, 0, P->pg->lists.add(pf), 0});
-            push_gemm(P, P->chol, it, 1);
+            seq.push_back({K_PANEL_FACTOR, (int)pf.size(), 256, 0, P->pg->lists.add(pf), 0});
+            push_gemm(P, seq, it, 1);
         }
         // trailing update right of the outer panel, K = p1 - p0
         std::vector<GemmTile> gt;
@@ -307,7 +457,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                          0, f.blkptr + nn + (i64)p0 * nj, nj, 1,
                                          1, f.updptr, na, na, na, p1 - p0, -1.0, 1.0, 1), true);
         }
-        push_gemm(P, P->chol, gt, 1);
+        push_gemm(P, seq, gt, 1);
     }
     // stored convention B21 = L21 L11^{-1}: groups of NB2/NB column blocks
     // from the right; inside a group block by block (X_b = R_b Dinv_b, update
@@ -320,8 +470,8 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
     // Nothing later in the factorization reads these blocks (parents read only
     // update matrices), so the conversion runs on side stream 2, overlapping
     // the next levels; the sweep joins it at its end.
-    const size_t conv0 = P->chol.size();
-    P->chol.push_back({K_FORK, 0, 0, 0, 0, 2});
+    const size_t conv0 = seq.size();
+    seq.push_back({K_FORK, 0, 0, 0, 0, 2});
     P->pg->split_side = true;
     std::vector<DiagTask> inv;
     for (int k : large) {
@@ -339,8 +489,8 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
             inv.push_back(t);
         }
     }
-    if (!inv.empty())
-        P->chol.push_back({K_DIAG, (int)inv.size(), 256, 0, P->pg->lists.add(inv), 0});
+    if (!dag && !inv.empty())
+        seq.push_back({K_DIAG, (int)inv.size(), 256, 0, P->pg->lists.add(inv), 0});
     const int GB = nb2 / NB;
     int maxgrp = 0;
     for (int k : large)
@@ -370,8 +520,8 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                              0, f.blkptr + nn + (i64)q0 * nj, nj,
                                              na, b0 - q0, b1 - b0, -1.0, 1.0, 0), false);
             }
-            push_gemm(P, P->chol, pt);
-            push_gemm(P, P->chol, it);
+            push_gemm(P, seq, pt);
+            push_gemm(P, seq, it);
         }
         std::vector<GemmTile> gt;
         for (int k : large) {
@@ -387,14 +537,14 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr) {
                                          0, f.blkptr + q0, nj, 0,
                                          0, f.blkptr + nn, nj, na, q0, q1 - q0, -1.0, 1.0, 0), false);
         }
-        push_gemm(P, P->chol, gt);
+        push_gemm(P, seq, gt);
     }
     P->pg->split_side = false;
-    if (P->chol.size() == conv0 + 1)
-        P->chol.pop_back();  // nothing to overlap
+    if (seq.size() == conv0 + 1)
+        seq.pop_back();  // nothing to overlap
     else
-        for (size_t i = conv0 + 1; i < P->chol.size(); ++i) P->chol[i].side = 2;
-    if (fork) P->chol.push_back({K_JOIN, 0, 0, 0, 0, 1});
+        for (size_t i = conv0 + 1; i < seq.size(); ++i) seq[i].side = 2;
+    if (fork) seq.push_back({K_JOIN, 0, 0, 0, 0, 1});
 }
 
 // Projected-inverse launches for one level (projected_inverse.c:25-73).
@@ -651,6 +801,11 @@ int alloc_doubles(double** d, i64 len) {
     return 0;
 }
 
+int alloc_ints(int** d, i64 len) {
+    HIP_CHECK(hipMalloc((void**)d, (size_t)std::max<i64>(len, 1) * sizeof(int)));
+    return 0;
+}
+
 // Profiled launches take their start/stop timestamps from the kernel's own
 // dispatch (hipExtLaunchKernelGGL events), not from separate event packets.
 #define CHP_LAUNCH(K, LDS, ...)                                                        \
@@ -732,6 +887,13 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_POTRF_DIAG:
             CHP_LAUNCH(k_potrf_diag, 0, (const int*)list, L.iarg, fi,
                        blkval, P->d_dinv, P->d_dinvptr, P->d_status);
+            break;
+        case K_CHOL_DAG:
+            CHP_LAUNCH(k_chol_dag, 0, (const int4*)list, blkval, bt.p[1], bt.p[3], P->d_dagcnt,
+                       P->d_status, P->dag_wait_limit, P->d_dagtrace);
+            break;
+        case K_FILL_INT:
+            CHP_LAUNCH(k_fill_int, 0, P->d_dagcnt, (i64)L.iarg, 0);
             break;
         case K_PANEL_FACTOR:
             CHP_LAUNCH(k_panel_factor, 0, (const int4*)list, fi, blkval, P->d_dinv,
@@ -880,9 +1042,14 @@ int run_sweep(chp_plan* P, const std::vector<Launch>& seq, chp_plan::Cached& G, 
 }
 
 int read_status(chp_plan* P, hipStream_t s, i64* info) {
-    unsigned long long h = CHP_STATUS_OK;
-    HIP_CHECK(hipMemcpyAsync(&h, P->d_status, sizeof(h), hipMemcpyDeviceToHost, s));
+    unsigned long long hw[2] = {CHP_STATUS_OK, 0};
+    HIP_CHECK(hipMemcpyAsync(hw, P->d_status, sizeof(hw), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    if (hw[1]) {
+        chp_set_error("internal: a dependency wait of the large-front factorization timed out");
+        return -3;
+    }
+    const unsigned long long h = hw[0];
     if (h == CHP_STATUS_OK) {
         if (info) { info[0] = -1; info[1] = 0; }
         return 0;
@@ -968,6 +1135,7 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
         P->nb2 = param(1);
     }
     if (param(2) > 0) P->splitk_max = param(2);
+    if (param(4) > 0) P->chol_dag = param(4) == 1;
     P->lds_cap = std::min<size_t>(param(3) > 0 ? (size_t)param(3) : 64 * 1024, lds_block);
     if (P->lds_cap > 64 * 1024) {  // small-front kernels may take more than 64 KB of LDS
         for (const void* k : {(const void*)k_chol_small, (const void*)k_projinv_small,
@@ -1041,21 +1209,30 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
         lds = std::max(lds, need(f));
         return true;
     };
-    for (int L = 0; L < H;) {
-        int L2 = L;
-        size_t lds = 0;
-        while (L2 < H && one_small(L2, chol_small_lds, lds)) ++L2;
-        if (L2 - L >= CHAIN_MIN) {
-            std::vector<int> run;
-            for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
-            P->chol.push_back(chain_launch(P, K_CHOL_SMALL, run, lds));
-            L = L2;
-            continue;
+    auto build_seq = [&](std::vector<Launch>& seq, bool dag) {
+        for (int L = 0; L < H;) {
+            int L2 = L;
+            size_t lds = 0;
+            while (L2 < H && one_small(L2, chol_small_lds, lds)) ++L2;
+            if (L2 - L >= CHAIN_MIN) {
+                std::vector<int> run;
+                for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
+                seq.push_back(chain_launch(P, K_CHOL_SMALL, run, lds));
+                L = L2;
+                continue;
+            }
+            build_chol_level(P, P->bylevel[L], seq, dag);
+            ++L;
         }
-        build_chol_level(P, P->bylevel[L]);
-        ++L;
+        seq.push_back({K_JOIN, 0, 0, 0, 0, 2});
+    };
+    build_seq(P->chol, P->chol_dag);
+    build_seq(P->chol_lp, false);
+    if (P->dag_bad) return fail("internal: tile-DAG task order");
+    if (P->dagcnt_len > 0) {  // the DAG launches' counters start from zero
+        const int g = (int)std::min<i64>(1024, (P->dagcnt_len + 255) / 256);
+        P->chol.insert(P->chol.begin(), Launch{K_FILL_INT, g, 256, 0, 0, (int)P->dagcnt_len});
     }
-    P->chol.push_back({K_JOIN, 0, 0, 0, 0, 2});
     // scratch of the projected inverse: X = L11^{-1} of every large front,
     // then the largest level's W, then the largest level's T
     i64 wbase = 0, wmax = 0;
@@ -1162,7 +1339,8 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     rc |= alloc_doubles(&P->d_partial, n);
     rc |= alloc_doubles(&P->d_splitp, P->main.split_doubles());
     if (!rc && hipMemset(P->d_splitp, 0, P->main.split_doubles() * sizeof(double)) != hipSuccess) rc = -1;
-    if (hipMalloc((void**)&P->d_status, sizeof(unsigned long long)) != hipSuccess) rc = -1;
+    if (hipMalloc((void**)&P->d_status, 2 * sizeof(unsigned long long)) != hipSuccess) rc = -1;
+    if (!rc) rc |= alloc_ints(&P->d_dagcnt, P->dagcnt_len);
     if (rc) {
         std::string e = g_err;
         chp_plan_destroy(P);
@@ -1193,7 +1371,7 @@ void chp_plan_destroy(chp_plan* P) {
     void* ptrs[] = {P->d_fi, P->d_fi_full, P->d_relidx, P->d_chidx, P->d_perm, P->d_snptr, P->d_dinvptr,
                     P->d_xptr, P->d_tptr, P->d_wunit, P->d_gptr, P->d_gent, P->d_lltptr, P->d_colfront, P->d_colidx, P->d_upd,
                     P->d_dinv, P->d_scratch, P->d_partial, P->d_splitp, P->d_work, P->d_status,
-                    P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
+                    P->d_dagcnt, P->d_dagtrace, P->main.d_lists, P->main.d_tasks, P->dense.d_lists, P->dense.d_tasks,
                     P->d_dscratch, P->d_ddinv, P->d_dsplitp, P->d_supd, P->d_ustack, P->d_uupd};
     for (SolvePlan* S : P->solves) {
         for (auto& g : S->graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1214,6 +1392,7 @@ void chp_plan_destroy(chp_plan* P) {
 int chp_plan_params(chp_plan* P, int* out) {
     if (!P || !out) { chp_set_error("null argument"); return -1; }
     out[0] = P->small_max; out[1] = P->nb2; out[2] = P->splitk_max; out[3] = (int)P->lds_cap;
+    out[4] = P->chol_dag ? 1 : 2;
     return 0;
 }
 
@@ -1253,9 +1432,53 @@ i64 chp_profile_launches(chp_plan* P, double* out, i64 max) {
 // ms[NKIDS], cnt[NKIDS], flops[NKIDS] accumulated per kernel kind.  Kinds:
 // 0 chol_small 1 extadd 2 potrf_diag 3 panel_mul 4 gemm 5 extract
 // 6 projinv_small 7 trtri_diag 8 neg_copy 9 llt_small 10 solve_fwd 11 solve_bwd
-int chp_profile_read(chp_plan* P, double* ms, i64* cnt, double* flops) {
+int64_t chp_dag_trace(chp_plan* P, int on) {
+    if (!P) { chp_set_error("null argument"); return -1; }
+    if (on && !P->d_dagtrace && P->dag_tasks > 0) {
+        if (hipMalloc((void**)&P->d_dagtrace, (size_t)P->dag_tasks * 4 * sizeof(i64)) != hipSuccess) {
+            chp_set_error("trace allocation failed");
+            return -2;
+        }
+        (void)hipMemset(P->d_dagtrace, 0, (size_t)P->dag_tasks * 4 * sizeof(i64));
+    } else if (!on && P->d_dagtrace) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(P->d_dagtrace);
+        P->d_dagtrace = nullptr;
+    }
+    // the captured graphs hold the trace pointer: drop them
+    for (chp_plan::Cached* G : {&P->g_chol}) {
+        if (G->exec) (void)hipGraphExecDestroy(G->exec);
+        G->exec = nullptr;
+        G->key = nullptr;
+    }
+    return P->dag_tasks;
+}
+
+int chp_set_dag_wait_limit(chp_plan* P, int polls) {
+    if (!P || polls < 0) { chp_set_error("invalid argument"); return -1; }
+    P->dag_wait_limit = polls;
+    for (chp_plan::Cached* G : {&P->g_chol}) {  // the graphs hold the old bound
+        if (G->exec) (void)hipGraphExecDestroy(G->exec);
+        G->exec = nullptr;
+        G->key = nullptr;
+    }
+    return 0;
+}
+
+int64_t chp_dag_trace_read(chp_plan* P, int64_t* out, int64_t max) {
+    if (!P || !out) { chp_set_error("null argument"); return -1; }
+    if (!P->d_dagtrace) return 0;
+    const i64 n = std::min<i64>(max, P->dag_tasks);
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(out, P->d_dagtrace, (size_t)n * 4 * sizeof(i64), hipMemcpyDeviceToHost));
+    return n;
+}
+
+int chp_profile_read(chp_plan* P, double* ms, i64* cnt, double* flops, int max) {
     if (!P) return -1;
-    for (int i = 0; i < NKIDS; ++i) { ms[i] = P->prof_ms[i]; cnt[i] = P->prof_cnt[i]; flops[i] = P->prof_flops[i]; }
+    for (int i = 0; i < std::min(NKIDS, max); ++i) {
+        ms[i] = P->prof_ms[i]; cnt[i] = P->prof_cnt[i]; flops[i] = P->prof_flops[i];
+    }
     return NKIDS;
 }
 
@@ -1316,7 +1539,7 @@ int chp_cholesky_lowprec(chp_plan* P, double* blkval, int schur, void* stream, i
     if (schur != 1 && schur != 2) { chp_set_error("schur must be 1 (FP32) or 2 (BF16x3)"); return -1; }
     hipStream_t s = (hipStream_t)stream;
     P->schur_mode = schur;
-    int rc = run_sweep(P, P->chol, schur == 1 ? P->g_chol32 : P->g_cholbf, blkval, s, true);
+    int rc = run_sweep(P, P->chol_lp, schur == 1 ? P->g_chol32 : P->g_cholbf, blkval, s, true);
     P->schur_mode = 0;
     if (rc) return rc;
     return read_status(P, s, info);

@@ -52,12 +52,14 @@ class Plan(object):
     and work arrays resident in HBM).  Built on first use, cached on the
     symbolic object, freed with it."""
 
-    PARAMS = ("small_max", "panel_width", "splitk_max", "lds_cap")
+    PARAMS = ("small_max", "panel_width", "splitk_max", "lds_cap", "chol_dag")
 
     def __init__(self, symb, device, small_max=None, **params):
         """``params``: panel_width (outer panel of large fronts, a multiple
         of 64), splitk_max (1 = no split-K), lds_cap (bytes of LDS a
-        small-front workgroup may use); omitted ones keep their defaults."""
+        small-front workgroup may use), chol_dag (large fronts of the FP64
+        cholesky: 1 one tile-DAG launch per level, 2 panel and GEMM
+        launches, the default); omitted ones keep their defaults."""
         torch = _torch()
         lib = _native.lib()
         self.symb_arrays = [_native.as_i64(a) for a in (
@@ -79,8 +81,8 @@ class Plan(object):
         self._lib = lib
 
     def params(self):
-        """The plan's parameters: small_max, panel_width, splitk_max, lds_cap."""
-        out = np.zeros(4, dtype=np.int32)
+        """The plan's parameters: small_max, panel_width, splitk_max, lds_cap, chol_dag."""
+        out = np.zeros(5, dtype=np.int32)
         self._lib.chp_plan_params(self.handle, out.ctypes.data)
         return dict(zip(self.PARAMS, [int(v) for v in out]))
 

@@ -86,14 +86,16 @@ chp_plan* chp_plan_create(int64_t n, int64_t nsn, const int64_t* snpost,
  * panel width of the large-front factorization (a multiple of 64, default
  * 256), [2] cap of the split-K factor of GEMM tasks (1 = no split-K,
  * default 16), [3] LDS bytes a small-front workgroup may use (default
- * 65536, capped by the device); entries <= 0 or beyond nparams keep their
- * defaults. */
+ * 65536, capped by the device), [4] the large fronts of the FP64 cholesky:
+ * 1 one tile-DAG launch per level, 2 panel and GEMM launches (default);
+ * entries <= 0 or beyond nparams keep their defaults. */
 chp_plan* chp_plan_create_params(int64_t n, int64_t nsn, const int64_t* snpost,
                                  const int64_t* snptr, const int64_t* relptr,
                                  const int64_t* relidx, const int64_t* chptr,
                                  const int64_t* chidx, const int64_t* blkptr,
                                  const int64_t* p, const int* params, int nparams);
-/* out[4]: the parameters of a plan (small_max, panel width, split-K cap, LDS cap). */
+/* out[5]: the parameters of a plan (small_max, panel width, split-K cap, LDS
+ * cap, large-front cholesky 1 tile DAG / 2 panels). */
 int chp_plan_params(chp_plan* plan, int* out);
 void chp_plan_destroy(chp_plan* plan);
 /* Update-arena layout of a plan (host only, no device call): updptr[k] =
@@ -122,17 +124,31 @@ int chp_set_graphs(chp_plan* plan, int on);
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
  * (synchronizing after each launch, every launch on that stream) and clears
- * the counters; chp_profile_read fills ms/cnt/flops[24] per kernel kind
- * (0 chol_small, 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm, 5 extract,
- * 6 projinv_small, 7 trtri_diag, 8 neg_copy, 9 llt_small, 10 solve_fwd,
- * 11 solve_bwd, 12 copy_top, 13 splitk_reduce, 14 diag, 15 matop,
- * 16 update_factor, 17 sgather_fwd, 18 sgather_bwd, 19 sscatter,
- * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish, 23 completion_chain) and
- * returns 24.
+ * the counters; chp_profile_read fills ms/cnt/flops[min(max, 27)] per
+ * kernel kind (0 chol_small, 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm,
+ * 5 extract, 6 projinv_small, 7 trtri_diag, 8 neg_copy, 9 llt_small,
+ * 10 solve_fwd, 11 solve_bwd, 12 copy_top, 13 splitk_reduce, 14 diag,
+ * 15 matop, 16 update_factor, 17 sgather_fwd, 18 sgather_bwd, 19 sscatter,
+ * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish, 23 completion_chain,
+ * 24 projinv_chain, 25 chol_dag, 26 fill_int) and returns the number of
+ * kinds (27).
  * chp_plan_flops: algorithmic GEMM flops of one cholesky (out[0]) and one
  * projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
-int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops);
+int chp_profile_read(chp_plan* plan, double* ms, int64_t* cnt, double* flops, int max);
+/* Per-task trace of the tile-DAG launches of the cholesky sweep (measurement
+ * support): chp_dag_trace(plan, 1) allocates a device buffer of 4 int64 per
+ * task that later sweeps fill (kind | k << 8 | i << 24 | j << 40 |
+ * workgroup << 52; the 100 MHz wall clock when the task was taken, when its
+ * dependencies were met, when it finished), 0 frees it; both return the
+ * number of tasks.  chp_dag_trace_read copies min(max, tasks) rows to out. */
+int64_t chp_dag_trace(chp_plan* plan, int on);
+int64_t chp_dag_trace_read(chp_plan* plan, int64_t* out, int64_t max);
+/* Bound of every dependency wait of the tile-DAG launches, in polls (default
+ * 262144, about a quarter of a second); a wait that runs out makes the sweep
+ * return -3 ("internal: a dependency wait ... timed out") after the launch
+ * has drained.  For tests of that error path. */
+int chp_set_dag_wait_limit(chp_plan* plan, int polls);
 /* Per-launch rows (kind, workgroups, algorithmic flops, ms) of the profiled
  * sweeps: copies min(rows, max) rows into out[4*max], returns the row count. */
 int64_t chp_profile_launches(chp_plan* plan, double* out, int64_t max);

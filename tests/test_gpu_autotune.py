@@ -1,9 +1,13 @@
 """GPU tests of the tunable plan parameters and the autotuner
 (chompack_amd.autotune): every parameter choice gives the oracle's
 cholesky, projected inverse and llt (1e-12 relative), including small-front
-kernels with more than 64 KB of LDS and persistent launches over
-largest-first front lists; tune() installs its fastest plan and the factor
-it produces matches the default plan's."""
+kernels with more than 64 KB of LDS, persistent launches over
+largest-first front lists and the tile-DAG factorization of large fronts
+(chol_dag=1, also with every front large); tune() installs its fastest plan
+and the factor it produces matches the default plan's.  The DAG launch's
+bounded waits: with a wait bound of 0 polls the sweep reports the internal
+timeout (the launch still drains), and the plan factors correctly again once
+the bound is restored."""
 import numpy as np
 import pytest
 
@@ -20,10 +24,13 @@ PARAMS = [
     dict(small_max=128, lds_cap=163840),
     dict(small_max=96, lds_cap=163840, panel_width=128),
     dict(small_max=32, panel_width=512, splitk_max=1),
+    dict(small_max=64, chol_dag=1),
+    dict(small_max=1, chol_dag=1),
 ]
 
 
-@pytest.mark.parametrize("params", PARAMS, ids=["small128", "small96_panel128", "small32_panel512_nosplit"])
+@pytest.mark.parametrize("params", PARAMS, ids=["small128", "small96_panel128", "small32_panel512_nosplit",
+                                                "dag", "dag_all_large"])
 @pytest.mark.parametrize("name", ["grid2d_64", "grid3d_12"])
 def test_plan_parameters_parity(name, params):
     import torch
@@ -70,3 +77,29 @@ def test_tune_batched_grids():
     cp.cholesky(Lt)
     assert _rel(Lt.blkval.cpu().numpy(), Ld.blkval.cpu().numpy()) <= TOL
     assert np.allclose(bs.logdet(Lt), bs.logdet(Ld), rtol=1e-12)
+
+
+def test_dag_wait_bound():
+    """chol_dag=1 with every front large (grid3d_12): a wait bound of 0 polls
+    makes some dependency wait run out (tasks of a level start together), the
+    sweep returns the internal timeout error after the launch drains; with
+    the default bound restored the same plan factors to the oracle's."""
+    from chompack_amd import _native
+    A = problems.grid3d(12, seed=1)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    plan = numeric.Plan(symb, "cuda:0", small_max=1, chol_dag=1)
+    symb._plans["cuda:0"] = plan
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    ref = X.blkval.cpu().numpy().copy()
+    assert on.cholesky(symb, ref) == 0
+    lib = _native.lib()
+    assert lib.chp_set_dag_wait_limit(plan.handle, 0) == 0
+    Y = X.copy()
+    with pytest.raises(RuntimeError, match="timed out"):
+        cp.cholesky(Y)
+    assert lib.chp_set_dag_wait_limit(plan.handle, 1 << 18) == 0
+    Y = X.copy()
+    cp.cholesky(Y)
+    e = _rel(Y.blkval.cpu().numpy(), ref)
+    print("after the timeout: cholesky rel err %.3e" % e, flush=True)
+    assert e <= TOL
