@@ -173,93 +173,6 @@ __device__ inline bool block_potrf4(double (&a)[DQ], double (*P)[5], double (*Q)
     return true;
 }
 
-// block_potrf4 with a second set of rows: b[] holds 64 rows below the block
-// (thread (w, l) owns panel row l, columns w + 4q), eliminated alongside the
-// block's own rows, so they end as the panel rows A_panel L^{-T} of the
-// factor.  P needs 2 * NB rows; Q is 16-byte aligned (rows read as pairs).
-// The step is latency-bound (dependent FP64 operations cost tens of cycles
-// each on one wave): the block-row and panel-row substitutions run side by
-// side without a branch, and the rank-4 update takes four columns at a time
-// with their chains interleaved, so the scheduler does not serialize them
-// through one temporary.
-__device__ inline bool block_potrf4_panel(double (&a)[DQ], double (&b)[DQ], double (*P)[5],
-                                          double (*Q)[4], int w, int l, int* bad) {
-#pragma unroll
-    for (int s = 0; s < NB / 4; ++s) {
-        const int r0 = 4 * s;
-        P[l][w] = a[s];
-        P[NB + l][w] = b[s];
-        __syncthreads();
-        const double d00 = P[r0][0], d10 = P[r0 + 1][0], d20 = P[r0 + 2][0], d30 = P[r0 + 3][0];
-        const double d11 = P[r0 + 1][1], d21 = P[r0 + 2][1], d31 = P[r0 + 3][1];
-        const double d22 = P[r0 + 2][2], d32 = P[r0 + 3][2], d33 = P[r0 + 3][3];
-        const double p0 = P[l][0], p1 = P[l][1], p2 = P[l][2], p3 = P[l][3];
-        const double q0 = P[NB + l][0], q1 = P[NB + l][1], q2 = P[NB + l][2], q3 = P[NB + l][3];
-        const double i0 = prsqrt(d00);
-        const double L10 = d10 * i0, L20 = d20 * i0, L30 = d30 * i0;
-        const double e11 = d11 - L10 * L10;
-        const double i1 = prsqrt(e11);
-        const double L21 = (d21 - L20 * L10) * i1, L31 = (d31 - L30 * L10) * i1;
-        const double e22 = d22 - L20 * L20 - L21 * L21;
-        const double i2 = prsqrt(e22);
-        const double L32 = (d32 - L30 * L20 - L31 * L21) * i2;
-        const double e33 = d33 - L30 * L30 - L31 * L31 - L32 * L32;
-        const double i3 = prsqrt(e33);
-        const int fail = !(d00 > 0.0) ? 0 : (!(e11 > 0.0) ? 1 : (!(e22 > 0.0) ? 2 : (!(e33 > 0.0) ? 3 : -1)));
-        if (fail >= 0) { *bad = r0 + fail; return false; }
-        double x0 = p0 * i0, y0 = q0 * i0;
-        double x1 = (p1 - x0 * L10) * i1, y1 = (q1 - y0 * L10) * i1;
-        double x2 = (p2 - x0 * L20 - x1 * L21) * i2, y2 = (q2 - y0 * L20 - y1 * L21) * i2;
-        double x3 = (p3 - x0 * L30 - x1 * L31 - x2 * L32) * i3;
-        const double y3 = (q3 - y0 * L30 - y1 * L31 - y2 * L32) * i3;
-        // rows of the pivot block and above it: the factor's own entries (zero above)
-        const int r = l - r0;
-        if (r < 4) {
-            x0 = r == 0 ? d00 * i0 : (r == 1 ? L10 : (r == 2 ? L20 : (r == 3 ? L30 : 0.0)));
-            x1 = r == 1 ? e11 * i1 : (r == 2 ? L21 : (r == 3 ? L31 : 0.0));
-            x2 = r == 2 ? e22 * i2 : (r == 3 ? L32 : 0.0);
-            x3 = r == 3 ? e33 * i3 : 0.0;
-        }
-        a[s] = w == 0 ? x0 : (w == 1 ? x1 : (w == 2 ? x2 : x3));
-        b[s] = w == 0 ? y0 : (w == 1 ? y1 : (w == 2 ? y2 : y3));
-        if (w == 0) { Q[l][0] = x0; Q[l][1] = x1; Q[l][2] = x2; Q[l][3] = x3; }
-        __syncthreads();
-        // rank-4 update of owned a(l, j), j = w + 4q > r0 + 3, l >= j, and of b
-#pragma unroll
-        for (int g = s + 1; g < DQ; g += 4) {
-            double c[4][4], ta[4], tb[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (g + u >= DQ) continue;
-                const int j = w + 4 * (g + u);
-                c[u][0] = Q[j][0]; c[u][1] = Q[j][1]; c[u][2] = Q[j][2]; c[u][3] = Q[j][3];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (g + u < DQ) { ta[u] = x0 * c[u][0]; tb[u] = y0 * c[u][0]; }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (g + u < DQ) { ta[u] = fma(x1, c[u][1], ta[u]); tb[u] = fma(y1, c[u][1], tb[u]); }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (g + u < DQ) { ta[u] = fma(x2, c[u][2], ta[u]); tb[u] = fma(y2, c[u][2], tb[u]); }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (g + u < DQ) { ta[u] = fma(x3, c[u][3], ta[u]); tb[u] = fma(y3, c[u][3], tb[u]); }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = g + u;
-                if (q >= DQ) continue;
-                const double m = (l >= w + 4 * q) ? 1.0 : 0.0;
-                a[q] -= m * ta[u];
-                b[q] -= tb[u];
-            }
-        }
-    }
-    __syncthreads();
-    return true;
-}
-
 // X = L^{-1} (64 x 64, in LDS) from L (lower, in LDS) and rdiag = 1/diag(L),
 // by recursive doubling.  W is a 64 x 65 LDS temporary.
 __device__ inline void block_trtri_rd(const double (*Ls)[NB + 1], const double* rdiag,
@@ -373,12 +286,22 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
 // front's inverse area (dinv + dinvptr[k] + (c0 / 64) * 4096, ld 64): writing
 // it into blkval here would race with workgroups of the same launch that
 // have not read the block yet.  A later launch moves it into blkval.
-__global__ __launch_bounds__(TPB) void k_panel_factor(
+// PANEL_WAVES (8) waves: thread (w, l) owns diagonal-block element
+// (l, w + 8 q) in a[q] and panel element (r0 + l, w + 8 q) in b[q].  Each of
+// the 16 steps eliminates 4 columns: the owners of those columns publish
+// them (P), every thread factors the 4 x 4 pivot block redundantly and forms
+// its row of the 4-column panel, and the rank-4 update of the columns right
+// of the step is spread over all 8 waves.  The step is latency bound (a
+// dependent FP64 operation costs tens of cycles): two waves per SIMD hide
+// part of that, one wave per SIMD (4 waves) measured about 20% slower.
+#define PANEL_WAVES 8
+__global__ __launch_bounds__(64 * PANEL_WAVES) void k_panel_factor(
     const int4* __restrict__ tasks, const FrontInfo* __restrict__ fi,
     double* __restrict__ blkval, double* __restrict__ dinv, const i64* __restrict__ dinvptr,
     unsigned long long* status) {
-    __shared__ double P[2 * NB][5];
-    __shared__ __attribute__((aligned(16))) double Q[NB][4];
+    constexpr int W = PANEL_WAVES, C = 4, DQW = NB / W;
+    __shared__ double P[2 * NB][C + 1];
+    __shared__ __attribute__((aligned(16))) double Q[NB][C];
     const int4 t = tasks[blockIdx.x];
     const int k = t.x, c0 = t.y, r0 = t.z;
     const FrontInfo f = fi[k];
@@ -387,22 +310,91 @@ __global__ __launch_bounds__(TPB) void k_panel_factor(
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     double* blk = blkval + f.blkptr;
     double* dg = blk + c0 + (i64)c0 * nj;
-    double a[DQ], b[DQ];
     const bool prow = r0 >= 0 && r0 + l < nj;
+    double a[DQW], b[DQW];
 #pragma unroll
-    for (int q = 0; q < DQ; ++q) {
-        const int j = w + 4 * q;
+    for (int q = 0; q < DQW; ++q) {
+        const int j = w + W * q;
         a[q] = (l < nb && j < nb && l >= j) ? dg[l + (i64)j * nj] : (l == j ? 1.0 : 0.0);
         b[q] = (prow && j < nb) ? blk[(r0 + l) + (i64)(c0 + j) * nj] : 0.0;
     }
-    int bad = 0;
-    if (!block_potrf4_panel(a, b, P, Q, w, l, &bad)) {
+    int bad = -1;
+#pragma unroll
+    for (int s = 0; s < NB / C; ++s) {
+        const int rs = C * s, qs = rs / W, lo = rs % W;
+        const bool own = w >= lo && w < lo + C;  // this thread holds a column of the step
+        if (own) { P[l][w - lo] = a[qs]; P[NB + l][w - lo] = b[qs]; }
+        __syncthreads();
+        // the pivot block's factor Lb (lower) and 1/diag
+        double Lb[C][C], inv[C], x[C], y[C];
+        int fail = -1;
+#pragma unroll
+        for (int tt = 0; tt < C; ++tt) {
+            double d = P[rs + tt][tt];
+#pragma unroll
+            for (int m = 0; m < tt; ++m) d -= Lb[tt][m] * Lb[tt][m];
+            if (fail < 0 && !(d > 0.0)) fail = tt;
+            inv[tt] = prsqrt(d);
+#pragma unroll
+            for (int i2 = tt + 1; i2 < C; ++i2) {
+                double v = P[rs + i2][tt];
+#pragma unroll
+                for (int m = 0; m < tt; ++m) v -= Lb[i2][m] * Lb[tt][m];
+                Lb[i2][tt] = v * inv[tt];
+            }
+        }
+        if (fail >= 0) { bad = rs + fail; break; }  // (the same in every thread)
+        // this thread's block row l and panel row r0 + l in the step's columns
+        // (the block's own rows come out as its factor's entries; zero above)
+#pragma unroll
+        for (int tt = 0; tt < C; ++tt) {
+            double v = P[l][tt], u = P[NB + l][tt];
+#pragma unroll
+            for (int m = 0; m < tt; ++m) { v -= x[m] * Lb[tt][m]; u -= y[m] * Lb[tt][m]; }
+            x[tt] = v * inv[tt];
+            y[tt] = u * inv[tt];
+        }
+        const int r = l - rs;
+#pragma unroll
+        for (int tt = 0; tt < C; ++tt)
+            if (r < tt) x[tt] = 0.0;
+        if (own) {
+            double xw = x[0], yw = y[0];
+#pragma unroll
+            for (int tt = 1; tt < C; ++tt)
+                if (tt == w - lo) { xw = x[tt]; yw = y[tt]; }
+            a[qs] = xw;
+            b[qs] = yw;
+        }
+        if (w == 0) {
+#pragma unroll
+            for (int tt = 0; tt < C; ++tt) Q[l][tt] = x[tt];
+        }
+        __syncthreads();
+        // rank-C update of the owned columns right of the step
+#pragma unroll
+        for (int q = qs; q < DQW; ++q) {
+            const int j = w + W * q;
+            if (j < rs + C) continue;
+            double ta = 0.0, tb = 0.0;
+#pragma unroll
+            for (int tt = 0; tt < C; ++tt) {
+                const double c = Q[j][tt];
+                ta = fma(x[tt], c, ta);
+                tb = fma(y[tt], c, tb);
+            }
+            const double m = (l >= j) ? 1.0 : 0.0;
+            a[q] -= m * ta;
+            b[q] -= tb;
+        }
+    }
+    if (bad >= 0) {
         if (r0 < 0 && threadIdx.x == 0) chp_record_failure(status, k, c0 + bad + 1);
         return;
     }
 #pragma unroll
-    for (int q = 0; q < DQ; ++q) {
-        const int j = w + 4 * q;
+    for (int q = 0; q < DQW; ++q) {
+        const int j = w + W * q;
         if (j >= nb) continue;
         if (r0 < 0) {
             if (l < nb) dinv[dinvptr[k] + (i64)(c0 / NB) * NB * NB + l + (i64)j * NB] = a[q];

@@ -437,10 +437,13 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr, std::vector<Launc
                                              0, f.blkptr + c1 + (i64)c1 * nj, nj,
                                              nj - c1, p1 - c1, c1 - c0, -1.0, 1.0, 1), true);
             }
-            seq.push_back({K_PANEL_FACTOR, (int)pf.size(), 256, 0, P->pg->lists.add(pf), 0});
+            seq.push_back({K_PANEL_FACTOR, (int)pf.size(), 512, 0, P->pg->lists.add(pf), 0});
             push_gemm(P, seq, it, 1);
         }
-        // trailing update right of the outer panel, K = p1 - p0
+        // trailing update right of the outer panel, K = p1 - p0.  (Splitting
+        // off the next outer panel's columns and running the rest on a side
+        // stream measured slower: the rest's workgroups occupy every CU, so
+        // the next panels wait for CUs instead of overlapping it.)
         std::vector<GemmTile> gt;
         for (int k : large) {
             const FrontInfo& f = P->fi[k];
