@@ -718,6 +718,193 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
             }
 }
 
+// The same tile for launches too small to fill the device (the host picks it
+// for at most LAT_TILES tiles and no split-K): 8 waves, two groups of four
+// that take alternate K chunks with a staging area each (the K loop runs in
+// half the chunk rounds), partial sums added through LDS; C is read at the
+// start, while the first chunks load, instead of after the K loop.  Groups:
+// g = tid / 256; inside a group the layout of gemm_tile.
+template <int OPA, int OPB, bool CHK>
+__device__ inline void gemm_tile_lat(const GemmTask& T, const double* __restrict__ A,
+                                     const double* __restrict__ B, double* __restrict__ C,
+                                     int row0, int col0, int kbase, int kend,
+                                     double (*As)[BK][TILE + 2], double (*Bs)[BK][TILE + 2]) {
+    const int g = threadIdx.x >> 8, tid = threadIdx.x & 255, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    constexpr bool AK = (OPA == 1);
+    constexpr bool BK_ = (OPB != 1);
+    const int ai = AK ? (tid >> 5) : (tid & 63);
+    const int ak = AK ? (tid & 31) : (tid >> 6);
+    const int bj = BK_ ? (tid >> 5) : (tid & 63);
+    const int bk = BK_ ? (tid & 31) : (tid >> 6);
+    double ra[8], rb[8];
+    const bool tile_in = row0 + TILE <= T.M && col0 + TILE <= T.N;
+    auto chunk_plain = [&](int kb) -> bool {
+        if (!CHK) return true;
+        if (!tile_in || kb + BK > T.K) return false;
+        if (T.triA == 1 && !(kb + BK - 1 <= row0)) return false;
+        if (T.triA == 2 && !(kb >= row0 + TILE - 1)) return false;
+        if (T.pad == 1 && !(kb >= col0 + TILE - 1)) return false;
+        if (T.pad == 2 && !(kb + BK - 1 <= col0)) return false;
+        return OPA != 2 && (OPB != 2 || kb >= col0 + TILE - 1);
+    };
+    auto fetch = [&](int kb) {
+        if (chunk_plain(kb)) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+                ra[q] = (OPA == 1) ? A[gk + (i64)gi * T.lda] : A[gi + (i64)gk * T.lda];
+                const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+                rb[q] = (OPB == 1) ? B[gj + (i64)gk2 * T.ldb] : B[gk2 + (i64)gj * T.ldb];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int gi = row0 + ai + (AK ? 8 * q : 0), gk = kb + ak + (AK ? 0 : 4 * q);
+                ra[q] = ldA_chk<OPA>(T, A, gi, gk);
+                const int gj = col0 + bj + (BK_ ? 8 * q : 0), gk2 = kb + bk + (BK_ ? 0 : 4 * q);
+                rb[q] = ldB_chk<OPB>(T, B, gk2, gj);
+            }
+        }
+    };
+    double (*as)[TILE + 2] = As[g];
+    double (*bs)[TILE + 2] = Bs[g];
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            as[ak + (AK ? 0 : 4 * q)][ai + (AK ? 8 * q : 0)] = ra[q];
+            bs[bk + (BK_ ? 0 : 4 * q)][bj + (BK_ ? 8 * q : 0)] = rb[q];
+        }
+    };
+    const double alpha = T.alpha, beta = T.beta;
+    // C of this thread's accumulator entries (group 0), loaded up front
+    double cpre[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
+                const int j = col0 + wc * 32 + b * 16 + (lane & 15);
+                const bool need = g == 0 && beta != 0.0 && (!CHK || (i < T.M && j < T.N)) &&
+                                  (T.tri == 0 || i >= j);
+                cpre[a][b][v] = need ? C[i + (i64)j * T.ldc] : 0.0;
+            }
+    dbl4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    const int nch = (kend - kbase + BK - 1) / BK;
+    // group g takes chunks g, g + 2, ...; both groups run the same rounds
+    if (g < nch) fetch(kbase + g * BK);
+    for (int c0 = 0; c0 < nch; c0 += 2) {
+        const int c = c0 + g;
+        if (c < nch) stash();
+        __syncthreads();
+        if (c + 2 < nch) fetch(kbase + (c + 2) * BK);
+        if (c < nch) {
+#pragma unroll 2
+            for (int ks = 0; ks < BK / 4; ++ks) {
+                const int kr = ks * 4 + (lane >> 4);
+                const double a0 = as[kr][wr * 32 + (lane & 15)];
+                const double a1 = as[kr][wr * 32 + 16 + (lane & 15)];
+                const double q0 = bs[kr][wc * 32 + (lane & 15)];
+                const double q1 = bs[kr][wc * 32 + 16 + (lane & 15)];
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, q0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, q1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q1, acc[1][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // group 1's partial sums through LDS (the staging areas are free now)
+    double (*red)[TILE + 2] = reinterpret_cast<double (*)[TILE + 2]>(&As[0][0][0]);
+    if (g == 1) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    red[wr * 32 + a * 16 + (lane >> 4) + 4 * v][wc * 32 + b * 16 + (lane & 15)] = acc[a][b][v];
+    }
+    __syncthreads();
+    if (g == 1) return;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int il = wr * 32 + a * 16 + (lane >> 4) + 4 * v;
+                const int jl = wc * 32 + b * 16 + (lane & 15);
+                const int i = row0 + il, j = col0 + jl;
+                if (CHK && (i >= T.M || j >= T.N)) continue;
+                const double sum = acc[a][b][v] + red[il][jl];
+                double* c = C + i + (i64)j * T.ldc;
+                if (T.tri == 0 || i >= j) *c = alpha * sum + (beta != 0.0 ? beta * cpre[a][b][v] : 0.0);
+                else if (T.tri == 2) *c = 0.0;
+            }
+}
+
+// Tiles of launches too small to fill the device (no split-K): 512 threads,
+// gemm_tile_lat.  Same tile list and task table as k_gemm_tiles.
+__global__ __launch_bounds__(2 * TPB) void k_gemm_tiles_lat(
+    const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
+    __shared__ double As[2][BK][TILE + 2];
+    __shared__ double Bs[2][BK][TILE + 2];
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    const double* A = bt.p[T.bufA] + T.offA;
+    const double* B = bt.p[T.bufB] + T.offB;
+    double* C = bt.p[T.bufC] + T.offC;
+    const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
+    int kbase = 0, kend = T.K;
+    if (T.triA == 1) kend = min(kend, row0 + TILE);
+    if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
+    if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
+    if (T.pad == 2) kend = min(kend, col0 + TILE);
+    if (T.tri == 2 && row0 + TILE - 1 < col0) {
+        for (int e = threadIdx.x; e < TILE * TILE; e += 2 * TPB) {
+            int i = row0 + e % TILE, j = col0 + e / TILE;
+            if (i < T.M && j < T.N) C[i + (i64)j * T.ldc] = 0.0;
+        }
+        return;
+    }
+    int opa = T.opA;
+    const bool full = row0 + TILE <= T.M && col0 + TILE <= T.N && ((kend - kbase) % BK) == 0 &&
+                      T.triA == 0 && T.pad == 0 && T.opB != 2;
+    bool chk = !full;
+    if (opa == 2 && full) {
+        if (row0 >= T.K) opa = 0;
+        else chk = true;
+    }
+    const int key = opa * 3 + T.opB;
+    if (!chk) {
+        switch (key) {
+        case 0: gemm_tile_lat<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 1: gemm_tile_lat<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 3: gemm_tile_lat<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        default: gemm_tile_lat<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        }
+    } else {
+        switch (key) {
+        case 0: gemm_tile_lat<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 1: gemm_tile_lat<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 2: gemm_tile_lat<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 3: gemm_tile_lat<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 4: gemm_tile_lat<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 5: gemm_tile_lat<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 6: gemm_tile_lat<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 7: gemm_tile_lat<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        default: gemm_tile_lat<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        }
+    }
+}
+
 // Four waves per SIMD (the LDS of four workgroups fits a CU): the register
 // budget is capped at 128 so that the K loop's load and LDS latency overlap
 // with the other workgroups' MFMA work.

@@ -49,6 +49,10 @@ __global__ void k_chol_dag(const int4* list, double* blkval, double* upd, double
                            unsigned long long* status, int spin_limit, i64* trace);
 __global__ void k_fill_int(int* p, i64 n, int v);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+// launches of at most LAT_TILES tiles without split-K: 512 threads, K split
+// between two wave groups (the Launch's block size selects it)
+#define LAT_TILES 256
+__global__ void k_gemm_tiles_lat(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles_bf3(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);

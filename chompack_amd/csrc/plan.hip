@@ -141,7 +141,12 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
                int schur) {
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
-    Launch L{K_GEMM, (int)tiles.size(), 256, 0, P->pg->lists.add(tiles), schur};
+    // launches too small to fill the device take the latency-oriented tile
+    // kernel (512 threads), unless a tile is a split-K slice
+    bool split = false;
+    for (const GemmTile& g : tiles) split = split || P->pg->tasks[g.task].ksplit > 1;
+    const int block = (!split && (int)tiles.size() <= LAT_TILES) ? 512 : 256;
+    Launch L{K_GEMM, (int)tiles.size(), block, 0, P->pg->lists.add(tiles), schur};
     double fl = 0.0;
     int last = -1;
     for (const GemmTile& g : tiles)
@@ -903,12 +908,17 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            if (L.iarg == 1 && P->schur_mode == 1)  // Schur update in FP32 arithmetic
-                CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);
-            else if (L.iarg == 1 && P->schur_mode == 2)  // ... in BF16x3 split precision
-                CHP_LAUNCH(k_gemm_tiles_bf3, 0, (const GemmTile*)list, pg.d_tasks, bt);
-            else
+            if (L.iarg == 1 && P->schur_mode) {  // low-precision Schur updates, 256 threads
+                b = dim3(256);
+                if (P->schur_mode == 1)  // FP32 arithmetic
+                    CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);
+                else                     // BF16x3 split precision
+                    CHP_LAUNCH(k_gemm_tiles_bf3, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            } else if (L.block == 512) {
+                CHP_LAUNCH(k_gemm_tiles_lat, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            } else {
                 CHP_LAUNCH(k_gemm_tiles, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            }
             break;
         case K_SPLITK_REDUCE:
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);

@@ -322,7 +322,10 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
             CHP_SLAUNCH(k_matop, (const MatOp*)list, bt);
             break;
         case K_GEMM:
-            CHP_SLAUNCH(k_gemm_tiles, (const GemmTile*)list, S->pg.d_tasks, bt);
+            if (L.block == 512)
+                CHP_SLAUNCH(k_gemm_tiles_lat, (const GemmTile*)list, S->pg.d_tasks, bt);
+            else
+                CHP_SLAUNCH(k_gemm_tiles, (const GemmTile*)list, S->pg.d_tasks, bt);
             break;
         case K_SPLITK_REDUCE:
             CHP_SLAUNCH(k_splitk_reduce, (const GemmTile*)list, S->pg.d_tasks, bt);
