@@ -908,7 +908,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            if (L.iarg == 1 && P->schur_mode) {  // low-precision Schur updates, 256 threads
+            // launches too small to fill the device are latency bound: they
+            // stay FP64 (k_gemm_tiles_lat) in the low-precision sweeps too
+            if (L.iarg == 1 && P->schur_mode && L.block != 512) {  // low-precision Schur updates
                 b = dim3(256);
                 if (P->schur_mode == 1)  // FP32 arithmetic
                     CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);

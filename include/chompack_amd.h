@@ -171,7 +171,9 @@ int chp_llt(chp_plan* plan, double* blkval, void* stream);
 
 /* Cholesky with the Schur-complement updates (the trailing and inner-panel
  * updates of large fronts) in FP32 arithmetic on MFMA; diagonal blocks,
- * panels, the stored-convention solves and small fronts stay FP64.  The
+ * panels, the stored-convention solves, small fronts and Schur launches of
+ * at most 256 tiles (latency bound: lower precision saves nothing there)
+ * stay FP64.  The
  * factor is accurate to about FP32 precision; pair it with iterative
  * refinement (chp_spmv_residual + chp_trsm).  No reference counterpart. */
 int chp_cholesky_mixed(chp_plan* plan, double* blkval, void* stream, int64_t* info);
