@@ -905,6 +905,71 @@ __global__ __launch_bounds__(2 * TPB) void k_gemm_tiles_lat(
     }
 }
 
+// GEMM tasks far smaller than a tile (the host sends single-tile tasks with
+// K <= SMALL_TASK_K = 32 whose C does not overlap A or B here, e.g. the
+// per-direction products of the Hessian's many tiny fronts): one task per
+// 256-thread workgroup, no MFMA tile padding.  op(A) (M x K) and op(B)
+// (K x N) are staged in LDS with all of a thread's loads in flight (bounds
+// and triangular / symmetric forms through ldA_chk / ldB_chk), then each
+// thread forms up to 16 outputs (rows across consecutive threads, so C
+// writes coalesce) as K-term FMA sums from LDS.
+template <int OPA, int OPB>
+__device__ inline void small_task(const GemmTask& T, const double* __restrict__ A,
+                                  const double* __restrict__ B, double* __restrict__ C,
+                                  double (*As)[TILE + 1], double (*Bs)[TILE + 1]) {
+    const int M = T.M, N = T.N, K = T.K, tid = threadIdx.x;
+    double va[8], vb[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {  // element e = tid + 256 q of the K x 64 staging
+        const int e = tid + TPB * q, k = e >> 6, x = e & 63;
+        va[q] = (k < K && x < M) ? ldA_chk<OPA>(T, A, x, k) : 0.0;
+        vb[q] = (k < K && x < N) ? ldB_chk<OPB>(T, B, k, x) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + TPB * q, k = e >> 6, x = e & 63;
+        if (k < SMALL_TASK_K) { As[k][x] = va[q]; Bs[k][x] = vb[q]; }
+    }
+    __syncthreads();
+    for (int e = tid; e < M * N; e += TPB) {
+        const int i = e % M, j = e / M;
+        double* c = C + i + (i64)j * T.ldc;
+        if (T.tri != 0 && i < j) {
+            if (T.tri == 2) *c = 0.0;
+            continue;
+        }
+        double s0 = 0.0, s1 = 0.0;
+        int k = 0;
+        for (; k + 1 < K; k += 2) {
+            s0 = fma(As[k][i], Bs[k][j], s0);
+            s1 = fma(As[k + 1][i], Bs[k + 1][j], s1);
+        }
+        if (k < K) s0 = fma(As[k][i], Bs[k][j], s0);
+        *c = T.alpha * (s0 + s1) + (T.beta != 0.0 ? T.beta * *c : 0.0);
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_gemm_small(const int* __restrict__ ids, int count,
+                                                    const GemmTask* __restrict__ tasks, BufTab bt) {
+    __shared__ double As[SMALL_TASK_K][TILE + 1];
+    __shared__ double Bs[SMALL_TASK_K][TILE + 1];
+    const GemmTask T = tasks[ids[blockIdx.x]];
+    const double* A = bt.p[T.bufA] + T.offA;
+    const double* B = bt.p[T.bufB] + T.offB;
+    double* C = bt.p[T.bufC] + T.offC;
+    switch (T.opA * 3 + T.opB) {
+    case 0: small_task<0, 0>(T, A, B, C, As, Bs); break;
+    case 1: small_task<0, 1>(T, A, B, C, As, Bs); break;
+    case 2: small_task<0, 2>(T, A, B, C, As, Bs); break;
+    case 3: small_task<1, 0>(T, A, B, C, As, Bs); break;
+    case 4: small_task<1, 1>(T, A, B, C, As, Bs); break;
+    case 5: small_task<1, 2>(T, A, B, C, As, Bs); break;
+    case 6: small_task<2, 0>(T, A, B, C, As, Bs); break;
+    case 7: small_task<2, 1>(T, A, B, C, As, Bs); break;
+    default: small_task<2, 2>(T, A, B, C, As, Bs); break;
+    }
+}
+
 // Four waves per SIMD (the LDS of four workgroups fits a CU): the register
 // budget is capped at 128 so that the K loop's load and LDS latency overlap
 // with the other workgroups' MFMA work.

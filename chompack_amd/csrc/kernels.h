@@ -52,6 +52,9 @@ __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTa
 // launches of at most LAT_TILES tiles without split-K: 512 threads, K split
 // between two wave groups (the Launch's block size selects it)
 #define LAT_TILES 256
+// single-tile GEMM tasks with K at most this go to k_gemm_small (a workgroup each)
+#define SMALL_TASK_K 32
+__global__ void k_gemm_small(const int* ids, int count, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles_lat(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);

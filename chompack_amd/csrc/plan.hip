@@ -137,8 +137,51 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
         }
 }
 
-void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& tiles,
+// True when C's storage overlaps A's or B's (in-place products such as the
+// solves by an explicit inverse): such tasks need every operand read before
+// any output is written, which the tile kernels do and k_gemm_small does not.
+static bool c_aliases(const GemmTask& t) {
+    auto span = [](i64 off, int ld, int rows, int cols) {
+        return std::make_pair(off, off + (i64)ld * std::max(cols - 1, 0) + rows);
+    };
+    const auto c = span(t.offC, t.ldc, t.M, t.N);
+    const auto a = t.opA == 1 ? span(t.offA, t.lda, t.K, t.M) : span(t.offA, t.lda, std::max(t.M, t.K), t.K);
+    const auto b = t.opB == 1 ? span(t.offB, t.ldb, t.N, t.K) : span(t.offB, t.ldb, std::max(t.K, t.N), t.N);
+    auto hit = [&](std::pair<i64, i64> o) { return c.first < o.second && o.first < c.second; };
+    return (t.bufA == t.bufC && hit(a)) || (t.bufB == t.bufC && hit(b));
+}
+
+void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile>& all,
                int schur) {
+    if (all.empty()) return;
+    // tasks far smaller than a tile take one wave each (k_gemm_small) instead
+    // of a padded 64 x 64 MFMA tile; a task of at most 64 x 64 has one tile
+    std::vector<GemmTile> tiles;
+    std::vector<int> small;
+    double fls = 0.0;
+    for (const GemmTile& g : all) {
+        const GemmTask& t = P->pg->tasks[g.task];
+        // (the low-precision sweep keeps its Schur updates on the low-precision tiles)
+        if (t.ksplit == 1 && t.M <= TILE && t.N <= TILE && t.K <= SMALL_TASK_K && !c_aliases(t) &&
+            !(schur && P->building_lowprec)) {
+            small.push_back(g.task);
+            fls += task_flops(t);
+        } else {
+            tiles.push_back(g);
+        }
+    }
+    // a few tiny tasks beside regular tiles stay in the tile launch (a launch
+    // of their own would add a step to the sweep's critical path)
+    if (!small.empty() && !tiles.empty() && small.size() < 256) {
+        tiles = all;
+        small.clear();
+    }
+    if (!small.empty()) {
+        Launch S{K_GEMM_SMALL, (int)small.size(), 256, 0, P->pg->lists.add(small),
+                 (int)small.size()};
+        S.flops = fls;
+        seq.push_back(S);
+    }
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
     // launches too small to fill the device take the latency-oriented tile
@@ -908,9 +951,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        P->d_dinvptr, P->d_status);
             break;
         case K_GEMM:
-            // launches too small to fill the device are latency bound: they
-            // stay FP64 (k_gemm_tiles_lat) in the low-precision sweeps too
-            if (L.iarg == 1 && P->schur_mode && L.block != 512) {  // low-precision Schur updates
+            if (L.iarg == 1 && P->schur_mode) {  // low-precision Schur updates (256 threads)
                 b = dim3(256);
                 if (P->schur_mode == 1)  // FP32 arithmetic
                     CHP_LAUNCH(k_gemm_tiles32, 0, (const GemmTile*)list, pg.d_tasks, bt);
@@ -924,6 +965,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_SPLITK_REDUCE:
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            break;
+        case K_GEMM_SMALL:
+            CHP_LAUNCH(k_gemm_small, 0, (const int*)list, L.iarg, pg.d_tasks, bt);
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, fi, blkval,
@@ -1242,7 +1286,9 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
         seq.push_back({K_JOIN, 0, 0, 0, 0, 2});
     };
     build_seq(P->chol, P->chol_dag);
+    P->building_lowprec = true;
     build_seq(P->chol_lp, false);
+    P->building_lowprec = false;
     if (P->dag_bad) return fail("internal: tile-DAG task order");
     if (P->dagcnt_len > 0) {  // the DAG launches' counters start from zero
         const int g = (int)std::min<i64>(1024, (P->dagcnt_len + 255) / 256);

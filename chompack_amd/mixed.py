@@ -4,8 +4,7 @@
 large fronts (the bulk of the flops) in low precision on the matrix cores --
 FP32 (``schur="fp32"``) or BF16x3 split precision with FP32 accumulation
 (``"bf16x3"``) -- while the diagonal blocks, the panels, the
-stored-convention solves, the small LDS-resident fronts and the Schur
-launches too small to fill the device (latency bound) stay FP64; the
+stored-convention solves and the small LDS-resident fronts stay FP64; the
 factor is accurate to about FP32 precision (FP32) or about 1e-5 relative
 (BF16x3).
 ``solve(A, L, B, tol)`` then solves A x = b to a relative residual

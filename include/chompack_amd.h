@@ -124,14 +124,14 @@ int chp_set_graphs(chp_plan* plan, int on);
 /* Measurement support (no counterpart in the reference).  chp_profile(plan, 1)
  * times every launch of the sweeps with HIP events on the caller's stream
  * (synchronizing after each launch, every launch on that stream) and clears
- * the counters; chp_profile_read fills ms/cnt/flops[min(max, 27)] per
+ * the counters; chp_profile_read fills ms/cnt/flops[min(max, 28)] per
  * kernel kind (0 chol_small, 1 extadd, 2 potrf_diag, 3 panel_factor, 4 gemm,
  * 5 extract, 6 projinv_small, 7 trtri_diag, 8 neg_copy, 9 llt_small,
  * 10 solve_fwd, 11 solve_bwd, 12 copy_top, 13 splitk_reduce, 14 diag,
  * 15 matop, 16 update_factor, 17 sgather_fwd, 18 sgather_bwd, 19 sscatter,
  * 20 trmm_fwd, 21 trmm_bwd, 22 trmm_finish, 23 completion_chain,
- * 24 projinv_chain, 25 chol_dag, 26 fill_int) and returns the number of
- * kinds (27).
+ * 24 projinv_chain, 25 chol_dag, 26 fill_int, 27 gemm_small) and returns
+ * the number of kinds (28).
  * chp_plan_flops: algorithmic GEMM flops of one cholesky (out[0]) and one
  * projected-inverse sweep (out[1]). */
 int chp_profile(chp_plan* plan, int on);
@@ -171,9 +171,7 @@ int chp_llt(chp_plan* plan, double* blkval, void* stream);
 
 /* Cholesky with the Schur-complement updates (the trailing and inner-panel
  * updates of large fronts) in FP32 arithmetic on MFMA; diagonal blocks,
- * panels, the stored-convention solves, small fronts and Schur launches of
- * at most 256 tiles (latency bound: lower precision saves nothing there)
- * stay FP64.  The
+ * panels, the stored-convention solves and small fronts stay FP64.  The
  * factor is accurate to about FP32 precision; pair it with iterative
  * refinement (chp_spmv_residual + chp_trsm).  No reference counterpart. */
 int chp_cholesky_mixed(chp_plan* plan, double* blkval, void* stream, int64_t* info);
