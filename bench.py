@@ -591,7 +591,7 @@ def main():
              for i in range(NK) if kcnt[i]}
     dom = int(np.argmax(kms))
     gemm_tf = kfl[4] / (kms[4] * 1e-3) / 1e12 if kms[4] > 0 else 0.0
-    roof = {"kernel": "k_gemm_tiles (FP64 MFMA Schur/projected-inverse products)",
+    roof = {"kernel": "k_gemm_tiles + k_gemm_tiles_lat (FP64 MFMA Schur/projected-inverse tile launches)",
             "bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK["fp64_tflops"],
             "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK["fp64_tflops"], 4),
             "traffic": None, "launches": int(kcnt[4]),
@@ -601,7 +601,7 @@ def main():
     gb = np.zeros(4)
     L.chp_plan_gemm_bytes(plan.handle, _native.ptr(gb))
     if args.side == 40:
-        roof["traffic"] = profiled_traffic("k_gemm_tiles")
+        roof["traffic"] = profiled_traffic("gemm_kind")
         roof["traffic_source"] = "profiles/hbm_traffic_grid40_cholesky_projinv.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-kernel calibrated)"
     roof["algorithmic_bytes_per_launch"] = float((gb[0] + gb[1]) / max(gb[2] + gb[3], 1))
 
