@@ -83,6 +83,15 @@ struct BufTab {
     double* p[8];
 };
 
+// bt.p[i] for an index known only at run time, without the copy of the
+// table into scratch memory that indexing the kernel argument would take.
+__device__ inline double* buf_sel(const BufTab& bt, int i) {
+    double* r = bt.p[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = (i == k) ? bt.p[k] : r;
+    return r;
+}
+
 // One diagonal block (order nb <= 64) of a dense lower-triangular operation,
 // addressed by descriptor: element (i, j) at p[buf][off + i + j*ld].
 //   mode 0: Cholesky in place; the inverse of the factor goes row-major to
@@ -159,6 +168,22 @@ enum { DAG_POTRF = 0, DAG_TRSM = 1, DAG_UPDATE = 2 };
 struct ChainItem {
     UFTask uf;
     int front, kind, nch, pad;
+};
+
+// One front of a chain launch of cholesky, llt or the projected inverse
+// (k_chol_chain, k_llt_chain, k_projinv_chain), in the order the launch
+// walks them.  `nbr` is the chain neighbour whose matrix the front takes
+// from LDS (cholesky and llt: the previous front, a child of this one, with
+// its update matrix; projected inverse: the previous front, this one's
+// parent, having extracted this front's V), or -1; nbr_na and nbr_relptr
+// describe the child's relative indices (cholesky and llt: of that child;
+// projected inverse: of the next front, which is then kept in LDS).  `keep`
+// says whether the matrix this front hands on stays in LDS for the next
+// front (otherwise it goes to the update arena).
+struct ChainFront {
+    i64 blkptr, updptr, nbr_relptr;
+    int front, nn, na, chbeg, chend;
+    int nbr, nbr_na, keep;
 };
 
 // Row tile of a front in the solve's scatter: rows r0 .. r0 + 63 of front k,

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
     const UFTask t = tasks[blockIdx.x];
     const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
     const int* r = relidx + t.relpos;
-    const double* F = bt.p[t.fbuf] + t.foff;
+    const double* F = buf_sel(bt, t.fbuf) + t.foff;
     const double* Lk = bt.p[1] + t.lkoff;
     double* Li = bt.p[1] + t.outoff;
     double* C = bt.p[2] + t.woff;
@@ -363,6 +363,76 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
     }
 }
 
+// Barrier of the one-wave kernels below that orders LDS only: their global
+// stores are read by later launches, never by the same one, and a full
+// __syncthreads() would wait for each of them to complete.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Triangular solves of one right-hand side held across the lanes of a wave
+// (lane l: row l), with L lower triangular of order n <= 64 at
+// L(i, j) = T[r0 + i][r0 + j] in LDS and rdg = 1 / L(l, l) (0 for l >= n).
+// Step i subtracts c_i(l) z_i from every lane, with z_i lane i's running
+// value (final by then) broadcast by readlane and c_i(l) = L(i, l) / L(i, i)
+// (L^T z = b, last row first; zero unless l < i) or L(l, i) / L(i, i)
+// (L w = b; zero unless l > i); the solution is z rdg.  The coefficients of
+// the next eight steps are read from LDS while the current eight run, so the
+// dependent chain is a broadcast and an fma per step.
+template <bool TRANS>
+__device__ inline double wave_solve(const double (*T)[65], int r0, int n, double z, double rdg) {
+    const int l = threadIdx.x & 63;
+    const int nb = (n + 7) / 8;
+    double cur[8], nxt[8];
+    auto fetch = [&](int b, double (&v)[8]) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = 8 * b + u;
+            const bool ok = i < n && (TRANS ? l < i : (l > i && l < n));
+            v[u] = TRANS ? T[ok ? r0 + i : 0][ok ? r0 + l : 0] : T[ok ? r0 + l : 0][ok ? r0 + i : 0];
+        }
+    };
+    auto scale = [&](int b, double (&v)[8]) {  // times 1 / L(i, i), zero where unused
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = 8 * b + u;
+            const bool ok = i < n && (TRANS ? l < i : (l > i && l < n));
+            v[u] = v[u] * (ok ? readlane_d(rdg, i & 63) : 0.0);
+        }
+    };
+#pragma unroll
+    for (int bb = 0; bb < 8; ++bb) {
+        const int b = TRANS ? 7 - bb : bb;  // block order of the sweep
+        if (b < nb) {
+            const bool first = TRANS ? (b == nb - 1) : (b == 0);
+            if (first) fetch(b, cur);
+            const int bn = TRANS ? b - 1 : b + 1;
+            if (bn >= 0 && bn < nb) fetch(bn, nxt);
+            scale(b, cur);
+#pragma unroll
+            for (int uu = 0; uu < 8; ++uu) {  // (steps i >= n subtract zero)
+                const int u = TRANS ? 7 - uu : uu;
+                z -= cur[u] * readlane_d(z, 8 * b + u);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
+        }
+    }
+    return z * rdg;
+}
+
+__device__ inline double wave_solve_lt(const double (*T)[65], int r0, int n, double z,
+                                       double rdg) {
+    return wave_solve<true>(T, r0, n, z, rdg);
+}
+
+__device__ inline double wave_solve_l(const double (*T)[65], int r0, int n, double w,
+                                      double rdg) {
+    return wave_solve<false>(T, r0, n, w, rdg);
+}
+
 // The same update for small tasks (N <= 64 and na <= 64): one wave, with C
 // and Li held in LDS; lanes walk rows for the reductions and the copies and
 // own a column each in the reflections, the solve and the rank-1 updates.
@@ -373,29 +443,49 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
 // Returns false after recording a failed reversed Cholesky.
 // FL / KL, when given, hold the parent's front F (FL[i][j] = F(i, j)) and the
 // factor Lk (KL[i][j] = Lk(i, j)) in LDS in place of global memory.
-__device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __restrict__ relidx,
+__device__ __forceinline__ bool update_factor_wave(const UFTask t, BufTab bt, const int* __restrict__ relidx,
                                    unsigned long long* status, double (*Cs)[65],
                                    double (*Ls)[65], const double (*FL)[65] = nullptr,
                                    const double (*KL)[65] = nullptr) {
     const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
     const int l = threadIdx.x;
     const int* r = relidx + t.relpos;
-    const double* F = bt.p[t.fbuf] + t.foff;
+    const double* F = buf_sel(bt, t.fbuf) + t.foff;
     const double* Lk = bt.p[1] + t.lkoff;
     double* Li = bt.p[1] + t.outoff;
     const int rl = l < N ? r[l] : 0;  // lane j's r[j] is broadcast by readlane below
     const int N1 = __popcll(__ballot(l < N && rl < nn)), N2 = N - N1;
-    __syncthreads();
+    wave_lds_sync();
     // eight columns at a time, loads first (from a valid address also where
     // the entry is zero), so that they are in flight together
-    if (FL) {
-#pragma unroll 8
-        for (int j = 0; j < N; ++j)
-            Ls[l][j] = (j < N1 && l < N && l >= j) ? FL[rl][__builtin_amdgcn_readlane(rl, j)] : 0.0;
-#pragma unroll 8
-        for (int j = 0; j < N2; ++j) {
-            const int cj = __builtin_amdgcn_readlane(rl, N1 + j) - nn;
-            Cs[l][j] = (l < na && l >= cj) ? KL[l][cj] : 0.0;
+    if (FL) {  // from LDS, eight columns at a time, reads before writes
+        for (int j0 = 0; j0 < N; j0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + u;
+                const bool ok = j < N1 && l < N && l >= j;
+                v[u] = FL[ok ? rl : 0][ok ? __builtin_amdgcn_readlane(rl, j & 63) : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + u;
+                if (j < N) Ls[l][j] = (j < N1 && l < N && l >= j) ? v[u] : 0.0;
+            }
+        }
+        for (int j0 = 0; j0 < N2; j0 += 8) {
+            double v[8];
+            bool okv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + u;
+                const int cj = j < N2 ? __builtin_amdgcn_readlane(rl, (N1 + j) & 63) - nn : 0;
+                okv[u] = j < N2 && l < na && l >= cj;
+                v[u] = KL[okv[u] ? l : 0][okv[u] ? cj : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j0 + u < N2) Cs[l][j0 + u] = okv[u] ? v[u] : 0.0;
         }
     } else {
     for (int j0 = 0; j0 < N; j0 += 8) {
@@ -426,7 +516,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
             if (j0 + u < N2) Cs[l][j0 + u] = v[u];
     }
     }
-    __syncthreads();
+    wave_lds_sync();
     // Reflection js has nothing to reduce when its band is empty, i.e. when
     // piv - jj = (na - 1 - js) - (r[N-1-js] - nn) is 0.  That difference
     // never decreases with js (piv drops by one, r by at least one), so the
@@ -454,10 +544,10 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
         const double beta = -copysign(hypot(alpha, xn), alpha);
         const double tau = (beta - alpha) / beta;
         const double scal = 1.0 / (alpha - beta);
-        __syncthreads();  // every lane has read the pivot
+        wave_lds_sync();  // every lane has read the pivot
         if (l >= jj && l < piv) Cs[l][col] = x * scal;
         if (l == piv) Cs[piv][col] = beta;
-        __syncthreads();
+        wave_lds_sync();
         if (l < col) {
             double d = Cs[piv][l];
             for (int i = jj; i < piv; ++i) d += Cs[i][col] * Cs[i][l];
@@ -465,30 +555,33 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
             for (int i = jj; i < piv; ++i) Cs[i][l] -= f * Cs[i][col];
             Cs[piv][l] -= f;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
-#pragma unroll 8
-    for (int j = 0; j < N2; ++j)
-        if (l >= j && l < N2) Ls[N1 + l][N1 + j] = Cs[na - N2 + l][j];
-    __syncthreads();
+    for (int j0 = 0; j0 < N2; j0 += 8) {  // Li22 into place, reads before writes
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            const bool ok = l >= j && l < N2 && j < N2;
+            v[u] = Cs[ok ? na - N2 + l : 0][ok ? j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + u;
+            if (l >= j && l < N2 && j < N2) Ls[N1 + l][N1 + j] = v[u];
+        }
+    }
+    wave_lds_sync();
     // Li21 from Li22^T Li21 = B, column by column, lane l holding row l: once
     // x_i is known, rows l < i subtract Li22(i, l) x_i
     // (multiplied by the reciprocal pivots, computed off the dependent chain)
     const double rdg = l < N2 ? 1.0 / Ls[N1 + l][N1 + l] : 0.0;
     for (int c = 0; c < N1; ++c) {
         double z = l < N2 ? Ls[N1 + l][c] : 0.0;
-        const int lc = min(N1 + l, 64);  // Li22 column of lane l (clamped into the tile)
-        double sv = Ls[N1 + max(N2 - 1, 0)][lc];
-        for (int i = N2 - 1; i >= 0; --i) {
-            const double sn = Ls[N1 + max(i - 1, 0)][lc];  // next row's, off the chain
-            const double zi = readlane_d(z, i) * readlane_d(rdg, i);
-            if (l == i) z = zi;
-            if (l < i) z -= sv * zi;
-            sv = sn;
-        }
+        z = wave_solve_lt(Ls, N1, N2, z, rdg);
         if (l < N2) Ls[N1 + l][c] = z;
     }
-    __syncthreads();
+    wave_lds_sync();
     // Li11 -= Li21^T Li21 (lower), row l
     if (l < N1 && N2 > 0) {
         for (int j = 0; j <= l; ++j) {
@@ -504,7 +597,7 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
             Ls[l][j] -= (s0 + s1) + (s2 + s3);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // reversed Cholesky of the (1,1) block, last row first
     for (int p = N1 - 1; p >= 0; --p) {
         const double d = Ls[p][p];
@@ -513,20 +606,20 @@ __device__ bool update_factor_wave(const UFTask& t, BufTab bt, const int* __rest
             return false;
         }
         const double sd = sqrt(d), rs = 1.0 / sd;
-        __syncthreads();  // every lane has read the pivot
+        wave_lds_sync();  // every lane has read the pivot
         if (l < p) Ls[p][l] *= rs;
         if (l == p) Ls[p][p] = sd;
-        __syncthreads();
+        wave_lds_sync();
         if (l < p) {
             const double ri = Ls[p][l];
             for (int j = 0; j <= l; ++j) Ls[l][j] -= ri * Ls[p][j];
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 #pragma unroll 8
     for (int j = 0; j < N; ++j)
         if (l < N) Li[l + (i64)j * N] = Ls[l][j];
-    __syncthreads();  // the LDS tiles are free again
+    wave_lds_sync();  // the LDS tiles are free again
     return true;
 }
 
@@ -572,7 +665,7 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
     bool have_v = false;
     const int l = threadIdx.x;
     for (int it = 0; it < nitems;) {
-        const ChainItem& ci = items[it];
+        const ChainItem ci = items[it];
         const FrontInfo f = fi[ci.front];
         const int nn = f.nn, na = f.na, nj = nn + na, nch = ci.nch;
         double* blk = bt.p[0] + f.blkptr;
@@ -601,11 +694,11 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
                     if (l < na && j0 + u < na) V[l][j0 + u] = v[u];
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         // the children's update factors, from B and V in LDS
         bool keep = false;
         for (int c = 1; c <= nch; ++c) {
-            const ChainItem& cu = items[it + c];
+            const ChainItem cu = items[it + c];
             if (!update_factor_wave(cu.uf, bt, relidx, status, Cs, Ls, B, V)) return;
             keep = cu.kind == 2;
         }
@@ -622,17 +715,10 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
             const double rdg = l < na ? 1.0 / V[l][l] : 0.0;
             for (int c = 0; c < nn; ++c) {
                 double z = l < na ? B[nn + l][c] : 0.0;
-                double sv = V[na - 1][l];
-                for (int i = na - 1; i >= 0; --i) {
-                    const double sn = V[max(i - 1, 0)][l];  // next row's, off the chain
-                    const double zi = readlane_d(z, i) * readlane_d(rdg, i);
-                    if (l == i) z = zi;
-                    if (l < i) z -= sv * zi;
-                    sv = sn;
-                }
+                z = wave_solve_lt(V, 0, na, z, rdg);
                 if (l < na) B[nn + l][c] = z;
             }
-            __syncthreads();
+            wave_lds_sync();
             if (l < nn) {  // X11 -= Z^T Z, row l
                 for (int j = 0; j <= l; ++j) {
                     double s0 = 0.0;
@@ -640,20 +726,13 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
                     B[l][j] -= s0;
                 }
             }
-            __syncthreads();
+            wave_lds_sync();
             for (int c = 0; c < nn; ++c) {
                 double w = l < na ? B[nn + l][c] : 0.0;
-                double sv = V[l][0];
-                for (int i = 0; i < na; ++i) {
-                    const double sn = V[l][min(i + 1, 63)];  // next column's, off the chain
-                    const double wi = readlane_d(w, i) * readlane_d(rdg, i);
-                    if (l == i) w = wi;
-                    if (l > i && l < na) w -= sv * wi;
-                    sv = sn;
-                }
+                w = wave_solve_l(V, 0, na, w, rdg);
                 if (l < na) B[nn + l][c] = -w;
             }
-            __syncthreads();
+            wave_lds_sync();
         }
         // X11 = Lr^T Lr, last row first
         for (int p = nn - 1; p >= 0; --p) {
@@ -663,15 +742,15 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
                 return;
             }
             const double sd = sqrt(d), rs = 1.0 / sd;
-            __syncthreads();  // every lane has read the pivot
+            wave_lds_sync();  // every lane has read the pivot
             if (l < p) B[p][l] *= rs;
             if (l == p) B[p][p] = sd;
-            __syncthreads();
+            wave_lds_sync();
             if (l < p) {
                 const double ri = B[p][l];
                 for (int j = 0; j <= l; ++j) B[l][j] -= ri * B[p][j];
             }
-            __syncthreads();
+            wave_lds_sync();
         }
         // L11 = Lr^{-1} into Cs, column l
         if (l < nn) {
@@ -682,12 +761,12 @@ __global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __rest
                 Cs[i][l] = v / B[i][i];
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         for (int j = 0; j < nn; ++j) {
             if (l < nn && l >= j) blk[l + (i64)j * nj] = Cs[l][j];
             if (l >= nn && l < nj) blk[l + (i64)j * nj] = B[l][j];
         }
-        __syncthreads();
+        wave_lds_sync();
         if (keep) {  // the chain child's factor is the next front's Lk
             double (*x)[65] = V;
             V = Ls;

@@ -467,8 +467,8 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
     const DiagTask t = tasks[blockIdx.x];
     const int nb = t.nb, ld = t.ld;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    double* blk = bt.p[t.buf] + t.off;
-    double* out = bt.p[t.obuf] + t.outoff;
+    double* blk = buf_sel(bt, t.buf) + t.off;
+    double* out = buf_sel(bt, t.obuf) + t.outoff;
     if (t.mode == 0) {
         double a[DQ];
 #pragma unroll
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
     }
     __syncthreads();
     if (t.mode == 2) {
-        double* cp2 = bt.p[t.buf2] + t.off2;
+        double* cp2 = buf_sel(bt, t.buf2) + t.off2;
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
             const int j = w + 4 * q;
@@ -532,8 +532,8 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
 // tile; consecutive threads walk rows.
 __global__ __launch_bounds__(TPB) void k_matop(const MatOp* __restrict__ ops, BufTab bt) {
     const MatOp o = ops[blockIdx.x];
-    const double* S = bt.p[o.sbuf] + o.src;
-    double* D = bt.p[o.dbuf] + o.dst;
+    const double* S = buf_sel(bt, o.sbuf) + o.src;
+    double* D = buf_sel(bt, o.dbuf) + o.dst;
     const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
     const int i = o.i0 + il;
     if (i >= o.m) return;

@@ -273,42 +273,6 @@ __device__ inline void projinv_small_front(
     }
 }
 
-// V_c = F_p[r, r] (both triangles) for child c of the finished front f,
-// by the whole workgroup: the element-by-element form of k_extract.
-__device__ inline void extract_child(int c, const FrontInfo& f, const FrontInfo* __restrict__ fi,
-                                     const double* __restrict__ blkval, double* __restrict__ upd,
-                                     const int* __restrict__ relidx) {
-    const FrontInfo g = fi[c];
-    const int N = g.na;
-    const int* r = relidx + g.relptr;
-    const int nnp = f.nn, njp = f.nn + f.na, nap = f.na;
-    const double* pblk = blkval + f.blkptr;
-    const double* pV = upd + f.updptr;
-    double* V = upd + g.updptr;
-    for (int e = threadIdx.x; e < N * N; e += TPB) {
-        const int i = e % N, j = e / N;
-        const int ra = r[max(i, j)], rb = r[min(i, j)];  // ra >= rb
-        V[e] = (rb < nnp) ? pblk[ra + (i64)rb * njp] : pV[(ra - nnp) + (i64)(rb - nnp) * nap];
-    }
-}
-
-// Projected inverse along a chain of small fronts (one per level, parent
-// first) in one workgroup: each front, then its children's V from it, so
-// the next front of the chain finds its V in its update slot.
-__global__ __launch_bounds__(TPB) void k_projinv_chain(
-    const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,
-    double* __restrict__ blkval, double* __restrict__ upd, const int* __restrict__ relidx,
-    const int* __restrict__ chidx, unsigned long long* status) {
-    for (int q = 0; q < count; ++q) {
-        const int k = fronts[q];
-        projinv_small_front(k, fi, blkval, upd, status);
-        __syncthreads();
-        const FrontInfo f = fi[k];
-        for (int l = f.chbeg; l < f.chend; ++l) extract_child(chidx[l], f, fi, blkval, upd, relidx);
-        __syncthreads();
-    }
-}
-
 // Persistent over the level's small fronts, as k_chol_small.
 __global__ __launch_bounds__(TPB) void k_projinv_small(
     const int* __restrict__ fronts, int count, const FrontInfo* __restrict__ fi,

@@ -21,9 +21,18 @@ __global__ void k_extract(const int4* tiles, const FrontInfo* fi, const double* 
                           double* upd, const int* relidx, i64 bstride, i64 ustride);
 __global__ void k_projinv_small(const int* fronts, int count, const FrontInfo* fi, double* blkval,
                                 const double* upd, unsigned long long* status);
-// a chain of small fronts (one workgroup, parent first), children's V extracted
-__global__ void k_projinv_chain(const int* fronts, int count, const FrontInfo* fi, double* blkval,
-                                double* upd, const int* relidx, const int* chidx,
+// chains of small fronts (nj <= CHAIN_NJ), one workgroup walking a
+// ChainFront list: cholesky and llt child first, the projected inverse
+// parent first (children's V extracted, the chain child's kept in LDS)
+#define CHAIN_NJ 64
+#define CHAIN_TPB 1024
+__global__ void k_chol_chain(const ChainFront* cf, int count, const FrontInfo* fi, double* blkval,
+                             double* upd, const int* relidx, const int* chidx,
+                             unsigned long long* status);
+__global__ void k_llt_chain(const ChainFront* cf, int count, const FrontInfo* fi, double* blkval,
+                            double* upd, const int* relidx, const int* chidx);
+__global__ void k_projinv_chain(const ChainFront* cf, int count, const FrontInfo* fi,
+                                double* blkval, double* upd, const int* relidx, const int* chidx,
                                 unsigned long long* status);
 __global__ void k_llt_small(const int* fronts, int count, const FrontInfo* fi, double* blkval,
                             double* upd, const int* relidx, const int* chidx);

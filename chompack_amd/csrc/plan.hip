@@ -276,15 +276,42 @@ Launch small_launch(chp_plan* P, Kid kid, std::vector<int> fronts, size_t lds) {
 }
 
 // Chains: runs of at least CHAIN_MIN consecutive levels that hold one small
-// front each (an elimination-tree path, as in banded and arrow patterns) go
-// to one launch of a single workgroup that walks the fronts in order: child
-// before parent for cholesky and llt (a front reads its chain child's update
-// matrix right after the child wrote it), parent before child for the
-// projected inverse.  A level schedule would spend launches per front there.
+// front each (an elimination-tree path, as in banded and arrow patterns), of
+// order at most CHAIN_NJ, go to one launch of a single workgroup that walks
+// the fronts in order: child before parent for cholesky and llt, parent
+// before child for the projected inverse.  Levels are heights above the
+// leaves, so the one front of a level is a child of the next level's one
+// front; the matrix handed from one to the other stays in LDS (keep / nbr of
+// ChainFront).  A level schedule would spend launches per front there.
 constexpr int CHAIN_MIN = 3;
 
-Launch chain_launch(chp_plan* P, Kid kid, const std::vector<int>& fronts, size_t lds) {
-    return Launch{kid, 1, 256, lds, P->pg->lists.add(fronts), (int)fronts.size()};
+Launch chain_launch(chp_plan* P, Kid kid, const std::vector<int>& fronts, bool child_first) {
+    const size_t n = fronts.size();
+    std::vector<ChainFront> cf(n);
+    for (size_t q = 0; q < n; ++q) {
+        const int k = fronts[q];
+        const FrontInfo& f = P->fi[k];
+        ChainFront& c = cf[q];
+        std::memset(&c, 0, sizeof(c));
+        c.blkptr = f.blkptr;
+        c.updptr = f.updptr;
+        c.front = k;
+        c.nn = f.nn;
+        c.na = f.na;
+        c.chbeg = f.chbeg;
+        c.chend = f.chend;
+        c.nbr = -1;
+        if (q + 1 < n)
+            c.keep = child_first ? (f.parent == fronts[q + 1]) : (P->fi[fronts[q + 1]].parent == k);
+        if (q > 0 && cf[q - 1].keep) c.nbr = fronts[q - 1];
+        // the relative indices of the child handed over in LDS
+        const int ch = child_first ? (c.nbr >= 0 ? c.nbr : -1) : (c.keep ? fronts[q + 1] : -1);
+        if (ch >= 0) {
+            c.nbr_na = P->fi[ch].na;
+            c.nbr_relptr = P->fi[ch].relptr;
+        }
+    }
+    return Launch{kid, 1, CHAIN_TPB, 0, P->pg->lists.add(cf), (int)n};
 }
 
 // Tile-DAG launch for the large fronts of one level (k_chol_dag).  The tasks
@@ -978,8 +1005,16 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
                        blkval, P->d_upd, P->d_status);
             break;
         case K_PROJINV_CHAIN:
-            CHP_LAUNCH(k_projinv_chain, L.lds, (const int*)list, L.iarg, fi, blkval, P->d_upd,
+            CHP_LAUNCH(k_projinv_chain, 0, (const ChainFront*)list, L.iarg, fi, blkval, P->d_upd,
                        P->d_relidx, P->d_chidx, P->d_status);
+            break;
+        case K_CHOL_CHAIN:
+            CHP_LAUNCH(k_chol_chain, 0, (const ChainFront*)list, L.iarg, fi, blkval, P->d_upd,
+                       P->d_relidx, P->d_chidx, P->d_status);
+            break;
+        case K_LLT_CHAIN:
+            CHP_LAUNCH(k_llt_chain, 0, (const ChainFront*)list, L.iarg, fi, blkval, P->d_upd,
+                       P->d_relidx, P->d_chidx);
             break;
         case K_TRTRI_DIAG:
             CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, fi, blkval,
@@ -1261,22 +1296,19 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     P->h_chidx.resize(chptr[nsn]);
     for (i64 i = 0; i < relptr[nsn]; ++i) P->h_relidx[i] = (int)relidx[i];
     for (i64 i = 0; i < chptr[nsn]; ++i) P->h_chidx[i] = (int)chidx[i];
-    auto one_small = [&](int L, size_t (*need)(const FrontInfo&), size_t& lds) {
+    auto one_small = [&](int L) {
         if (P->bylevel[L].size() != 1) return false;
         const FrontInfo& f = P->fi[P->bylevel[L][0]];
-        if (f.nn + f.na > P->small_max || need(f) > P->lds_cap) return false;
-        lds = std::max(lds, need(f));
-        return true;
+        return f.nn + f.na <= std::min(P->small_max, CHAIN_NJ);
     };
     auto build_seq = [&](std::vector<Launch>& seq, bool dag) {
         for (int L = 0; L < H;) {
             int L2 = L;
-            size_t lds = 0;
-            while (L2 < H && one_small(L2, chol_small_lds, lds)) ++L2;
+            while (L2 < H && one_small(L2)) ++L2;
             if (L2 - L >= CHAIN_MIN) {
                 std::vector<int> run;
                 for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
-                seq.push_back(chain_launch(P, K_CHOL_SMALL, run, lds));
+                seq.push_back(chain_launch(P, K_CHOL_CHAIN, run, true));
                 L = L2;
                 continue;
             }
@@ -1313,12 +1345,11 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     P->projinv.push_back({K_FORK, 0, 0, 0, 0, 2});
     for (int L = H - 1; L >= 0;) {
         int L2 = L;
-        size_t lds = 0;
-        while (L2 >= 0 && one_small(L2, projinv_small_lds, lds)) --L2;
+        while (L2 >= 0 && one_small(L2)) --L2;
         if (L - L2 >= CHAIN_MIN) {
             std::vector<int> run;
             for (int M = L; M > L2; --M) run.push_back(P->bylevel[M][0]);
-            P->projinv.push_back(chain_launch(P, K_PROJINV_CHAIN, run, lds));
+            P->projinv.push_back(chain_launch(P, K_PROJINV_CHAIN, run, false));
             L = L2;
             continue;
         }
@@ -1329,12 +1360,11 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     if (xcur != wbase) return fail("projected-inverse scratch layout");
     for (int L = 0; L < H;) {
         int L2 = L;
-        size_t lds = 0;
-        while (L2 < H && one_small(L2, llt_small_lds, lds)) ++L2;
+        while (L2 < H && one_small(L2)) ++L2;
         if (L2 - L >= CHAIN_MIN) {
             std::vector<int> run;
             for (int M = L; M < L2; ++M) run.push_back(P->bylevel[M][0]);
-            P->llt.push_back(chain_launch(P, K_LLT_SMALL, run, lds));
+            P->llt.push_back(chain_launch(P, K_LLT_CHAIN, run, true));
             L = L2;
             continue;
         }
