@@ -441,12 +441,9 @@ __device__ inline double wave_solve_l(const double (*T)[65], int r0, int n, doub
 // their norm in every lane, and the solve runs down the columns with the
 // rows across the lanes, multiplying by reciprocal pivots.
 // Returns false after recording a failed reversed Cholesky.
-// FL / KL, when given, hold the parent's front F (FL[i][j] = F(i, j)) and the
-// factor Lk (KL[i][j] = Lk(i, j)) in LDS in place of global memory.
 __device__ __forceinline__ bool update_factor_wave(const UFTask t, BufTab bt, const int* __restrict__ relidx,
                                    unsigned long long* status, double (*Cs)[65],
-                                   double (*Ls)[65], const double (*FL)[65] = nullptr,
-                                   const double (*KL)[65] = nullptr) {
+                                   double (*Ls)[65]) {
     const int nn = t.nn, na = t.na, N = t.N, nj = nn + na;
     const int l = threadIdx.x;
     const int* r = relidx + t.relpos;
@@ -458,36 +455,7 @@ __device__ __forceinline__ bool update_factor_wave(const UFTask t, BufTab bt, co
     wave_lds_sync();
     // eight columns at a time, loads first (from a valid address also where
     // the entry is zero), so that they are in flight together
-    if (FL) {  // from LDS, eight columns at a time, reads before writes
-        for (int j0 = 0; j0 < N; j0 += 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = j0 + u;
-                const bool ok = j < N1 && l < N && l >= j;
-                v[u] = FL[ok ? rl : 0][ok ? __builtin_amdgcn_readlane(rl, j & 63) : 0];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = j0 + u;
-                if (j < N) Ls[l][j] = (j < N1 && l < N && l >= j) ? v[u] : 0.0;
-            }
-        }
-        for (int j0 = 0; j0 < N2; j0 += 8) {
-            double v[8];
-            bool okv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = j0 + u;
-                const int cj = j < N2 ? __builtin_amdgcn_readlane(rl, (N1 + j) & 63) - nn : 0;
-                okv[u] = j < N2 && l < na && l >= cj;
-                v[u] = KL[okv[u] ? l : 0][okv[u] ? cj : 0];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j0 + u < N2) Cs[l][j0 + u] = okv[u] ? v[u] : 0.0;
-        }
-    } else {
+    {
     for (int j0 = 0; j0 < N; j0 += 8) {
         double v[8];
 #pragma unroll
@@ -635,143 +603,4 @@ __global__ __launch_bounds__(64) void k_update_factor_small(const UFTask* __rest
         return;
     }
     update_factor_wave(t, bt, relidx, status, Cs, Ls);
-}
-
-// Completion (factored updates, completion.c:24-118) of a chain of small
-// fronts (nn + na <= 64, one front per level) in one 64-thread workgroup.
-// The items run in order: a front (kind 1, carrying the number nch of its
-// children) is followed by the update factors of its children (kind 0, and
-// kind 2 for the child that is the next front of the chain, which comes
-// last).  The front's columns B = [X11; X21] and the factor Lk of its update
-// matrix (V_k = Lk^T Lk) stay in LDS: the children's update factors read
-// them there, and the chain child's factor stays in LDS as the next front's
-// Lk (it is also stored to the update arena, as every child's is).  The step:
-//   X21 <- Lk^{-T} X21, X11 -= X21^T X21, X21 <- -Lk^{-1} X21 (substitution
-//   down each column, the rows across the lanes, by reciprocal pivots),
-//   X11 = Lr^T Lr (reversed Cholesky) and L11 = Lr^{-1} (forward
-//   substitution, a column per lane).
-// A zero pivot of Lk or a non-positive one of the reversed Cholesky records
-// a failure and ends the chain.
-__global__ __launch_bounds__(64) void k_completion_chain(const ChainItem* __restrict__ items,
-                                                         int nitems,
-                                                         const FrontInfo* __restrict__ fi,
-                                                         BufTab bt, const int* __restrict__ relidx,
-                                                         unsigned long long* status) {
-    __shared__ double T0[64][65], T1[64][65], T2[64][65], T3[64][65];
-    double (*B)[65] = T0;   // the front's columns
-    double (*V)[65] = T1;   // Lk of the front
-    double (*Cs)[65] = T2;  // update-factor workspace
-    double (*Ls)[65] = T3;  // a child's update factor
-    bool have_v = false;
-    const int l = threadIdx.x;
-    for (int it = 0; it < nitems;) {
-        const ChainItem ci = items[it];
-        const FrontInfo f = fi[ci.front];
-        const int nn = f.nn, na = f.na, nj = nn + na, nch = ci.nch;
-        double* blk = bt.p[0] + f.blkptr;
-        // loads first, eight columns at a time
-        for (int j0 = 0; j0 < nn; j0 += 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                v[u] = blk[(l < nj && j0 + u < nn) ? l + (i64)(j0 + u) * nj : 0];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (l < nj && j0 + u < nn) B[l][j0 + u] = v[u];
-        }
-        if (!have_v) {
-            const double* Lk = bt.p[1] + f.updptr;
-            for (int j0 = 0; j0 < na; j0 += 8) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool ok = l < na && j0 + u < na && l >= j0 + u;
-                    v[u] = Lk[ok ? l + (i64)(j0 + u) * na : 0];
-                    v[u] = ok ? v[u] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (l < na && j0 + u < na) V[l][j0 + u] = v[u];
-            }
-        }
-        wave_lds_sync();
-        // the children's update factors, from B and V in LDS
-        bool keep = false;
-        for (int c = 1; c <= nch; ++c) {
-            const ChainItem cu = items[it + c];
-            if (!update_factor_wave(cu.uf, bt, relidx, status, Cs, Ls, B, V)) return;
-            keep = cu.kind == 2;
-        }
-        it += 1 + nch;
-        if (na > 0) {
-            const int zero = __ffsll(__ballot(l < na && V[l][l] == 0.0));
-            if (zero) {
-                if (l == 0) chp_record_failure(status, ci.front, zero);
-                return;
-            }
-            // Lk^T Z = X21 and, below, Lk W = Z: column by column, lane l
-            // holding row l
-            // (multiplied by the reciprocal pivots)
-            const double rdg = l < na ? 1.0 / V[l][l] : 0.0;
-            for (int c = 0; c < nn; ++c) {
-                double z = l < na ? B[nn + l][c] : 0.0;
-                z = wave_solve_lt(V, 0, na, z, rdg);
-                if (l < na) B[nn + l][c] = z;
-            }
-            wave_lds_sync();
-            if (l < nn) {  // X11 -= Z^T Z, row l
-                for (int j = 0; j <= l; ++j) {
-                    double s0 = 0.0;
-                    for (int q = 0; q < na; ++q) s0 += B[nn + q][l] * B[nn + q][j];
-                    B[l][j] -= s0;
-                }
-            }
-            wave_lds_sync();
-            for (int c = 0; c < nn; ++c) {
-                double w = l < na ? B[nn + l][c] : 0.0;
-                w = wave_solve_l(V, 0, na, w, rdg);
-                if (l < na) B[nn + l][c] = -w;
-            }
-            wave_lds_sync();
-        }
-        // X11 = Lr^T Lr, last row first
-        for (int p = nn - 1; p >= 0; --p) {
-            const double d = B[p][p];
-            if (!(d > 0.0)) {
-                if (l == 0) chp_record_failure(status, ci.front, nn - p);
-                return;
-            }
-            const double sd = sqrt(d), rs = 1.0 / sd;
-            wave_lds_sync();  // every lane has read the pivot
-            if (l < p) B[p][l] *= rs;
-            if (l == p) B[p][p] = sd;
-            wave_lds_sync();
-            if (l < p) {
-                const double ri = B[p][l];
-                for (int j = 0; j <= l; ++j) B[l][j] -= ri * B[p][j];
-            }
-            wave_lds_sync();
-        }
-        // L11 = Lr^{-1} into Cs, column l
-        if (l < nn) {
-            Cs[l][l] = 1.0 / B[l][l];
-            for (int i = l + 1; i < nn; ++i) {
-                double v = 0.0;
-                for (int q = l; q < i; ++q) v -= B[i][q] * Cs[q][l];
-                Cs[i][l] = v / B[i][i];
-            }
-        }
-        wave_lds_sync();
-        for (int j = 0; j < nn; ++j) {
-            if (l < nn && l >= j) blk[l + (i64)j * nj] = Cs[l][j];
-            if (l >= nn && l < nj) blk[l + (i64)j * nj] = B[l][j];
-        }
-        wave_lds_sync();
-        if (keep) {  // the chain child's factor is the next front's Lk
-            double (*x)[65] = V;
-            V = Ls;
-            Ls = x;
-        }
-        have_v = keep;
-    }
 }
