@@ -27,7 +27,8 @@ KINDS = ["chol_small", "extadd", "potrf_diag", "panel_factor", "gemm", "extract"
          "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
          "copy_top", "splitk_reduce", "diag", "matop", "update_factor", "sgather_fwd",
          "sgather_bwd", "sscatter", "trmm_fwd", "trmm_bwd", "trmm_finish", "completion_chain",
-         "projinv_chain", "chol_dag", "fill_int", "gemm_small", "chol_chain", "llt_chain"]
+         "projinv_chain", "chol_dag", "fill_int", "gemm_small", "chol_chain", "llt_chain", "uf_setup",
+         "uf_block", "uf_apply"]
 NK = len(KINDS)
 
 

@@ -139,7 +139,23 @@ struct UFTask {
     // come from multi-workgroup triangular inverse and GEMM launches);
     // defer_rchol: leave A - Li21^T Li21 in Li11 (its reversed Cholesky runs
     // afterwards as a blocked multi-workgroup potrf)
-    unsigned char fbuf, defer_solve, defer_rchol, pad[5];
+    // ext_reduce: the copies and the reflections ran as launches of their own
+    // (k_uf_setup, k_uf_block, k_uf_apply); the kernel starts at Li22
+    unsigned char fbuf, defer_solve, defer_rchol, ext_reduce, pad[4];
+};
+
+// One block of consecutive reflections js0 .. js0 + nbk - 1 of a large
+// update-factor task (k_uf_block), or one 16-column tile [c0, c0 + 16) of the
+// columns left of the block that take its reflections in compact WY form
+// (k_uf_apply).  C (na x N2, ld na) is the task's workspace in buffer 2;
+// the block's bands cover rows lo .. lo + len - 1 of C; T (UF_BMAX x UF_BMAX),
+// tau and the band starts of the block go to buffer 2 at toff.
+#define UF_BMAX 16
+struct UFBlock {
+    i64 woff, relpos, toff;
+    int nn, na, N, N2;
+    int js0, nbk, lo, len;
+    int colmin, c0, key, pad;
 };
 
 // Tile-DAG factorization of a level's large fronts.  A front is cut into

@@ -68,12 +68,15 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
     }
     __syncthreads();
     const int N1 = s_n1, N2 = N - N1;
+    const bool ext = t.ext_reduce;  // copies and reflections done by k_uf_*
     // element loops: one column per wave, its rows over the lanes
+    if (!ext)
     for (int j = wv; j < N; j += UF_WAVES) {
         const i64 fj = (i64)r[j] * nj;
         for (int i = lane; i < N; i += 64)
             Li[i + (i64)j * N] = (j < N1 && i >= j) ? F[r[i] + fj] : 0.0;
     }
+    if (!ext)
     for (int j = wv; j < N2; j += UF_WAVES) {
         const int cj = r[N1 + j] - nn;
         for (int i = lane; i < na; i += 64)
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(UF_TPB) void k_update_factor(const UFTask* __restri
         // LAPACK dlarft forward recurrence), one wave per column.  A block
         // whose bands span more than UF_VMAX rows applies its reflections one
         // by one instead.
-        for (int js0 = 0; js0 < N2; js0 += UF_NBH) {
+        for (int js0 = 0; js0 < (ext ? 0 : N2); js0 += UF_NBH) {
             const int nb = min(UF_NBH, N2 - js0);
             const int colmin = N2 - js0 - nb;  // leftmost column of the block
             for (int b = 0; b < nb; ++b) {

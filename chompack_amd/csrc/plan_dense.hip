@@ -179,6 +179,38 @@ UFTask uf_task(const chp_plan* P, int c, int fbuf) {
 
 // Update matrices of the children of the fronts `fr`, read from the fronts
 // [blk (buffer fbuf); V or its factor (update arena)].
+// Blocks of consecutive reflections of a task with many of them (k_uf_block):
+// up to UF_BMAX reflections whose bands, rows lo .. na - 1 - js0, fit UF_VCAP
+// doubles together; runs of reflections with empty bands are left out.
+std::vector<UFBlock> uf_blocks(const chp_plan* P, const UFTask& t, int n1, i64 toff) {
+    const int* r = P->h_relidx.data() + t.relpos;
+    const int N = t.N, N2 = N - n1, nn = t.nn, na = t.na;
+    std::vector<UFBlock> out;
+    int js = 0;
+    while (js < N2) {
+        const int start = js, hi = na - 1 - start;
+        int lo = INT32_MAX, cnt = 0;
+        while (js < N2 && cnt < UF_BMAX) {
+            const int jj = r[N - 1 - js] - nn, piv = na - 1 - js;
+            const int nlo = piv - jj > 0 ? std::min(lo, jj) : lo;
+            const int nlen = nlo == INT32_MAX ? 1 : hi - nlo + 1;
+            if (cnt > 0 && (i64)nlen * (cnt + 1) > UF_VCAP) break;
+            lo = nlo;
+            ++cnt;
+            ++js;
+        }
+        if (lo == INT32_MAX) continue;
+        UFBlock B;
+        std::memset(&B, 0, sizeof(B));
+        B.woff = t.woff; B.relpos = t.relpos; B.toff = toff;
+        B.nn = nn; B.na = na; B.N = N; B.N2 = N2;
+        B.js0 = start; B.nbk = cnt; B.lo = lo; B.len = hi - lo + 1;
+        B.colmin = N2 - start - cnt; B.key = t.key;
+        out.push_back(B);
+    }
+    return out;
+}
+
 void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector<int>& fr,
                         bool factored, int fbuf, Bump& sc, Bump& dv) {
     const std::vector<int> ch = children_of(P, fr);
@@ -197,6 +229,12 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
     //     through reversed-transpose copies.
     // Smaller tasks are latency chains that stay inside the kernel.
     constexpr int UF_BLOCKED_N1 = 192, UF_SOLVE_N1 = 1, UF_SOLVE_N2 = 64;
+    // tasks with at least UF_EXT_N2 reflections reduce C on the whole device
+    // (k_uf_setup, k_uf_block, k_uf_apply) before k_update_factor's tail
+    constexpr int UF_EXT_N2 = 64;
+    std::vector<UFTask> xt;
+    std::vector<int4> xcols;
+    std::vector<std::vector<UFBlock>> xblocks;
     std::vector<MatOp> rt1, rt2;
     std::vector<Mat> R, L22s, Xs;
     std::vector<i64> Ws;
@@ -236,7 +274,37 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
         }
         // tasks that fit one wave's LDS tiles run in 64-thread workgroups
         const bool small = N <= 64 && f.na <= 64 && !t.defer_solve && !t.defer_rchol;
+        if (!small && n2 >= UF_EXT_N2 && f.na <= UF_VCAP) {
+            t.ext_reduce = 1;
+            const int ti = (int)xt.size();
+            xt.push_back(t);
+            for (int j = 0; j < N + n2; ++j) xcols.push_back(make_int4(ti, j, n1, 0));
+            xblocks.push_back(uf_blocks(P, t, n1, sc.take(UF_BMAX * UF_BMAX + 2 * UF_BMAX)));
+        }
         (small ? us : ut).push_back(t);
+    }
+    if (!xt.empty()) {
+        std::vector<char> lb(xt.size() * sizeof(UFTask) + xcols.size() * sizeof(int4));
+        std::memcpy(lb.data(), xt.data(), xt.size() * sizeof(UFTask));
+        std::memcpy(lb.data() + xt.size() * sizeof(UFTask), xcols.data(), xcols.size() * sizeof(int4));
+        seq.push_back(Launch{K_UF_SETUP, (int)xcols.size(), 256, 0, P->pg->lists.add(lb),
+                             (int)xt.size()});
+        size_t nbl = 0;
+        for (const auto& v : xblocks) nbl = std::max(nbl, v.size());
+        for (size_t b = 0; b < nbl; ++b) {
+            std::vector<UFBlock> bl, tl;
+            for (const auto& v : xblocks) {
+                if (b >= v.size()) continue;
+                bl.push_back(v[b]);
+                for (int c0 = 0; c0 < v[b].colmin; c0 += 16) {
+                    UFBlock T = v[b];
+                    T.c0 = c0;
+                    tl.push_back(T);
+                }
+            }
+            push_tasks(P, seq, K_UF_BLOCK, bl, 0);
+            push_tasks(P, seq, K_UF_APPLY, tl, 0);
+        }
     }
     push_tasks(P, seq, K_UPDATE_FACTOR, us);
     if (!us.empty()) seq.back().block = 64;
