@@ -140,22 +140,27 @@ struct UFTask {
     // defer_rchol: leave A - Li21^T Li21 in Li11 (its reversed Cholesky runs
     // afterwards as a blocked multi-workgroup potrf)
     // ext_reduce: the copies and the reflections ran as launches of their own
-    // (k_uf_setup, k_uf_block, k_uf_apply); the kernel starts at Li22
+    // (k_uf_setup, k_uf_step); the kernel starts at Li22
     unsigned char fbuf, defer_solve, defer_rchol, ext_reduce, pad[4];
 };
 
-// One block of consecutive reflections js0 .. js0 + nbk - 1 of a large
-// update-factor task (k_uf_block), or one 16-column tile [c0, c0 + 16) of the
-// columns left of the block that take its reflections in compact WY form
-// (k_uf_apply).  C (na x N2, ld na) is the task's workspace in buffer 2;
-// the block's bands cover rows lo .. lo + len - 1 of C; T (UF_BMAX x UF_BMAX),
-// tau and the band starts of the block go to buffer 2 at toff.
+// One item of a k_uf_step launch for a large update-factor task.  A block is
+// a run of consecutive reflections js0 .. js0 + nbk - 1 whose bands cover
+// rows lo .. lo + len - 1 of C (na x N2, ld na, the task's workspace in
+// buffer 2); its T (UF_BMAX x UF_BMAX), tau and band starts go to buffer 2
+// at toff.  role 0: the block's compact WY form applied to those of the 16
+// columns c0 .. c0 + 15 of C that lie below the bound colmin and outside
+// xlo .. xhi - 1; role 1: the block's reflections on its columns colmin ..
+// colmin + nbk - 1, after the previous block (pjs0, pnbk, plo, plen, ptoff;
+// pnbk = 0 if none) has been applied to them.
 #define UF_BMAX 16
 struct UFBlock {
-    i64 woff, relpos, toff;
+    i64 woff, relpos, toff, ptoff;
     int nn, na, N, N2;
     int js0, nbk, lo, len;
-    int colmin, c0, key, pad;
+    int colmin, c0, key, role;
+    int pjs0, pnbk, plo, plen;
+    int xlo, xhi;
 };
 
 // Tile-DAG factorization of a level's large fronts.  A front is cut into

@@ -76,13 +76,12 @@ __global__ void k_update_factor(const UFTask* tasks, BufTab bt, const int* relid
 __global__ void k_completion_chain(const ChainItem* items, int nitems, const FrontInfo* fi,
                                    BufTab bt, const int* relidx, unsigned long long* status);
 // large update-factor tasks on the whole device: the element copies (list:
-// UFTask[ntask] then int2 (task, column) items), one block of reflections in
-// LDS per task, and the block's compact WY form applied to 16-column tiles
-// of the columns left of it
+// UFTask[ntask] then int4 (task, column, N1, 0) items), then launches that
+// each run one block of reflections per task in LDS and apply the previous
+// block's compact WY form to 16-column tiles of the columns left of it
 #define UF_VCAP 12288  // len x nbk of a block (its V staged in LDS)
 __global__ void k_uf_setup(const char* list, int ntask, BufTab bt, const int* relidx);
-__global__ void k_uf_block(const UFBlock* blocks, BufTab bt, const int* relidx);
-__global__ void k_uf_apply(const UFBlock* tiles, BufTab bt);
+__global__ void k_uf_step(const UFBlock* items, BufTab bt, const int* relidx);
 // N <= 64 and na <= 64; 64-thread workgroups
 __global__ void k_update_factor_small(const UFTask* tasks, BufTab bt, const int* relidx,
                                       unsigned long long* status);

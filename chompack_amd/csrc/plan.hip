@@ -1015,11 +1015,8 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_UF_SETUP:
             CHP_LAUNCH(k_uf_setup, 0, (const char*)list, L.iarg, bt, P->d_relidx);
             break;
-        case K_UF_BLOCK:
-            CHP_LAUNCH(k_uf_block, 0, (const UFBlock*)list, bt, P->d_relidx);
-            break;
-        case K_UF_APPLY:
-            CHP_LAUNCH(k_uf_apply, 0, (const UFBlock*)list, bt);
+        case K_UF_STEP:
+            CHP_LAUNCH(k_uf_step, 0, (const UFBlock*)list, bt, P->d_relidx);
             break;
         case K_LLT_CHAIN:
             CHP_LAUNCH(k_llt_chain, 0, (const ChainFront*)list, L.iarg, fi, blkval, P->d_upd,

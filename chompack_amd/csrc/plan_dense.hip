@@ -179,9 +179,10 @@ UFTask uf_task(const chp_plan* P, int c, int fbuf) {
 
 // Update matrices of the children of the fronts `fr`, read from the fronts
 // [blk (buffer fbuf); V or its factor (update arena)].
-// Blocks of consecutive reflections of a task with many of them (k_uf_block):
+// Blocks of consecutive reflections of a task with many of them (k_uf_step):
 // up to UF_BMAX reflections whose bands, rows lo .. na - 1 - js0, fit UF_VCAP
-// doubles together; runs of reflections with empty bands are left out.
+// doubles together; runs of reflections with empty bands are left out.  The
+// blocks' T go to the two slots at toff in turn.
 std::vector<UFBlock> uf_blocks(const chp_plan* P, const UFTask& t, int n1, i64 toff) {
     const int* r = P->h_relidx.data() + t.relpos;
     const int N = t.N, N2 = N - n1, nn = t.nn, na = t.na;
@@ -202,7 +203,8 @@ std::vector<UFBlock> uf_blocks(const chp_plan* P, const UFTask& t, int n1, i64 t
         if (lo == INT32_MAX) continue;
         UFBlock B;
         std::memset(&B, 0, sizeof(B));
-        B.woff = t.woff; B.relpos = t.relpos; B.toff = toff;
+        B.woff = t.woff; B.relpos = t.relpos;
+        B.toff = toff + (out.size() & 1) * (UF_BMAX * UF_BMAX + 2 * UF_BMAX);
         B.nn = nn; B.na = na; B.N = N; B.N2 = N2;
         B.js0 = start; B.nbk = cnt; B.lo = lo; B.len = hi - lo + 1;
         B.colmin = N2 - start - cnt; B.key = t.key;
@@ -230,7 +232,7 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
     // Smaller tasks are latency chains that stay inside the kernel.
     constexpr int UF_BLOCKED_N1 = 192, UF_SOLVE_N1 = 1, UF_SOLVE_N2 = 64;
     // tasks with at least UF_EXT_N2 reflections reduce C on the whole device
-    // (k_uf_setup, k_uf_block, k_uf_apply) before k_update_factor's tail
+    // (k_uf_setup, k_uf_step) before k_update_factor's tail
     constexpr int UF_EXT_N2 = 64;
     std::vector<UFTask> xt;
     std::vector<int4> xcols;
@@ -279,7 +281,7 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
             const int ti = (int)xt.size();
             xt.push_back(t);
             for (int j = 0; j < N + n2; ++j) xcols.push_back(make_int4(ti, j, n1, 0));
-            xblocks.push_back(uf_blocks(P, t, n1, sc.take(UF_BMAX * UF_BMAX + 2 * UF_BMAX)));
+            xblocks.push_back(uf_blocks(P, t, n1, sc.take(2 * (UF_BMAX * UF_BMAX + 2 * UF_BMAX))));
         }
         (small ? us : ut).push_back(t);
     }
@@ -289,21 +291,39 @@ void emit_child_updates(chp_plan* P, std::vector<Launch>& seq, const std::vector
         std::memcpy(lb.data() + xt.size() * sizeof(UFTask), xcols.data(), xcols.size() * sizeof(int4));
         seq.push_back(Launch{K_UF_SETUP, (int)xcols.size(), 256, 0, P->pg->lists.add(lb),
                              (int)xt.size()});
+        // step s: block s of each task (role 1, first in the list: the
+        // critical path) and block s - 1 applied to the columns left of it
+        // but for block s's own (role 0)
         size_t nbl = 0;
         for (const auto& v : xblocks) nbl = std::max(nbl, v.size());
-        for (size_t b = 0; b < nbl; ++b) {
-            std::vector<UFBlock> bl, tl;
+        for (size_t s = 0; s <= nbl; ++s) {
+            std::vector<UFBlock> items, tiles;
             for (const auto& v : xblocks) {
-                if (b >= v.size()) continue;
-                bl.push_back(v[b]);
-                for (int c0 = 0; c0 < v[b].colmin; c0 += 16) {
-                    UFBlock T = v[b];
+                if (s < v.size()) {
+                    UFBlock B = v[s];
+                    B.role = 1;
+                    if (s > 0) {
+                        const UFBlock& Q = v[s - 1];
+                        B.pjs0 = Q.js0; B.pnbk = Q.nbk; B.plo = Q.lo; B.plen = Q.len;
+                        B.ptoff = Q.toff;
+                    }
+                    items.push_back(B);
+                }
+                if (s == 0 || s > v.size()) continue;
+                UFBlock T = v[s - 1];
+                T.role = 0;
+                if (s < v.size()) {
+                    T.xlo = v[s].colmin;
+                    T.xhi = v[s].colmin + v[s].nbk;
+                }
+                for (int c0 = 0; c0 < T.colmin; c0 += 16) {
+                    if (c0 >= T.xlo && c0 + 16 <= T.xhi) continue;  // all in the excluded range
                     T.c0 = c0;
-                    tl.push_back(T);
+                    tiles.push_back(T);
                 }
             }
-            push_tasks(P, seq, K_UF_BLOCK, bl, 0);
-            push_tasks(P, seq, K_UF_APPLY, tl, 0);
+            items.insert(items.end(), tiles.begin(), tiles.end());
+            push_tasks(P, seq, K_UF_STEP, items, 0);
         }
     }
     push_tasks(P, seq, K_UPDATE_FACTOR, us);

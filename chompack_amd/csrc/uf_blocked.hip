@@ -7,14 +7,17 @@
 //   * k_uf_setup: the element copies of Li's first N1 columns from the front
 //     and of C = Lk[:, r2 - nn] from the parent's factor, a workgroup per
 //     column;
-//   * k_uf_block: one block of up to UF_BMAX consecutive reflections per
-//     task, with the block's columns (rows lo .. lo + len - 1) in LDS; it
-//     leaves the reflected columns, the block's tau and band starts, and the
-//     upper triangular T of the compact WY form
+//   * k_uf_step, launched once per block of up to UF_BMAX consecutive
+//     reflections and once more: one workgroup per task applies the previous
+//     block's reflections to the block's own columns and then forms the
+//     block's reflections with those columns (rows lo .. lo + len - 1) in
+//     LDS, leaving the reflected columns, the block's tau and band starts,
+//     and the upper triangular T of the compact WY form
 //     H_{nb-1} ... H_0 = I - V T^T V^T (the dlarft forward recurrence) in
-//     global memory;
-//   * k_uf_apply: that form applied to 16-column tiles of the columns left of
-//     the block: w = V^T c, y = T^T w, c -= V y.
+//     global memory; the other workgroups apply the previous block's form to
+//     16-column tiles of the columns left of it, w = V^T c, y = T^T w,
+//     c -= V y.  T alternates between two slots per task, so the step that
+//     reads one block's T never writes it.
 // The arithmetic of each reflection is k_update_factor's (dlarfg convention
 // beta = -sign(alpha) ||[alpha; x]||, a pivot below PIVOT_NOISE ||x|| taken
 // as +0); k_update_factor then continues from Li22 (UFTask::ext_reduce).
@@ -64,19 +67,22 @@ __global__ __launch_bounds__(UB_TPB) void k_uf_setup(const char* __restrict__ li
     }
 }
 
-// One block of reflections of one task (see the file comment).  Reflection
+// One block of reflections of one task (see the file comment); its columns
+// have taken every earlier block's reflections.  Reflection
 // b of the block is js = js0 + b, on column col = N2 - 1 - js of C, band
 // rows jj .. piv - 1 with jj = r[N - 1 - js] - nn and pivot row
 // piv = na - 1 - js.
-__global__ __launch_bounds__(UB_TPB) void k_uf_block(const UFBlock* __restrict__ blocks, BufTab bt,
-                                                     const int* __restrict__ relidx) {
-    __shared__ double S[UF_VCAP];  // S[b * len + i] = C(lo + i, col_b)
-    __shared__ double red[UB_WAVES];
-    __shared__ double gram[UF_BMAX][UF_BMAX], tmat[UF_BMAX][UF_BMAX];
-    __shared__ double gpart[UB_WAVES][UF_BMAX][UF_BMAX];
-    __shared__ double s_tau[UF_BMAX], s_scal[UF_BMAX], s_beta[UF_BMAX];
-    __shared__ int s_jj[UF_BMAX];
-    const UFBlock B = blocks[blockIdx.x];
+__device__ void uf_reflect_block(const UFBlock& B, BufTab bt, const int* __restrict__ relidx,
+                                 double* lds) {
+    double* S = lds;  // S[b * len + i] = C(lo + i, col_b)
+    double* red = S + UF_VCAP;
+    double (*gram)[UF_BMAX] = reinterpret_cast<double (*)[UF_BMAX]>(red + UB_WAVES);
+    double (*tmat)[UF_BMAX] = gram + UF_BMAX;
+    double (*gpart)[UF_BMAX][UF_BMAX] = reinterpret_cast<double (*)[UF_BMAX][UF_BMAX]>(tmat + UF_BMAX);
+    double* s_tau = &gpart[UB_WAVES][0][0];
+    double* s_scal = s_tau + UF_BMAX;
+    double* s_beta = s_scal + UF_BMAX;
+    int* s_jj = reinterpret_cast<int*>(s_beta + UF_BMAX);
     const int nn = B.nn, na = B.na, N = B.N, N2 = B.N2;
     const int js0 = B.js0, nbk = B.nbk, lo = B.lo, len = B.len;
     const int* r = relidx + B.relpos;
@@ -294,20 +300,22 @@ __global__ __launch_bounds__(UB_TPB) void k_uf_block(const UFBlock* __restrict__
     }
 }
 
-// The block's reflections applied to columns c0 .. c0 + 15 (< colmin) of C:
-// w = V^T c, y = T^T w, c -= V y.  Thread t: column c0 + t / 16, rows
-// lo + (t % 16) + 16 m.
-__global__ __launch_bounds__(UB_TPB) void k_uf_apply(const UFBlock* __restrict__ tiles, BufTab bt) {
-    __shared__ double Vb[UF_VCAP];  // Vb[b * len + i] = V_b(lo + i)
-    __shared__ double Wp[16][UF_BMAX][17];
-    __shared__ double Ts[UF_BMAX][UF_BMAX], Wt[UF_BMAX][17], Yt[UF_BMAX][17];
-    const UFBlock B = tiles[blockIdx.x];
+// The block's reflections applied to those of the columns c0 .. c0 + 15 of
+// C below `bound` and outside xlo .. xhi - 1: w = V^T c, y = T^T w,
+// c -= V y.  Thread t: column c0 + t / 16, rows lo + (t % 16) + 16 m.
+__device__ void uf_apply_tile(const UFBlock& B, int c0, int bound, int xlo, int xhi, BufTab bt,
+                              double* lds) {
+    double* Vb = lds;  // Vb[b * len + i] = V_b(lo + i)
+    double (*Wp)[UF_BMAX][17] = reinterpret_cast<double (*)[UF_BMAX][17]>(Vb + UF_VCAP);
+    double (*Ts)[UF_BMAX] = reinterpret_cast<double (*)[UF_BMAX]>(&Wp[16][0][0]);
+    double (*Wt)[17] = reinterpret_cast<double (*)[17]>(&Ts[UF_BMAX][0]);
+    double (*Yt)[17] = Wt + UF_BMAX;
     const int na = B.na, N2 = B.N2, js0 = B.js0, nbk = B.nbk, lo = B.lo, len = B.len;
     double* C = bt.p[2] + B.woff;
     const double* Tg = bt.p[2] + B.toff;
     const int tid = threadIdx.x;
     const int cl = tid >> 4, rg = tid & 15;
-    const int col = B.c0 + cl;
+    const int col = c0 + cl;
     for (int b = 0; b < nbk; ++b) {  // eight loads in flight per thread
         const int pb = na - 1 - (js0 + b), jb = (int)Tg[UF_BMAX * UF_BMAX + UF_BMAX + b];
         const bool on = Tg[UF_BMAX * UF_BMAX + b] != 0.0;
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(UB_TPB) void k_uf_apply(const UFBlock* __restrict__
     }
     Ts[tid / UF_BMAX][tid % UF_BMAX] = Tg[tid];
     __syncthreads();
-    const bool live = col < B.colmin;
+    const bool live = col < bound && (col < xlo || col >= xhi);
     double* cc = C + (i64)col * na + lo;
     double w[UF_BMAX];
 #pragma unroll
@@ -370,4 +378,35 @@ __global__ __launch_bounds__(UB_TPB) void k_uf_apply(const UFBlock* __restrict__
             if (b < nbk) x -= Vb[b * len + i] * y[b];
         cc[i] = x;
     }
+}
+
+// LDS of a k_uf_step workgroup: the larger of the two roles' layouts
+// (V or the block's columns, then the 16-row-group partial sums and the
+// 16 x 16 T, W and Y of a tile; or the block's reduction scratch)
+#define UF_STEP_LDS (UF_VCAP + 16 * UF_BMAX * 17 + UF_BMAX * UF_BMAX + 2 * UF_BMAX * 17)
+static_assert(UF_STEP_LDS >= UF_VCAP + UB_WAVES + (2 + UB_WAVES) * UF_BMAX * UF_BMAX + 4 * UF_BMAX,
+              "k_uf_step LDS");
+
+// One step of the blocked reduction: role 0 items apply a block's compact WY
+// form to a 16-column tile; role 1 items apply the previous block's form to
+// the block's own columns, then run the block's reflections.
+__global__ __launch_bounds__(UB_TPB) void k_uf_step(const UFBlock* __restrict__ items, BufTab bt,
+                                                    const int* __restrict__ relidx) {
+    __shared__ double lds[UF_STEP_LDS];
+    const UFBlock B = items[blockIdx.x];
+    if (B.role == 0) {
+        uf_apply_tile(B, B.c0, B.colmin, B.xlo, B.xhi, bt, lds);
+        return;
+    }
+    if (B.pnbk > 0) {
+        UFBlock P = B;
+        P.js0 = B.pjs0;
+        P.nbk = B.pnbk;
+        P.lo = B.plo;
+        P.len = B.plen;
+        P.toff = B.ptoff;
+        uf_apply_tile(P, B.colmin, B.colmin + B.nbk, 0, 0, bt, lds);
+        __syncthreads();  // the updated columns are read back from C below
+    }
+    uf_reflect_block(B, bt, relidx, lds);
 }
