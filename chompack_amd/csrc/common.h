@@ -79,9 +79,15 @@ struct GemmTile {
 };
 
 // Base pointers of the buffers a launch addresses (kernel argument by value).
+// gate: when not null, the dense kernels (k_diag_tasks, k_matop and the GEMM
+// kernels) return at once unless *gate != 0, so that a launch sequence can be
+// made conditional on a flag the device sets without a host round trip.
 struct BufTab {
     double* p[8];
+    const int* gate;
 };
+
+__device__ inline bool gated_off(const BufTab& bt) { return bt.gate != nullptr && *bt.gate == 0; }
 
 // bt.p[i] for an index known only at run time, without the copy of the
 // table into scratch memory that indexing the kernel argument would take.

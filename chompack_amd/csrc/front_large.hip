@@ -474,6 +474,7 @@ __global__ __launch_bounds__(TPB) void k_trtri_diag(
 // and Hessian sweeps): mode 0 factors the block in place and writes the
 // inverse of the factor row-major to the output; mode 1 writes the inverse
 // of the lower-triangular block column-major (zero above the diagonal).
+// Launches whose BufTab gate reads 0 do nothing.
 __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__ tasks,
                                                     BufTab bt, unsigned long long* status) {
     __shared__ double Ls[NB][NB + 1];
@@ -482,6 +483,7 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
     __shared__ double P[NB][5], Q[NB][5];
     __shared__ double rdiag[NB];
     __shared__ int zero_at;
+    if (gated_off(bt)) return;
     const DiagTask t = tasks[blockIdx.x];
     const int nb = t.nb, ld = t.ld;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -534,10 +536,9 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
         rdiag[l] = 1.0 / Ls[l][l];
     }
     __syncthreads();
-    if (zero_at < NB) {
-        if (threadIdx.x == 0) chp_record_failure(status, t.key, t.col + zero_at + 1);
-        return;
-    }
+    // a zero pivot is recorded; the inverse is still formed (non-finite, as
+    // a triangular solve with the block would be)
+    if (zero_at < NB && threadIdx.x == 0) chp_record_failure(status, t.key, t.col + zero_at + 1);
     block_trtri_rd(Ls, rdiag, X, W);
 #pragma unroll
     for (int q = 0; q < DQ; ++q) {
@@ -549,6 +550,7 @@ __global__ __launch_bounds__(TPB) void k_diag_tasks(const DiagTask* __restrict__
 // Element-wise tile operations (see MatOpKind).  One workgroup per 64 x 64
 // tile; consecutive threads walk rows.
 __global__ __launch_bounds__(TPB) void k_matop(const MatOp* __restrict__ ops, BufTab bt) {
+    if (gated_off(bt)) return;
     const MatOp o = ops[blockIdx.x];
     const double* S = buf_sel(bt, o.sbuf) + o.src;
     double* D = buf_sel(bt, o.dbuf) + o.dst;
@@ -874,6 +876,7 @@ __global__ __launch_bounds__(2 * TPB) void k_gemm_tiles_lat(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     __shared__ double As[2][BK][TILE + 2];
     __shared__ double Bs[2][BK][TILE + 2];
+    if (gated_off(bt)) return;
     const GemmTile tl = tiles[blockIdx.x];
     const GemmTask T = tasks[tl.task];
     const double* A = bt.p[T.bufA] + T.offA;
@@ -971,6 +974,7 @@ __global__ __launch_bounds__(TPB) void k_gemm_small(const int* __restrict__ ids,
                                                     const GemmTask* __restrict__ tasks, BufTab bt) {
     __shared__ double As[SMALL_TASK_K][TILE + 1];
     __shared__ double Bs[SMALL_TASK_K][TILE + 1];
+    if (gated_off(bt)) return;
     const GemmTask T = tasks[ids[blockIdx.x]];
     const double* A = bt.p[T.bufA] + T.offA;
     const double* B = bt.p[T.bufB] + T.offB;
@@ -995,6 +999,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     __shared__ double As[BK][TILE + 2];
     __shared__ double Bs[BK][TILE + 2];
+    if (gated_off(bt)) return;
     const GemmTile tl = tiles[blockIdx.x];
     const GemmTask T = tasks[tl.task];
     const double* A = bt.p[T.bufA] + T.offA;
@@ -1441,6 +1446,7 @@ __global__ __launch_bounds__(TPB) void k_splitk_reduce(
     const GemmTile* __restrict__ tiles, const GemmTask* __restrict__ tasks, BufTab bt) {
     // tiles[i / 8] is the output tile; workgroup i % 8 reduces its 8 columns
     // i % 8 * 8 .. +7 (slices summed in slice order)
+    if (gated_off(bt)) return;
     const GemmTile tl = tiles[blockIdx.x >> 3];
     const int q = blockIdx.x & 7;
     const GemmTask T = tasks[tl.task];

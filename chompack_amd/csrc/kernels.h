@@ -97,6 +97,7 @@ __global__ void k_solve_bwd(const int2* items, const FrontInfo* fi, const double
                             const i64* snptr, const int* relidx, const i64* perm,
                             double* B, int ldb, int nrhs, double alpha, double* work,
                             const i64* wptr);
+__global__ void k_flag_clear(int* flag);
 __global__ void k_l11_check(const STile* tiles, const FrontInfo* fi, const double* blkval,
                             double* copy, int* changed);
 

@@ -898,6 +898,7 @@ BufTab main_tab(chp_plan* P, double* blkval) {
     BufTab t;
     t.p[0] = blkval; t.p[1] = P->d_upd; t.p[2] = P->d_scratch; t.p[3] = P->d_dinv;
     t.p[4] = P->d_splitp; t.p[5] = nullptr; t.p[6] = nullptr; t.p[7] = nullptr;
+    t.gate = nullptr;
     return t;
 }
 

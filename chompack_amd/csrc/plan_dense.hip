@@ -821,6 +821,7 @@ BufTab dense_tab(chp_plan* P, double* x, const double* L, const double* Y) {
     t.p[0] = x; t.p[1] = P->d_upd; t.p[2] = P->d_dscratch; t.p[3] = P->d_ddinv;
     t.p[4] = P->d_dsplitp; t.p[5] = const_cast<double*>(L); t.p[6] = const_cast<double*>(Y);
     t.p[7] = nullptr;
+    t.gate = nullptr;
     return t;
 }
 

@@ -241,10 +241,13 @@ void build_trmm(chp_plan* P, SolvePlan* S) {
     P->pg = &P->main;
 }
 
+// gate: when not null, the dense launches of seq do nothing unless *gate != 0.
 int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
-              const double* blkval, double* B, int ldb, int nrhs, double alpha, hipStream_t s) {
+              const double* blkval, double* B, int ldb, int nrhs, double alpha, hipStream_t s,
+              const int* gate = nullptr) {
     BufTab bt;
     for (int i = 0; i < 8; ++i) bt.p[i] = nullptr;
+    bt.gate = gate;
     bt.p[0] = const_cast<double*>(blkval);
     bt.p[2] = S->d_w;
     bt.p[4] = S->d_splitp;
@@ -420,22 +423,26 @@ int triangular_op(chp_plan* P, const double* blkval, double* B, int ldb, int nrh
         HIP_CHECK(hipMalloc((void**)&P->d_work, need * sizeof(double)));
         P->work_len = need;
     }
-    hipLaunchKernelGGL(k_status_init, dim3(1), dim3(1), 0, s, P->d_status);
-    // the inverses X_k: formed again when L11 of a large front is not the one
-    // they were formed from (an exact comparison with a copy)
-    if (!product && !S->chk.empty()) {
-        int changed = 0;
-        HIP_CHECK(hipMemsetAsync(S->d_changed, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_l11_check, dim3((unsigned)S->chk.size()), dim3(256), 0, s, S->d_chk,
+    // The inverses X_k of the large fronts' L11: formed on first use, then
+    // formed again only when L11 is not the one they were formed from.  The
+    // exact comparison with a copy (k_l11_check) sets the gate that the
+    // re-inversion's launches test on the device, so no call waits for the
+    // host.  A zero pivot gives non-finite inverses and results, as a
+    // triangular solve with the block would.
+    const bool inv = !product && !S->chk.empty();
+    auto refresh = [&](hipStream_t st, bool gated) -> int {
+        hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, st, S->d_changed);
+        hipLaunchKernelGGL(k_l11_check, dim3((unsigned)S->chk.size()), dim3(256), 0, st, S->d_chk,
                            P->d_fi, blkval, S->d_l11, S->d_changed);
-        HIP_CHECK(hipMemcpyAsync(&changed, S->d_changed, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        if (changed || !S->inv_valid) {
-            S->inv_valid = false;
-            int rc = run_solve(P, S, S->inv, blkval, B, ldb, nrhs, alpha, s);
-            if (rc) return rc;
-            S->inv_valid = true;
-        }
+        return run_solve(P, S, S->inv, blkval, B, ldb, nrhs, alpha, st,
+                         gated ? S->d_changed : nullptr);
+    };
+    bool fresh = false;
+    if (inv && !S->inv_valid) {
+        int rc = refresh(s, false);
+        if (rc) return rc;
+        S->inv_valid = true;
+        fresh = true;
     }
     const std::vector<Launch>& seq = product ? (trans == 'N' ? S->mfwd : S->mbwd)
                                              : (trans == 'N' ? S->fwd : S->bwd);
@@ -459,7 +466,8 @@ int triangular_op(chp_plan* P, const double* blkval, double* B, int ldb, int nrh
             }
             hipGraph_t graph = nullptr;
             HIP_CHECK(hipStreamBeginCapture(P->gstream, hipStreamCaptureModeThreadLocal));
-            int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, P->gstream);
+            int rc = inv ? refresh(P->gstream, true) : 0;
+            if (!rc) rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, P->gstream);
             hipError_t e = hipStreamEndCapture(P->gstream, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
             if (e != hipSuccess) { chp_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return -2; }
@@ -478,14 +486,9 @@ int triangular_op(chp_plan* P, const double* blkval, double* B, int ldb, int nrh
         HIP_CHECK(hipStreamWaitEvent(s, P->gev_out, 0));
     } else {
         S->last = key;
-        int rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, s);
+        int rc = (inv && !fresh) ? refresh(s, true) : 0;
+        if (!rc) rc = run_solve(P, S, seq, blkval, B, ldb, nrhs, alpha, s);
         if (rc) return rc;
-    }
-    i64 info[2];
-    if (read_status(P, s, info) != 0) {
-        S->inv_valid = false;
-        chp_set_error("trsm: singular diagonal block in supernode " + std::to_string(info[0]));
-        return -4;
     }
     return 0;
 }

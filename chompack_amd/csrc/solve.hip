@@ -207,6 +207,11 @@ __global__ __launch_bounds__(64) void k_solve_bwd(
     }
 }
 
+// *flag = 0 (a kernel rather than a memset, so that a captured graph replays it).
+__global__ void k_flag_clear(int* flag) {
+    if (threadIdx.x == 0) *flag = 0;
+}
+
 // Compares the lower triangle of L11 of the listed fronts (16-column tiles,
 // t.r0 = first column) with the copy the solve's inverses were formed from
 // (at t.tofs, ld nn); refreshes the copy and sets *changed where they differ.

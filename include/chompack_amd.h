@@ -219,7 +219,10 @@ int chp_hessian(chp_plan* plan, const double* lblkval, const double* yblkval,
 
 /* B := alpha L^{-1} B (trans 'N') or alpha L^{-T} B (trans 'T'), B a device
  * column-major n x nrhs array with leading dimension ldb in the original
- * ordering; replaces trsm() (trsm.c:3-138, cbase.c:1042-1049). */
+ * ordering; replaces trsm() (trsm.c:3-138, cbase.c:1042-1049).  Asynchronous
+ * on `stream` (no host round trip once the schedule for nrhs exists); a zero
+ * pivot gives non-finite entries in B, as the reference's dtrsm does, not an
+ * error. */
 int chp_trsm(chp_plan* plan, const double* blkval, double* B, int ldb, int nrhs,
              double alpha, int trans, void* stream);
 

@@ -830,6 +830,40 @@ def test_trsm_offset_and_leading_dimension():
     assert np.array_equal(got[mask], store[mask])
 
 
+def test_trsm_zero_pivot_nonfinite_not_error():
+    """A zero pivot in L11 of the root front (a large front whose inverse the
+    solve keeps): trsm gives non-finite entries and no error, as the
+    reference's dtrsm does; the inverse is formed again on the device when
+    L11 changes, first to the singular block and then back."""
+    import torch
+    A = problems.grid3d(12, seed=6)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    cp.cholesky(X)
+    k = symb.Nsn - 1
+    assert symb.snptr[k + 1] - symb.snptr[k] > 64
+    rng = np.random.default_rng(2)
+    B0 = torch.as_tensor(np.asfortranarray(rng.standard_normal((symb.n, 3)))).to("cuda:0")
+    B0 = B0.t().contiguous().t()
+    outs = []
+    for rep in range(3):  # eager, then the captured graph
+        B = B0.clone().t().contiguous().t()
+        cp.trsm(X, B)
+        outs.append(B.cpu().numpy())
+    assert np.isfinite(outs[0]).all()
+    assert all(np.array_equal(o, outs[0]) for o in outs)
+    p = int(symb.blkptr[k])
+    d = X.blkval[p].item()
+    X.blkval[p] = 0.0
+    B = B0.clone().t().contiguous().t()
+    cp.trsm(X, B)
+    assert not np.isfinite(B.cpu().numpy()).all()
+    X.blkval[p] = d
+    B = B0.clone().t().contiguous().t()
+    cp.trsm(X, B)
+    assert np.array_equal(B.cpu().numpy(), outs[0])
+
+
 @pytest.mark.parametrize("name", ["grid2d_64", "grid3d_12", "ref31"])
 def test_factor_export_on_gpu(name):
     """spmatrix() of a GPU factor (L21 = B21 L11 on the GPU) against the same
