@@ -1440,6 +1440,146 @@ __global__ __launch_bounds__(TPB) void k_copy_top(
         for (int i = threadIdx.x; i < nn; i += TPB) S[i + (i64)j * nj] = blk[i + (i64)j * nj];
 }
 
+// ---------------------------------------------------------------------------
+// GEMM tasks with at most NARROW_N columns (the solves' products with a few
+// right-hand sides): one workgroup per 64-row tile (and split-K slice), the
+// four waves taking four consecutive parts of the slice's K range, summed in
+// LDS in wave order.  A wave steps through K sixteen at a time with
+// v_mfma_f64_16x16x4f64 on four 16-row blocks; in the step's MFMA j, lane
+// group g = lane / 16 supplies k = k0 + 4 g + j, so a lane reads four
+// consecutive k of one row of op(A) (or column of op(B)), which is what makes
+// the transposed operand's reads contiguous.  The next step's operands are
+// loaded before the current step's MFMAs.
+template <int OPA, int OPB>
+__device__ inline void narrow_load(const GemmTask& T, const double* __restrict__ A,
+                                   const double* __restrict__ B, const int* row, int c, int k0,
+                                   int klim, bool mask, double (&a)[4][4], double (&b)[4]) {
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int k = k0 + 4 * g + j;
+        if (mask) k = k < klim ? k : k0;  // a valid address; zeroed at use
+        const int cc = mask ? (c < T.N ? c : 0) : c;
+        b[j] = (OPB == 0) ? B[k + (i64)cc * T.ldb] : B[cc + (i64)k * T.ldb];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int r = mask ? (row[rb] < T.M ? row[rb] : 0) : row[rb];
+            a[rb][j] = (OPA == 0) ? A[r + (i64)k * T.lda] : A[k + (i64)r * T.lda];
+        }
+    }
+}
+
+template <int OPA, int OPB, int TRIA>
+__device__ inline void gemm_narrow(const GemmTask& T, const double* __restrict__ A,
+                                   const double* __restrict__ B, double* __restrict__ C,
+                                   int row0, int kbase, int kend, double* Pp,
+                                   double (*red)[TILE][NARROW_N + 1]) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    // a triangular op(A) (TRIA 1: zero for k > i; 2: zero for k < i) limits K
+    if (TRIA == 1) kend = min(kend, row0 + TILE);
+    if (TRIA == 2) kbase = max(kbase, row0);
+    const int q = ((max(kend - kbase, 0) + 63) / 64) * 16;  // each wave's share, a multiple of 16
+    const int k0w = min(kend, kbase + w * q), k1w = max(k0w, min(kend, k0w + q));
+    int row[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) row[rb] = row0 + rb * 16 + c;
+    // steps [kc0, kc1) need no masks: full rows and columns, whole steps, and
+    // clear of the diagonal of a triangular op(A)
+    int kc0 = k0w, kc1 = k0w + ((k1w - k0w) & ~15);
+    if (TRIA == 2 && row0 + TILE - 1 > k0w) kc0 = k0w + ((row0 + TILE - 1 - k0w + 15) & ~15);
+    if (TRIA == 1) kc1 = min(kc1, row0 + 1 > k0w ? k0w + ((row0 + 1 - k0w) & ~15) : k0w);
+    if (!(row0 + TILE <= T.M && T.N == NARROW_N)) kc0 = kc1 = k0w;
+    kc0 = min(kc0, k1w);
+    kc1 = max(kc1, kc0);
+    dbl4 acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[rb] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    double a[4][4], b[4], na[4][4], nb[4];
+    auto masked_step = [&](int k0) {
+        narrow_load<OPA, OPB>(T, A, B, row, c, k0, k1w, true, a, b);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + 4 * g + j;
+            const bool kok = k < k1w;
+            const double bj = (kok && c < T.N) ? b[j] : 0.0;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const bool on = kok && row[rb] < T.M && (TRIA != 1 || k <= row[rb]) &&
+                                (TRIA != 2 || k >= row[rb]);
+                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? a[rb][j] : 0.0, bj, acc[rb], 0, 0, 0);
+            }
+        }
+    };
+    for (int k0 = k0w; k0 < kc0; k0 += 16) masked_step(k0);
+    if (kc0 < kc1) narrow_load<OPA, OPB>(T, A, B, row, c, kc0, k1w, false, a, b);
+    for (int k0 = kc0; k0 < kc1; k0 += 16) {
+        const bool more = k0 + 16 < kc1;
+        if (more) narrow_load<OPA, OPB>(T, A, B, row, c, k0 + 16, k1w, false, na, nb);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][j], b[j], acc[rb], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                b[j] = nb[j];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) a[rb][j] = na[rb][j];
+            }
+        }
+    }
+    for (int k0 = kc1; k0 < k1w; k0 += 16) masked_step(k0);
+    // accumulator lane l, element v: row 16 rb + l / 16 + 4 v, column l % 16
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[w][rb * 16 + g + 4 * v][c] = acc[rb][v];
+    __syncthreads();
+    const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = tid + u * TPB, i = e & 63, j = e >> 6;
+        const int gi = row0 + i;
+        if (gi >= T.M || j >= T.N) continue;
+        const double s = (red[0][i][j] + red[1][i][j]) + (red[2][i][j] + red[3][i][j]);
+        if (Pp) { Pp[gi + (i64)j * T.M] = alpha * s; continue; }
+        double* cp = C + gi + (i64)j * T.ldc;
+        *cp = alpha * s + (beta != 0.0 ? beta * (*cp) : 0.0);
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_gemm_narrow(const GemmTile* __restrict__ tiles,
+                                                     const GemmTask* __restrict__ tasks, BufTab bt) {
+    __shared__ double red[4][TILE][NARROW_N + 1];
+    if (gated_off(bt)) return;
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmTask T = tasks[tl.task];
+    const double* A = buf_sel(bt, T.bufA) + T.offA;
+    const double* B = buf_sel(bt, T.bufB) + T.offB;
+    double* C = buf_sel(bt, T.bufC) + T.offC;
+    const int row0 = tl.tm * TILE;
+    int kbase = 0, kend = T.K;
+    double* Pp = nullptr;
+    if (T.ksplit > 1) {
+        kbase = tl.ks * T.kchunk;
+        kend = min(T.K, kbase + T.kchunk);
+        Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
+    }
+    switch (T.triA * 4 + T.opA * 2 + T.opB) {
+    case 0: gemm_narrow<0, 0, 0>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 1: gemm_narrow<0, 1, 0>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 2: gemm_narrow<1, 0, 0>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 3: gemm_narrow<1, 1, 0>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 4: gemm_narrow<0, 0, 1>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 6: gemm_narrow<1, 0, 1>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 8: gemm_narrow<0, 0, 2>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    case 10: gemm_narrow<1, 0, 2>(T, A, B, C, row0, kbase, kend, Pp, red); break;
+    default: break;  // not a narrow task (the planner sends none)
+    }
+}
+
 // C = beta*C + sum_ks partial_ks over one 64 x 64 tile of a split-K task, in
 // slice order (deterministic), honouring the task's triangular output mode.
 __global__ __launch_bounds__(TPB) void k_splitk_reduce(

@@ -66,6 +66,11 @@ __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTa
 __global__ void k_gemm_small(const int* ids, int count, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles_lat(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_splitk_reduce(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
+// tasks with N <= NARROW_N (opA, opB in {0, 1}, no triangular output or B;
+// a triangular A with opB 0): a 64 x N tile per workgroup, K split over its
+// four waves
+#define NARROW_N 16
+__global__ void k_gemm_narrow(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles32(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 __global__ void k_gemm_tiles_bf3(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
 // descriptor-addressed dense operations (completion, Hessian)

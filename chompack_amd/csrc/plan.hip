@@ -89,6 +89,13 @@ double task_flops(const GemmTask& t) {
 
 // Adds the tiles of one task.  Tasks with few output tiles and a long K are
 // split along K (deterministic split-K) so the launch fills the GPU.
+// Tasks k_gemm_narrow takes: at most NARROW_N columns, op(A) plain or
+// triangular, op(B) plain (op(B) = B with a triangular A).
+bool narrow_task(const GemmTask& t) {
+    return t.N <= NARROW_N && t.opA <= 1 && t.opB <= 1 && t.tri == 0 && t.pad == 0 &&
+           (t.triA == 0 || t.opB == 0);
+}
+
 void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool skip_upper) {
     if (t.M <= 0 || t.N <= 0) return;
     int id = (int)P->pg->tasks.size();
@@ -100,6 +107,9 @@ void add_gemm_tiles(chp_plan* P, std::vector<GemmTile>& tiles, GemmTask t, bool 
     int ks = 1;
     if (P->splitk_max <= 1) {
         ks = 1;
+    } else if (narrow_task(t) && !skip_upper) {
+        // narrow tiles are cheap: slices of about 256 k (64 per wave)
+        ks = std::max(1, std::min(t.K / 256, std::min(16, P->splitk_max)));
     } else if (ntiles < 128 && t.K >= 512) {
         ks = std::max(1, std::min(t.K / 256, (512 + ntiles - 1) / ntiles));
         ks = std::min(ks, std::min(16, P->splitk_max));
@@ -186,10 +196,14 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     P->n_gemm_tiles += (i64)tiles.size();
     // launches too small to fill the device take the latency-oriented tile
     // kernel (512 threads), unless a tile is a split-K slice
-    bool split = false;
-    for (const GemmTile& g : tiles) split = split || P->pg->tasks[g.task].ksplit > 1;
-    const int block = (!split && (int)tiles.size() <= LAT_TILES) ? 512 : 256;
-    Launch L{K_GEMM, (int)tiles.size(), block, 0, P->pg->lists.add(tiles), schur};
+    bool split = false, narrow = !schur;
+    for (const GemmTile& g : tiles) {
+        split = split || P->pg->tasks[g.task].ksplit > 1;
+        narrow = narrow && narrow_task(P->pg->tasks[g.task]);
+    }
+    const int block = (!narrow && !split && (int)tiles.size() <= LAT_TILES) ? 512 : 256;
+    Launch L{narrow ? K_GEMM_NARROW : K_GEMM, (int)tiles.size(), block, 0, P->pg->lists.add(tiles),
+             schur};
     double fl = 0.0;
     int last = -1;
     for (const GemmTile& g : tiles)
@@ -993,6 +1007,9 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_SPLITK_REDUCE:
             CHP_LAUNCH(k_splitk_reduce, 0, (const GemmTile*)list, pg.d_tasks, bt);
+            break;
+        case K_GEMM_NARROW:
+            CHP_LAUNCH(k_gemm_narrow, 0, (const GemmTile*)list, pg.d_tasks, bt);
             break;
         case K_GEMM_SMALL:
             CHP_LAUNCH(k_gemm_small, 0, (const int*)list, L.iarg, pg.d_tasks, bt);

@@ -333,6 +333,9 @@ int run_solve(chp_plan* P, const SolvePlan* S, const std::vector<Launch>& seq,
         case K_SPLITK_REDUCE:
             CHP_SLAUNCH(k_splitk_reduce, (const GemmTile*)list, S->pg.d_tasks, bt);
             break;
+        case K_GEMM_NARROW:
+            CHP_SLAUNCH(k_gemm_narrow, (const GemmTile*)list, S->pg.d_tasks, bt);
+            break;
         case K_GEMM_SMALL:
             CHP_SLAUNCH(k_gemm_small, (const int*)list, L.iarg, S->pg.d_tasks, bt);
             break;

@@ -221,14 +221,30 @@ __global__ __launch_bounds__(TPB) void k_l11_check(
     const STile t = tiles[blockIdx.x];
     const FrontInfo f = fi[t.k];
     const int nn = f.nn, nj = nn + f.na;
-    const int c1 = min(t.r0 + 16, nn);
+    const int c0 = t.r0, ncol = min(c0 + 16, nn) - c0;
+    const int total = ncol * nn;  // the tile's columns, rows 0 .. nn - 1 (those above the diagonal skipped)
+    const double* src = blkval + f.blkptr + (i64)c0 * nj;
+    double* dst = copy + t.tofs + (i64)c0 * nn;
     bool diff = false;
-    for (int c = t.r0; c < c1; ++c) {
-        const double* src = blkval + f.blkptr + (i64)c * nj;
-        double* dst = copy + t.tofs + (i64)c * nn;
-        for (int i = c + threadIdx.x; i < nn; i += TPB) {
-            const double v = src[i];
-            if (__double_as_longlong(v) != __double_as_longlong(dst[i])) { dst[i] = v; diff = true; }
+    for (int e0 = 0; e0 < total; e0 += 8 * TPB) {  // eight element pairs in flight per thread
+        double v[8], o[8];
+        int at[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * TPB + threadIdx.x;
+            const int c = e / nn, i = e - c * nn;
+            const bool ok = e < total && i >= c0 + c;
+            at[u] = ok ? e : -1;
+            const int ee = ok ? e : 0;
+            const int cc = ok ? c : 0;
+            v[u] = src[(ee - cc * nn) + (i64)cc * nj];
+            o[u] = dst[ee];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (at[u] < 0 || __double_as_longlong(v[u]) == __double_as_longlong(o[u])) continue;
+            dst[at[u]] = v[u];
+            diff = true;
         }
     }
     if (diff) *changed = 1;

@@ -830,6 +830,32 @@ def test_trsm_offset_and_leading_dimension():
     assert np.array_equal(got[mask], store[mask])
 
 
+@pytest.mark.parametrize("nrhs", [1, 7, 16, 17])
+def test_trsm_rhs_counts_large_fronts(nrhs):
+    """trsm N and T with 1-17 right-hand sides on a grid whose top fronts are
+    large (the products with <= 16 columns run as narrow tiles: masked
+    columns below 16, edge row tiles, K not a multiple of 16, split K)
+    against the oracle's supernodal solve."""
+    import torch
+    print("case trsm nrhs", nrhs, flush=True)
+    A = problems.grid3d(16, seed=4)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    ref = X.blkval.cpu().numpy().copy()
+    assert on.cholesky(symb, ref) == 0
+    cp.cholesky(X)
+    rng = np.random.default_rng(nrhs)
+    Bh = rng.standard_normal((symb.n, nrhs))
+    for tr in "NT":
+        B = torch.as_tensor(np.asfortranarray(Bh)).to("cuda:0").t().contiguous().t()
+        Bref = np.asfortranarray(Bh.copy())
+        on.trsm(symb, ref, Bref, trans=tr)
+        cp.trsm(X, B, trans=tr)
+        e = _rel(B.cpu().numpy(), Bref)
+        print("trsm %s rel err %.3e" % (tr, e), flush=True)
+        assert e <= TOL
+
+
 def test_trsm_zero_pivot_nonfinite_not_error():
     """A zero pivot in L11 of the root front (a large front whose inverse the
     solve keeps): trsm gives non-finite entries and no error, as the
