@@ -8,6 +8,8 @@
 
 #define TPB 256
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 // ---------------------------------------------------------------------------
 // Deterministic extend-add for large fronts.  One workgroup owns the target
 // tile rows [r0, r0+256) x columns [c0, c0+16) of front k (lower part).  The
@@ -295,11 +297,16 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(
     }
 }
 
-// One inner step of the large-front Cholesky: tasks[i] = (front, c0, r0).
+// One inner step of the large-front Cholesky: tasks[i] = (front, c0, r0, kp).
 // Every workgroup factors the 64 x 64 diagonal block at (c0, c0) from the
 // front's current values (redundantly: the chain of dependent pivots is the
 // same length for all of them) together with the 64 rows r0 .. r0 + 63 of the
-// panel below it, and writes those rows of the factor.  The workgroup with
+// panel below it, and writes those rows of the factor.  Columns kp .. c0 - 1
+// (the earlier inner panels of the outer panel, already factored) are applied
+// first, left-looking: D -= Lc Lc^T and R -= Lr Lc^T with Lc, Lr their rows
+// c0 .. c0 + 63 and r0 .. r0 + 63, on v_mfma_f64_16x16x4f64 from LDS, 64
+// columns at a time (each workgroup forms the diagonal block's update
+// itself, as it factors the block itself).  The workgroup with
 // r0 < 0 reports a failed pivot and stages the factored diagonal block in the
 // front's inverse area (dinv + dinvptr[k] + (c0 / 64) * 4096, ld 64): writing
 // it into blkval here would race with workgroups of the same launch that
@@ -320,8 +327,9 @@ __global__ __launch_bounds__(64 * PANEL_WAVES) void k_panel_factor(
     constexpr int W = PANEL_WAVES, C = 4, DQW = NB / W;
     __shared__ double P[2 * NB][C + 1];
     __shared__ __attribute__((aligned(16))) double Q[NB][C];
+    __shared__ double S[2][NB][NB + 2];  // left-looking update: [0] Lc, [1] Lr as [k][row]; then the updates as [col][row]
     const int4 t = tasks[blockIdx.x];
-    const int k = t.x, c0 = t.y, r0 = t.z;
+    const int k = t.x, c0 = t.y, r0 = t.z, kp = t.w;
     const FrontInfo f = fi[k];
     const int nn = f.nn, nj = nn + f.na;
     const int nb = min(NB, nn - c0);
@@ -335,6 +343,84 @@ __global__ __launch_bounds__(64 * PANEL_WAVES) void k_panel_factor(
         const int j = w + W * q;
         a[q] = (l < nb && j < nb && l >= j) ? dg[l + (i64)j * nj] : (l == j ? 1.0 : 0.0);
         b[q] = (prow && j < nb) ? blk[(r0 + l) + (i64)(c0 + j) * nj] : 0.0;
+    }
+    if (kp < c0) {
+        // output blocks (I, J) of 16 x 16: slots 0 .. 9 the lower blocks of
+        // D (J <= I), 10 .. 25 the blocks of R; wave w takes slots w + 8 s
+        const int lane = l;
+        const int nslot = r0 >= 0 ? 26 : 10;
+        dbl4 acc[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) acc[s4] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        double vc[DQW], vr[DQW];
+        auto load_chunk = [&](int kc) {
+#pragma unroll
+            for (int m = 0; m < DQW; ++m) {  // thread (w, l): row l, chunk column w + 8 m
+                const i64 col = (i64)(kc + w + W * m) * nj;
+                vc[m] = blk[(l < nb ? c0 + l : c0) + col];
+                vr[m] = blk[(prow ? r0 + l : c0) + col];
+            }
+        };
+        load_chunk(kp);
+        for (int kc = kp; kc < c0; kc += NB) {
+            if (kc > kp) __syncthreads();  // the previous chunk's reads are done
+#pragma unroll
+            for (int m = 0; m < DQW; ++m) {
+                S[0][w + W * m][l] = l < nb ? vc[m] : 0.0;
+                S[1][w + W * m][l] = prow ? vr[m] : 0.0;
+            }
+            __syncthreads();
+            if (kc + NB < c0) load_chunk(kc + NB);  // in flight during this chunk's products
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int slot = w + W * s4;
+                if (slot >= nslot) break;
+                int I, J, src;
+                if (slot < 10) {
+                    I = slot >= 6 ? 3 : slot >= 3 ? 2 : slot >= 1 ? 1 : 0;
+                    J = slot - I * (I + 1) / 2;
+                    src = 0;
+                } else {
+                    I = (slot - 10) >> 2;
+                    J = (slot - 10) & 3;
+                    src = 1;
+                }
+#pragma unroll 4
+                for (int k4 = 0; k4 < NB; k4 += 4) {
+                    const int kr = k4 + (lane >> 4);
+                    const double av = S[src][kr][I * 16 + (lane & 15)];
+                    const double bv = S[0][kr][J * 16 + (lane & 15)];
+                    acc[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[s4], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // the last chunk's reads are done: S takes the updates
+        // accumulator lane l, element v: row 16 I + l / 16 + 4 v, column 16 J + l % 16
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int slot = w + W * s4;
+            if (slot >= nslot) break;
+            int I, J, src;
+            if (slot < 10) {
+                I = slot >= 6 ? 3 : slot >= 3 ? 2 : slot >= 1 ? 1 : 0;
+                J = slot - I * (I + 1) / 2;
+                src = 0;
+            } else {
+                I = (slot - 10) >> 2;
+                J = (slot - 10) & 3;
+                src = 1;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                S[src][J * 16 + (lane & 15)][I * 16 + (lane >> 4) + 4 * v] = acc[s4][v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DQW; ++q) {
+            const int j = w + W * q;
+            if (l < nb && j < nb && l >= j) a[q] -= S[0][j][l];
+            if (prow && j < nb) b[q] -= S[1][j][l];
+        }
     }
     int bad = -1;
 #pragma unroll
@@ -584,7 +670,6 @@ __global__ __launch_bounds__(TPB) void k_matop(const MatOp* __restrict__ ops, Bu
 // chunks of 32 staged through LDS, the next chunk prefetched into registers
 // while the current one is multiplied.  Accumulator layout (16 x 16 f64): lane l,
 // element v holds row (l / 16) + 4 v, column l % 16.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 #define BK 32
 
