@@ -365,6 +365,17 @@ def differentiation_section(dev, with_cpu):
             ref.completion(bsymb, bh)
             t_cc = (time.perf_counter() - t0) * 1e3
             gc = Xb.blkval.cpu().numpy()
+            # cholesky + projected inverse of the band-plus-arrow chain
+            ch = Xb0.blkval.cpu().numpy().copy()
+            t0 = time.perf_counter()
+            ref.cholesky(bsymb, ch)
+            ref.projected_inverse(bsymb, ch)
+            t_cpc = (time.perf_counter() - t0) * 1e3
+            gpc = Xc.blkval.cpu().numpy()
+            out.update({"cpu_band_arrow_cholesky_projinv_ms": round(t_cpc, 1),
+                        "band_arrow_cholesky_projinv_speedup_vs_cpu": round(t_cpc / t_cp, 3),
+                        "band_arrow_cholesky_projinv_rel_err_vs_reference":
+                            float(np.abs(gpc - ch).max() / np.abs(ch).max())})
             out.update({"cpu_reference_threads": threads,
                         "cpu_hessian_32_directions_ms": round(t_hc, 1),
                         "hessian_speedup_vs_cpu": round(t_hc / t_hess, 2),
