@@ -126,14 +126,35 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
 // the mirrored upper-triangle writes are coalesced too.
 __global__ __launch_bounds__(TPB) void k_extract(
     const int4* __restrict__ tiles, const FrontInfo* __restrict__ fi,
-    const double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx, i64 bstride, i64 ustride) {
+    double* __restrict__ blkval, double* __restrict__ upd,
+    const int* __restrict__ relidx, i64 bstride, i64 ustride,
+    const double* __restrict__ scratch, const i64* __restrict__ tptr) {
     __shared__ double S[256][17];
     // grid y: the instance (a Hessian direction), as in k_extadd_tiles
     blkval += (i64)blockIdx.y * bstride;
     upd += (i64)blockIdx.y * ustride;
     const int4 t = tiles[blockIdx.x];
     const int k = t.x, j0 = t.y, ib = t.z;
+    if (t.w == 2) {
+        // F21 = -T for columns j0 .. j0 + 15 of front k (T at scratch +
+        // tptr[k], ld na): the projected inverse's last step for the front,
+        // in the launch that extracts its children's V from T directly
+        const FrontInfo f = fi[k];
+        const int nn = f.nn, na = f.na, nj = nn + na;
+        double* blk = blkval + f.blkptr + nn;
+        const double* T = scratch + tptr[k];
+        const int nc = min(j0 + 16, nn) - j0;
+        for (int c = 0; c < nc; c += 4)
+            for (int i = threadIdx.x; i < na; i += TPB) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = (c + u < nc) ? T[i + (i64)(j0 + c + u) * na] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (c + u < nc) blk[i + (i64)(j0 + c + u) * nj] = -v[u];
+            }
+        return;
+    }
     const FrontInfo f = fi[k];
     const int na = f.na;
     const FrontInfo p = fi[f.parent];
@@ -141,6 +162,8 @@ __global__ __launch_bounds__(TPB) void k_extract(
     const int* r = relidx + f.relptr;
     const double* pblk = blkval + p.blkptr;
     const double* pV = upd + p.updptr;
+    // t.w == 1: the parent's F21 is still -T in scratch (ld nap)
+    const double* pT = t.w == 1 ? scratch + tptr[f.parent] : nullptr;
     double* V = upd + f.updptr;
     const int j1 = min(j0 + 16, na);
     const int tid = threadIdx.x;
@@ -156,8 +179,9 @@ __global__ __launch_bounds__(TPB) void k_extract(
             v[jj] = 0.0;
             if (j < j1 && i < i1 && i >= j) {
                 const int rj = r[j];
-                v[jj] = (rj < nnp) ? pblk[ri + (i64)rj * njp]
-                                   : pV[(ri - nnp) + (i64)(rj - nnp) * nap];
+                v[jj] = (rj >= nnp) ? pV[(ri - nnp) + (i64)(rj - nnp) * nap]
+                        : (pT && ri >= nnp) ? -pT[(ri - nnp) + (i64)rj * nap]
+                                            : pblk[ri + (i64)rj * njp];
             }
         }
 #pragma unroll

@@ -17,8 +17,9 @@ __global__ void k_status_init(unsigned long long* status);
 __global__ void k_chol_small(const int* fronts, int count, const FrontInfo* fi, double* blkval,
                              double* upd, const int* relidx, const int* chidx,
                              unsigned long long* status);
-__global__ void k_extract(const int4* tiles, const FrontInfo* fi, const double* blkval,
-                          double* upd, const int* relidx, i64 bstride, i64 ustride);
+__global__ void k_extract(const int4* tiles, const FrontInfo* fi, double* blkval,
+                          double* upd, const int* relidx, i64 bstride, i64 ustride,
+                          const double* scratch, const i64* tptr);
 __global__ void k_projinv_small(const int* fronts, int count, const FrontInfo* fi, double* blkval,
                                 const double* upd, unsigned long long* status);
 // chains of small fronts (nj <= CHAIN_NJ), one workgroup walking a

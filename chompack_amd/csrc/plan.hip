@@ -783,14 +783,20 @@ void build_projinv_level(chp_plan* P, const std::vector<int>& fr, i64 wbase, i64
     }
     push_gemm(P, P->projinv, g3);
     push_gemm(P, P->projinv, g4);
-    std::vector<int2> nt;
+    if (fork) P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 1});
+    // one launch for the children's V and the large fronts' F21 = -T: the
+    // children of large fronts read F21 as -T from scratch (tile w = 1), so
+    // the negated copy (tiles w = 2) does not have to precede them
+    std::vector<int4> xl = xt;
+    std::vector<char> is_large(P->fi.size(), 0);
+    for (int k : large) is_large[k] = 1;
+    for (int4& x : xl)
+        if (is_large[P->fi[x.x].parent] && P->fi[P->fi[x.x].parent].na > 0) x.w = 1;
     for (int k : large)
         if (P->fi[k].na > 0)
-            for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) nt.push_back(make_int2(k, c0));
-    if (!nt.empty())
-        P->projinv.push_back({K_NEG_COPY, (int)nt.size(), 256, 0, P->pg->lists.add(nt), 0});
-    if (fork) P->projinv.push_back({K_JOIN, 0, 0, 0, 0, 1});
-    extract_children();
+            for (int c0 = 0; c0 < P->fi[k].nn; c0 += 16) xl.push_back(make_int4(k, c0, 0, 2));
+    if (!xl.empty())
+        P->projinv.push_back({K_EXTRACT, (int)xl.size(), 256, 0, P->pg->lists.add(xl), 0});
 }
 
 // llt launches for one level (llt.c:26-58).  Large fronts: keep the diagonal
@@ -1042,7 +1048,7 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             break;
         case K_EXTRACT:
             CHP_LAUNCH(k_extract, 0, (const int4*)list, fi, blkval,
-                       bt.p[1], P->d_relidx, L.bstride, L.ustride);
+                       bt.p[1], P->d_relidx, L.bstride, L.ustride, P->d_scratch, P->d_tptr);
             break;
         case K_PROJINV_SMALL:
             CHP_LAUNCH(k_projinv_small, L.lds, (const int*)list, L.iarg, fi,
