@@ -213,6 +213,13 @@ struct ChainFront {
     int nbr, nbr_na, keep;
 };
 
+// One piece of the solve's L11 comparison: columns c0 .. c0 + 15 and rows
+// r0 .. r1 - 1 of front k's L11, whose copy starts at tofs (ld nn).
+struct CheckTile {
+    i64 tofs;
+    int k, c0, r0, r1;
+};
+
 // Row tile of a front in the solve's scatter: rows r0 .. r0 + 63 of front k,
 // whose solved block sits at offset tofs of the solve's block buffer.
 struct STile {

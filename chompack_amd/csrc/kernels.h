@@ -104,7 +104,7 @@ __global__ void k_solve_bwd(const int2* items, const FrontInfo* fi, const double
                             double* B, int ldb, int nrhs, double alpha, double* work,
                             const i64* wptr);
 __global__ void k_flag_clear(int* flag);
-__global__ void k_l11_check(const STile* tiles, const FrontInfo* fi, const double* blkval,
+__global__ void k_l11_check(const CheckTile* tiles, const FrontInfo* fi, const double* blkval,
                             double* copy, int* changed);
 
 __global__ void k_sgather_fwd(const int2* tiles, const FrontInfo* fi, const i64* snptr,

@@ -83,8 +83,8 @@ struct SolvePlan {
     std::vector<Launch> mfwd, mbwd;     // products with L ('N') and L^T ('T')
     bool mbuilt = false;
     i64 t2base = 0;                     // offset of T2 = B21^T x_A in d_tmp
-    std::vector<STile> chk;             // (front, 16-column tile, X offset) of the L11 check
-    STile* d_chk = nullptr;
+    std::vector<CheckTile> chk;         // pieces of the L11 check
+    CheckTile* d_chk = nullptr;
     int* d_changed = nullptr;           // set by k_l11_check when L11 differs from its copy
     double* d_l11 = nullptr;            // copy of the L11 the inverses were formed from
     bool inv_valid = false;             // d_inv holds the inverses of d_l11

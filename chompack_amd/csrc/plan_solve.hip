@@ -81,7 +81,10 @@ SolvePlan* build_solve(chp_plan* P, int nrhs) {
         Xs.push_back({6, xo[k], f.nn, f.nn});
         Ws.push_back(bw.take((i64)f.nn * f.nn));
         keys.push_back((int)k);
-        for (int c0 = 0; c0 < f.nn; c0 += 16) S->chk.push_back({(int)k, c0, xo[k]});
+        // 16 columns x at most 1024 rows per workgroup
+        for (int c0 = 0; c0 < f.nn; c0 += 16)
+            for (int r0 = c0; r0 < f.nn; r0 += 1024)
+                S->chk.push_back({xo[k], (int)k, c0, r0, std::min(r0 + 1024, f.nn)});
     }
     S->inv_len = bx.cur;
     S->w_len = bw.cur;

@@ -212,19 +212,20 @@ __global__ void k_flag_clear(int* flag) {
     if (threadIdx.x == 0) *flag = 0;
 }
 
-// Compares the lower triangle of L11 of the listed fronts (16-column tiles,
-// t.r0 = first column) with the copy the solve's inverses were formed from
-// (at t.tofs, ld nn); refreshes the copy and sets *changed where they differ.
+// Compares the lower triangle of L11 of the listed pieces (columns c0 ..
+// c0 + 15, rows r0 .. r1 - 1) with the copy the solve's inverses were formed
+// from (at tofs, ld nn); refreshes the copy and sets *changed where they
+// differ.
 __global__ __launch_bounds__(TPB) void k_l11_check(
-    const STile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
+    const CheckTile* __restrict__ tiles, const FrontInfo* __restrict__ fi,
     const double* __restrict__ blkval, double* __restrict__ copy, int* __restrict__ changed) {
-    const STile t = tiles[blockIdx.x];
+    const CheckTile t = tiles[blockIdx.x];
     const FrontInfo f = fi[t.k];
     const int nn = f.nn, nj = nn + f.na;
-    const int c0 = t.r0, ncol = min(c0 + 16, nn) - c0;
-    const int total = ncol * nn;  // the tile's columns, rows 0 .. nn - 1 (those above the diagonal skipped)
-    const double* src = blkval + f.blkptr + (i64)c0 * nj;
-    double* dst = copy + t.tofs + (i64)c0 * nn;
+    const int c0 = t.c0, ncol = min(c0 + 16, nn) - c0, r0 = t.r0, nr = t.r1 - t.r0;
+    const int total = ncol * nr;  // rows r0 .. r1 - 1 of each column (those above the diagonal skipped)
+    const double* src = blkval + f.blkptr + (i64)c0 * nj + r0;
+    double* dst = copy + t.tofs + (i64)c0 * nn + r0;
     bool diff = false;
     for (int e0 = 0; e0 < total; e0 += 8 * TPB) {  // eight element pairs in flight per thread
         double v[8], o[8];
@@ -232,13 +233,12 @@ __global__ __launch_bounds__(TPB) void k_l11_check(
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * TPB + threadIdx.x;
-            const int c = e / nn, i = e - c * nn;
-            const bool ok = e < total && i >= c0 + c;
-            at[u] = ok ? e : -1;
-            const int ee = ok ? e : 0;
-            const int cc = ok ? c : 0;
-            v[u] = src[(ee - cc * nn) + (i64)cc * nj];
-            o[u] = dst[ee];
+            const int c = e / nr, i = e - c * nr;
+            const bool ok = e < total && r0 + i >= c0 + c;
+            const int cc = ok ? c : 0, ii = ok ? i : 0;
+            at[u] = ok ? ii + cc * nn : -1;
+            v[u] = src[ii + (i64)cc * nj];
+            o[u] = dst[ii + (i64)cc * nn];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
