@@ -61,8 +61,10 @@ def memory_bytes(symb, small_max=64):
     """Algorithmic HBM bytes of the memory-bound front kernels of one step:
     extend-add into large fronts (12 N (N + 1) B per child of N rows: read the
     child's update triangle, read and write the target entries) and the
-    extraction of update matrices for large-front children (read the parent
-    triangle, write both triangles: 8 N (N + 1) / 2 + 8 N^2 B)."""
+    extraction launches: update matrices of the children (read the parent
+    triangle, write both triangles: 8 N (N + 1) / 2 + 8 N^2 B) and, in the
+    same launches, the large fronts' F21 = -T copies of the projected
+    inverse (read T, write F21: 16 na nn B)."""
     import numpy as np
     nn = np.diff(symb.snptr)
     na = np.diff(symb.relptr)
@@ -77,6 +79,7 @@ def memory_bytes(symb, small_max=64):
     # k_extract runs for every front with a separator (small and large parents)
     N = na.astype(float)
     xtr = float(np.sum(4.0 * N * (N + 1) + 8.0 * N * N))
+    xtr += float(np.sum(16.0 * na[large].astype(float) * nn[large].astype(float)))
     return ext, xtr
 
 
