@@ -830,6 +830,32 @@ def test_trsm_offset_and_leading_dimension():
     assert np.array_equal(got[mask], store[mask])
 
 
+@pytest.mark.parametrize("panel_width", [256, 128])
+def test_cholesky_projinv_grid20_outer_panels(panel_width):
+    """Cholesky and projected inverse on the 20^3 grid (root front of 608
+    columns): every outer panel of the large fronts has several inner
+    panels, so the left-looking update inside the panel launches runs with
+    64 to panel_width - 64 columns; parity with the oracle's supernodal
+    sweeps."""
+    print("case grid20 outer panels", panel_width, flush=True)
+    A = problems.grid3d(20, seed=2)
+    symb = cp.symbolic(A, p=cp.amd.order)
+    X = cp.cspmatrix(symb, device="cuda:0") + A
+    dev = X.blkval.device
+    symb._plans[str(dev)] = numeric.Plan(symb, dev, panel_width=panel_width)
+    ref = X.blkval.cpu().numpy().copy()
+    assert on.cholesky(symb, ref) == 0
+    cp.cholesky(X)
+    e = _rel(X.blkval.cpu().numpy(), ref)
+    print("cholesky rel err %.3e" % e, flush=True)
+    assert e <= TOL
+    on.projected_inverse(symb, ref)
+    cp.projected_inverse(X)
+    e = _rel(X.blkval.cpu().numpy(), ref)
+    print("projected_inverse rel err %.3e" % e, flush=True)
+    assert e <= TOL
+
+
 @pytest.mark.parametrize("nrhs", [1, 7, 16, 17])
 def test_trsm_rhs_counts_large_fronts(nrhs):
     """trsm N and T with 1-17 right-hand sides on a grid whose top fronts are
