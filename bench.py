@@ -24,7 +24,7 @@ sys.path.insert(0, HERE)
 
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 KINDS = ["chol_small", "extadd", "potrf_diag", "panel_factor", "gemm", "extract",
-         "projinv_small", "trtri_diag", "neg_copy", "llt_small", "solve_fwd", "solve_bwd",
+         "projinv_small", "trtri_diag", "llt_small", "solve_fwd", "solve_bwd",
          "copy_top", "splitk_reduce", "diag", "matop", "update_factor", "sgather_fwd",
          "sgather_bwd", "sscatter", "trmm_fwd", "trmm_bwd", "trmm_finish", "completion_chain",
          "projinv_chain", "chol_dag", "fill_int", "gemm_small", "chol_chain", "llt_chain", "uf_setup",

@@ -1480,32 +1480,6 @@ __global__ __launch_bounds__(TPB) void k_gemm_tiles_bf3(
     }
 }
 
-// blk[nn + i, j] = -S[i + j*na] for the listed (front, 16-column) tiles:
-// F21 = -V_k B21 written back after F11 has consumed B21
-// (projected_inverse.c:51-56).
-__global__ __launch_bounds__(TPB) void k_neg_copy(
-    const int2* __restrict__ tiles, const FrontInfo* __restrict__ fi,
-    double* __restrict__ blkval, const double* __restrict__ scratch,
-    const i64* __restrict__ sptr) {
-    const int2 t = tiles[blockIdx.x];
-    const int k = t.x, c0 = t.y;
-    const FrontInfo f = fi[k];
-    const int nn = f.nn, na = f.na, nj = nn + na;
-    double* blk = blkval + f.blkptr + nn;
-    const double* S = scratch + sptr[k];
-    const int nc = min(c0 + 16, nn) - c0;
-    // 16 columns x na rows: rows over the threads, four columns in flight
-    for (int c = 0; c < nc; c += 4)
-        for (int i = threadIdx.x; i < na; i += TPB) {
-            double v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                v[u] = (c + u < nc) ? S[i + (i64)(c0 + c + u) * na] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (c + u < nc) blk[i + (i64)(c0 + c + u) * nj] = -v[u];
-        }
-}
 
 // scratch[sptr[k] + i + j*nj] = blk[i + j*nj] for the rows i < nn of the
 // listed (front, 16-column) tiles: the diagonal block as stored, kept for

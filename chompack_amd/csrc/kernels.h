@@ -91,8 +91,6 @@ __global__ void k_uf_step(const UFBlock* items, BufTab bt, const int* relidx);
 // N <= 64 and na <= 64; 64-thread workgroups
 __global__ void k_update_factor_small(const UFTask* tasks, BufTab bt, const int* relidx,
                                       unsigned long long* status);
-__global__ void k_neg_copy(const int2* tiles, const FrontInfo* fi, double* blkval,
-                           const double* scratch, const i64* sptr);
 
 // triangular solve with the chordal factor
 __global__ void k_solve_fwd(const int2* items, const FrontInfo* fi, const double* blkval,

@@ -12,14 +12,14 @@ namespace chpi {
 
 enum Kid {
     K_CHOL_SMALL, K_EXTADD, K_POTRF_DIAG, K_PANEL_FACTOR, K_GEMM, K_EXTRACT,
-    K_PROJINV_SMALL, K_TRTRI_DIAG, K_NEG_COPY, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
+    K_PROJINV_SMALL, K_TRTRI_DIAG, K_LLT_SMALL, K_SOLVE_FWD, K_SOLVE_BWD,
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
     K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_TRMM_FWD, K_TRMM_BWD, K_TRMM_FIN, K_COMP_CHAIN,
     K_PROJINV_CHAIN, K_CHOL_DAG, K_FILL_INT, K_GEMM_SMALL, K_CHOL_CHAIN, K_LLT_CHAIN,
     K_UF_SETUP, K_UF_STEP, K_GEMM_NARROW,
     K_FORK, K_JOIN
 };
-const int NKIDS = 33;  // kernel kinds (K_FORK / K_JOIN are stream markers)
+const int NKIDS = 32;  // kernel kinds (K_FORK / K_JOIN are stream markers)
 
 struct Launch {
     Kid kid;

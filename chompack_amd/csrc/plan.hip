@@ -1076,10 +1076,6 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             CHP_LAUNCH(k_trtri_diag, 0, (const int2*)list, fi, blkval,
                        P->d_dinv, P->d_dinvptr, P->d_scratch, P->d_xptr, P->d_status);
             break;
-        case K_NEG_COPY:
-            CHP_LAUNCH(k_neg_copy, 0, (const int2*)list, fi, blkval,
-                       P->d_scratch, P->d_tptr);
-            break;
         case K_LLT_SMALL:
             CHP_LAUNCH(k_llt_small, L.lds, (const int*)list, L.iarg, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx);
@@ -1576,9 +1572,6 @@ i64 chp_profile_launches(chp_plan* P, double* out, i64 max) {
     return rows;
 }
 
-// ms[NKIDS], cnt[NKIDS], flops[NKIDS] accumulated per kernel kind.  Kinds:
-// 0 chol_small 1 extadd 2 potrf_diag 3 panel_mul 4 gemm 5 extract
-// 6 projinv_small 7 trtri_diag 8 neg_copy 9 llt_small 10 solve_fwd 11 solve_bwd
 int64_t chp_dag_trace(chp_plan* P, int on) {
     if (!P) { chp_set_error("null argument"); return -1; }
     if (on && !P->d_dagtrace && P->dag_tasks > 0) {
