@@ -29,7 +29,7 @@ from .symbolic import cspmatrix
 __all__ = ["tune"]
 
 SMALL = [(32, 65536), (64, 65536), (96, 163840), (128, 163840)]
-PANEL = [128, 256, 512]
+PANEL = [128, 192, 256, 512]
 SPLITK = [1, 16]
 DAG = [1, 2]
 
