@@ -740,15 +740,17 @@ __device__ inline void gemm_tile(const GemmTask& T, const double* __restrict__ A
 }
 
 // The same tile for launches too small to fill the device (the host picks it
-// for at most LAT_TILES tiles and no split-K): 8 waves, two groups of four
+// for at most LAT_TILES tiles, split-K slices included): 8 waves, two groups of four
 // that take alternate K chunks with a staging area each (the K loop runs in
 // half the chunk rounds), partial sums added through LDS; C is read at the
 // start, while the first chunks load, instead of after the K loop.  Groups:
-// g = tid / 256; inside a group the layout of gemm_tile.
+// g = tid / 256; inside a group the layout of gemm_tile.  A split-K slice
+// (Pp not null) writes alpha times its partial sum to Pp (ld M), as
+// gemm_tile does.
 template <int OPA, int OPB, bool CHK>
 __device__ inline void gemm_tile_lat(const GemmTask& T, const double* __restrict__ A,
                                      const double* __restrict__ B, double* __restrict__ C,
-                                     int row0, int col0, int kbase, int kend,
+                                     int row0, int col0, int kbase, int kend, double* Pp,
                                      double (*As)[BK][TILE + 2], double (*Bs)[BK][TILE + 2]) {
     const int g = threadIdx.x >> 8, tid = threadIdx.x & 255, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
@@ -797,7 +799,7 @@ __device__ inline void gemm_tile_lat(const GemmTask& T, const double* __restrict
             bs[bk + (BK_ ? 0 : 4 * q)][bj + (BK_ ? 8 * q : 0)] = rb[q];
         }
     };
-    const double alpha = T.alpha, beta = T.beta;
+    const double alpha = T.alpha, beta = Pp ? 0.0 : T.beta;
     // C of this thread's accumulator entries (group 0), loaded up front
     double cpre[2][2][4];
 #pragma unroll
@@ -865,6 +867,7 @@ __device__ inline void gemm_tile_lat(const GemmTask& T, const double* __restrict
                 const int i = row0 + il, j = col0 + jl;
                 if (CHK && (i >= T.M || j >= T.N)) continue;
                 const double sum = acc[a][b][v] + red[il][jl];
+                if (Pp) { Pp[i + (i64)j * T.M] = alpha * sum; continue; }
                 double* c = C + i + (i64)j * T.ldc;
                 if (T.tri == 0 || i >= j) *c = alpha * sum + (beta != 0.0 ? beta * cpre[a][b][v] : 0.0);
                 else if (T.tri == 2) *c = 0.0;
@@ -885,6 +888,13 @@ __global__ __launch_bounds__(2 * TPB) void k_gemm_tiles_lat(
     double* C = bt.p[T.bufC] + T.offC;
     const int row0 = tl.tm * TILE, col0 = tl.tn * TILE;
     int kbase = 0, kend = T.K;
+    double* Pp = nullptr;
+    if (T.ksplit > 1) {
+        kbase = tl.ks * T.kchunk;
+        kend = min(T.K, kbase + T.kchunk);
+        Pp = bt.p[4] + T.offP + (i64)tl.ks * T.M * T.N;
+        if (T.tri != 0 && row0 + TILE - 1 < col0) return;  // the reduction handles it
+    }
     if (T.triA == 1) kend = min(kend, row0 + TILE);
     if (T.triA == 2) kbase = max(kbase, row0 & ~(BK - 1));
     if (T.pad == 1) kbase = max(kbase, col0 & ~(BK - 1));
@@ -907,22 +917,22 @@ __global__ __launch_bounds__(2 * TPB) void k_gemm_tiles_lat(
     const int key = opa * 3 + T.opB;
     if (!chk) {
         switch (key) {
-        case 0: gemm_tile_lat<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 1: gemm_tile_lat<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 3: gemm_tile_lat<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        default: gemm_tile_lat<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 0: gemm_tile_lat<0, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile_lat<0, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile_lat<1, 0, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile_lat<1, 1, false>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     } else {
         switch (key) {
-        case 0: gemm_tile_lat<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 1: gemm_tile_lat<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 2: gemm_tile_lat<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 3: gemm_tile_lat<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 4: gemm_tile_lat<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 5: gemm_tile_lat<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 6: gemm_tile_lat<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        case 7: gemm_tile_lat<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
-        default: gemm_tile_lat<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, As, Bs); break;
+        case 0: gemm_tile_lat<0, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 1: gemm_tile_lat<0, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 2: gemm_tile_lat<0, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 3: gemm_tile_lat<1, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 4: gemm_tile_lat<1, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 5: gemm_tile_lat<1, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 6: gemm_tile_lat<2, 0, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        case 7: gemm_tile_lat<2, 1, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
+        default: gemm_tile_lat<2, 2, true>(T, A, B, C, row0, col0, kbase, kend, Pp, As, Bs); break;
         }
     }
 }

@@ -63,7 +63,7 @@ __global__ void k_chol_dag(const int4* list, double* blkval, double* upd, double
                            unsigned long long* status, int spin_limit, i64* trace);
 __global__ void k_fill_int(int* p, i64 n, int v);
 __global__ void k_gemm_tiles(const GemmTile* tiles, const GemmTask* tasks, BufTab bt);
-// launches of at most LAT_TILES tiles without split-K: 512 threads, K split
+// launches of at most LAT_TILES tiles (split-K slices included): 512 threads, K split
 // between two wave groups (the Launch's block size selects it)
 #define LAT_TILES 256
 // single-tile GEMM tasks with K at most this go to k_gemm_small (a workgroup each)

@@ -195,13 +195,10 @@ void push_gemm(chp_plan* P, std::vector<Launch>& seq, const std::vector<GemmTile
     if (tiles.empty()) return;
     P->n_gemm_tiles += (i64)tiles.size();
     // launches too small to fill the device take the latency-oriented tile
-    // kernel (512 threads), unless a tile is a split-K slice
-    bool split = false, narrow = !schur;
-    for (const GemmTile& g : tiles) {
-        split = split || P->pg->tasks[g.task].ksplit > 1;
-        narrow = narrow && narrow_task(P->pg->tasks[g.task]);
-    }
-    const int block = (!narrow && !split && (int)tiles.size() <= LAT_TILES) ? 512 : 256;
+    // kernel (512 threads), split-K slices included
+    bool narrow = !schur;
+    for (const GemmTile& g : tiles) narrow = narrow && narrow_task(P->pg->tasks[g.task]);
+    const int block = (!narrow && (int)tiles.size() <= LAT_TILES) ? 512 : 256;
     Launch L{narrow ? K_GEMM_NARROW : K_GEMM, (int)tiles.size(), block, 0, P->pg->lists.add(tiles),
              schur};
     double fl = 0.0;
