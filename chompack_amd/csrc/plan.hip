@@ -1393,13 +1393,15 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     auto build_seq = [&](std::vector<Launch>& seq, bool dag) {
         int nbig = 0;
         for (int L = 0; L < H; ++L) nbig += has_large(L);
-        // the conversions of every level but the last with large fronts run
-        // together, forked once before that level (each fork of side stream 2
-        // from the sweep's stream costs the sweep's next launch about 9 us in
-        // a graph replay; the last level is long enough to hide them).
-        // Measured on the 40^3 grid: 13.78 ms per cholesky + projected
-        // inverse step against 13.99 ms with a fork per level, and 14.20 ms
-        // with every conversion after the last level.
+        // the conversions of the levels with large fronts run in groups of
+        // four levels, forked after the group's last level, and the group
+        // before the last level is forked before that level (each fork of
+        // side stream 2 from the sweep's stream costs the sweep's next launch
+        // about 9 us in a graph replay).  Measured on the 40^3 grid (cholesky
+        // + projected inverse) / 512 batched grids (cholesky): a fork per
+        // level 13.99 / 8.27 ms; everything but the last level in one group
+        // 13.78 / 8.34 ms; groups of four 13.67 / 8.10 ms; every conversion
+        // after the last level 14.20 ms.
         const int flush_at = std::max(0, nbig - 1);
         std::vector<int> pending;
         auto flush = [&]() {
@@ -1423,7 +1425,7 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
             if (big && bi == flush_at) flush();
             build_chol_level(P, P->bylevel[L], seq, dag, pending);
             if (big) {
-                if (bi >= flush_at) flush();
+                if (bi >= flush_at || (bi + 1) % 4 == 0) flush();
                 ++bi;
             }
             ++L;
