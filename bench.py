@@ -28,7 +28,7 @@ KINDS = ["chol_small", "extadd", "potrf_diag", "panel_factor", "gemm", "extract"
          "copy_top", "splitk_reduce", "diag", "matop", "update_factor", "sgather_fwd",
          "sgather_bwd", "sscatter", "trmm_fwd", "trmm_bwd", "trmm_finish", "completion_chain",
          "projinv_chain", "chol_dag", "fill_int", "gemm_small", "chol_chain", "llt_chain", "uf_setup",
-         "uf_step", "gemm_narrow", "chol_head"]
+         "uf_step", "gemm_narrow"]
 NK = len(KINDS)
 
 

@@ -6,7 +6,6 @@
 // block layout of blkval so block loads/stores are straight copies.
 #include "common.h"
 #include "kernels.h"
-#include "extadd_tile.h"
 
 #define TPB 256
 
@@ -115,28 +114,6 @@ __global__ __launch_bounds__(TPB) void k_chol_small(
     for (int q = blockIdx.x; q < count; q += gridDim.x) {
         chol_small_front(fronts[q], fi, blkval, upd, relidx, chidx, status);
         __syncthreads();  // the front's LDS is free before the next one loads
-    }
-}
-
-// A cholesky level's small fronts and its large fronts' extend-add in one
-// launch (the two read only the level's children and write disjoint front
-// and update storage): workgroups 0 .. nsg - 1 walk the small fronts as
-// k_chol_small does, the others take one extend-add tile each.  One launch
-// instead of a second stream saves the sweep the fork and the join, each
-// about 9 us in a graph replay.  Dynamic LDS: the larger of the two needs.
-__global__ __launch_bounds__(TPB) void k_chol_head(
-    const int* __restrict__ fronts, int count, int nsg, const int4* __restrict__ tiles,
-    const FrontInfo* __restrict__ fi, double* __restrict__ blkval, double* __restrict__ upd,
-    const int* __restrict__ relidx, const int* __restrict__ chidx,
-    unsigned long long* status) {
-    extern __shared__ double S[];
-    if ((int)blockIdx.x < nsg) {
-        for (int q = blockIdx.x; q < count; q += nsg) {
-            chol_small_front(fronts[q], fi, blkval, upd, relidx, chidx, status);
-            __syncthreads();
-        }
-    } else {
-        extadd_tile(tiles[blockIdx.x - nsg], tiles, fi, blkval, upd, relidx, 1, S);
     }
 }
 

@@ -17,10 +17,6 @@ __global__ void k_status_init(unsigned long long* status);
 __global__ void k_chol_small(const int* fronts, int count, const FrontInfo* fi, double* blkval,
                              double* upd, const int* relidx, const int* chidx,
                              unsigned long long* status);
-// small fronts (workgroups < nsg) and extend-add tiles of one cholesky level
-__global__ void k_chol_head(const int* fronts, int count, int nsg, const int4* tiles,
-                            const FrontInfo* fi, double* blkval, double* upd, const int* relidx,
-                            const int* chidx, unsigned long long* status);
 __global__ void k_extract(const int4* tiles, const FrontInfo* fi, double* blkval,
                           double* upd, const int* relidx, i64 bstride, i64 ustride,
                           const double* scratch, const i64* tptr);

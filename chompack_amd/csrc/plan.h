@@ -16,10 +16,10 @@ enum Kid {
     K_COPY_TOP, K_SPLITK_REDUCE, K_DIAG, K_MATOP, K_UPDATE_FACTOR,
     K_SGATHER_F, K_SGATHER_B, K_SSCATTER, K_TRMM_FWD, K_TRMM_BWD, K_TRMM_FIN, K_COMP_CHAIN,
     K_PROJINV_CHAIN, K_CHOL_DAG, K_FILL_INT, K_GEMM_SMALL, K_CHOL_CHAIN, K_LLT_CHAIN,
-    K_UF_SETUP, K_UF_STEP, K_GEMM_NARROW, K_CHOL_HEAD,
+    K_UF_SETUP, K_UF_STEP, K_GEMM_NARROW,
     K_FORK, K_JOIN
 };
-const int NKIDS = 33;  // kernel kinds (K_FORK / K_JOIN are stream markers)
+const int NKIDS = 32;  // kernel kinds (K_FORK / K_JOIN are stream markers)
 
 struct Launch {
     Kid kid;
@@ -36,8 +36,6 @@ struct Launch {
     int gy = 1;          // extend-add / extract: instances along grid y, their
     i64 bstride = 0;     // blkval and update arenas bstride / ustride doubles apart
     i64 ustride = 0;
-    i64 list2 = 0;       // cholesky level head: the extend-add tile list,
-    int iarg2 = 0;       // and the number of small-front workgroups
 };
 
 struct ListBuf {

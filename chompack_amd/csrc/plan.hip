@@ -571,27 +571,19 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr, std::vector<Launc
         else large.push_back(k);
     }
     // small fronts (LDS kernel) and large fronts of one level write disjoint
-    // front and update storage, and the large fronts' extend-add reads only
-    // the children, as the small fronts do: with both, the small fronts and
-    // the extend-add share one launch (k_chol_head, small-front workgroups
-    // first: they run longest)
-    if (large.empty()) {
-        if (!small.empty()) seq.push_back(small_launch(P, K_CHOL_SMALL, small, lds));
-        return;
+    // front and update storage: the small-front launch runs on the side stream
+    const bool fork = !small.empty() && !large.empty();
+    if (fork) seq.push_back({K_FORK, 0, 0, 0, 0, 1});
+    if (!small.empty()) {
+        Launch Ls = small_launch(P, K_CHOL_SMALL, small, lds);
+        Ls.side = fork ? 1 : 0;
+        seq.push_back(Ls);
     }
+    if (large.empty()) return;
     if (dag) P->n_large_chol += (int)large.size();
     int net = 0;
     const std::vector<int4> et = extadd_tiles(P, large, &net);
-    if (small.empty()) {
-        seq.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1});
-    } else {
-        Launch H = small_launch(P, K_CHOL_HEAD, small, lds);
-        H.iarg2 = H.grid;
-        H.grid += net;
-        H.lds = std::max(H.lds, (size_t)(256 * 17 * sizeof(double)));
-        H.list2 = P->pg->lists.add(et);
-        seq.push_back(H);
-    }
+    seq.push_back({K_EXTADD, net, 256, 0, P->pg->lists.add(et), 1});
     // Two-level blocking: 64-wide inner panels (each launch first applies the
     // outer panel's earlier inner panels to its own columns, left-looking,
     // then factors the diagonal block and solves the panel), then one
@@ -670,6 +662,7 @@ void build_chol_level(chp_plan* P, const std::vector<int>& fr, std::vector<Launc
         push_gemm(P, seq, gt, 1);
     }
     convert.insert(convert.end(), large.begin(), large.end());
+    if (fork) seq.push_back({K_JOIN, 0, 0, 0, 0, 1});
 }
 
 // Projected-inverse launches for one level (projected_inverse.c:25-73).
@@ -999,8 +992,6 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             continue;
         }
         const hipStream_t s = (two && L.side) ? P->sstream[L.side] : s0;
-        Kid rec = L.kid;  // the kind and grid a profiled launch is recorded under
-        int rec_grid = L.grid;
         const void* list = pg.d_lists + L.list;
         const FrontInfo* fi = L.fisel ? P->d_fi_full : P->d_fi;
         double* blkval = bt.p[L.bsel];
@@ -1009,29 +1000,6 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
         case K_CHOL_SMALL:
             CHP_LAUNCH(k_chol_small, L.lds, (const int*)list, L.iarg, fi, blkval,
                        P->d_upd, P->d_relidx, P->d_chidx, P->d_status);
-            break;
-        case K_CHOL_HEAD:
-            if (P->prof) {  // timed as its two parts, one after the other
-                const dim3 gs(L.iarg2), ge(L.grid - L.iarg2);
-                hipExtLaunchKernelGGL(k_chol_small, gs, b, L.lds, s, P->ev0, P->ev1, 0,
-                                      (const int*)list, L.iarg, fi, blkval, P->d_upd,
-                                      P->d_relidx, P->d_chidx, P->d_status);
-                HIP_CHECK(hipEventSynchronize(P->ev1));
-                float ms = 0.f;
-                HIP_CHECK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
-                P->prof_ms[K_CHOL_SMALL] += ms;
-                P->prof_cnt[K_CHOL_SMALL] += 1;
-                P->prof_rows.insert(P->prof_rows.end(), {(double)K_CHOL_SMALL, (double)L.iarg2, 0.0, (double)ms});
-                hipExtLaunchKernelGGL(k_extadd_tiles, ge, b, 0, s, P->ev0, P->ev1, 0,
-                                      (const int4*)(pg.d_lists + L.list2), fi, blkval,
-                                      bt.p[1], P->d_relidx, P->d_chidx, 1, (i64)0, (i64)0);
-                rec = K_EXTADD;
-                rec_grid = L.grid - L.iarg2;
-            } else {
-                CHP_LAUNCH(k_chol_head, L.lds, (const int*)list, L.iarg, L.iarg2,
-                           (const int4*)(pg.d_lists + L.list2), fi, blkval, P->d_upd,
-                           P->d_relidx, P->d_chidx, P->d_status);
-            }
             break;
         case K_EXTADD:
             CHP_LAUNCH(k_extadd_tiles, 0, (const int4*)list, fi, blkval,
@@ -1137,10 +1105,10 @@ int run_seq(chp_plan* P, const Program& pg, const std::vector<Launch>& seq, cons
             HIP_CHECK(hipEventSynchronize(P->ev1));
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
-            P->prof_ms[rec] += ms;
-            P->prof_cnt[rec] += 1;
-            P->prof_flops[rec] += L.flops;
-            P->prof_rows.insert(P->prof_rows.end(), {(double)rec, (double)rec_grid, L.flops, (double)ms});
+            P->prof_ms[L.kid] += ms;
+            P->prof_cnt[L.kid] += 1;
+            P->prof_flops[L.kid] += L.flops;
+            P->prof_rows.insert(P->prof_rows.end(), {(double)L.kid, (double)L.grid, L.flops, (double)ms});
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -1317,7 +1285,7 @@ chp_plan* chp_plan_create_params(i64 n, i64 nsn, const i64* snpost, const i64* s
     P->lds_cap = std::min<size_t>(param(3) > 0 ? (size_t)param(3) : 64 * 1024, lds_block);
     if (P->lds_cap > 64 * 1024) {  // small-front kernels may take more than 64 KB of LDS
         for (const void* k : {(const void*)k_chol_small, (const void*)k_projinv_small,
-                              (const void*)k_llt_small, (const void*)k_chol_head})
+                              (const void*)k_llt_small})
             if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)P->lds_cap) != hipSuccess)
                 return fail("the device refuses " + std::to_string(P->lds_cap) +
